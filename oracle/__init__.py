@@ -1,0 +1,295 @@
+"""The oracle: ctypes bindings to the f64 C restatement of the reference twin (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY. The shipped library never imports this package; only tests/, the smoke check and the
+bench's CPU-baseline leg use it, and only as the checker. See sro.h for what each C function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+NS = 10
+COMPONENTS = 14
+EDGE_BOUNCE, EDGE_LEAVE = 0, 1
+OPACITY_REFERENCE, OPACITY_KRAMERS = 0, 1
+
+
+class Physics(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("G", C.c_double), ("dx", C.c_double),
+        ("sigma_vac", C.c_double), ("sigma_floor", C.c_double), ("t_floor", C.c_double),
+        ("cfl", C.c_double), ("eta_g", C.c_double),
+        ("a_r", C.c_double), ("c_sb", C.c_double), ("kappa0", C.c_double), ("k_cond", C.c_double),
+        ("kappa_dust", C.c_double), ("t_dust", C.c_double),
+        ("opacity", C.c_int), ("mu", C.c_double), ("edge", C.c_int),
+        ("pin_temperature", C.c_int), ("constant_diffusivity", C.c_double), ("flux_limiter", C.c_int),
+        ("max_padded", C.c_double),
+    ]
+
+
+class HeatReport(C.Structure):
+    _fields_ = [("passes", C.c_int), ("rounds", C.c_int), ("converged", C.c_int),
+                ("delta", C.c_double), ("radiated", C.c_double)]
+
+
+def reference_physics(**over) -> Physics:
+    """The reference's assets/physics.json constants the path reads, gamma = 2, dx = 1, the reference opacity."""
+    d = dict(gamma=2.0, G=1.0, dx=1.0, sigma_vac=1e-6, sigma_floor=1e-8, t_floor=0.05, cfl=0.4, eta_g=0.3,
+             a_r=6.25e-7, c_sb=12800.0, kappa0=1.0, k_cond=0.0, kappa_dust=0.0, t_dust=0.0,
+             opacity=OPACITY_REFERENCE, mu=0.0, edge=EDGE_LEAVE, pin_temperature=0,
+             constant_diffusivity=0.0, flux_limiter=1, max_padded=2048.0)
+    d.update(over)
+    return Physics(**d)
+
+
+def _build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        _build()
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_sz = C.c_size_t
+_PP = C.POINTER(Physics)
+
+
+def _opt(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+lib.sro_padded_size.argtypes = [_sz, _sz, C.POINTER(_sz)]
+lib.sro_fft.argtypes = [_dp, _sz, C.c_int]
+lib.sro_fft2.argtypes = [_dp, _sz, _sz, C.c_int]
+lib.sro_grav_kernel.argtypes = [C.c_double, C.c_double, C.c_long, C.c_long]
+lib.sro_grav_kernel.restype = C.c_double
+lib.sro_gravity_field.argtypes = [_PP, _dp, _sz, _sz, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+lib.sro_hydro_dt.argtypes = [_PP, _dp, _sz, _sz, C.c_void_p, C.c_void_p]
+lib.sro_hydro_dt.restype = C.c_double
+lib.sro_hydro_step.argtypes = [_PP, _dp, _sz, _sz, C.c_double] + [C.c_void_p] * 7
+lib.sro_heat_step.argtypes = [_PP, _dp, _sz, _sz, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.POINTER(HeatReport)]
+lib.sro_heat_law.argtypes = [_PP, _dp, _sz, _sz, _dp, _dp]
+lib.sro_heat_first_system.argtypes = [_PP, _dp, _sz, _sz, C.c_double, _dp, _dp, _dp, _dp, _u8, C.c_void_p]
+lib.sro_flux_cells.argtypes = [_dp, _dp, _sz, _sz, _dp, _dp]
+lib.sro_fld_matvec.argtypes = [_sz, _sz, _dp, _dp, _dp, C.c_void_p, _dp, _dp]
+lib.sro_fld_pcg.argtypes = [_sz, _sz, _dp, _dp, _dp, C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int,
+                            C.c_int, C.c_int, C.POINTER(C.c_double)]
+lib.sro_mg_apply.argtypes = [_sz, _sz, _dp, _dp, _dp, C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_int]
+lib.sro_floors_apply.argtypes = [_PP, _dp, _sz, _sz, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+lib.sro_twin_new.argtypes = [_PP, _sz, _sz, _dp, C.c_int, C.c_int, C.c_int]
+lib.sro_twin_new.restype = C.c_void_p
+lib.sro_twin_free.argtypes = [C.c_void_p]
+lib.sro_twin_step.argtypes = [C.c_void_p]
+lib.sro_twin_state.argtypes = [C.c_void_p, _dp]
+lib.sro_twin_times.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_uint64)]
+lib.sro_twin_cap_dt_next.argtypes = [C.c_void_p, C.c_double]
+lib.sro_twin_last_heat.argtypes = [C.c_void_p, C.POINTER(HeatReport)]
+lib.sro_twin_escaped.argtypes = [C.c_void_p, _dp]
+lib.sro_lcg_fill.argtypes = [C.c_uint64, _sz, _dp]
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(code):
+    if code != 0:
+        raise OracleError(f"oracle refused: code {code}")
+
+
+def padded_size(n: int, max_padded: int = 2048) -> int:
+    out = _sz()
+    _check(lib.sro_padded_size(n, max_padded, C.byref(out)))
+    return out.value
+
+
+def fft(x: np.ndarray, inverse=False) -> np.ndarray:
+    d = np.ascontiguousarray(np.stack([x.real, x.imag], -1).astype(np.float64).ravel())
+    _check(lib.sro_fft(d, len(x), int(inverse)))
+    d = d.reshape(-1, 2)
+    return d[:, 0] + 1j * d[:, 1]
+
+
+def fft2(x: np.ndarray, inverse=False) -> np.ndarray:
+    ny, nx = x.shape
+    d = np.ascontiguousarray(np.stack([x.real, x.imag], -1).astype(np.float64).ravel())
+    _check(lib.sro_fft2(d, nx, ny, int(inverse)))
+    d = d.reshape(ny, nx, 2)
+    return d[..., 0] + 1j * d[..., 1]
+
+
+def grav_kernel(i: int, j: int, G=1.0, dx=1.0) -> float:
+    return lib.sro_grav_kernel(G, dx, i, j)
+
+
+def gravity_field(ph: Physics, sigma: np.ndarray, ringed=False):
+    """phi, gx, gy over the box (and phi on the box grown by one ring) — grav.rs:115-121."""
+    h, w = sigma.shape
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    phi, gx, gy = (np.empty((h, w)) for _ in range(3))
+    pr = np.empty((h + 2, w + 2))
+    _check(lib.sro_gravity_field(C.byref(ph), s, w, h, phi.ctypes.data, gx.ctypes.data, gy.ctypes.data,
+                                 pr.ctypes.data))
+    return (phi, gx, gy, pr) if ringed else (phi, gx, gy)
+
+
+def hydro_dt(ph: Physics, planes: np.ndarray, gx=None, gy=None) -> float:
+    _, h, w = planes.shape
+    p = np.ascontiguousarray(planes, dtype=np.float64)
+    gx = None if gx is None else np.ascontiguousarray(gx, dtype=np.float64)
+    gy = None if gy is None else np.ascontiguousarray(gy, dtype=np.float64)
+    return lib.sro_hydro_dt(C.byref(ph), p, w, h, _opt(gx), _opt(gy))
+
+
+def hydro_step(ph: Physics, planes: np.ndarray, dt: float, gravity=None, radiation=None, escaped=None,
+               want_mid=False):
+    """One P4 step (hydro.rs:649); gravity = (phi, gx, gy) of P2 on the start state; radiation = (fx, fy)."""
+    _, h, w = planes.shape
+    p = np.ascontiguousarray(planes, dtype=np.float64).copy()
+    phi = gx = gy = rfx = rfy = None
+    if gravity is not None:
+        phi, gx, gy = (np.ascontiguousarray(a, dtype=np.float64) for a in gravity)
+    if radiation is not None:
+        rfx, rfy = (np.ascontiguousarray(a, dtype=np.float64) for a in radiation)
+    mid = np.zeros((h, w)) if want_mid else None
+    _check(lib.sro_hydro_step(C.byref(ph), p, w, h, dt, _opt(phi), _opt(gx), _opt(gy), _opt(rfx), _opt(rfy),
+                              _opt(escaped), _opt(mid)))
+    return (p, mid) if want_mid else p
+
+
+def heat_step(ph: Physics, planes: np.ndarray, dt: float):
+    """One P5 step (heat.rs:453): returns (planes, report, flux_x, flux_y, emitted)."""
+    _, h, w = planes.shape
+    p = np.ascontiguousarray(planes, dtype=np.float64).copy()
+    fx = np.zeros((h, w + 1))
+    fy = np.zeros((h + 1, w))
+    em = np.zeros((h, w))
+    rep = HeatReport()
+    _check(lib.sro_heat_step(C.byref(ph), p, w, h, dt, fx.ctypes.data, fy.ctypes.data, em.ctypes.data,
+                             C.byref(rep)))
+    return p, rep, fx, fy, em
+
+
+def heat_law(ph: Physics, planes: np.ndarray):
+    _, h, w = planes.shape
+    p = np.ascontiguousarray(planes, dtype=np.float64)
+    fx = np.zeros((h, w + 1))
+    fy = np.zeros((h + 1, w))
+    _check(lib.sro_heat_law(C.byref(ph), p, w, h, fx, fy))
+    return fx, fy
+
+
+def flux_cells(fx_faces: np.ndarray, fy_faces: np.ndarray):
+    h, w1 = fx_faces.shape
+    w = w1 - 1
+    a, b = np.zeros((h, w)), np.zeros((h, w))
+    lib.sro_flux_cells(np.ascontiguousarray(fx_faces), np.ascontiguousarray(fy_faces), w, h, a, b)
+    return a, b
+
+
+def heat_first_system(ph: Physics, planes: np.ndarray, dt: float):
+    """The first Newton pass's system: own, kx, ky, rhs, gas, u0 (heat.rs:474-488 in face form)."""
+    _, h, w = planes.shape
+    p = np.ascontiguousarray(planes, dtype=np.float64)
+    own, rhs, u0 = np.zeros((h, w)), np.zeros((h, w)), np.zeros((h, w))
+    kx, ky = np.zeros((h, w + 1)), np.zeros((h + 1, w))
+    gas = np.zeros((h, w), np.uint8)
+    _check(lib.sro_heat_first_system(C.byref(ph), p, w, h, dt, own, kx, ky, rhs, gas, u0.ctypes.data))
+    return own, kx, ky, rhs, gas, u0
+
+
+def fld_matvec(own, kx, ky, gas, v):
+    h, w = own.shape
+    out = np.zeros((h, w))
+    g = None if gas is None else np.ascontiguousarray(gas, np.uint8)
+    lib.sro_fld_matvec(w, h, own, kx, ky, None if g is None else g.ctypes.data, np.ascontiguousarray(v), out)
+    return out
+
+
+def fld_pcg(own, kx, ky, gas, b, x0=None, tol=1e-8, max_iter=10000, precond="mg", levels=0, nu1=2, nu2=2):
+    h, w = own.shape
+    x = np.zeros((h, w)) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+    g = None if gas is None else np.ascontiguousarray(gas, np.uint8)
+    rel = C.c_double()
+    it = lib.sro_fld_pcg(w, h, np.ascontiguousarray(own), np.ascontiguousarray(kx), np.ascontiguousarray(ky),
+                         None if g is None else g.ctypes.data, np.ascontiguousarray(b), x, tol, max_iter,
+                         1 if precond == "mg" else 0, levels, nu1, nu2, C.byref(rel))
+    return x, it, rel.value
+
+
+def mg_apply(own, kx, ky, gas, r, levels=0, nu1=2, nu2=2):
+    h, w = own.shape
+    z = np.zeros((h, w))
+    g = None if gas is None else np.ascontiguousarray(gas, np.uint8)
+    lib.sro_mg_apply(w, h, np.ascontiguousarray(own), np.ascontiguousarray(kx), np.ascontiguousarray(ky),
+                     None if g is None else g.ctypes.data, np.ascontiguousarray(r), z, levels, nu1, nu2)
+    return z
+
+
+def floors_apply(ph: Physics, planes: np.ndarray):
+    _, h, w = planes.shape
+    p = np.ascontiguousarray(planes, dtype=np.float64).copy()
+    vr, fa = C.c_double(), C.c_double()
+    lib.sro_floors_apply(C.byref(ph), p, w, h, C.byref(vr), C.byref(fa))
+    return p, vr.value, fa.value
+
+
+def lcg(seed: int, count: int) -> np.ndarray:
+    """The reference's 64-bit LCG (fft.rs:292-300), uniform on [0, 1)."""
+    out = np.empty(count)
+    lib.sro_lcg_fill(seed, count, out)
+    return out
+
+
+class Twin:
+    """The twin's step driver (mod.rs:129-297) over f64 planes shaped (14, h, w)."""
+
+    def __init__(self, ph: Physics, planes: np.ndarray, gravity=True, hydro=True, heat=False):
+        _, self.h, self.w = planes.shape
+        p = np.ascontiguousarray(planes, dtype=np.float64)
+        self._t = lib.sro_twin_new(C.byref(ph), self.w, self.h, p, int(gravity), int(hydro), int(heat))
+
+    def step(self, n=1):
+        for _ in range(n):
+            _check(lib.sro_twin_step(self._t))
+
+    def state(self) -> np.ndarray:
+        out = np.empty((COMPONENTS, self.h, self.w))
+        lib.sro_twin_state(self._t, out)
+        return out
+
+    def times(self):
+        dt, dtn, st = C.c_double(), C.c_double(), C.c_double()
+        steps = C.c_uint64()
+        lib.sro_twin_times(self._t, C.byref(dt), C.byref(dtn), C.byref(st), C.byref(steps))
+        return dt.value, dtn.value, st.value, steps.value
+
+    def cap_dt_next(self, cap):
+        lib.sro_twin_cap_dt_next(self._t, cap)
+
+    def last_heat(self) -> HeatReport:
+        r = HeatReport()
+        lib.sro_twin_last_heat(self._t, C.byref(r))
+        return r
+
+    def escaped(self) -> np.ndarray:
+        out = np.zeros((COMPONENTS, 2 * (self.w + self.h)))
+        lib.sro_twin_escaped(self._t, out)
+        return out
+
+    def __del__(self):
+        if getattr(self, "_t", None):
+            lib.sro_twin_free(self._t)
+            self._t = None
