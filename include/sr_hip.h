@@ -1,0 +1,173 @@
+/* sr_hip.h — the C ABI of the MI355X-native grid-physics path (P2 FFT gravity, P4 MUSCL-Hancock/HLL hydro,
+ * P5 implicit flux-limited diffusion by MG-PCG) of a sandbox_reactions-style engine.
+ *
+ * The reference engine has no plugin registry: a pass is a Rust type contributing dispatch lists to `Step`, and the
+ * operator API is `Step`'s methods (crates/sr-engine/src/step/mod.rs:120-169, :367-527, :691-777). This header is
+ * the C surface a HIP backend crate would bind through `extern "C"`; every entry point names the reference
+ * interface it replaces. Plain pointers and sizes only. All functions return SR_OK (0) or a negative SR_ERR_* code
+ * and leave a message for sr_last_error(); none aborts across the boundary.
+ *
+ * Ownership: the caller owns host arrays and passes their lengths; a world owns its device memory until
+ * sr_destroy. Functions taking `*_dev` pointers read and write device memory the caller owns, on the caller's
+ * stream (a hipStream_t passed as void*, NULL = the default stream).
+ * Threading: a world is confined to one host thread; it runs on its own hipStream. sr_read_* calls block.
+ */
+#ifndef SR_HIP_H
+#define SR_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: the reference's named refusals (crates/sr-engine/src/step/fft.rs:35-51, state.rs:93-100,
+ * gpu.rs:45-80) as integers */
+#define SR_OK 0
+#define SR_ERR_WORLD_SIZE (-1)  /* StateError::WorldSize: an axis not a multiple of 8 or outside [8, 4096] */
+#define SR_ERR_BOX_AXIS (-2)    /* FftError::BoxAxis / BoxOutOfRange: a padded FFT axis past the cap */
+#define SR_ERR_LENGTH (-3)      /* FftError::LengthMismatch: a buffer whose length is not what the call needs */
+#define SR_ERR_DEVICE (-4)      /* a HIP runtime error (the reference's device error, exit 3) */
+#define SR_ERR_ARGUMENT (-5)    /* a null handle or pointer, or a parameter out of range */
+#define SR_ERR_NOT_POW2 (-6)    /* FftError::NotPowerOfTwo */
+#define SR_ERR_NON_FINITE (-7)  /* the non-finite guard tripped (the reference's exit 5) */
+
+#define SR_COMPONENTS 14 /* Sigma, Sigma u, Sigma v, E, then 10 species fractions (state.rs:26-46) */
+#define SR_SPECIES 10
+
+enum sr_edge { SR_EDGE_BOUNCE = 0, SR_EDGE_LEAVE = 1 };
+enum sr_opacity { SR_OPACITY_REFERENCE = 0, SR_OPACITY_KRAMERS = 1 };
+enum sr_precond { SR_PRECOND_JACOBI = 0, SR_PRECOND_MULTIGRID = 1, SR_PRECOND_LEARNED = 2 };
+enum sr_stop { SR_STOP_REFERENCE = 0, SR_STOP_RELATIVE = 1 };
+enum sr_field { SR_FIELD_PHI = 0, SR_FIELD_GX = 1, SR_FIELD_GY = 2, SR_FIELD_FLUX_X = 3, SR_FIELD_FLUX_Y = 4,
+                SR_FIELD_SIGMA_MID = 5, SR_FIELD_PHI_MID = 6 };
+
+/* The constants the path reads (assets/physics.json keys, plus the generalisations the workloads need: gamma,
+ * dx, G, the opacity law and a fixed mu). Defaults via sr_physics_defaults() are the reference's. */
+typedef struct {
+    double gamma;       /* adiabatic index; the reference hard-wires 2 (crates/sr-physics/src/eos.rs:207-247) */
+    double G;           /* gravitational constant (reference/grav.rs:19) */
+    double dx;          /* cell side; the reference's cells are 1 */
+    double sigma_vac, sigma_floor, t_floor;
+    double cfl, eta_g;
+    double a_r, c_sb, kappa0, k_cond, kappa_dust, t_dust;
+    int opacity;        /* enum sr_opacity */
+    double mu;          /* > 0: fixed mean molecular weight; <= 0: from the species (registry.rs:146-154) */
+    int edge;           /* enum sr_edge */
+    int pin_temperature;         /* heat test flags (reference/heat.rs:62-74) */
+    double constant_diffusivity; /* > 0 switches it on */
+    int flux_limiter;
+    int max_padded;     /* FFT cap per padded axis; the reference's is 2048 (step/fft.rs:26-63), this build's 8192 */
+} sr_physics;
+
+typedef struct {
+    int gravity, hydro, heat; /* scene switches (reference/mod.rs:47-55) */
+} sr_switches;
+
+/* FLD solver settings (the heat pass's stopping rules and multigrid shape) */
+typedef struct {
+    int stop;              /* enum sr_stop: the reference's two energy-weighted measures, or ||b-Ax||/||b|| */
+    double solve_tol;      /* the reference GPU's 1e-5 (heat.wgsl:355-360) or a relative residual such as 1e-8 */
+    double newton_heat;    /* Newton pass stop on sum C|dT| / sum C T0 (reference GPU 1e-6) */
+    double newton_cell;    /* Newton pass stop per cell |dT| / T (reference GPU 1e-3) */
+    int max_passes;        /* reference 128 */
+    int max_rounds;        /* reference 4096 */
+    int levels;            /* multigrid levels, 0 = down to 4x4 */
+    int nu1, nu2;          /* red-black Gauss-Seidel sweeps before / after the coarse correction */
+} sr_heat_settings;
+
+typedef struct {
+    double dt, dt_next, sim_time; /* the step's dt, the next step's, sim time in f64 */
+    uint64_t steps;
+    int non_finite;               /* 1 when the guard found a non-finite value */
+    int nf_x, nf_y, nf_channel;
+} sr_dt_report;
+
+typedef struct {
+    double dt;
+    int passes, rounds, converged;
+    double delta, radiated;
+    double final_residual;        /* ||b - A x|| / ||b|| of the last pass, FP64 */
+} sr_heat_report;
+
+typedef struct {
+    uint64_t steps;
+    double step_ms;               /* mean wall ms per step over the timed steps */
+    double pass_ms[10];           /* per pass, the reference's Pass::ORDER slots P0..P9 (step/mod.rs:200-219) */
+} sr_step_cost;
+
+typedef struct sr_world sr_world;
+typedef struct sr_gravity sr_gravity;
+typedef struct sr_fld sr_fld;
+
+/* ------------------------------------------------------------------------------------------------ errors / device */
+const char *sr_last_error(void);
+int sr_device_info(char *name, size_t name_len, int *cu_count, size_t *hbm_bytes);
+void sr_physics_defaults(sr_physics *ph);
+void sr_heat_settings_defaults(sr_heat_settings *hs);
+
+/* ------------------------------------------------------------------------------------------------ P2 gravity / FFT */
+/* Fft::transform (crates/sr-engine/src/step/fft.rs:116-134): an in-place complex 2D transform of an nx x ny
+ * row-major interleaved (re, im) f32 array in device memory; forward unnormalised, inverse divided by nx*ny. */
+int sr_fft2(float *data_dev, int nx, int ny, int inverse, void *stream);
+
+/* Convolution::new + Gravity::new (step/fft.rs:136-182, step/grav.rs:133-200; twin reference/grav.rs:99-121):
+ * a gravity solver for a w x h box, its kernel's spectrum built once (f64 on the host, stored f32). */
+int sr_gravity_create(int w, int h, double G, double dx, double sigma_vac, int max_padded, sr_gravity **out);
+/* Gravity::field (reference/grav.rs:115-121): phi on the box grown by one ring ((w+2) x (h+2)), and g = -grad phi
+ * over the box (gx, gy may be NULL). sigma_dev is the w x h surface density; cells below sigma_vac are no source. */
+int sr_gravity_field(sr_gravity *g, const float *sigma_dev, float *phi_ringed_dev, float *gx_dev, float *gy_dev,
+                     void *stream);
+void sr_gravity_destroy(sr_gravity *g);
+
+/* ------------------------------------------------------------------------------------------------ P5 standalone FLD */
+/* The heat pass's linear system (step/heat.rs:609-673 heat_first_system / apply_heat_precond): a solver for the
+ * 5-point SPD operator (A v)_i = own_i v_i + sum_f k_f (v_i - v_j) over a w x h grid, face coefficients kx
+ * ((w+1) x h, face x between cells x-1 and x) and ky (w x (h+1)), couplings to non-gas cells and the edge already
+ * folded into own. gas_dev (w x h bytes, may be NULL: all gas) marks the rows in the system. */
+int sr_fld_create(int w, int h, const sr_heat_settings *hs, sr_fld **out);
+int sr_fld_set_system(sr_fld *f, const float *own_dev, const float *kx_dev, const float *ky_dev,
+                      const uint8_t *gas_dev, void *stream);
+/* Solves A x = b by preconditioned flexible CG with FP64 x, r and dots, FP32 preconditioner, to
+ * ||b - A x||_2 <= tol ||b||_2 (FP64 true residual); x_dev holds the start and receives the answer. */
+int sr_fld_solve(sr_fld *f, const double *b_dev, double *x_dev, double tol, int max_iter, int precond,
+                 int *iterations, double *final_rel, void *stream);
+/* z = M^-1 r for one preconditioner application (symmetry / positivity tests) */
+int sr_fld_apply_precond(sr_fld *f, const float *r_dev, float *z_dev, int precond, void *stream);
+/* out = A v in FP64 */
+int sr_fld_matvec(sr_fld *f, const double *v_dev, double *out_dev, void *stream);
+int sr_fld_levels(sr_fld *f, int *levels);
+void sr_fld_destroy(sr_fld *f);
+
+/* ------------------------------------------------------------------------------------------------ the world / step */
+/* Step::with_heat (step/mod.rs:415-527) + State::new (state.rs:139): refuses an axis not a multiple of 8 in
+ * [8, 4096] (StateError::WorldSize) and a padded FFT axis past max_padded (FftError::BoxAxis). */
+int sr_world_create(int w, int h, const sr_physics *ph, const sr_switches *sw, const sr_heat_settings *hs,
+                    sr_world **out);
+/* State::upload (state.rs:251): 14 canonical f32 planes, channel-major, row-major; n = 14 w h. Resets the clock
+ * and solves P2 on the state for the first dt (reference/mod.rs:173-175). */
+int sr_upload(sr_world *wd, const float *planes, size_t n);
+/* State::readback (state.rs:273) */
+int sr_readback(sr_world *wd, float *planes, size_t n);
+/* Step::run (step/mod.rs:703): `steps` steps, P1 -> P2 -> P4 -> P5 -> P8 -> P9 (reference/mod.rs:264-297) */
+int sr_run(sr_world *wd, uint32_t steps);
+int sr_read_dt(sr_world *wd, sr_dt_report *out);              /* Step::read_dt (step/mod.rs:691) */
+int sr_read_heat(sr_world *wd, sr_heat_report *out);          /* read_heat (step/heat.rs:499) */
+int sr_read_field(sr_world *wd, int field, float *out, size_t n); /* side fields: phi, g, F (step/heat.rs:549) */
+/* FP64 totals of the 14 conserved quantities (Sigma, Sigma u, Sigma v, E, each species' mass) and what has left
+ * through the edges (reference/hydro.rs:210-224, :954-976) */
+int sr_read_totals(sr_world *wd, double *totals, double *escaped);
+int sr_set_precond(sr_world *wd, int precond);                /* set_heat_precond (step/heat.rs:595) */
+int sr_set_dt_cap(sr_world *wd, double cap);                  /* Twin::cap_dt_next (reference/mod.rs:225) */
+int sr_set_timing(sr_world *wd, int on);                      /* Step::set_timing (step/mod.rs:722) */
+int sr_read_cost(sr_world *wd, sr_step_cost *out);            /* Step::read_cost (step/mod.rs:753) */
+void sr_destroy(sr_world *wd);
+
+/* Timing of the dominant kernel inside the last sr_run with timing on: launches and mean device ms per launch,
+ * measured with HIP events on the world's stream; `kernel` names: "hydro_update", "fft_columns", "mg_smooth". */
+int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launches, double *mean_ms, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

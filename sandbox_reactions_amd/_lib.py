@@ -1,0 +1,113 @@
+"""ctypes declarations of include/sr_hip.h (the library is libsrhip.so in this directory)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsrhip.so")
+
+SR_OK = 0
+ERRORS = {
+    -1: "WorldSize", -2: "BoxAxis", -3: "LengthMismatch", -4: "Device", -5: "Argument", -6: "NotPowerOfTwo",
+    -7: "NonFinite",
+}
+SR_ERR_WORLD_SIZE, SR_ERR_BOX_AXIS, SR_ERR_LENGTH, SR_ERR_DEVICE, SR_ERR_ARGUMENT, SR_ERR_NOT_POW2, \
+    SR_ERR_NON_FINITE = -1, -2, -3, -4, -5, -6, -7
+
+EDGE_BOUNCE, EDGE_LEAVE = 0, 1
+OPACITY_REFERENCE, OPACITY_KRAMERS = 0, 1
+PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
+STOP_REFERENCE, STOP_RELATIVE = 0, 1
+FIELD_PHI, FIELD_GX, FIELD_GY, FIELD_FLUX_X, FIELD_FLUX_Y, FIELD_SIGMA_MID, FIELD_PHI_MID = range(7)
+COMPONENTS = 14
+
+
+class Physics(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("G", C.c_double), ("dx", C.c_double),
+        ("sigma_vac", C.c_double), ("sigma_floor", C.c_double), ("t_floor", C.c_double),
+        ("cfl", C.c_double), ("eta_g", C.c_double),
+        ("a_r", C.c_double), ("c_sb", C.c_double), ("kappa0", C.c_double), ("k_cond", C.c_double),
+        ("kappa_dust", C.c_double), ("t_dust", C.c_double),
+        ("opacity", C.c_int), ("mu", C.c_double), ("edge", C.c_int),
+        ("pin_temperature", C.c_int), ("constant_diffusivity", C.c_double), ("flux_limiter", C.c_int),
+        ("max_padded", C.c_int),
+    ]
+
+
+class Switches(C.Structure):
+    _fields_ = [("gravity", C.c_int), ("hydro", C.c_int), ("heat", C.c_int)]
+
+
+class HeatSettings(C.Structure):
+    _fields_ = [("stop", C.c_int), ("solve_tol", C.c_double), ("newton_heat", C.c_double),
+                ("newton_cell", C.c_double), ("max_passes", C.c_int), ("max_rounds", C.c_int),
+                ("levels", C.c_int), ("nu1", C.c_int), ("nu2", C.c_int)]
+
+
+class DtReport(C.Structure):
+    _fields_ = [("dt", C.c_double), ("dt_next", C.c_double), ("sim_time", C.c_double), ("steps", C.c_uint64),
+                ("non_finite", C.c_int), ("nf_x", C.c_int), ("nf_y", C.c_int), ("nf_channel", C.c_int)]
+
+
+class HeatReport(C.Structure):
+    _fields_ = [("dt", C.c_double), ("passes", C.c_int), ("rounds", C.c_int), ("converged", C.c_int),
+                ("delta", C.c_double), ("radiated", C.c_double), ("final_residual", C.c_double)]
+
+
+class StepCost(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("step_ms", C.c_double), ("pass_ms", C.c_double * 10)]
+
+
+# (name, restype, argtypes): every entry point include/sr_hip.h declares
+_vp, _i, _d, _sz, _u32 = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_uint32
+_pp = C.POINTER(C.c_void_p)
+SIGNATURES = [
+    ("sr_last_error", C.c_char_p, []),
+    ("sr_device_info", _i, [C.c_char_p, _sz, C.POINTER(_i), C.POINTER(_sz)]),
+    ("sr_physics_defaults", None, [C.POINTER(Physics)]),
+    ("sr_heat_settings_defaults", None, [C.POINTER(HeatSettings)]),
+    ("sr_fft2", _i, [_vp, _i, _i, _i, _vp]),
+    ("sr_gravity_create", _i, [_i, _i, _d, _d, _d, _i, _pp]),
+    ("sr_gravity_field", _i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("sr_gravity_destroy", None, [_vp]),
+    ("sr_fld_create", _i, [_i, _i, C.POINTER(HeatSettings), _pp]),
+    ("sr_fld_set_system", _i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("sr_fld_solve", _i, [_vp, _vp, _vp, _d, _i, _i, C.POINTER(_i), C.POINTER(_d), _vp]),
+    ("sr_fld_apply_precond", _i, [_vp, _vp, _vp, _i, _vp]),
+    ("sr_fld_matvec", _i, [_vp, _vp, _vp, _vp]),
+    ("sr_fld_levels", _i, [_vp, C.POINTER(_i)]),
+    ("sr_fld_destroy", None, [_vp]),
+    ("sr_world_create", _i, [_i, _i, C.POINTER(Physics), C.POINTER(Switches), C.POINTER(HeatSettings), _pp]),
+    ("sr_upload", _i, [_vp, _vp, _sz]),
+    ("sr_readback", _i, [_vp, _vp, _sz]),
+    ("sr_run", _i, [_vp, _u32]),
+    ("sr_read_dt", _i, [_vp, C.POINTER(DtReport)]),
+    ("sr_read_heat", _i, [_vp, C.POINTER(HeatReport)]),
+    ("sr_read_field", _i, [_vp, _i, _vp, _sz]),
+    ("sr_read_totals", _i, [_vp, _vp, _vp]),
+    ("sr_set_precond", _i, [_vp, _i]),
+    ("sr_set_dt_cap", _i, [_vp, _d]),
+    ("sr_set_timing", _i, [_vp, _i]),
+    ("sr_read_cost", _i, [_vp, C.POINTER(StepCost)]),
+    ("sr_destroy", None, [_vp]),
+    ("sr_kernel_time", _i, [_vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_d), C.POINTER(_d)]),
+]
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise OSError(f"{path} is not built: run `python -m sandbox_reactions_amd.build` first")
+    lib = C.CDLL(path)
+    missing = []
+    for name, res, args in SIGNATURES:
+        try:
+            fn = getattr(lib, name)
+        except AttributeError:
+            missing.append(name)
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    lib.missing_symbols = missing
+    return lib

@@ -1,0 +1,73 @@
+// P5 linear solver: FP32 multigrid-preconditioned flexible CG inside FP64 iterative refinement.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace sr {
+
+// One level of the hierarchy: (A v)_i = own_i v_i + sum_f a_f (v_i - v_j); ax is (w+1) x h (face x between cells
+// x-1 and x, 0 at the edges), ay is w x (h+1).
+struct FldLevel {
+    int w = 0, h = 0;
+    float *own = nullptr, *ax = nullptr, *ay = nullptr, *z = nullptr, *b = nullptr, *zp = nullptr;
+};
+
+struct FldStats {
+    int iterations = 0, refinements = 0;
+    double rel = 0.0;      // ||b - A x|| / ||b|| in FP64
+    bool converged = false;
+    bool finite = true;
+};
+
+// The reference's energy-weighted stopping rule (crates/sr-physics/src/reference/heat.rs:424-431): r^T D^-1 r <=
+// u_bound and sum heat_i (r_i slope_i / D_i)^2 <= t_bound, D the diagonal own + sum of couplings.
+struct RefStop {
+    const float *heat = nullptr, *slope = nullptr;
+    double u_bound = 0.0, t_bound = 0.0;
+};
+
+class FldSolver {
+  public:
+    ~FldSolver();
+    int init(int w, int h, int levels, int nu1, int nu2);
+    // level 0's operator from device arrays (own, couplings ax/ay, gas mask for the coarsening); builds the
+    // Galerkin hierarchy. own/ax/ay are copied.
+    int set_system(const float *own, const float *ax, const float *ay, const uint8_t *gas, hipStream_t s);
+    // the operator is level 0's own arrays, filled in place by a caller kernel; then rebuild() the coarse levels
+    float *own0() { return lv[0].own; }
+    float *ax0() { return lv[0].ax; }
+    float *ay0() { return lv[0].ay; }
+    uint8_t *gas0() { return gas; }
+    int rebuild(bool faces, hipStream_t s);
+    // A x = b to ||b - Ax|| <= tol ||b|| (or the reference rule when ref != nullptr); x holds the start
+    int solve(const double *b, double *x, double tol, int max_iter, int precond, const RefStop *ref, FldStats *st,
+              hipStream_t s);
+    int apply_precond(const float *r, float *z, int precond, hipStream_t s);
+    int matvec64(const double *v, double *out, hipStream_t s);
+    int nlevels() const { return (int)lv.size(); }
+    int width() const { return w0; }
+    int height() const { return h0; }
+    double bytes_per_iteration() const;
+    // kernel timing of the level-0 smoother (the solver's dominant kernel)
+    void set_timing(bool on) { timing = on; }
+    bool timing = false;
+    double smooth_ms = 0.0;
+    unsigned long long smooth_launches = 0;
+
+  private:
+    int w0 = 0, h0 = 0, nu1 = 2, nu2 = 2, coarse_from = 0;
+    std::vector<FldLevel> lv;
+    uint8_t *gas = nullptr;
+    // level-0 vectors
+    double *r64 = nullptr;
+    float *r32 = nullptr, *za = nullptr, *zb = nullptr, *p = nullptr, *q = nullptr;
+    // device scalars, block partials and the last-block counters
+    double *sc = nullptr, *part = nullptr;
+    unsigned *counter = nullptr;
+    double *host_sc = nullptr; // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int vcycle(const float *r, float *z, const float *zold, bool dots, hipStream_t s);
+};
+
+} // namespace sr

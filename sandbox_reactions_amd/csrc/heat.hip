@@ -1,0 +1,445 @@
+// P5: one implicit (backward Euler) flux-limited-diffusion step in u = a_r T^4 with the Levermore-Pomraning limiter,
+// Newton passes on T(u), each pass's SPD system solved by FldSolver, then the delta-closure commit (or the limited
+// commit past the ceiling). The scheme is the reference's (crates/sr-physics/src/reference/heat.rs:453-592; GPU
+// crates/sr-engine/shaders/heat.wgsl:190-285, :547-565, :914-1160); the kernels are this library's own: one pass over
+// the cells for the sides, one over the faces for the law, one per Newton pass for the system and one for its update,
+// FP64 sums by the deterministic last-block reduction.
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+#include "heat.hpp"
+
+namespace sr {
+
+#define NT 256
+#define NS 10
+
+__constant__ float h_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f / 20.0f,
+                                13.0f / 24.0f, 15.0f / 28.0f, 17.0f / 32.0f, 27.0f / 56.0f, 1.0f};
+
+enum { HS_UB = 0, HS_TB, HS_HEAT0, HS_MOVED, HS_BAD, HS_RAD, HS_GAIN, HS_HELD, HS_DELTA, HS_N = 16 };
+enum { HC_SYS = 0, HC_UPD, HC_COMMIT, HC_N = 4 };
+
+__device__ __forceinline__ double u_of(const HeatParams &P, double t) { return P.cd ? t : P.a_r * t * t * t * t; }
+__device__ __forceinline__ double t_of(const HeatParams &P, double u) { return P.cd ? u : sqrt(sqrt(u / P.a_r)); }
+__device__ __forceinline__ double dt_du(const HeatParams &P, double t) {
+    return P.cd ? 1.0 : 1.0 / (4.0 * P.a_r * t * t * t);
+}
+
+// the sides of every cell (heat.rs:202-219): gas flag, T with the floor, kappa, 1/mu, heat capacity C
+__global__ void k_heat_cells(HeatParams P, const double *__restrict__ sigma, const float *__restrict__ mx,
+                             const float *__restrict__ my, const float *__restrict__ e, const float *__restrict__ x,
+                             HeatPlanes H, double *__restrict__ sums, double *part, unsigned *counter) {
+    const int n = P.w * P.h;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        float xs0 = x[i];
+        float im = 0.0f;
+        for (int s = 0; s < NS; s++) im += x[(size_t)s * n + i] * h_imu[s];
+        if (P.fixed_imu > 0.0f) im = P.fixed_imu;
+        const double sig = sigma[i];
+        const bool gas = sig >= (double)P.sigma_vac;
+        double t, s;
+        if (!gas) {
+            t = P.t_floor;
+            s = P.sigma_floor;
+        } else {
+            s = sig;
+            const double m0 = mx[i], m1 = my[i];
+            const double raw = (double)e[i] / s - 0.5 * (m0 * m0 + m1 * m1) / (s * s);
+            const double fl = (double)P.t_floor * im / P.gm1;
+            const double eps = raw < fl ? fl : raw;
+            t = P.gm1 * eps / im;
+        }
+        float kappa;
+        if (P.opacity == 1) {
+            kappa = (float)(P.kappa0 * s * pow(t, -3.5));
+        } else {
+            float dust = 0.0f;
+            if (t < P.t_dust) dust = P.kappa_dust * (1.0f - xs0 - x[(size_t)n + i] - x[(size_t)9 * n + i]);
+            kappa = P.kappa0 * (1.0f + xs0) + dust;
+        }
+        H.gas[i] = gas ? 1 : 0;
+        H.t0[i] = t;
+        H.t[i] = t;
+        H.kappa[i] = kappa;
+        H.imu[i] = im;
+        H.cap[i] = (float)(s * im / P.gm1);
+        H.sig[i] = (float)s;
+        if (gas) v[0] += (double)H.cap[i] * t;
+    }
+    double out[1];
+    if (reduce_finish<1>(v, part, gridDim.x, counter, out) && threadIdx.x == 0) sums[HS_HEAT0] = out[0];
+}
+
+struct SideV {
+    bool gas;
+    double t;
+    float sig, kappa, imu;
+};
+
+__device__ inline SideV side_at(const HeatPlanes &H, int i) {
+    return {H.gas[i] != 0, H.t0[i], H.sig[i], H.kappa[i], H.imu[i]};
+}
+
+// D_f / dx^2 of a face between sides a and b, either standing for the vacuum beyond (heat.rs:251-282)
+__device__ inline float face_k(const HeatParams &P, bool has_a, SideV a, bool has_b, SideV b) {
+    SideV A, B;
+    auto vac = [&](const SideV &g) { return SideV{false, (double)P.t_floor, P.sigma_floor, g.kappa, g.imu}; };
+    if (has_a && has_b) {
+        if (!a.gas && !b.gas) return 0.0f;
+        A = a.gas ? a : vac(b);
+        B = b.gas ? b : vac(a);
+    } else if (has_a) {
+        if (!a.gas) return 0.0f;
+        A = a;
+        B = vac(a);
+    } else {
+        if (!b.gas) return 0.0f;
+        A = vac(b);
+        B = b;
+    }
+    const double idx2 = (double)P.inv_dx * P.inv_dx;
+    if (P.cd) {
+        const double ca = (double)A.sig * A.imu / P.gm1, cb = (double)B.sig * B.imu / P.gm1;
+        return (float)(P.chi * 0.5 * (ca + cb) * idx2);
+    }
+    const double ta = A.t, tb = B.t;
+    const double era = P.a_r * ta * ta * ta * ta, erb = P.a_r * tb * tb * tb * tb;
+    const double fer = 0.5 * (era + erb);
+    const double ks = 0.5 * ((double)A.kappa * A.sig + (double)B.kappa * B.sig);
+    const double r = fer > 0.0 ? fabs(erb - era) / ((double)P.dx * ks * fer) : 0.0;
+    const double lam = P.limiter ? (2.0 + r) / (6.0 + 3.0 * r + r * r) : 1.0 / 3.0;
+    double d = P.c_sb * lam / ks;
+    if (P.k_cond != 0.0) d += P.k_cond / (P.a_r * (ta + tb) * (ta * ta + tb * tb));
+    return (float)(d * idx2);
+}
+
+// the face law on the start state: raw k on every face, the couplings (both sides gas) into the solver's level 0
+__global__ void k_heat_law(HeatParams P, HeatPlanes H, float *__restrict__ ax, float *__restrict__ ay) {
+    const int w = P.w, h = P.h;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (w + 1) * h) {
+        const int x = t % (w + 1), y = t / (w + 1);
+        const bool ha = x > 0, hb = x < w;
+        const SideV a = ha ? side_at(H, y * w + x - 1) : SideV{}, b = hb ? side_at(H, y * w + x) : SideV{};
+        const float k = face_k(P, ha, a, hb, b);
+        H.kx[t] = k;
+        ax[t] = (ha && hb && a.gas && b.gas) ? k : 0.0f;
+    }
+    if (t < w * (h + 1)) {
+        const int x = t % w, y = t / w;
+        const bool ha = y > 0, hb = y < h;
+        const SideV a = ha ? side_at(H, (y - 1) * w + x) : SideV{}, b = hb ? side_at(H, y * w + x) : SideV{};
+        const float k = face_k(P, ha, a, hb, b);
+        H.ky[t] = k;
+        ay[t] = (ha && hb && a.gas && b.gas) ? k : 0.0f;
+    }
+}
+
+// the sum of k over a gas cell's vacuum and edge faces
+__global__ void k_heat_kvac(HeatParams P, HeatPlanes H) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = P.w, h = P.h;
+    if (i >= w * h) return;
+    const int x = i % w, y = i / w;
+    float s = 0.0f;
+    if (H.gas[i]) {
+        if (!(x > 0 && H.gas[i - 1])) s += H.kx[y * (w + 1) + x];
+        if (!(x + 1 < w && H.gas[i + 1])) s += H.kx[y * (w + 1) + x + 1];
+        if (!(y > 0 && H.gas[i - w])) s += H.ky[y * w + x];
+        if (!(y + 1 < h && H.gas[i + w])) s += H.ky[(y + 1) * w + x];
+    }
+    H.kvac[i] = s;
+}
+
+// a Newton pass's system at the iterate t (heat.rs:474-488): own = c + k_vac, rhs = c u - (C/dt)(t - t0) + k_vac u_vac,
+// x = u; the two stopping bounds eps^2 sum c u^2 and eps^2 sum (C/dt) t^2
+__global__ void k_heat_system(HeatParams P, HeatPlanes H, const double *__restrict__ scal, float *__restrict__ own,
+                              double *__restrict__ rhs, double *__restrict__ xs, double *sums, double eps2,
+                              double *part, unsigned *counter) {
+    const int n = P.w * P.h;
+    const double dt = scal[0];
+    const double uvac = u_of(P, P.t_floor);
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        if (!H.gas[i]) {
+            own[i] = 1.0f;
+            rhs[i] = 0.0;
+            xs[i] = 0.0;
+            H.heat[i] = 0.0f;
+            H.slope[i] = 0.0f;
+            continue;
+        }
+        const double t = H.t[i], heat = (double)H.cap[i] / dt, slope = dt_du(P, t), ci = heat * slope;
+        const double u = u_of(P, t), kv = H.kvac[i];
+        own[i] = (float)(ci + kv);
+        rhs[i] = ci * u - heat * (t - (double)H.t0[i]) + kv * uvac;
+        xs[i] = u;
+        H.heat[i] = (float)heat;
+        H.slope[i] = (float)slope;
+        v[0] += ci * u * u;
+        v[1] += heat * t * t;
+    }
+    double out[2];
+    if (reduce_finish<2>(v, part, gridDim.x, counter + HC_SYS, out) && threadIdx.x == 0) {
+        sums[HS_UB] = eps2 * out[0];
+        sums[HS_TB] = eps2 * out[1];
+    }
+}
+
+// the pass's update (heat.rs:502-515): t = clamp(T(max(x, u_vac)), t/2, 2t); sums C |dT| and counts the cells that
+// moved more than newton_cell of their T
+__global__ void k_heat_update(HeatParams P, HeatPlanes H, const double *__restrict__ xs, double newton_cell,
+                              double *sums, double *part, unsigned *counter) {
+    const int n = P.w * P.h;
+    const double uvac = u_of(P, P.t_floor);
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        if (!H.gas[i]) continue;
+        const double t = H.t[i];
+        double nx = t_of(P, fmax(xs[i], uvac));
+        nx = fmin(fmax(nx, 0.5 * t), 2.0 * t);
+        const double d = fabs(nx - t);
+        v[0] += (double)H.cap[i] * d;
+        if (!(d <= newton_cell * t)) v[1] += 1.0;
+        H.t[i] = nx;
+    }
+    double out[2];
+    if (reduce_finish<2>(v, part, gridDim.x, counter + HC_UPD, out) && threadIdx.x == 0) {
+        sums[HS_MOVED] = out[0];
+        sums[HS_BAD] = out[1];
+    }
+}
+
+// the commit's sums (heat.rs:522-530): radiated = sum dt k_vac (u - u_vac), gained = sum C (t - t0), held = sum C t;
+// then delta = -(gained + radiated) / held
+__global__ void k_heat_commit_sums(HeatParams P, HeatPlanes H, const double *__restrict__ scal, double *sums,
+                                   double *part, unsigned *counter) {
+    const int n = P.w * P.h;
+    const double dt = scal[0], uvac = u_of(P, P.t_floor);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        if (!H.gas[i]) continue;
+        const double t = H.t[i], c = H.cap[i];
+        v[0] += dt * (double)H.kvac[i] * (u_of(P, t) - uvac);
+        v[1] += c * (t - (double)H.t0[i]);
+        v[2] += c * t;
+    }
+    double out[3];
+    if (reduce_finish<3>(v, part, gridDim.x, counter + HC_COMMIT, out) && threadIdx.x == 0) {
+        sums[HS_RAD] = out[0];
+        sums[HS_GAIN] = out[1];
+        sums[HS_HELD] = out[2];
+        sums[HS_DELTA] = out[2] != 0.0 ? -(out[1] + out[0]) / out[2] : 0.0;
+    }
+}
+
+// F = -k dx (v_b - v_a) at a face, v the iterate's u on gas cells, u_vac elsewhere (heat.rs:340-362)
+__device__ inline double face_flux(const HeatParams &P, const HeatPlanes &H, float k, int a, int b) {
+    if (k == 0.0f) return 0.0;
+    const double uvac = u_of(P, P.t_floor);
+    const double va = (a >= 0 && H.gas[a]) ? u_of(P, H.t[a]) : uvac;
+    const double vb = (b >= 0 && H.gas[b]) ? u_of(P, H.t[b]) : uvac;
+    return -(double)k * P.dx * (vb - va);
+}
+
+// E += C ((1 + delta) t - t0) on the gas, and the cell-centred F the next P4 reads (heat.rs:89-100, :531-534); with
+// `limited`, the ceiling's commit instead: every gas cell's outflow scaled to the heat it holds above its floor
+// (heat.rs:544-592), applied by k_heat_limited.
+__global__ void k_heat_commit(HeatParams P, HeatPlanes H, const double *__restrict__ scal, const double *sums,
+                              float *__restrict__ e, float *__restrict__ fx, float *__restrict__ fy, int pinned) {
+    const int w = P.w, h = P.h;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const int x = i % w, y = i / w;
+    const double fl = face_flux(P, H, H.kx[y * (w + 1) + x], x > 0 ? i - 1 : -1, i);
+    const double fr = face_flux(P, H, H.kx[y * (w + 1) + x + 1], i, x + 1 < w ? i + 1 : -1);
+    const double fu = face_flux(P, H, H.ky[y * w + x], y > 0 ? i - w : -1, i);
+    const double fd = face_flux(P, H, H.ky[(y + 1) * w + x], i, y + 1 < h ? i + w : -1);
+    fx[i] = (float)(0.5 * (fl + fr));
+    fy[i] = (float)(0.5 * (fu + fd));
+    if (pinned || !H.gas[i]) return;
+    const double delta = sums[HS_DELTA];
+    e[i] = (float)((double)e[i] + (double)H.cap[i] * ((1.0 + delta) * (double)H.t[i] - (double)H.t0[i]));
+}
+
+// the limited commit: scale = min(1, C (t0 - T_floor) / outflow) per gas cell (1 elsewhere)
+__global__ void k_heat_scale(HeatParams P, HeatPlanes H, const double *__restrict__ scal) {
+    const int w = P.w, h = P.h;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    if (!H.gas[i]) {
+        H.scale[i] = 1.0f;
+        return;
+    }
+    const int x = i % w, y = i / w;
+    const double dt = scal[0];
+    const double o0 = -face_flux(P, H, H.kx[y * (w + 1) + x], x > 0 ? i - 1 : -1, i);
+    const double o1 = face_flux(P, H, H.kx[y * (w + 1) + x + 1], i, x + 1 < w ? i + 1 : -1);
+    const double o2 = -face_flux(P, H, H.ky[y * w + x], y > 0 ? i - w : -1, i);
+    const double o3 = face_flux(P, H, H.ky[(y + 1) * w + x], i, y + 1 < h ? i + w : -1);
+    const double out = dt * P.inv_dx * (fmax(o0, 0.0) + fmax(o1, 0.0) + fmax(o2, 0.0) + fmax(o3, 0.0));
+    const double held = (double)H.cap[i] * fmax((double)H.t0[i] - P.t_floor, 0.0);
+    H.scale[i] = (float)(out > held ? held / out : 1.0);
+}
+
+__device__ inline double limited_flux(const HeatParams &P, const HeatPlanes &H, float k, int a, int b) {
+    double f = face_flux(P, H, k, a, b);
+    const int from = f > 0.0 ? a : b;
+    return f * (from >= 0 ? (double)H.scale[from] : 1.0);
+}
+
+__global__ void k_heat_limited(HeatParams P, HeatPlanes H, const double *__restrict__ scal, float *__restrict__ e,
+                               float *__restrict__ fx, float *__restrict__ fy, double *sums, double *part,
+                               unsigned *counter) {
+    const int w = P.w, h = P.h, n = w * h;
+    const double dt = scal[0];
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        const int x = i % w, y = i / w;
+        const double fl = limited_flux(P, H, H.kx[y * (w + 1) + x], x > 0 ? i - 1 : -1, i);
+        const double fr = limited_flux(P, H, H.kx[y * (w + 1) + x + 1], i, x + 1 < w ? i + 1 : -1);
+        const double fu = limited_flux(P, H, H.ky[y * w + x], y > 0 ? i - w : -1, i);
+        const double fd = limited_flux(P, H, H.ky[(y + 1) * w + x], i, y + 1 < h ? i + w : -1);
+        fx[i] = (float)(0.5 * (fl + fr));
+        fy[i] = (float)(0.5 * (fu + fd));
+        if (!H.gas[i]) continue;
+        const double o[4] = {-fl, fr, -fu, fd};
+        const bool vac[4] = {!(x > 0 && H.gas[i - 1]), !(x + 1 < w && H.gas[i + 1]), !(y > 0 && H.gas[i - w]),
+                             !(y + 1 < h && H.gas[i + w])};
+        double em = 0.0;
+        for (int f = 0; f < 4; f++)
+            if (vac[f]) em += dt * P.inv_dx * o[f];
+        v[0] += em;
+        e[i] = (float)((double)e[i] - dt * P.inv_dx * (o[0] + o[1] + o[2] + o[3]));
+    }
+    double out[1];
+    if (reduce_finish<1>(v, part, gridDim.x, counter + HC_COMMIT, out) && threadIdx.x == 0) sums[HS_RAD] = out[0];
+}
+
+// ------------------------------------------------------------------------------------------------------------ host
+
+static int grid_cells(size_t n) { return (int)std::min<size_t>(4096, (n + NT * 4 - 1) / (NT * 4)); }
+
+HeatPass::~HeatPass() {
+    void *ps[] = {H.gas, H.t0, H.t, H.kappa, H.imu, H.cap, H.sig, H.kx, H.ky, H.kvac, H.heat, H.slope, H.scale,
+                  rhs, xs, sums, part, counter};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (host) (void)hipHostFree(host);
+}
+
+int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
+    P = p;
+    hs = hs_;
+    const size_t n = (size_t)P.w * P.h, nx = (size_t)(P.w + 1) * P.h, ny = (size_t)P.w * (P.h + 1);
+    SR_HIP(hipMalloc(&H.gas, n));
+    float **fp[] = {&H.kappa, &H.imu, &H.cap, &H.sig, &H.kvac, &H.heat, &H.slope, &H.scale};
+    for (float **q : fp) SR_HIP(hipMalloc(q, sizeof(float) * n));
+    SR_HIP(hipMalloc(&H.t0, sizeof(double) * n));
+    SR_HIP(hipMalloc(&H.t, sizeof(double) * n));
+    SR_HIP(hipMalloc(&H.kx, sizeof(float) * nx));
+    SR_HIP(hipMalloc(&H.ky, sizeof(float) * ny));
+    SR_HIP(hipMalloc(&rhs, sizeof(double) * n));
+    SR_HIP(hipMalloc(&xs, sizeof(double) * n));
+    SR_HIP(hipMalloc(&sums, sizeof(double) * HS_N));
+    SR_HIP(hipMemset(sums, 0, sizeof(double) * HS_N));
+    SR_HIP(hipMalloc(&part, sizeof(double) * 4 * 4096));
+    SR_HIP(hipMalloc(&counter, sizeof(unsigned) * HC_N));
+    SR_HIP(hipMemset(counter, 0, sizeof(unsigned) * HC_N));
+    SR_HIP(hipHostMalloc(&host, sizeof(double) * HS_N));
+    return solver.init(P.w, P.h, hs.levels, hs.nu1, hs.nu2);
+}
+
+int HeatPass::read_sums(hipStream_t s) {
+    SR_HIP(hipMemcpyAsync(host, sums, sizeof(double) * HS_N, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    return SR_OK;
+}
+
+int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, float *fx, float *fy, int precond,
+                   hipStream_t s) {
+    const size_t n = (size_t)P.w * P.h;
+    const int g = grid_cells(n);
+    report = sr_heat_report{};
+    report.dt = dt_host;
+    report.converged = 1;
+    hipLaunchKernelGGL(k_heat_cells, dim3(g), dim3(NT), 0, s, P, st.sigma, st.mx, st.my, st.e, st.x, H, sums, part,
+                       counter + 3);
+    const int nf = std::max((P.w + 1) * P.h, P.w * (P.h + 1));
+    hipLaunchKernelGGL(k_heat_law, dim3((nf + 255) / 256), dim3(256), 0, s, P, H, solver.ax0(), solver.ay0());
+    hipLaunchKernelGGL(k_heat_kvac, dim3((n + 255) / 256), dim3(256), 0, s, P, H);
+    SR_HIP(hipGetLastError());
+    if (P.pinned || !(dt_host > 0.0)) {
+        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal, sums, st.e, fx, fy, 1);
+        SR_HIP(hipGetLastError());
+        return SR_OK;
+    }
+    SR_HIP(hipMemcpyAsync(solver.gas0(), H.gas, n, hipMemcpyDeviceToDevice, s));
+    SR_TRY(read_sums(s));
+    const double heat0 = host[HS_HEAT0];
+    bool any_gas = heat0 > 0.0;
+    if (!any_gas) {
+        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal, sums, st.e, fx, fy, 1);
+        SR_HIP(hipGetLastError());
+        return SR_OK;
+    }
+    const double eps = hs.stop == SR_STOP_REFERENCE ? hs.solve_tol : 0.0;
+    int passes = 0, rounds = 0;
+    bool converged = false, faces = true;
+    double last_rel = 0.0;
+    while (passes < hs.max_passes && rounds < hs.max_rounds) {
+        passes++;
+        hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, sums, eps * eps,
+                           part, counter);
+        SR_TRY(solver.rebuild(faces, s));
+        faces = false;
+        SR_TRY(read_sums(s));
+        if (!(std::isfinite(host[HS_UB]) && std::isfinite(host[HS_TB]))) {
+            passes--;
+            break;
+        }
+        RefStop ref;
+        ref.heat = H.heat;
+        ref.slope = H.slope;
+        ref.u_bound = host[HS_UB];
+        ref.t_bound = host[HS_TB];
+        FldStats fs;
+        SR_TRY(solver.solve(rhs, xs, hs.solve_tol, hs.max_rounds - rounds, precond,
+                            hs.stop == SR_STOP_REFERENCE ? &ref : nullptr, &fs, s));
+        rounds += fs.iterations;
+        last_rel = fs.rel;
+        if (!fs.finite) break;
+        hipLaunchKernelGGL(k_heat_update, dim3(g), dim3(NT), 0, s, P, H, xs, hs.newton_cell, sums, part, counter);
+        SR_TRY(read_sums(s));
+        if (host[HS_MOVED] <= hs.newton_heat * heat0 && host[HS_BAD] == 0.0) {
+            converged = true;
+            break;
+        }
+    }
+    report.passes = passes;
+    report.rounds = rounds;
+    report.final_residual = last_rel;
+    if (converged) {
+        hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, sums, part, counter);
+        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal, sums, st.e, fx, fy, 0);
+        SR_HIP(hipGetLastError());
+        SR_TRY(read_sums(s));
+        report.delta = host[HS_DELTA];
+        report.radiated = host[HS_RAD];
+        report.converged = 1;
+    } else {
+        hipLaunchKernelGGL(k_heat_scale, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal);
+        hipLaunchKernelGGL(k_heat_limited, dim3(g), dim3(NT), 0, s, P, H, scal, st.e, fx, fy, sums, part, counter);
+        SR_HIP(hipGetLastError());
+        SR_TRY(read_sums(s));
+        report.delta = 0.0;
+        report.radiated = host[HS_RAD];
+        report.converged = 0;
+    }
+    return SR_OK;
+}
+
+} // namespace sr

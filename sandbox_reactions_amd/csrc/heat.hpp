@@ -1,0 +1,62 @@
+// P5 heat pass: the implicit FLD step's kernels and Newton driver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/sr_hip.h"
+#include "fld.hpp"
+
+namespace sr {
+
+struct HeatParams {
+    int w, h;
+    double dx;
+    float inv_dx;
+    double gm1;             // gamma - 1
+    double t_floor;
+    float sigma_vac, sigma_floor;
+    float fixed_imu;        // 1/mu when fixed, else 0
+    double a_r, c_sb, k_cond;
+    float kappa0, kappa_dust, t_dust;
+    int opacity;            // 0 reference, 1 Kramers
+    int cd;                 // constant_diffusivity on
+    double chi;
+    int limiter;
+    int pinned;
+};
+
+struct HeatPlanes {
+    uint8_t *gas = nullptr;
+    double *t0 = nullptr, *t = nullptr; // the start temperature and the Newton iterate, FP64
+    float *kappa = nullptr, *imu = nullptr, *cap = nullptr, *sig = nullptr;
+    float *kx = nullptr, *ky = nullptr, *kvac = nullptr, *heat = nullptr, *slope = nullptr, *scale = nullptr;
+};
+
+// the state planes the pass reads (E is written)
+struct HydroLike {
+    const double *sigma;
+    const float *mx, *my;
+    float *e;
+    const float *x;
+};
+
+class HeatPass {
+  public:
+    ~HeatPass();
+    int init(const HeatParams &p, const sr_heat_settings &hs);
+    // one P5 step on the state (E in place); writes the cell-centred flux the next P4 reads
+    int step(const HydroLike &st, const double *scal, double dt_host, float *fx, float *fy, int precond,
+             hipStream_t s);
+    sr_heat_report report{};
+    FldSolver solver;
+    HeatParams P{};
+    sr_heat_settings hs{};
+
+  private:
+    int read_sums(hipStream_t s);
+    HeatPlanes H;
+    double *rhs = nullptr, *xs = nullptr, *sums = nullptr, *part = nullptr;
+    unsigned *counter = nullptr;
+    double *host = nullptr;
+};
+
+} // namespace sr

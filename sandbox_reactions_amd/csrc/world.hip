@@ -1,0 +1,643 @@
+// The world: device state, the step driver (P1 -> P2 -> P4 -> P5 -> P8 -> P9, reference/mod.rs:264-297) and the
+// C ABI of include/sr_hip.h for it. The reference's Step records every pass into one submission and keeps dt, the
+// sim time and the solver's round counts on the GPU (crates/sr-engine/src/step/mod.rs:610-718); here dt and the sim
+// time live in a small device array that every kernel reads, so a step with no heat pass issues its kernels
+// back to back on the world's stream without a host round trip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "grav.hpp"
+#include "heat.hpp"
+#include "hydro.hpp"
+
+namespace sr {
+
+#define NT 256
+#define NS 10
+
+enum { SCAL_DT = 0, SCAL_DT_NEXT, SCAL_SIM_TIME, SCAL_CAP, SCAL_RATE, SCAL_GMAX, SCAL_NF, SCAL_N = 16 };
+
+__constant__ float w_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f / 20.0f,
+                                13.0f / 24.0f, 15.0f / 28.0f, 17.0f / 32.0f, 27.0f / 56.0f, 1.0f};
+
+struct FloorParams {
+    int w, h;
+    float gamma, gm1, t_floor_g, fixed_imu;
+    double sigma_vac;          // the hydro's test (f64)
+    float svac32, sfloor32;    // the floors' f32 constants (floors.rs:110)
+    double cfl, eta_g, dx;
+};
+
+__global__ void k_upload(const float *__restrict__ planes, HydroState st, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    st.sigma[i] = (double)planes[i];
+    st.mx[i] = planes[(size_t)n + i];
+    st.my[i] = planes[(size_t)2 * n + i];
+    st.e[i] = planes[(size_t)3 * n + i];
+    for (int s = 0; s < NS; s++) st.x[(size_t)s * n + i] = planes[(size_t)(4 + s) * n + i];
+}
+
+__global__ void k_readback(HydroState st, float *__restrict__ planes, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    planes[i] = (float)st.sigma[i];
+    planes[(size_t)n + i] = st.mx[i];
+    planes[(size_t)2 * n + i] = st.my[i];
+    planes[(size_t)3 * n + i] = st.e[i];
+    for (int s = 0; s < NS; s++) planes[(size_t)(4 + s) * n + i] = st.x[(size_t)s * n + i];
+}
+
+// P1: this step's dt is the last P9's (capped), and the sim time advances by it (reference/mod.rs:266-270)
+__global__ void k_p1(double *scal) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double dt = scal[SCAL_DT_NEXT];
+        if (scal[SCAL_CAP] > 0.0 && scal[SCAL_CAP] < dt) dt = scal[SCAL_CAP];
+        scal[SCAL_DT] = dt;
+        scal[SCAL_SIM_TIME] += dt;
+    }
+}
+
+// P8 floors (floors.rs:124-175, no hold) then P9's dt (hydro.rs:618-634) and the non-finite guard: the largest
+// wave-speed rate and |g| by a max reduction, the first non-finite (cell, channel) as the max of -key.
+__global__ void __launch_bounds__(NT) k_floors_dt(FloorParams P, HydroState st, const float *__restrict__ gx,
+                                                  const float *__restrict__ gy, int apply, double *scal, double *part,
+                                                  unsigned *counter) {
+    const int n = P.w * P.h;
+    double v[3] = {0.0, 0.0, -HUGE_VAL};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        float xs[NS];
+        bool finite = true;
+        for (int s = 0; s < NS; s++) {
+            xs[s] = st.x[(size_t)s * n + i];
+            finite &= isfinite(xs[s]);
+        }
+        double sig = st.sigma[i];
+        float mx = st.mx[i], my = st.my[i], e = st.e[i];
+        if (apply && finite) {
+            float sum = 0.0f;
+            for (int s = 0; s < NS; s++) {
+                xs[s] = fmaxf(xs[s], 0.0f);
+                sum += xs[s];
+            }
+            if (sum > 0.0f) {
+                for (int s = 0; s < NS; s++) xs[s] /= sum;
+            } else {
+                for (int s = 0; s < NS; s++) xs[s] = s == 0 ? 1.0f : 0.0f;
+            }
+            float im = 0.0f;
+            for (int s = 0; s < NS; s++) im += xs[s] * w_imu[s];
+            if (P.fixed_imu > 0.0f) im = P.fixed_imu;
+            const float efl = P.t_floor_g * im;
+            if (sig < (double)P.svac32) {
+                sig = (double)P.sfloor32;
+                mx = my = 0.0f;
+                e = P.sfloor32 * efl;
+            } else {
+                const float s = (float)sig;
+                const float raw = e / s - 0.5f * (mx * mx + my * my) / (s * s);
+                if (raw < efl) e = e + s * (efl - raw);
+            }
+            st.sigma[i] = sig;
+            st.mx[i] = mx;
+            st.my[i] = my;
+            st.e[i] = e;
+            for (int s = 0; s < NS; s++) st.x[(size_t)s * n + i] = xs[s];
+        }
+        // the rate as P9 reads the cell (hydro.rs:424-441): the floor state below sigma_vac
+        float im = 0.0f;
+        for (int s = 0; s < NS; s++) im += xs[s] * w_imu[s];
+        if (P.fixed_imu > 0.0f) im = P.fixed_imu;
+        float u = 0.0f, vv = 0.0f, c2;
+        if (sig < P.sigma_vac) {
+            c2 = P.gamma * P.gm1 * P.t_floor_g * im;
+        } else {
+            const float s = (float)sig;
+            u = mx / s;
+            vv = my / s;
+            const float raw = e / s - 0.5f * (u * u + vv * vv);
+            const float eps = fmaxf(raw, P.t_floor_g * im);
+            c2 = P.gamma * P.gm1 * eps;
+        }
+        const float c = sqrtf(fmaxf(c2, 0.0f));
+        const double rate = (double)((fabsf(u) + c) + (fabsf(vv) + c));
+        v[0] = fmax(v[0], rate);
+        if (gx) v[1] = fmax(v[1], (double)hypotf(gx[i], gy[i]));
+        if (!(finite && isfinite(sig) && isfinite(mx) && isfinite(my) && isfinite(e))) {
+            int ch = 0;
+            if (!isfinite(sig)) ch = 0;
+            else if (!isfinite(mx)) ch = 1;
+            else if (!isfinite(my)) ch = 2;
+            else if (!isfinite(e)) ch = 3;
+            else
+                for (int s = NS - 1; s >= 0; s--)
+                    if (!isfinite(xs[s])) ch = 4 + s;
+            v[2] = fmax(v[2], -((double)i * 16.0 + ch));
+        }
+        if (!isfinite(rate)) v[0] = HUGE_VAL;
+    }
+    double out[3];
+    if (reduce_max_finish<3>(v, part, gridDim.x, counter, out) && threadIdx.x == 0) {
+        scal[SCAL_RATE] = out[0];
+        scal[SCAL_GMAX] = out[1];
+        scal[SCAL_NF] = out[2];
+        double dt = P.cfl * P.dx / out[0];
+        if (out[1] > 0.0) dt = fmin(dt, P.eta_g * sqrt(P.dx / out[1]));
+        scal[SCAL_DT_NEXT] = dt;
+    }
+}
+
+// FP64 totals: Sigma, Sigma u, Sigma v, E, Sigma X_s (hydro.rs:210-224), and the escaped sums per component
+__global__ void __launch_bounds__(NT) k_totals(HydroState st, int n, double *out, double *part, unsigned *counter) {
+    double v[14];
+    for (int k = 0; k < 14; k++) v[k] = 0.0;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        const double s = st.sigma[i];
+        v[0] += s;
+        v[1] += st.mx[i];
+        v[2] += st.my[i];
+        v[3] += st.e[i];
+        for (int k = 0; k < NS; k++) v[4 + k] += s * (double)st.x[(size_t)k * n + i];
+    }
+    double o[14];
+    if (reduce_finish<14>(v, part, gridDim.x, counter, o) && threadIdx.x == 0)
+        for (int k = 0; k < 14; k++) out[k] = o[k];
+}
+
+__global__ void k_escaped_sum(const double *__restrict__ esc, int nf, double *out) {
+    __shared__ double sh[4];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) s += esc[(size_t)k * nf + i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void k_extract_box(const float *__restrict__ ringed, int w, int h, float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const int x = i % w, y = i / w;
+    out[i] = ringed[(size_t)(y + 1) * (w + 2) + x + 1];
+}
+
+} // namespace sr
+
+using namespace sr;
+
+struct KernelClock {
+    unsigned long long launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct sr_world {
+    int w = 0, h = 0;
+    size_t n = 0;
+    sr_physics ph{};
+    sr_switches sw{};
+    sr_heat_settings hs{};
+    int precond = SR_PRECOND_MULTIGRID;
+    hipStream_t s = nullptr;
+    HydroState st[2]{};
+    int cur = 0;
+    HydroParams hp{};
+    FloorParams fp{};
+    double *scal = nullptr, *host_scal = nullptr;
+    double *takeback = nullptr;
+    HydroPartials part{};
+    double *rpart = nullptr;
+    unsigned *counters = nullptr;
+    Gravity grav;
+    float *phi = nullptr, *gx = nullptr, *gy = nullptr, *phi_mid = nullptr, *sigma_mid = nullptr;
+    float *pushmag = nullptr, *rfx = nullptr, *rfy = nullptr;
+    bool have_flux = false;
+    double *escaped = nullptr, *totals = nullptr;
+    HeatPass heat;
+    uint64_t steps = 0;
+    sr_heat_report last_heat{};
+    bool timing = false;
+    hipEvent_t ev[12]{};
+    sr_step_cost cost{};
+    KernelClock k_update, k_mid, k_grav, k_pre0;
+    std::vector<void *> allocs;
+};
+
+static int world_alloc(sr_world *wd, void **p, size_t bytes) {
+    SR_HIP(hipMalloc(p, bytes));
+    SR_HIP(hipMemset(*p, 0, bytes));
+    wd->allocs.push_back(*p);
+    return SR_OK;
+}
+
+static int hydro_state_alloc(sr_world *wd, HydroState &st) {
+    SR_TRY(world_alloc(wd, (void **)&st.sigma, sizeof(double) * wd->n));
+    SR_TRY(world_alloc(wd, (void **)&st.mx, sizeof(float) * wd->n));
+    SR_TRY(world_alloc(wd, (void **)&st.my, sizeof(float) * wd->n));
+    SR_TRY(world_alloc(wd, (void **)&st.e, sizeof(float) * wd->n));
+    SR_TRY(world_alloc(wd, (void **)&st.x, sizeof(float) * wd->n * NS));
+    return SR_OK;
+}
+
+static int grid_n(size_t n) { return (int)std::min<size_t>(4096, (n + NT * 4 - 1) / (NT * 4)); }
+
+extern "C" void sr_destroy(sr_world *wd) {
+    if (!wd) return;
+    if (wd->s) (void)hipStreamSynchronize(wd->s);
+    for (void *p : wd->allocs) (void)hipFree(p);
+    if (wd->host_scal) (void)hipHostFree(wd->host_scal);
+    for (auto &e : wd->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (wd->s) (void)hipStreamDestroy(wd->s);
+    delete wd;
+}
+
+extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_switches *sw, const sr_heat_settings *hs,
+                               sr_world **out) {
+    if (!out || !ph || !sw) return fail(SR_ERR_ARGUMENT, "sr_world_create: null pointer");
+    *out = nullptr;
+    if (w < 8 || h < 8 || w > 4096 || h > 4096 || w % 8 || h % 8)
+        return fail(SR_ERR_WORLD_SIZE, "a " + std::to_string(w) + "x" + std::to_string(h) +
+                                            " world: each axis a multiple of 8 in [8, 4096]");
+    if (!(ph->gamma > 1.0) || !(ph->dx > 0.0) || !(ph->t_floor > 0.0))
+        return fail(SR_ERR_ARGUMENT, "gamma > 1, dx > 0 and t_floor > 0 are required");
+    auto *wd = new sr_world();
+    wd->w = w;
+    wd->h = h;
+    wd->n = (size_t)w * h;
+    wd->ph = *ph;
+    wd->sw = *sw;
+    if (hs)
+        wd->hs = *hs;
+    else
+        sr_heat_settings_defaults(&wd->hs);
+    auto bail = [&](int c) {
+        sr_destroy(wd);
+        return c;
+    };
+    int c;
+    if (hipStreamCreateWithFlags(&wd->s, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SR_ERR_DEVICE, "hipStreamCreate"));
+    for (auto &e : wd->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(SR_ERR_DEVICE, "hipEventCreate"));
+    const size_t n = wd->n;
+    if ((c = hydro_state_alloc(wd, wd->st[0])) || (c = hydro_state_alloc(wd, wd->st[1]))) return bail(c);
+    if ((c = world_alloc(wd, (void **)&wd->scal, sizeof(double) * SCAL_N))) return bail(c);
+    if (hipHostMalloc(&wd->host_scal, sizeof(double) * SCAL_N) != hipSuccess)
+        return bail(fail(SR_ERR_DEVICE, "hipHostMalloc"));
+    const int nb = hydro_blocks(w, h);
+    if ((c = world_alloc(wd, (void **)&wd->takeback, sizeof(double) * 8)) ||
+        (c = world_alloc(wd, (void **)&wd->part.sums, sizeof(double) * 5 * nb)) ||
+        (c = world_alloc(wd, (void **)&wd->part.keys, sizeof(unsigned long long) * nb)) ||
+        (c = world_alloc(wd, (void **)&wd->rpart, sizeof(double) * 16 * 4096)) ||
+        (c = world_alloc(wd, (void **)&wd->counters, sizeof(unsigned) * 8)) ||
+        (c = world_alloc(wd, (void **)&wd->phi, sizeof(float) * (w + 2) * (h + 2))) ||
+        (c = world_alloc(wd, (void **)&wd->phi_mid, sizeof(float) * (w + 2) * (h + 2))) ||
+        (c = world_alloc(wd, (void **)&wd->gx, sizeof(float) * n)) ||
+        (c = world_alloc(wd, (void **)&wd->gy, sizeof(float) * n)) ||
+        (c = world_alloc(wd, (void **)&wd->sigma_mid, sizeof(float) * n)) ||
+        (c = world_alloc(wd, (void **)&wd->pushmag, sizeof(float) * n)) ||
+        (c = world_alloc(wd, (void **)&wd->rfx, sizeof(float) * n)) ||
+        (c = world_alloc(wd, (void **)&wd->rfy, sizeof(float) * n)) ||
+        (c = world_alloc(wd, (void **)&wd->escaped, sizeof(double) * 14 * 2 * (w + h))) ||
+        (c = world_alloc(wd, (void **)&wd->totals, sizeof(double) * 32)))
+        return bail(c);
+    if (sw->gravity) {
+        if ((c = wd->grav.init(w, h, ph->G, ph->dx, ph->sigma_vac, ph->max_padded > 0 ? ph->max_padded : 4096)))
+            return bail(c);
+    } else {
+        int px;
+        if ((c = padded_size(w, ph->max_padded > 0 ? ph->max_padded : 4096, &px))) return bail(c);
+        if ((c = padded_size(h, ph->max_padded > 0 ? ph->max_padded : 4096, &px))) return bail(c);
+    }
+    HydroParams &hp = wd->hp;
+    hp.w = w;
+    hp.h = h;
+    hp.gamma = (float)ph->gamma;
+    hp.gm1 = (float)(ph->gamma - 1.0);
+    hp.t_floor_g = (float)(ph->t_floor / (ph->gamma - 1.0));
+    hp.sigma_vac = (float)ph->sigma_vac;
+    hp.sigma_floor = (float)ph->sigma_floor;
+    hp.fixed_imu = ph->mu > 0 ? (float)(1.0 / ph->mu) : 0.0f;
+    hp.edge = ph->edge;
+    hp.inv_dx = (float)(1.0 / ph->dx);
+    hp.kappa0 = (float)ph->kappa0;
+    hp.c_sb = (float)ph->c_sb;
+    hp.kappa_dust = (float)ph->kappa_dust;
+    hp.t_dust = (float)ph->t_dust;
+    hp.opacity = ph->opacity;
+    FloorParams &fp = wd->fp;
+    fp.w = w;
+    fp.h = h;
+    fp.gamma = hp.gamma;
+    fp.gm1 = hp.gm1;
+    fp.t_floor_g = hp.t_floor_g;
+    fp.fixed_imu = hp.fixed_imu;
+    fp.sigma_vac = ph->sigma_vac;
+    fp.svac32 = (float)ph->sigma_vac;
+    fp.sfloor32 = (float)ph->sigma_floor;
+    fp.cfl = ph->cfl;
+    fp.eta_g = ph->eta_g;
+    fp.dx = ph->dx;
+    if (sw->heat) {
+        HeatParams P{};
+        P.w = w;
+        P.h = h;
+        P.dx = ph->dx;
+        P.inv_dx = (float)(1.0 / ph->dx);
+        P.gm1 = ph->gamma - 1.0;
+        P.t_floor = ph->t_floor;
+        P.sigma_vac = (float)ph->sigma_vac;
+        P.sigma_floor = (float)ph->sigma_floor;
+        P.fixed_imu = hp.fixed_imu;
+        P.a_r = ph->a_r;
+        P.c_sb = ph->c_sb;
+        P.k_cond = ph->k_cond;
+        P.kappa0 = (float)ph->kappa0;
+        P.kappa_dust = (float)ph->kappa_dust;
+        P.t_dust = (float)ph->t_dust;
+        P.opacity = ph->opacity;
+        P.cd = ph->constant_diffusivity > 0.0;
+        P.chi = ph->constant_diffusivity;
+        P.limiter = ph->flux_limiter;
+        P.pinned = ph->pin_temperature;
+        if ((c = wd->heat.init(P, wd->hs))) return bail(c);
+    }
+    *out = wd;
+    return SR_OK;
+}
+
+static int state_ok(sr_world *wd, size_t n) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (n != SR_COMPONENTS * wd->n)
+        return fail(SR_ERR_LENGTH, "planes hold " + std::to_string(n) + " values, expected " +
+                                       std::to_string(SR_COMPONENTS * wd->n));
+    return SR_OK;
+}
+
+static int p2(sr_world *wd) {
+    if (!wd->sw.gravity) return SR_OK;
+    SR_TRY(wd->grav.potential<double>(wd->st[wd->cur].sigma, wd->phi, wd->s));
+    return wd->grav.gradient(wd->phi, wd->gx, wd->gy, wd->s);
+}
+
+static int p89(sr_world *wd, int apply) {
+    HydroState &st = wd->st[wd->cur];
+    hipLaunchKernelGGL(k_floors_dt, dim3(grid_n(wd->n)), dim3(NT), 0, wd->s, wd->fp, st,
+                       wd->sw.gravity ? wd->gx : nullptr, wd->sw.gravity ? wd->gy : nullptr, apply, wd->scal,
+                       wd->rpart, wd->counters + 1);
+    SR_HIP(hipGetLastError());
+    return SR_OK;
+}
+
+extern "C" int sr_upload(sr_world *wd, const float *planes, size_t n) {
+    SR_TRY(state_ok(wd, n));
+    if (!planes) return fail(SR_ERR_ARGUMENT, "sr_upload: null planes");
+    float *tmp = nullptr;
+    SR_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * n, wd->s));
+    SR_HIP(hipMemcpyAsync(tmp, planes, sizeof(float) * n, hipMemcpyHostToDevice, wd->s));
+    wd->cur = 0;
+    hipLaunchKernelGGL(k_upload, dim3((wd->n + 255) / 256), dim3(256), 0, wd->s, tmp, wd->st[0], (int)wd->n);
+    SR_HIP(hipFreeAsync(tmp, wd->s));
+    SR_HIP(hipMemsetAsync(wd->scal, 0, sizeof(double) * SCAL_N, wd->s));
+    SR_HIP(hipMemsetAsync(wd->escaped, 0, sizeof(double) * 14 * 2 * (wd->w + wd->h), wd->s));
+    wd->have_flux = false;
+    wd->steps = 0;
+    wd->last_heat = sr_heat_report{};
+    // the twin's start: P2 on the state and dt_0 from it (reference/mod.rs:173-175), no floors
+    SR_TRY(p2(wd));
+    SR_TRY(p89(wd, 0));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+extern "C" int sr_readback(sr_world *wd, float *planes, size_t n) {
+    SR_TRY(state_ok(wd, n));
+    if (!planes) return fail(SR_ERR_ARGUMENT, "sr_readback: null planes");
+    float *tmp = nullptr;
+    SR_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * n, wd->s));
+    hipLaunchKernelGGL(k_readback, dim3((wd->n + 255) / 256), dim3(256), 0, wd->s, wd->st[wd->cur], tmp, (int)wd->n);
+    SR_HIP(hipMemcpyAsync(planes, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipFreeAsync(tmp, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+static void mark(sr_world *wd, int k) {
+    if (wd->timing) (void)hipEventRecord(wd->ev[k], wd->s);
+}
+
+static double between(sr_world *wd, int a, int b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, wd->ev[a], wd->ev[b]) != hipSuccess) return 0.0;
+    return ms;
+}
+
+static int one_step(sr_world *wd) {
+    hipStream_t s = wd->s;
+    mark(wd, 0);
+    hipLaunchKernelGGL(k_p1, dim3(1), dim3(64), 0, s, wd->scal);
+    mark(wd, 1);
+    SR_TRY(p2(wd));
+    mark(wd, 2);
+    if (wd->sw.hydro) {
+        HydroState &in = wd->st[wd->cur], &out = wd->st[1 - wd->cur];
+        const bool grav = wd->sw.gravity, rad = wd->sw.heat && wd->have_flux;
+        if (grav) {
+            mark(wd, 8);
+            SR_TRY(hydro_launch(HYDRO_MID, wd->hp, in, out, wd->scal, wd->gx, wd->gy, nullptr, nullptr, nullptr,
+                                nullptr, wd->sigma_mid, nullptr, wd->escaped, wd->part, s));
+            mark(wd, 9);
+            SR_TRY(wd->grav.potential<float>(wd->sigma_mid, wd->phi_mid, s));
+        }
+        mark(wd, 10);
+        SR_TRY(hydro_launch(HYDRO_UPDATE, wd->hp, in, out, wd->scal, grav ? wd->gx : nullptr,
+                            grav ? wd->gy : nullptr, grav ? wd->sigma_mid : nullptr, grav ? wd->phi_mid : nullptr,
+                            rad ? wd->rfx : nullptr, rad ? wd->rfy : nullptr, nullptr, wd->pushmag, wd->escaped,
+                            wd->part, s));
+        mark(wd, 11);
+        if (grav || rad) SR_TRY(hydro_takeback(wd->hp, out, wd->pushmag, wd->part, wd->takeback, rad, s));
+        wd->cur = 1 - wd->cur;
+    }
+    mark(wd, 3);
+    if (wd->sw.heat) {
+        SR_HIP(hipMemcpyAsync(wd->host_scal, wd->scal, sizeof(double) * SCAL_N, hipMemcpyDeviceToHost, s));
+        SR_HIP(hipStreamSynchronize(s));
+        HydroState &st = wd->st[wd->cur];
+        HydroLike hl = {st.sigma, st.mx, st.my, st.e, st.x};
+        SR_TRY(wd->heat.step(hl, wd->scal, wd->host_scal[SCAL_DT], wd->rfx, wd->rfy, wd->precond, s));
+        wd->last_heat = wd->heat.report;
+        wd->have_flux = true;
+    }
+    mark(wd, 4);
+    SR_TRY(p89(wd, 1));
+    mark(wd, 5);
+    wd->steps++;
+    if (wd->timing) {
+        SR_HIP(hipEventSynchronize(wd->ev[5]));
+        wd->cost.pass_ms[1] += between(wd, 0, 1);
+        wd->cost.pass_ms[2] += between(wd, 1, 2);
+        wd->cost.pass_ms[4] += between(wd, 2, 3);
+        wd->cost.pass_ms[5] += between(wd, 3, 4);
+        wd->cost.pass_ms[8] += between(wd, 4, 5);
+        wd->cost.step_ms += between(wd, 0, 5);
+        wd->cost.steps++;
+        if (wd->sw.hydro) {
+            wd->k_update.ms += between(wd, 10, 11);
+            wd->k_update.launches++;
+            if (wd->sw.gravity) {
+                wd->k_mid.ms += between(wd, 8, 9);
+                wd->k_mid.launches++;
+            }
+        }
+    }
+    return SR_OK;
+}
+
+extern "C" int sr_run(sr_world *wd, uint32_t steps) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    for (uint32_t k = 0; k < steps; k++) SR_TRY(one_step(wd));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+extern "C" int sr_read_dt(sr_world *wd, sr_dt_report *out) {
+    if (!wd || !out) return fail(SR_ERR_ARGUMENT, "null pointer");
+    SR_HIP(hipMemcpyAsync(wd->host_scal, wd->scal, sizeof(double) * SCAL_N, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    const double *h = wd->host_scal;
+    std::memset(out, 0, sizeof *out);
+    out->dt = h[SCAL_DT];
+    out->dt_next = h[SCAL_DT_NEXT];
+    out->sim_time = h[SCAL_SIM_TIME];
+    out->steps = wd->steps;
+    if (h[SCAL_NF] > -HUGE_VAL) {
+        const long key = (long)(-h[SCAL_NF]);
+        out->non_finite = 1;
+        const long cell = key / 16;
+        out->nf_x = (int)(cell % wd->w);
+        out->nf_y = (int)(cell / wd->w);
+        out->nf_channel = (int)(key % 16);
+    }
+    return SR_OK;
+}
+
+extern "C" int sr_read_heat(sr_world *wd, sr_heat_report *out) {
+    if (!wd || !out) return fail(SR_ERR_ARGUMENT, "null pointer");
+    *out = wd->last_heat;
+    return SR_OK;
+}
+
+extern "C" int sr_read_field(sr_world *wd, int field, float *out, size_t n) {
+    if (!wd || !out) return fail(SR_ERR_ARGUMENT, "null pointer");
+    if (n != wd->n) return fail(SR_ERR_LENGTH, "a side field holds w*h values");
+    const float *src = nullptr;
+    float *tmp = nullptr;
+    switch (field) {
+    case SR_FIELD_PHI:
+    case SR_FIELD_PHI_MID:
+        SR_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * n, wd->s));
+        hipLaunchKernelGGL(k_extract_box, dim3((n + 255) / 256), dim3(256), 0, wd->s,
+                           field == SR_FIELD_PHI ? wd->phi : wd->phi_mid, wd->w, wd->h, tmp);
+        src = tmp;
+        break;
+    case SR_FIELD_GX: src = wd->gx; break;
+    case SR_FIELD_GY: src = wd->gy; break;
+    case SR_FIELD_FLUX_X: src = wd->rfx; break;
+    case SR_FIELD_FLUX_Y: src = wd->rfy; break;
+    case SR_FIELD_SIGMA_MID: src = wd->sigma_mid; break;
+    default: return fail(SR_ERR_ARGUMENT, "unknown field");
+    }
+    SR_HIP(hipMemcpyAsync(out, src, sizeof(float) * n, hipMemcpyDeviceToHost, wd->s));
+    if (tmp) SR_HIP(hipFreeAsync(tmp, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+extern "C" int sr_read_totals(sr_world *wd, double *totals, double *escaped) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    hipLaunchKernelGGL(k_totals, dim3(grid_n(wd->n)), dim3(NT), 0, wd->s, wd->st[wd->cur], (int)wd->n, wd->totals,
+                       wd->rpart, wd->counters + 2);
+    hipLaunchKernelGGL(k_escaped_sum, dim3(14), dim3(256), 0, wd->s, wd->escaped, 2 * (wd->w + wd->h),
+                       wd->totals + 16);
+    SR_HIP(hipGetLastError());
+    double h[32];
+    SR_HIP(hipMemcpyAsync(h, wd->totals, sizeof(double) * 32, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    if (totals) std::memcpy(totals, h, sizeof(double) * 14);
+    if (escaped) std::memcpy(escaped, h + 16, sizeof(double) * 14);
+    return SR_OK;
+}
+
+extern "C" int sr_set_precond(sr_world *wd, int precond) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (precond == SR_PRECOND_LEARNED) return fail(SR_ERR_ARGUMENT, "the learned preconditioner is not built");
+    if (precond != SR_PRECOND_JACOBI && precond != SR_PRECOND_MULTIGRID)
+        return fail(SR_ERR_ARGUMENT, "unknown preconditioner");
+    wd->precond = precond;
+    return SR_OK;
+}
+
+extern "C" int sr_set_dt_cap(sr_world *wd, double cap) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (!(cap > 0.0) || !std::isfinite(cap)) return fail(SR_ERR_ARGUMENT, "a dt cap must be positive and finite");
+    SR_HIP(hipMemcpyAsync(wd->scal + SCAL_CAP, &cap, sizeof(double), hipMemcpyHostToDevice, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+extern "C" int sr_set_timing(sr_world *wd, int on) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    wd->timing = on != 0;
+    wd->cost = sr_step_cost{};
+    wd->k_update = wd->k_mid = wd->k_grav = wd->k_pre0 = KernelClock{};
+    if (wd->sw.heat) {
+        wd->heat.solver.set_timing(wd->timing);
+        wd->heat.solver.smooth_ms = 0.0;
+        wd->heat.solver.smooth_launches = 0;
+    }
+    return SR_OK;
+}
+
+extern "C" int sr_read_cost(sr_world *wd, sr_step_cost *out) {
+    if (!wd || !out) return fail(SR_ERR_ARGUMENT, "null pointer");
+    *out = wd->cost;
+    if (wd->cost.steps) {
+        out->step_ms = wd->cost.step_ms / wd->cost.steps;
+        for (int k = 0; k < 10; k++) out->pass_ms[k] = wd->cost.pass_ms[k] / wd->cost.steps;
+    }
+    return SR_OK;
+}
+
+extern "C" int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launches, double *mean_ms,
+                              double *bytes_per_launch) {
+    if (!wd || !kernel) return fail(SR_ERR_ARGUMENT, "null pointer");
+    const std::string k = kernel;
+    const double n = (double)wd->n;
+    KernelClock c;
+    double bytes = 0.0;
+    if (k == "hydro_update") {
+        c = wd->k_update;
+        // state in and out (Sigma f64 + 13 f32 planes) + g + Sigma_mid + phi_mid (+ F and |push| with radiation)
+        bytes = n * (2 * 60.0 + (wd->sw.gravity ? 16.0 : 0.0) + (wd->sw.heat ? 12.0 : 0.0));
+    } else if (k == "hydro_mid") {
+        c = wd->k_mid;
+        bytes = n * (60.0 + 8.0 + 4.0);
+    } else if (k == "mg_pre0") {
+        if (!wd->sw.heat) return fail(SR_ERR_ARGUMENT, "no heat pass");
+        c.ms = wd->heat.solver.smooth_ms;
+        c.launches = wd->heat.solver.smooth_launches;
+        bytes = n * 21.0; // b, own, ax, ay in; z and the coarse b out
+    } else {
+        return fail(SR_ERR_ARGUMENT, "unknown kernel " + k);
+    }
+    if (launches) *launches = c.launches;
+    if (mean_ms) *mean_ms = c.launches ? c.ms / c.launches : 0.0;
+    if (bytes_per_launch) *bytes_per_launch = bytes;
+    return SR_OK;
+}

@@ -1,0 +1,65 @@
+"""The heat pass's linear solver on its own (step/heat.rs:609-673: heat_first_system / apply_heat_precond): FP32
+multigrid-preconditioned flexible CG inside FP64 iterative refinement, on device tensors."""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+
+
+class Fld:
+    """Solver for the 5-point SPD system own_i v_i + sum_f k_f (v_i - v_j) on a w x h grid (float32 own, kx
+    (h, w+1), ky (h+1, w) couplings on CUDA tensors; gas uint8 (h, w) or None)."""
+
+    def __init__(self, w: int, h: int, levels: int = 0, nu1: int = 2, nu2: int = 2):
+        from . import _check, heat_settings, lib
+        self.w, self.h = w, h
+        hs = heat_settings(levels=levels, nu1=nu1, nu2=nu2)
+        p = C.c_void_p()
+        _check(lib().sr_fld_create(w, h, C.byref(hs), C.byref(p)))
+        self._p = p
+
+    @staticmethod
+    def _s(stream):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        return C.c_void_p(s.cuda_stream)
+
+    def set_system(self, own, kx, ky, gas=None, stream=None):
+        from . import _check, lib
+        for t in (own, kx, ky):
+            assert t.is_contiguous()
+        _check(lib().sr_fld_set_system(self._p, C.c_void_p(own.data_ptr()), C.c_void_p(kx.data_ptr()),
+                                       C.c_void_p(ky.data_ptr()), C.c_void_p(gas.data_ptr()) if gas is not None
+                                       else None, self._s(stream)))
+
+    def solve(self, b, x, tol=1e-8, max_iter=10000, precond=_lib.PRECOND_MULTIGRID, stream=None):
+        """Solves in place into x (float64 CUDA tensor); returns (iterations, ||b - Ax|| / ||b||)."""
+        from . import _check, lib
+        it = C.c_int()
+        rel = C.c_double()
+        _check(lib().sr_fld_solve(self._p, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), tol, max_iter,
+                                  precond, C.byref(it), C.byref(rel), self._s(stream)))
+        return it.value, rel.value
+
+    def apply_precond(self, r, z, precond=_lib.PRECOND_MULTIGRID, stream=None):
+        from . import _check, lib
+        _check(lib().sr_fld_apply_precond(self._p, C.c_void_p(r.data_ptr()), C.c_void_p(z.data_ptr()), precond,
+                                          self._s(stream)))
+
+    def matvec(self, v, out, stream=None):
+        from . import _check, lib
+        _check(lib().sr_fld_matvec(self._p, C.c_void_p(v.data_ptr()), C.c_void_p(out.data_ptr()), self._s(stream)))
+
+    @property
+    def levels(self) -> int:
+        from . import _check, lib
+        n = C.c_int()
+        _check(lib().sr_fld_levels(self._p, C.byref(n)))
+        return n.value
+
+    def __del__(self):
+        from . import _LIB
+        if getattr(self, "_p", None) and _LIB is not None:
+            _LIB.sr_fld_destroy(self._p)
+            self._p = None

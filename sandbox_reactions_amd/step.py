@@ -1,0 +1,105 @@
+"""The world: the reference's Step + State surface over the C ABI (crates/sr-engine/src/step/mod.rs:367-777,
+crates/sr-engine/src/state.rs:139-280; the twin's driver crates/sr-physics/src/reference/mod.rs:129-297)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DtReport, HeatReport, StepCost, Switches
+
+
+class World:
+    """A w x h world on the GPU. `upload` takes the 14 canonical planes (channel-major, row-major, f32), `run` steps
+    P1 -> P2 -> P4 -> P5 -> P8 -> P9 (the passes switched on), `readback` returns the planes."""
+
+    def __init__(self, w: int, h: int, physics=None, gravity=True, hydro=True, heat=False, heat_settings=None,
+                 precond=_lib.PRECOND_MULTIGRID):
+        from . import _check, lib, physics as mkphys, heat_settings as mkhs
+        self.w, self.h = w, h
+        self._ph = physics if physics is not None else mkphys()
+        self._sw = Switches(int(gravity), int(hydro), int(heat))
+        self._hs = heat_settings if heat_settings is not None else mkhs()
+        p = C.c_void_p()
+        _check(lib().sr_world_create(w, h, C.byref(self._ph), C.byref(self._sw), C.byref(self._hs), C.byref(p)))
+        self._p = p
+        if precond != _lib.PRECOND_MULTIGRID:
+            self.set_precond(precond)
+
+    def _c(self, code):
+        from . import _check
+        _check(code)
+
+    @property
+    def lib(self):
+        from . import lib
+        return lib()
+
+    def upload(self, planes: np.ndarray):
+        a = np.ascontiguousarray(planes, dtype=np.float32)
+        if a.shape != (_lib.COMPONENTS, self.h, self.w):
+            from . import SrError
+            raise SrError(_lib.SR_ERR_LENGTH, f"planes must be (14, {self.h}, {self.w}), got {a.shape}")
+        self._c(self.lib.sr_upload(self._p, a.ctypes.data, a.size))
+
+    def readback(self) -> np.ndarray:
+        out = np.empty((_lib.COMPONENTS, self.h, self.w), np.float32)
+        self._c(self.lib.sr_readback(self._p, out.ctypes.data, out.size))
+        return out
+
+    def run(self, steps: int = 1):
+        self._c(self.lib.sr_run(self._p, steps))
+
+    def read_dt(self) -> DtReport:
+        r = DtReport()
+        self._c(self.lib.sr_read_dt(self._p, C.byref(r)))
+        return r
+
+    def read_heat(self) -> HeatReport:
+        r = HeatReport()
+        self._c(self.lib.sr_read_heat(self._p, C.byref(r)))
+        return r
+
+    def read_field(self, field: int) -> np.ndarray:
+        out = np.empty((self.h, self.w), np.float32)
+        self._c(self.lib.sr_read_field(self._p, field, out.ctypes.data, out.size))
+        return out
+
+    def totals(self):
+        t = np.zeros(14)
+        e = np.zeros(14)
+        self._c(self.lib.sr_read_totals(self._p, t.ctypes.data, e.ctypes.data))
+        return t, e
+
+    def set_precond(self, precond: int):
+        self._c(self.lib.sr_set_precond(self._p, precond))
+
+    def set_dt_cap(self, cap: float):
+        self._c(self.lib.sr_set_dt_cap(self._p, cap))
+
+    def set_timing(self, on: bool = True):
+        self._c(self.lib.sr_set_timing(self._p, int(on)))
+
+    def read_cost(self) -> StepCost:
+        c = StepCost()
+        self._c(self.lib.sr_read_cost(self._p, C.byref(c)))
+        return c
+
+    def kernel_time(self, name: str):
+        n = C.c_uint64()
+        ms = C.c_double()
+        b = C.c_double()
+        self._c(self.lib.sr_kernel_time(self._p, name.encode(), C.byref(n), C.byref(ms), C.byref(b)))
+        return n.value, ms.value, b.value
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self.lib.sr_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,101 @@
+"""The FLD linear solver on the GPU against the oracle's f64 model: the same Galerkin hierarchy, the same V-cycle,
+flexible CG to ||b - Ax|| / ||b|| <= 1e-8, the residual checked in f64 by the oracle's own matvec."""
+import numpy as np
+import pytest
+
+import oracle
+from sandbox_reactions_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def system(n, seed=0x5EED0000, stiffness=1e3):
+    c, kx, ky, b, dt = scenes.fld_system(n, seed=seed, stiffness=stiffness)
+    own = c.astype(np.float32).astype(np.float64)
+    kx = kx.astype(np.float32).astype(np.float64)
+    ky = ky.astype(np.float32).astype(np.float64)
+    return own, kx, ky, b
+
+
+def upload(sr, n, own, kx, ky, gas=None, levels=0):
+    import torch
+    f = sr.Fld(n, n, levels=levels)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    g = None if gas is None else torch.tensor(gas.astype(np.uint8), device="cuda")
+    f.set_system(t(own), t(kx), t(ky), g)
+    return f
+
+
+@pytest.mark.parametrize("n", [64, 128, 256])
+def test_mg_pcg_solves_to_1e8_and_matches_the_f64_model(gpu_lib, n):
+    import torch
+    print(f"fld {n}", flush=True)
+    own, kx, ky, b = system(n)
+    f = upload(gpu_lib, n, own, kx, ky)
+    x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    bt = torch.tensor(b, device="cuda")
+    it, rel = f.solve(bt, x, tol=1e-8)
+    xg = x.cpu().numpy()
+    r = b - oracle.fld_matvec(own, kx, ky, None, xg)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert rel <= 1e-8 and true_rel <= 1.01e-8, (rel, true_rel)
+    xo, ito, relo = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-10, precond="mg")
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-7
+    _, it_model, _ = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-8, precond="mg")
+    print(f"iterations gpu {it} model {it_model}")
+    assert it <= 2 * it_model + 6
+
+
+def test_the_v_cycle_is_the_models(gpu_lib):
+    import torch
+    n = 128
+    own, kx, ky, _ = system(n)
+    f = upload(gpu_lib, n, own, kx, ky)
+    r = np.random.default_rng(1).standard_normal((n, n))
+    z = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    f.apply_precond(torch.tensor(r.astype(np.float32), device="cuda"), z)
+    zm = oracle.mg_apply(own, kx, ky, None, r.astype(np.float32).astype(np.float64))
+    assert np.linalg.norm(z.cpu().numpy() - zm) / np.linalg.norm(zm) <= 1e-5
+    assert f.levels == 6  # 128 -> 64 -> 32 -> 16 -> 8 -> 4
+
+
+def test_the_preconditioner_is_symmetric_and_positive(gpu_lib):
+    import torch
+    n = 64
+    own, kx, ky, _ = system(n)
+    f = upload(gpu_lib, n, own, kx, ky)
+    rng = np.random.default_rng(2)
+    u, v = rng.standard_normal((2, n, n)).astype(np.float32)
+    zu = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    zv = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    f.apply_precond(torch.tensor(u, device="cuda"), zu)
+    f.apply_precond(torch.tensor(v, device="cuda"), zv)
+    zu, zv = zu.cpu().numpy().astype(np.float64), zv.cpu().numpy().astype(np.float64)
+    a, b = np.sum(v * zu), np.sum(u * zv)
+    assert abs(a - b) <= 1e-5 * np.sqrt(np.sum(u * zu) * np.sum(v * zv))
+    assert np.sum(u * zu) > 0 and np.sum(v * zv) > 0
+
+
+def test_gas_mask_and_jacobi(gpu_lib):
+    import torch
+    n = 64
+    own, kx, ky, b = system(n)
+    gas = np.ones((n, n), np.uint8)
+    gas[:, :8] = 0
+    b = np.where(gas == 1, b, 0.0)
+    # couplings to non-gas cells are zero in the face form
+    kx[:, 1:9] = 0.0
+    own_g = np.where(gas == 1, own, 1.0)
+    f = upload(gpu_lib, n, own_g, kx, ky, gas=gas)
+    for pc, name in ((1, "mg"), (0, "jacobi")):
+        x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        it, rel = f.solve(torch.tensor(b, device="cuda"), x, tol=1e-8, precond=pc, max_iter=20000)
+        xg = x.cpu().numpy()
+        assert np.all(xg[:, :8] == 0.0)
+        xo, _, _ = oracle.fld_pcg(own_g, kx, ky, gas, b, tol=1e-10, precond="mg")
+        assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-7, name
+
+
+def test_solver_refusals(gpu_lib):
+    with pytest.raises(gpu_lib.SrError):
+        gpu_lib.Fld(64, 64, nu1=0)

@@ -1,0 +1,103 @@
+"""P5 on the GPU against the oracle twin: the heat pass alone (the reference's heat scopes, crates/sr-engine/tests/
+gpu/heat.rs: a hot slab beside vacuum, a pulse under constant diffusivity) and the full step with it."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import both, f32, rel_l1
+from sandbox_reactions_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def tight(sr, **over):
+    d = dict(stop=1, solve_tol=1e-10, newton_heat=1e-9, newton_cell=1e-6)
+    d.update(over)
+    return sr.heat_settings(**d)
+
+
+def hot_slab(n=64, sigma=30.0, t=18800.0, he=True):
+    """A 32 x 32 slab of hot helium (scenes/heat_hot.json's gas) in vacuum: light leaves through its rim."""
+    p = np.zeros((14, n, n))
+    p[0] = 1e-9
+    p[3] = 1e-9
+    p[4] = 1.0
+    sl = slice(n // 4, n // 4 + n // 2)
+    p[0][sl, sl] = sigma
+    if he:
+        p[4][sl, sl] = 0.0
+        p[5][sl, sl] = 1.0
+    inv_mu = 0.75 if he else 2.0
+    p[3][sl, sl] = sigma * t * inv_mu  # gamma 2: E = Sigma eps, eps = T / mu
+    return p
+
+
+def heat_pair(p, hs_over=None, **phys):
+    """One heat-only step on the GPU at the dt P9 gives the uploaded state, and the oracle's P5 at the same dt."""
+    import sandbox_reactions_amd as sr
+    op, gp = both(**phys)
+    _, h, w = p.shape
+    start = f32(p)
+    world = sr.World(w, h, physics=gp, gravity=False, hydro=False, heat=True,
+                     heat_settings=tight(sr, **(hs_over or {})))
+    world.upload(start.astype(np.float32))
+    dt = world.read_dt().dt_next
+    world.run(1)
+    assert world.read_dt().dt == dt
+    planes, rep, fxf, fyf, em = oracle.heat_step(op, start, dt)
+    return planes, rep, fxf, fyf, world
+
+
+def test_hot_slab_beside_vacuum(gpu_lib):
+    print("hot slab", flush=True)
+    p = hot_slab()
+    planes, rep, fxf, fyf, world = heat_pair(p)
+    assert rep.converged and rep.passes >= 2
+    g = world.readback()
+    r = world.read_heat()
+    assert r.converged and r.passes >= 2
+    assert rel_l1(g[3], planes[3]) <= 1e-5, rel_l1(g[3], planes[3])
+    assert abs(r.radiated - rep.radiated) <= 1e-4 * rep.radiated
+    fx, fy = oracle.flux_cells(fxf, fyf)
+    gfx = world.read_field(gpu_lib.FIELD_FLUX_X)
+    scale = np.abs(fx).max()
+    assert np.abs(gfx - fx).max() <= 1e-4 * scale
+
+
+def test_constant_diffusivity_pulse(gpu_lib):
+    print("pulse", flush=True)
+    n = 64
+    p = np.zeros((14, n, n))
+    p[0] = 1.0
+    p[4] = 1.0
+    yy, xx = np.mgrid[0:n, 0:n]
+    T = 0.05 + 2.0 * np.exp(-((xx - n / 2 + 0.5) ** 2 + (yy - n / 2 + 0.5) ** 2) / (2 * 4.0 ** 2))
+    p[3] = T * 2.0  # gamma 2, 1/mu = 2 (hydrogen): E = Sigma T / mu
+    planes, rep, _, _, world = heat_pair(p, constant_diffusivity=0.2, edge=0)
+    g = world.readback()
+    assert rel_l1(g[3], planes[3]) <= 1e-5
+
+
+@pytest.mark.parametrize("opacity", [0, 1])
+def test_full_step_with_heat_matches_the_twin(gpu_lib, opacity):
+    import sandbox_reactions_amd as sr
+    print(f"full step opacity {opacity}", flush=True)
+    n = 64
+    p = scenes.turbulent_scene(n)
+    kw = dict(scenes.config_physics_kwargs(n), opacity=opacity)
+    op, gp = both(**kw)
+    start = f32(p)
+    twin = oracle.Twin(op, start, gravity=True, hydro=True, heat=True)
+    world = sr.World(n, n, physics=gp, gravity=True, hydro=True, heat=True, heat_settings=tight(sr))
+    world.upload(start.astype(np.float32))
+    for _ in range(3):
+        dtg = world.read_dt().dt_next
+        twin.cap_dt_next(dtg)
+        twin.step()
+        world.set_dt_cap(dtg)
+        world.run(1)
+    g, t = world.readback(), twin.state()
+    for c in range(4):
+        assert rel_l1(g[c], t[c]) <= 5e-5, (c, rel_l1(g[c], t[c]))
+    r = world.read_heat()
+    assert r.converged
