@@ -7,9 +7,10 @@
  *
  * Preconditioners: Jacobi (the reference twin's, heat.rs:419) or a multigrid V-cycle whose coarse operators are the
  * Galerkin products of a piecewise-constant 2x2 aggregation (the reference's GPU hierarchy, heat.wgsl:1160-1526,
- * builds the same aggregation). Smoothing is exact red-black Gauss-Seidel, red first on the way down and black first
- * on the way up (the adjoint order, so the preconditioner is symmetric); the coarsest level (<= 16 cells) takes 16
- * forward and 16 backward sweeps. The CG is the flexible (Polak-Ribiere) form. The device solver applies the same
+ * builds the same aggregation), coarsened until a level holds at most 16 cells. Smoothing is exact red-black
+ * Gauss-Seidel, red first on the way down and black first on the way up (the adjoint order, so the preconditioner
+ * is symmetric); the coarsest level is solved exactly (dense Cholesky) when it holds at most 16 cells, else (a
+ * capped level count) by 16 forward and 16 backward sweeps. The CG is the flexible (Polak-Ribiere) form. The device solver applies the same
  * operator in f32; this model pins it.
  */
 #include <math.h>
@@ -54,7 +55,7 @@ static void level_free(level_t *l) {
     free(l->r);
 }
 
-static int is_coarsest(size_t w, size_t h) { return w * h <= 16 || w == 1 || h == 1; }
+static int is_coarsest(size_t w, size_t h) { return w * h <= 16; }
 
 /* level 0 from the reference-form system, then Galerkin aggregation down */
 static void hier_build(hier_t *H, size_t w, size_t h, const double *own, const double *kx, const double *ky,
@@ -141,6 +142,47 @@ static void vcycle(hier_t *H, int L, int nu1, int nu2) {
     level_t *l = &H->lv[L];
     size_t n = l->w * l->h;
     memset(l->z, 0, sizeof(double) * n);
+    if (L == H->n - 1 && n <= 16) {
+        /* the exact solve: A = L L^T on the level's dense matrix */
+        double A[16 * 16], y[16];
+        size_t w = l->w, m = n;
+        memset(A, 0, sizeof A);
+        for (size_t i = 0; i < m; i++) {
+            size_t x = i % w, yy = i / w;
+            double a[4] = {l->ax[yy * (w + 1) + x], l->ax[yy * (w + 1) + x + 1], l->ay[yy * w + x],
+                           l->ay[(yy + 1) * w + x]};
+            long nb[4] = {(long)i - 1, (long)i + 1, (long)i - (long)w, (long)i + (long)w};
+            double d = l->own[i];
+            for (int f = 0; f < 4; f++)
+                if (a[f] != 0.0) {
+                    d += a[f];
+                    A[i * m + nb[f]] -= a[f];
+                }
+            A[i * m + i] += d;
+        }
+        for (size_t j = 0; j < m; j++) {
+            double s = A[j * m + j];
+            for (size_t k = 0; k < j; k++) s -= A[j * m + k] * A[j * m + k];
+            double d = sqrt(s > 1e-300 ? s : 1e-300);
+            A[j * m + j] = d;
+            for (size_t i = j + 1; i < m; i++) {
+                double t = A[i * m + j];
+                for (size_t k = 0; k < j; k++) t -= A[i * m + k] * A[j * m + k];
+                A[i * m + j] = t / d;
+            }
+        }
+        for (size_t i = 0; i < m; i++) {
+            double t = l->b[i];
+            for (size_t k = 0; k < i; k++) t -= A[i * m + k] * y[k];
+            y[i] = t / A[i * m + i];
+        }
+        for (size_t i = m; i-- > 0;) {
+            double t = y[i];
+            for (size_t k = i + 1; k < m; k++) t -= A[k * m + i] * l->z[k];
+            l->z[i] = t / A[i * m + i];
+        }
+        return;
+    }
     if (L == H->n - 1) {
         for (int k = 0; k < COARSE_SWEEPS; k++) {
             half_sweep(l, 0);

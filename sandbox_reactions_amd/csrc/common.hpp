@@ -126,96 +126,49 @@ __device__ inline double block_sum(double v, double *scratch) {
 }
 
 // ------------------------------------------------------------------------------------------- deterministic reduction
-// Each block adds its K sums to part[k * nb + block]; the last block to arrive sums them in block order into out[k].
-// Returns true in every thread of the last block (after out is written and visible to it).
-template <int K>
-__device__ inline bool reduce_finish(double (&v)[K], double *part, int nb, unsigned *counter, double *out) {
+// Block sums (OP = 0) or maxima (OP = 1) of K values into part[k * nb + block]. No fence and no counter: the partials
+// are read by a later kernel or by the host, which sum them in block order (so the result is deterministic). A
+// device-wide fence here would write back the whole L2 of an XCD in every block.
+template <int K, int OP = 0> __device__ inline void block_partials(double (&v)[K], double *part, int nb) {
     __shared__ double sh[K][16];
-    __shared__ bool last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-#pragma unroll
-    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < K; k++) sh[k][wid] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < K; k++) {
-            double s = 0.0;
-            for (int i = 0; i < nw; i++) s += sh[k][i];
-            part[(size_t)k * nb + bid] = s;
-        }
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        last = prev == (unsigned)(nb - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    for (int k = 0; k < K; k++) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x)
-            s += __hip_atomic_load(&part[(size_t)k * nb + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = wave_sum(s);
-        __syncthreads();
-        if (lane == 0) sh[k][wid] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int i = 0; i < nw; i++) t += sh[k][i];
-            out[k] = t;
-        }
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-    __syncthreads();
-    return true;
-}
-
-// As reduce_finish with max in place of sum (order-independent, so deterministic by construction).
-template <int K>
-__device__ inline bool reduce_max_finish(double (&v)[K], double *part, int nb, unsigned *counter, double *out) {
-    __shared__ double shm[K][16];
-    __shared__ bool lastm;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
     for (int k = 0; k < K; k++)
-        for (int o = 32; o > 0; o >>= 1) v[k] = fmax(v[k], __shfl_xor(v[k], o, 64));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double t = __shfl_xor(v[k], o, 64);
+            v[k] = OP ? fmax(v[k], t) : v[k] + t;
+        }
     if (lane == 0)
 #pragma unroll
-        for (int k = 0; k < K; k++) shm[k][wid] = v[k];
+        for (int k = 0; k < K; k++) sh[k][wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = sh[threadIdx.x][0];
+        for (int i = 1; i < nw; i++) s = OP ? fmax(s, sh[threadIdx.x][i]) : s + sh[threadIdx.x][i];
+        part[(size_t)threadIdx.x * nb + bid] = s;
+    }
+}
+
+// The sum of nb partials in a fixed order; every thread of the block gets the result.
+__device__ inline double sum_partials(const double *part, int nb) {
+    __shared__ double shp[16];
+    __shared__ double res;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+    s = wave_sum(s);
+    __syncthreads();
+    if (lane == 0) shp[wid] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 0; k < K; k++) {
-            double s = shm[k][0];
-            for (int i = 1; i < nw; i++) s = fmax(s, shm[k][i]);
-            part[(size_t)k * nb + bid] = s;
-        }
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        lastm = prev == (unsigned)(nb - 1);
+        double t = 0.0;
+        for (int i = 0; i < nw; i++) t += shp[i];
+        res = t;
     }
     __syncthreads();
-    if (!lastm) return false;
-    __threadfence();
-    for (int k = 0; k < K; k++) {
-        double s = -HUGE_VAL;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x)
-            s = fmax(s, __hip_atomic_load(&part[(size_t)k * nb + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
-        __syncthreads();
-        if (lane == 0) shm[k][wid] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = shm[k][0];
-            for (int i = 1; i < nw; i++) t = fmax(t, shm[k][i]);
-            out[k] = t;
-        }
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-    __syncthreads();
-    return true;
+    return res;
 }
 
 inline int ilog2(long n) {

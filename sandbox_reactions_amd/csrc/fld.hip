@@ -7,14 +7,17 @@
 // machinery is this library's own:
 //   * outer loop in FP64: r = b - A x with FP64 x and b, the stopping rule on that true residual;
 //   * inner loop in FP32: flexible (Polak-Ribiere) PCG on the residual, FP64 dot products, x updated in FP64;
-//   * the pre-smoother of a level, its residual and its restriction are one kernel: a 32x32 tile with a halo of
+//   * the pre-smoother of a level, its residual and its restriction are one kernel: a 64x32 tile with a halo of
 //     2 nu1 + 1 cells in LDS, each half sweep recomputed redundantly on the shrinking halo, so the result is exact
 //     red-black Gauss-Seidel with one pass over HBM; the prolongation and the post-smoother are another such kernel;
-//   * every level of at most 1024 cells is solved inside one 1024-thread block (LDS-resident V-cycle);
-//   * dot products reduce by wave64 shuffles, per-block partials summed in a fixed order by the last block to finish
-//     (deterministic), which also forms alpha and beta on the device.
+//   * every level of at most 1024 cells is handled inside one 1024-thread block (an LDS-resident V-cycle), the
+//     coarsest (at most 16 cells) by an exact dense solve;
+//   * dot products reduce by wave64 shuffles into per-block partials that the next kernel sums itself in a fixed
+//     order (deterministic, and no device-wide fence: a fence at agent scope writes back the whole L2 of an XCD);
+//     alpha and beta are formed on the device by every block alike.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.hpp"
@@ -22,16 +25,13 @@
 
 namespace sr {
 
-#define TILE 32
 #define NT 256
+#define NB_MAX 1024
 #define COARSE_SWEEPS 16
-#define COARSE_MAX 1024
-
-enum {
-    SC_RZ = 0, SC_PQ, SC_ALPHA, SC_BETA, SC_RR, SC_RZN, SC_RZO, SC_BB, SC_R64, SC_UMEAS, SC_HMEAS, SC_RESTART,
-    SC_N = 16
-};
-enum { CNT_TRUE = 0, CNT_PQ, CNT_RR, CNT_DOTS, CNT_N };
+#define COARSE_MAX 4096
+#define COARSE_NT 512
+#define COARSE_LDS (150 * 1024)
+#define DENSE_MAX 16
 
 // ------------------------------------------------------------------------------------------------ elementwise kernels
 
@@ -43,11 +43,13 @@ struct L0 {
 
 __device__ __forceinline__ bool is_gas(const L0 &a, size_t i) { return a.gas == nullptr || a.gas[i]; }
 
-// r = b - A x in FP64 (x zeroed off the gas), r32 = (float) r; sums ||r||^2, ||b||^2 and the reference's two measures
+#define UNROLL 4
+
+// r = b - A x in FP64 (x zeroed off the gas), r32 = (float) r; partials of ||r||^2, ||b||^2 and the reference's two
+// measures
 __global__ void __launch_bounds__(NT) k_true_residual(L0 a, const double *__restrict__ b, double *__restrict__ x,
-                                                      float *__restrict__ r32,
-                                                      const float *__restrict__ heat, const float *__restrict__ slope,
-                                                      double *sc, double *part, unsigned *counter) {
+                                                      float *__restrict__ r32, const float *__restrict__ heat,
+                                                      const float *__restrict__ slope, double *part) {
     const int n = a.w * a.h;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
@@ -76,78 +78,88 @@ __global__ void __launch_bounds__(NT) k_true_residual(L0 a, const double *__rest
         r32[i] = (float)r;
         v[0] += r * r;
     }
-    double out[4];
-    if (reduce_finish<4>(v, part, gridDim.x, counter + CNT_TRUE, out) && threadIdx.x == 0) {
-        sc[SC_R64] = out[0];
-        sc[SC_BB] = out[1];
-        sc[SC_UMEAS] = out[2];
-        sc[SC_HMEAS] = out[3];
-        sc[SC_RESTART] = 1.0;
-        sc[SC_BETA] = 0.0;
-    }
+    block_partials<4>(v, part, gridDim.x);
 }
 
-// p = z + beta p_old (into p_new), q = A p_new in FP32; sums p.q and forms alpha = rz / pq
+// beta from the preconditioner's partials: (r.z_new - r.z_old) / (r.z of the previous iteration), 0 on a restart
+__device__ inline float beta_of(const double *dots_cur, const double *dots_prev, int nbd, int nbp, int restart) {
+    if (restart) return 0.0f;
+    const double rzn = sum_partials(dots_cur, nbd), rzo = sum_partials(dots_cur + nbd, nbd);
+    const double rzp = sum_partials(dots_prev, nbp);
+    return rzp != 0.0 ? (float)((rzn - rzo) / rzp) : 0.0f;
+}
+
+// p = z + beta p_old (into p_new), q = A p_new in FP32; partials of p.q. Blocks of 256 columns x `rows` rows.
+#define MV_ROWS 16
 __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__ z, const float *__restrict__ pold,
-                                                 float *__restrict__ pnew, float *__restrict__ q, double *sc,
-                                                 double *part, unsigned *counter) {
-    const int n = a.w * a.h;
-    const float beta = (float)sc[SC_BETA];
-    const bool fresh = sc[SC_RESTART] != 0.0;
-    auto P = [&](int j) { return fresh ? z[j] : z[j] + beta * pold[j]; };
+                                                 float *__restrict__ pnew, float *__restrict__ q,
+                                                 const double *dots_cur, const double *dots_prev, int nbd, int nbp,
+                                                 int restart, int rows, double *part) {
+    const float beta = beta_of(dots_cur, dots_prev, nbd, nbp, restart);
+    const int w = a.w, h = a.h;
+    const int x = blockIdx.x * NT + threadIdx.x, y0 = blockIdx.y * rows;
+    auto P = [&](size_t j) { return z[j] + beta * pold[j]; };
     double v[1] = {0.0};
-    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
-        const int xx = i % a.w, yy = i / a.w;
-        const float pi = P(i);
-        const float al = a.ax[(size_t)yy * (a.w + 1) + xx], ar = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
-        const float au = a.ay[(size_t)yy * a.w + xx], ad = a.ay[(size_t)(yy + 1) * a.w + xx];
-        float s = a.own[i] * pi;
-        if (al != 0.0f) s += al * (pi - P(i - 1));
-        if (ar != 0.0f) s += ar * (pi - P(i + 1));
-        if (au != 0.0f) s += au * (pi - P(i - a.w));
-        if (ad != 0.0f) s += ad * (pi - P(i + a.w));
-        pnew[i] = pi;
-        q[i] = s;
-        v[0] += (double)pi * (double)s;
+    if (x < w) {
+#pragma unroll 4
+        for (int r = 0; r < rows; r++) {
+            const int y = y0 + r;
+            if (y >= h) break;
+            const size_t i = (size_t)y * w + x;
+            const float pi = P(i);
+            const float al = a.ax[(size_t)y * (w + 1) + x], ar = a.ax[(size_t)y * (w + 1) + x + 1];
+            const float au = a.ay[i], ad = a.ay[i + w];
+            float s = a.own[i] * pi;
+            if (al != 0.0f) s += al * (pi - P(i - 1));
+            if (ar != 0.0f) s += ar * (pi - P(i + 1));
+            if (au != 0.0f) s += au * (pi - P(i - w));
+            if (ad != 0.0f) s += ad * (pi - P(i + w));
+            pnew[i] = pi;
+            q[i] = s;
+            v[0] += (double)pi * (double)s;
+        }
     }
-    double out[1];
-    if (reduce_finish<1>(v, part, gridDim.x, counter + CNT_PQ, out) && threadIdx.x == 0) {
-        sc[SC_PQ] = out[0];
-        sc[SC_ALPHA] = out[0] != 0.0 ? sc[SC_RZ] / out[0] : 0.0;
-        sc[SC_RESTART] = 0.0;
-    }
+    block_partials<1>(v, part, gridDim.x * gridDim.y);
 }
 
-// x += alpha p (FP64), r -= alpha q; sums ||r||^2
+// x += alpha p (FP64), r -= alpha q, alpha = r.z / p.q from the partials; partials of ||r||^2
 __global__ void __launch_bounds__(NT) k_update(int n, const float *__restrict__ p, const float *__restrict__ q,
-                                               double *__restrict__ x, float *__restrict__ r, double *sc,
-                                               double *part, unsigned *counter) {
-    const double alpha = sc[SC_ALPHA];
+                                               double *__restrict__ x, float *__restrict__ r, const double *dots_cur,
+                                               int nbd, const double *pq_part, int nbq, double *part) {
+    const double rz = sum_partials(dots_cur, nbd), pq = sum_partials(pq_part, nbq);
+    const double alpha = pq != 0.0 ? rz / pq : 0.0;
     const float af = (float)alpha;
     double v[1] = {0.0};
-    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
-        x[i] += alpha * (double)p[i];
-        const float ri = r[i] - af * q[i];
-        r[i] = ri;
-        v[0] += (double)ri * ri;
+    for (int base = blockIdx.x * NT * UNROLL + threadIdx.x; base < n; base += gridDim.x * NT * UNROLL) {
+        float pv[UNROLL], qv[UNROLL], rv[UNROLL];
+        double xv[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int i = base + u * NT;
+            if (i < n) {
+                pv[u] = p[i];
+                qv[u] = q[i];
+                rv[u] = r[i];
+                xv[u] = x[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int i = base + u * NT;
+            if (i < n) {
+                x[i] = xv[u] + alpha * (double)pv[u];
+                const float ri = rv[u] - af * qv[u];
+                r[i] = ri;
+                v[0] += (double)ri * ri;
+            }
+        }
     }
-    double out[1];
-    if (reduce_finish<1>(v, part, gridDim.x, counter + CNT_RR, out) && threadIdx.x == 0) sc[SC_RR] = out[0];
+    block_partials<1>(v, part, gridDim.x);
 }
 
-// the flexible beta from r.z_new and r.z_old (sums made by the preconditioner's last kernel)
-__device__ inline void finish_beta(double *sc, double rzn, double rzo) {
-    const bool fresh = sc[SC_RESTART] != 0.0;
-    sc[SC_RZN] = rzn;
-    sc[SC_RZO] = rzo;
-    sc[SC_BETA] = fresh || sc[SC_RZ] == 0.0 ? 0.0 : (rzn - rzo) / sc[SC_RZ];
-    sc[SC_RZ] = rzn;
-}
-
-// Jacobi: z = r / D, D = own + sum of couplings; sums r.z and r.z_old
+// Jacobi: z = r / D, D = own + sum of couplings; partials of r.z and r.z_old
 __global__ void __launch_bounds__(NT) k_jacobi(L0 a, const float *__restrict__ r, float *__restrict__ z,
-                                               const float *__restrict__ zold, double *sc, double *part,
-                                               unsigned *counter, int dots) {
+                                               const float *__restrict__ zold, double *part) {
     const int n = a.w * a.h;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
@@ -159,24 +171,18 @@ __global__ void __launch_bounds__(NT) k_jacobi(L0 a, const float *__restrict__ r
         v[0] += (double)r[i] * zi;
         if (zold) v[1] += (double)r[i] * zold[i];
     }
-    if (!dots) return;
-    double out[2];
-    if (reduce_finish<2>(v, part, gridDim.x, counter + CNT_DOTS, out) && threadIdx.x == 0)
-        finish_beta(sc, out[0], out[1]);
+    if (part) block_partials<2>(v, part, gridDim.x);
 }
 
-// r.z and r.z_old after a preconditioner that made no sums itself; forms beta
+// partials of r.z and r.z_old after a preconditioner that made none
 __global__ void __launch_bounds__(NT) k_dots(int n, const float *__restrict__ r, const float *__restrict__ z,
-                                             const float *__restrict__ zold, double *sc, double *part,
-                                             unsigned *counter) {
+                                             const float *__restrict__ zold, double *part) {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
         v[0] += (double)r[i] * z[i];
         if (zold) v[1] += (double)r[i] * zold[i];
     }
-    double out[2];
-    if (reduce_finish<2>(v, part, gridDim.x, counter + CNT_DOTS, out) && threadIdx.x == 0)
-        finish_beta(sc, out[0], out[1]);
+    block_partials<2>(v, part, gridDim.x);
 }
 
 // ------------------------------------------------------------------------------------------------ tile smoothers
@@ -186,185 +192,209 @@ struct Lv {
     const float *own, *ax, *ay;
 };
 
-// Region of a tile plus halo `hw`: LDS arrays of (T + 2 hw + 1)^2 entries (one extra column/row for the right and
-// bottom faces).
-template <int HW> struct Region {
-    static constexpr int RW = TILE + 2 * HW;
-    static constexpr int RS = RW + 1;
-    float z[RS * RS], b[RS * RS], own[RS * RS], al[RS * RS], au[RS * RS];
+// The region of a TX x TY tile plus a halo of HW cells: (TX + 2 HW) x (TY + 2 HW) entries per array. A cell at
+// least one cell inside the region reads its right and bottom faces from its neighbours' left and top entries.
+template <int TX, int TY, int HW> struct Region {
+    static constexpr int RX = TX + 2 * HW, RY = TY + 2 * HW, N = RX * RY;
+    float z[N], b[N], own[N], al[N], au[N];
 };
 
-template <int HW>
-__device__ inline void load_region(Region<HW> &R, const Lv &L, int x0, int y0, const float *__restrict__ b) {
-    constexpr int RS = Region<HW>::RS;
-    for (int t = threadIdx.x; t < RS * RS; t += NT) {
-        const int lx = t % RS, ly = t / RS;
+template <int TX, int TY, int HW>
+__device__ inline void load_region(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0, const float *__restrict__ b) {
+    using RG = Region<TX, TY, HW>;
+    constexpr int IT = (RG::N + NT - 1) / NT;
+    // every load of the region issued before any is used: one memory round trip per block
+    float vo[IT], vb[IT], vl[IT], vu[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int t = threadIdx.x + k * NT;
+        const int lx = t % RG::RX, ly = t / RG::RX;
         const int gx = x0 + lx - HW, gy = y0 + ly - HW;
-        const bool in = gx >= 0 && gx < L.w && gy >= 0 && gy < L.h;
+        const bool in = t < RG::N && gx >= 0 && gx < L.w && gy >= 0 && gy < L.h;
         const size_t i = (size_t)gy * L.w + gx;
-        R.own[t] = in ? L.own[i] : 1.0f;
-        R.b[t] = in && b ? b[i] : 0.0f;
-        // the face on the cell's left / top; for a column or row past the grid, the grid's edge face (0)
-        R.al[t] = (gy >= 0 && gy < L.h && gx >= 0 && gx <= L.w) ? L.ax[(size_t)gy * (L.w + 1) + gx] : 0.0f;
-        R.au[t] = (gx >= 0 && gx < L.w && gy >= 0 && gy <= L.h) ? L.ay[(size_t)gy * L.w + gx] : 0.0f;
+        vo[k] = in ? L.own[i] : 1.0f;
+        vb[k] = in && b ? b[i] : 0.0f;
+        vl[k] = in ? L.ax[(size_t)gy * (L.w + 1) + gx] : 0.0f;
+        vu[k] = in ? L.ay[i] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int t = threadIdx.x + k * NT;
+        if (t < RG::N) {
+            R.own[t] = vo[k];
+            R.b[t] = vb[k];
+            R.al[t] = vl[k];
+            R.au[t] = vu[k];
+        }
     }
 }
 
-// one half sweep of `color` over the region cells at least `k` cells from the region's border
-template <int HW>
-__device__ inline void half_sweep(Region<HW> &R, const Lv &L, int x0, int y0, int color, int k) {
-    constexpr int RW = Region<HW>::RW, RS = Region<HW>::RS;
-    const int span = RW - 2 * k;
-    for (int t = threadIdx.x; t < span * span; t += NT) {
-        const int lx = k + t % span, ly = k + t / span;
+// One half sweep of colour `color` over the region cells at least K cells from its border (colour-compact: every
+// thread takes a cell of the colour). Parity of a region cell: (lx + ly) & 1 (tile origins and 2 HW are even).
+template <int TX, int TY, int HW, int K>
+__device__ inline void half_sweep(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0, int color) {
+    using RG = Region<TX, TY, HW>;
+    constexpr int SX = RG::RX - 2 * K, SY = RG::RY - 2 * K, HALF = SX / 2, TOTAL = HALF * SY;
+    constexpr int IT = (TOTAL + NT - 1) / NT;
+    static_assert(SX % 2 == 0, "even spans");
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int t = threadIdx.x + k * NT;
+        if (t >= TOTAL) break;
+        const int row = t / HALF, j = t - row * HALF;
+        const int ly = K + row;
+        const int lx = K + ((color - (K + ly)) & 1) + 2 * j;
         const int gx = x0 + lx - HW, gy = y0 + ly - HW;
-        if (((gx + gy) & 1) != color) continue;
         if (gx < 0 || gx >= L.w || gy < 0 || gy >= L.h) continue;
-        const int c = ly * RS + lx;
-        const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RS];
-        float num = R.b[c], den = R.own[c];
-        if (al != 0.0f) { num += al * R.z[c - 1]; den += al; }
-        if (ar != 0.0f) { num += ar * R.z[c + 1]; den += ar; }
-        if (au != 0.0f) { num += au * R.z[c - RS]; den += au; }
-        if (ad != 0.0f) { num += ad * R.z[c + RS]; den += ad; }
-        R.z[c] = num / den;
+        const int c = ly * RG::RX + lx;
+        const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RG::RX];
+        const float num = R.b[c] + al * R.z[c - 1] + ar * R.z[c + 1] + au * R.z[c - RG::RX] + ad * R.z[c + RG::RX];
+        R.z[c] = num / (R.own[c] + al + ar + au + ad);
     }
     __syncthreads();
 }
 
-// Pre-smoothing of level L from z = 0 (nu1 sweeps, red first), then the residual on the tile and its 2x2
-// restriction into the next level's b. Halo 2 nu1 + 1.
-template <int NU>
+template <int TX, int TY, int HW, int K0, int NH, bool DOWN>
+__device__ inline void sweeps(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0) {
+    if constexpr (NH > 0) {
+        // down: red (0) first; up: black (1) first
+        constexpr int color = DOWN ? (K0 - 1) & 1 : 1 - ((K0 - 1) & 1);
+        half_sweep<TX, TY, HW, K0>(R, L, x0, y0, color);
+        sweeps<TX, TY, HW, K0 + 1, NH - 1, DOWN>(R, L, x0, y0);
+    }
+}
+
+// Pre-smoothing of level L from z = 0 (nu sweeps, red first), then the residual on the tile and its 2x2
+// restriction into the next level's b. Halo 2 nu + 1.
+template <int TX, int TY, int NU>
 __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, float *__restrict__ z,
                                             float *__restrict__ bc, int wc) {
     constexpr int HW = 2 * NU + 1;
-    constexpr int RS = Region<HW>::RS;
-    __shared__ Region<HW> R;
-    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
-    load_region<HW>(R, L, x0, y0, b);
-    for (int t = threadIdx.x; t < RS * RS; t += NT) R.z[t] = 0.0f;
+    using RG = Region<TX, TY, HW>;
+    __shared__ RG R;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    load_region<TX, TY, HW>(R, L, x0, y0, b);
+#pragma unroll 4
+    for (int t = threadIdx.x; t < RG::N; t += NT) R.z[t] = 0.0f;
     __syncthreads();
-    for (int s = 0; s < 2 * NU; s++) half_sweep<HW>(R, L, x0, y0, s & 1, s + 1);
-    // the residual of the tile's cells, summed over each 2x2 aggregate (tile origin even: aggregates lie inside)
-    for (int t = threadIdx.x; t < (TILE / 2) * (TILE / 2); t += NT) {
-        const int cx = t % (TILE / 2), cy = t / (TILE / 2);
+    sweeps<TX, TY, HW, 1, 2 * NU, true>(R, L, x0, y0);
+    constexpr int CX = TX / 2, CY = TY / 2;
+    for (int t = threadIdx.x; t < CX * CY; t += NT) {
+        const int cx = t % CX, cy = t / CX;
         const int X = x0 / 2 + cx, Y = y0 / 2 + cy;
+        if (2 * X >= L.w || 2 * Y >= L.h) continue;
         float sum = 0.0f;
-        bool any = false;
+#pragma unroll
         for (int dy = 0; dy < 2; dy++)
+#pragma unroll
             for (int dx = 0; dx < 2; dx++) {
                 const int lx = HW + 2 * cx + dx, ly = HW + 2 * cy + dy;
-                const int gx = x0 + lx - HW, gy = y0 + ly - HW;
-                if (gx >= L.w || gy >= L.h) continue;
-                any = true;
-                const int c = ly * RS + lx;
+                if (x0 + lx - HW >= L.w || y0 + ly - HW >= L.h) continue;
+                const int c = ly * RG::RX + lx;
                 const float zc = R.z[c];
-                float az = R.own[c] * zc;
-                const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RS];
-                if (al != 0.0f) az += al * (zc - R.z[c - 1]);
-                if (ar != 0.0f) az += ar * (zc - R.z[c + 1]);
-                if (au != 0.0f) az += au * (zc - R.z[c - RS]);
-                if (ad != 0.0f) az += ad * (zc - R.z[c + RS]);
+                const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RG::RX];
+                const float az = R.own[c] * zc + al * (zc - R.z[c - 1]) + ar * (zc - R.z[c + 1]) +
+                                 au * (zc - R.z[c - RG::RX]) + ad * (zc - R.z[c + RG::RX]);
                 sum += R.b[c] - az;
             }
-        if (any) bc[(size_t)Y * wc + X] = sum;
+        bc[(size_t)Y * wc + X] = sum;
     }
-    for (int t = threadIdx.x; t < TILE * TILE; t += NT) {
-        const int lx = HW + t % TILE, ly = HW + t / TILE;
+    for (int t = threadIdx.x; t < TX * TY; t += NT) {
+        const int lx = HW + t % TX, ly = HW + t / TX;
         const int gx = x0 + lx - HW, gy = y0 + ly - HW;
-        if (gx < L.w && gy < L.h) z[(size_t)gy * L.w + gx] = R.z[ly * RS + lx];
+        if (gx < L.w && gy < L.h) z[(size_t)gy * L.w + gx] = R.z[ly * RG::RX + lx];
     }
 }
 
-// Prolongation of the coarse correction and post-smoothing of level L (nu2 sweeps, black first). Halo 2 nu2.
-// With `dots` (level 0), sums r.z_new and r.z_old and forms beta.
-template <int NU>
+// Prolongation of the coarse correction and post-smoothing of level L (nu sweeps, black first). Halo 2 nu. With
+// `part` (level 0), partials of r.z_new and r.z_old.
+template <int TX, int TY, int NU>
 __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, const float *__restrict__ zin,
                                              const float *__restrict__ zc, int wc, float *__restrict__ zout,
-                                             const float *__restrict__ zold, double *sc, double *part,
-                                             unsigned *counter, int dots) {
+                                             const float *__restrict__ zold, double *part) {
     constexpr int HW = 2 * NU;
-    constexpr int RS = Region<HW>::RS;
-    __shared__ Region<HW> R;
-    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
-    load_region<HW>(R, L, x0, y0, b);
-    for (int t = threadIdx.x; t < RS * RS; t += NT) {
-        const int lx = t % RS, ly = t / RS;
-        const int gx = x0 + lx - HW, gy = y0 + ly - HW;
-        const bool in = gx >= 0 && gx < L.w && gy >= 0 && gy < L.h;
-        R.z[t] = in ? zin[(size_t)gy * L.w + gx] + zc[(size_t)(gy >> 1) * wc + (gx >> 1)] : 0.0f;
+    using RG = Region<TX, TY, HW>;
+    __shared__ RG R;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    load_region<TX, TY, HW>(R, L, x0, y0, b);
+    {
+        constexpr int IT = (RG::N + NT - 1) / NT;
+        float zv[IT];
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int t = threadIdx.x + k * NT;
+            const int lx = t % RG::RX, ly = t / RG::RX;
+            const int gx = x0 + lx - HW, gy = y0 + ly - HW;
+            const bool in = t < RG::N && gx >= 0 && gx < L.w && gy >= 0 && gy < L.h;
+            zv[k] = in ? zin[(size_t)gy * L.w + gx] + zc[(size_t)(gy >> 1) * wc + (gx >> 1)] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int t = threadIdx.x + k * NT;
+            if (t < RG::N) R.z[t] = zv[k];
+        }
     }
     __syncthreads();
-    for (int s = 0; s < 2 * NU; s++) half_sweep<HW>(R, L, x0, y0, 1 - (s & 1), s + 1);
+    sweeps<TX, TY, HW, 1, 2 * NU, false>(R, L, x0, y0);
     double v[2] = {0.0, 0.0};
-    for (int t = threadIdx.x; t < TILE * TILE; t += NT) {
-        const int lx = HW + t % TILE, ly = HW + t / TILE;
+    for (int t = threadIdx.x; t < TX * TY; t += NT) {
+        const int lx = HW + t % TX, ly = HW + t / TX;
         const int gx = x0 + lx - HW, gy = y0 + ly - HW;
         if (gx < L.w && gy < L.h) {
             const size_t i = (size_t)gy * L.w + gx;
-            const float zn = R.z[ly * RS + lx];
+            const float zn = R.z[ly * RG::RX + lx];
             zout[i] = zn;
-            if (dots) {
-                const float bi = R.b[ly * RS + lx];
+            if (part) {
+                const float bi = R.b[ly * RG::RX + lx];
                 v[0] += (double)bi * zn;
                 if (zold) v[1] += (double)bi * zold[i];
             }
         }
     }
-    if (!dots) return;
-    double out[2];
-    if (reduce_finish<2>(v, part, gridDim.x * gridDim.y, counter + CNT_DOTS, out) && threadIdx.x == 0)
-        finish_beta(sc, out[0], out[1]);
+    if (part) block_partials<2>(v, part, gridDim.x * gridDim.y);
 }
 
-// The V-cycle over every level of at most COARSE_MAX cells in one block: levels [first, last], the first one's b
-// from global memory, its z written back. The coarsest takes COARSE_SWEEPS forward and backward sweeps.
+// The V-cycle over every level of at most COARSE_MAX cells in one block. The levels' own, ax and ay arrays (and the
+// coarsest's inverse) lie back to back in one packed buffer, so one coalesced sweep fills the LDS; z and b of every
+// level follow it in LDS. The first level's b comes from global memory and its z goes back there. The coarsest is
+// solved exactly by the inverse formed at the rebuild when it holds at most DENSE_MAX cells, else by COARSE_SWEEPS
+// forward and backward sweeps.
 struct CoarseSet {
     int n;
     int w[16], h[16];
-    const float *own[16], *ax[16], *ay[16];
+    int pack_floats;   // own/ax/ay of every level, then the inverse
+    int offo[16], offx[16], offy[16], offz[16], offb[16], offinv, total;  // LDS offsets (floats)
 };
 
-__global__ void __launch_bounds__(COARSE_MAX) k_coarse(CoarseSet C, const float *__restrict__ b0, float *__restrict__ z0,
-                                                       int nu1, int nu2, int level0_gasless) {
+__global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *__restrict__ pack,
+                                                      const float *__restrict__ b0, float *__restrict__ z0, int nu1,
+                                                      int nu2) {
     extern __shared__ float sm[];
-    // per level: own, al ((w+1) h), au (w (h+1)), z, b
-    int off[16][5];
-    int o = 0;
-    for (int l = 0; l < C.n; l++) {
-        const int w = C.w[l], h = C.h[l];
-        off[l][0] = o; o += w * h;
-        off[l][1] = o; o += (w + 1) * h;
-        off[l][2] = o; o += w * (h + 1);
-        off[l][3] = o; o += w * h;
-        off[l][4] = o; o += w * h;
-    }
-    for (int l = 0; l < C.n; l++) {
-        const int w = C.w[l], h = C.h[l];
-        for (int t = threadIdx.x; t < w * h; t += blockDim.x) {
-            sm[off[l][0] + t] = C.own[l][t];
-            sm[off[l][3] + t] = 0.0f;
-            sm[off[l][4] + t] = l == 0 ? b0[t] : 0.0f;
-        }
-        for (int t = threadIdx.x; t < (w + 1) * h; t += blockDim.x) sm[off[l][1] + t] = C.ax[l][t];
-        for (int t = threadIdx.x; t < w * (h + 1); t += blockDim.x) sm[off[l][2] + t] = C.ay[l][t];
-    }
+    const int *offo = C.offo, *offx = C.offx, *offy = C.offy, *offz = C.offz, *offb = C.offb;
+    const int offinv = C.offinv, total = C.total;
+#pragma unroll 8
+    for (int t = threadIdx.x; t < C.pack_floats; t += COARSE_NT) sm[t] = pack[t];
+    for (int t = C.pack_floats + threadIdx.x; t < total; t += COARSE_NT) sm[t] = 0.0f;
     __syncthreads();
+    for (int t = threadIdx.x; t < C.w[0] * C.h[0]; t += COARSE_NT) sm[offb[0] + t] = b0[t];
+    __syncthreads();
+    // one half sweep of `color` on level l, colour-compact
     auto sweep = [&](int l, int color) {
-        const int w = C.w[l], h = C.h[l];
-        float *own = sm + off[l][0], *al = sm + off[l][1], *au = sm + off[l][2], *z = sm + off[l][3],
-              *b = sm + off[l][4];
-        for (int t = threadIdx.x; t < w * h; t += blockDim.x) {
-            const int x = t % w, y = t / w;
-            if (((x + y) & 1) != color) continue;
-            const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[y * w + x],
-                        a3 = au[(y + 1) * w + x];
-            float num = b[t], den = own[t];
-            if (a0 != 0.0f) { num += a0 * z[t - 1]; den += a0; }
-            if (a1 != 0.0f) { num += a1 * z[t + 1]; den += a1; }
-            if (a2 != 0.0f) { num += a2 * z[t - w]; den += a2; }
-            if (a3 != 0.0f) { num += a3 * z[t + w]; den += a3; }
-            z[t] = num / den;
+        const int w = C.w[l], h = C.h[l], hw = (w + 1) >> 1;
+        const float *own = sm + offo[l], *al = sm + offx[l], *au = sm + offy[l], *b = sm + offb[l];
+        float *z = sm + offz[l];
+        for (int t = threadIdx.x; t < hw * h; t += COARSE_NT) {
+            const int y = t / hw;
+            const int x = 2 * (t - y * hw) + ((color + y) & 1);
+            if (x >= w) continue;
+            const int i = y * w + x;
+            const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[i], a3 = au[i + w];
+            float num = b[i], den = own[i];
+            if (a0 != 0.0f) { num += a0 * z[i - 1]; den += a0; }
+            if (a1 != 0.0f) { num += a1 * z[i + 1]; den += a1; }
+            if (a2 != 0.0f) { num += a2 * z[i - w]; den += a2; }
+            if (a3 != 0.0f) { num += a3 * z[i + w]; den += a3; }
+            z[i] = num / den;
         }
         __syncthreads();
     };
@@ -374,19 +404,18 @@ __global__ void __launch_bounds__(COARSE_MAX) k_coarse(CoarseSet C, const float 
             sweep(l, 1);
         }
         const int w = C.w[l], h = C.h[l], wc = C.w[l + 1], hc = C.h[l + 1];
-        float *own = sm + off[l][0], *al = sm + off[l][1], *au = sm + off[l][2], *z = sm + off[l][3],
-              *b = sm + off[l][4];
-        float *bc = sm + off[l + 1][4];
-        for (int t = threadIdx.x; t < wc * hc; t += blockDim.x) {
-            const int X = t % wc, Y = t / wc;
+        const float *own = sm + offo[l], *al = sm + offx[l], *au = sm + offy[l], *z = sm + offz[l],
+                    *b = sm + offb[l];
+        float *bc = sm + offb[l + 1];
+        for (int t = threadIdx.x; t < wc * hc; t += COARSE_NT) {
+            const int Y = t / wc, X = t - Y * wc;
             float s = 0.0f;
             for (int dy = 0; dy < 2; dy++)
                 for (int dx = 0; dx < 2; dx++) {
                     const int x = 2 * X + dx, y = 2 * Y + dy;
                     if (x >= w || y >= h) continue;
                     const int i = y * w + x;
-                    const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[y * w + x],
-                                a3 = au[(y + 1) * w + x];
+                    const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[i], a3 = au[i + w];
                     float az = own[i] * z[i];
                     if (a0 != 0.0f) az += a0 * (z[i] - z[i - 1]);
                     if (a1 != 0.0f) az += a1 * (z[i] - z[i + 1]);
@@ -399,20 +428,33 @@ __global__ void __launch_bounds__(COARSE_MAX) k_coarse(CoarseSet C, const float 
         __syncthreads();
     }
     const int lc = C.n - 1;
-    for (int k = 0; k < COARSE_SWEEPS; k++) {
-        sweep(lc, 0);
-        sweep(lc, 1);
-    }
-    for (int k = 0; k < COARSE_SWEEPS; k++) {
-        sweep(lc, 1);
-        sweep(lc, 0);
+    const int m = C.w[lc] * C.h[lc];
+    if (m <= DENSE_MAX) {
+        // the exact solve by the inverse formed at the system's rebuild
+        float *z = sm + offz[lc];
+        const float *b = sm + offb[lc], *ainv = sm + offinv;
+        if (threadIdx.x < m) {
+            float t = 0.0f;
+            for (int j = 0; j < m; j++) t += ainv[threadIdx.x * m + j] * b[j];
+            z[threadIdx.x] = t;
+        }
+        __syncthreads();
+    } else {
+        for (int k = 0; k < COARSE_SWEEPS; k++) {
+            sweep(lc, 0);
+            sweep(lc, 1);
+        }
+        for (int k = 0; k < COARSE_SWEEPS; k++) {
+            sweep(lc, 1);
+            sweep(lc, 0);
+        }
     }
     for (int l = C.n - 2; l >= 0; l--) {
         const int w = C.w[l], h = C.h[l], wc = C.w[l + 1];
-        float *z = sm + off[l][3];
-        const float *zc = sm + off[l + 1][3];
-        for (int t = threadIdx.x; t < w * h; t += blockDim.x) {
-            const int x = t % w, y = t / w;
+        float *z = sm + offz[l];
+        const float *zc = sm + offz[l + 1];
+        for (int t = threadIdx.x; t < w * h; t += COARSE_NT) {
+            const int y = t / w, x = t - y * w;
             z[t] += zc[(y >> 1) * wc + (x >> 1)];
         }
         __syncthreads();
@@ -421,7 +463,7 @@ __global__ void __launch_bounds__(COARSE_MAX) k_coarse(CoarseSet C, const float 
             sweep(l, 0);
         }
     }
-    for (int t = threadIdx.x; t < C.w[0] * C.h[0]; t += blockDim.x) z0[t] = sm[off[0][3] + t];
+    for (int t = threadIdx.x; t < C.w[0] * C.h[0]; t += COARSE_NT) z0[t] = sm[offz[0] + t];
 }
 
 // ------------------------------------------------------------------------------------------------ hierarchy build
@@ -472,6 +514,40 @@ __global__ void k_coarsen_faces(Lv f, float *__restrict__ axc, float *__restrict
     }
 }
 
+// The inverse of the coarsest level's dense matrix (at most DENSE_MAX cells) by Gauss-Jordan elimination, one
+// thread per row; the matrix is SPD, so no pivoting.
+__global__ void k_coarse_inverse(int w, int h, const float *__restrict__ own, const float *__restrict__ ax,
+                                 const float *__restrict__ ay, float *__restrict__ ainv) {
+    __shared__ float A[DENSE_MAX][2 * DENSE_MAX];
+    const int m = w * h, i = threadIdx.x;
+    if (i < m) {
+        for (int j = 0; j < 2 * m; j++) A[i][j] = 0.0f;
+        const int x = i % w, y = i / w;
+        const float a[4] = {ax[y * (w + 1) + x], ax[y * (w + 1) + x + 1], ay[y * w + x], ay[(y + 1) * w + x]};
+        const int nb[4] = {i - 1, i + 1, i - w, i + w};
+        float d = own[i];
+        for (int f = 0; f < 4; f++)
+            if (a[f] != 0.0f) {
+                d += a[f];
+                A[i][nb[f]] -= a[f];
+            }
+        A[i][i] += d;
+        A[i][m + i] = 1.0f;
+    }
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        const float piv = A[k][k];
+        float f = 0.0f;
+        if (i < m && i != k) f = A[i][k] / piv;
+        __syncthreads();
+        if (i < m && i != k)
+            for (int j = 0; j < 2 * m; j++) A[i][j] -= f * A[k][j];
+        __syncthreads();
+    }
+    if (i < m)
+        for (int j = 0; j < m; j++) ainv[i * m + j] = A[i][m + j] / A[i][i];
+}
+
 // the grid's edge faces carry no coupling (a neighbour past the edge does not exist)
 __global__ void k_zero_edges(int w, int h, float *__restrict__ ax, float *__restrict__ ay) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -507,31 +583,31 @@ __global__ void k_matvec64(L0 a, const double *__restrict__ v, double *__restric
 
 // ------------------------------------------------------------------------------------------------ host
 
-static bool is_coarsest(int w, int h) { return w * h <= 16 || w == 1 || h == 1; }
+#define PTX 64
+#define PTY 32
+
+static bool is_coarsest(int w, int h) { return w * h <= DENSE_MAX; }
 
 FldSolver::~FldSolver() {
-    for (auto &l : lv) {
-        hipFree(l.own);
-        hipFree(l.ax);
-        hipFree(l.ay);
-        hipFree(l.z);
-        hipFree(l.b);
-        hipFree(l.zp);
+    for (size_t l = 0; l < lv.size(); l++) {
+        if ((int)l < coarse_from) {
+            (void)hipFree(lv[l].own);
+            (void)hipFree(lv[l].ax);
+            (void)hipFree(lv[l].ay);
+        }
+        (void)hipFree(lv[l].z);
+        (void)hipFree(lv[l].b);
+        (void)hipFree(lv[l].zp);
     }
-    hipFree(gas);
-    hipFree(r64);
-    hipFree(r32);
-    hipFree(za);
-    hipFree(zb);
-    hipFree(p);
-    hipFree(q);
-    hipFree(sc);
-    hipFree(part);
-    hipFree(counter);
-    if (host_sc) hipHostFree(host_sc);
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
+    void *ps[] = {gas, r32, za, zb, p, q, pack, part_true, part_pq, part_rr, part_dots};
+    for (void *x : ps)
+        if (x) (void)hipFree(x);
+    if (host_part) (void)hipHostFree(host_part);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
 }
+
+static size_t level_floats(int w, int h) { return (size_t)w * h + (size_t)(w + 1) * h + (size_t)w * (h + 1); }
 
 int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_) {
     if (w <= 0 || h <= 0) return fail(SR_ERR_ARGUMENT, "an empty FLD grid");
@@ -540,55 +616,91 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_) {
     h0 = h;
     nu1 = nu1_;
     nu2 = nu2_;
+    // the hierarchy's shape: halve until a level holds at most DENSE_MAX cells (or the level cap)
     int cw = w, ch = h;
     const int maxl = levels > 0 ? levels : 64;
     while (true) {
         FldLevel l;
         l.w = cw;
         l.h = ch;
-        SR_HIP(hipMalloc(&l.own, sizeof(float) * cw * ch));
-        SR_HIP(hipMalloc(&l.ax, sizeof(float) * (cw + 1) * ch));
-        SR_HIP(hipMalloc(&l.ay, sizeof(float) * cw * (ch + 1)));
-        SR_HIP(hipMalloc(&l.z, sizeof(float) * cw * ch));
-        SR_HIP(hipMalloc(&l.b, sizeof(float) * cw * ch));
-        SR_HIP(hipMalloc(&l.zp, sizeof(float) * cw * ch));
         lv.push_back(l);
         if ((int)lv.size() >= maxl || is_coarsest(cw, ch)) break;
         cw = (cw + 1) / 2;
         ch = (ch + 1) / 2;
     }
-    // the first level handled by the single-block V-cycle: every level from there down fits COARSE_MAX cells and
-    // the LDS budget
+    // the first level handled by the single-block V-cycle: from there down every level fits COARSE_MAX cells and
+    // the packed arrays plus z and b fit the block's LDS
     coarse_from = (int)lv.size() - 1;
     while (coarse_from > 0) {
         const int c = coarse_from - 1;
-        if (lv[c].w * lv[c].h > COARSE_MAX) break;
-        size_t bytes = 0;
-        for (int l = c; l < (int)lv.size(); l++)
-            bytes += sizeof(float) * (3 * lv[l].w * lv[l].h + (lv[l].w + 1) * lv[l].h + lv[l].w * (lv[l].h + 1));
-        if (bytes > 60000 || (int)lv.size() - c > 16) break;
+        if (lv[c].w * lv[c].h > COARSE_MAX || (int)lv.size() - c > 16) break;
+        size_t floats = DENSE_MAX * DENSE_MAX;
+        for (int l = c; l < (int)lv.size(); l++) floats += level_floats(lv[l].w, lv[l].h) + 2 * lv[l].w * lv[l].h;
+        if (floats * sizeof(float) > COARSE_LDS) break;
         coarse_from = c;
     }
+    {
+        const FldLevel &c = lv.back();
+        size_t floats = DENSE_MAX * DENSE_MAX + level_floats(c.w, c.h) + 2 * c.w * c.h;
+        if (c.w * c.h > COARSE_MAX || floats * sizeof(float) > COARSE_LDS)
+            return fail(SR_ERR_ARGUMENT, "too few multigrid levels: the coarsest level must fit one block (" +
+                                             std::to_string(COARSE_MAX) + " cells)");
+    }
+    pack_floats = DENSE_MAX * DENSE_MAX;
+    for (int l = coarse_from; l < (int)lv.size(); l++) pack_floats += (int)level_floats(lv[l].w, lv[l].h);
+    SR_HIP(hipMalloc(&pack, sizeof(float) * pack_floats));
+    SR_HIP(hipMemset(pack, 0, sizeof(float) * pack_floats));
+    {
+        float *o = pack;
+        for (int l = coarse_from; l < (int)lv.size(); l++) {
+            FldLevel &L = lv[l];
+            L.own = o;
+            o += L.w * L.h;
+            L.ax = o;
+            o += (L.w + 1) * L.h;
+            L.ay = o;
+            o += L.w * (L.h + 1);
+        }
+        ainv = o;
+    }
+    for (int l = 0; l < (int)lv.size(); l++) {
+        FldLevel &L = lv[l];
+        if (l < coarse_from) {
+            SR_HIP(hipMalloc(&L.own, sizeof(float) * L.w * L.h));
+            SR_HIP(hipMalloc(&L.ax, sizeof(float) * (L.w + 1) * L.h));
+            SR_HIP(hipMalloc(&L.ay, sizeof(float) * L.w * (L.h + 1)));
+        }
+        SR_HIP(hipMalloc(&L.z, sizeof(float) * L.w * L.h));
+        SR_HIP(hipMalloc(&L.b, sizeof(float) * L.w * L.h));
+        SR_HIP(hipMalloc(&L.zp, sizeof(float) * L.w * L.h));
+    }
     const size_t n = (size_t)w * h;
+    nbe = (int)std::max<size_t>(1, std::min<size_t>(NB_MAX, (n + NT * UNROLL - 1) / (NT * UNROLL)));
+    // the level smoothers' tile width: 64 (less halo, 2 waves per SIMD) or 32 (4 waves per SIMD); SR_FLD_TILE picks
+    const char *env = std::getenv("SR_FLD_TILE");
+    tile_x = (env && std::atoi(env) == 64 && nu1 <= 2 && nu2 <= 2) ? PTX : 32;
+    const int tx = tile_x, ty = PTY;
+    ntiles0 = ((w + tx - 1) / tx) * ((h + ty - 1) / ty);
+    nbd_max = std::max(nbe, ntiles0);
     SR_HIP(hipMalloc(&gas, n));
     SR_HIP(hipMemset(gas, 1, n));
-    SR_HIP(hipMalloc(&r64, sizeof(double) * n));
     SR_HIP(hipMalloc(&r32, sizeof(float) * n));
     SR_HIP(hipMalloc(&za, sizeof(float) * n));
     SR_HIP(hipMalloc(&zb, sizeof(float) * n));
     SR_HIP(hipMalloc(&p, sizeof(float) * 2 * n));
     SR_HIP(hipMalloc(&q, sizeof(float) * n));
     SR_HIP(hipMemset(za, 0, sizeof(float) * n));
+    SR_HIP(hipMemset(zb, 0, sizeof(float) * n));
     SR_HIP(hipMemset(p, 0, sizeof(float) * 2 * n));
-    SR_HIP(hipMalloc(&sc, sizeof(double) * SC_N));
-    SR_HIP(hipMemset(sc, 0, sizeof(double) * SC_N));
-    const size_t nb = std::max<size_t>(4096, ((w + TILE - 1) / TILE) * ((h + TILE - 1) / TILE));
-    SR_HIP(hipMalloc(&part, sizeof(double) * 4 * nb));
-    SR_HIP(hipMalloc(&counter, sizeof(unsigned) * CNT_N));
-    SR_HIP(hipMemset(counter, 0, sizeof(unsigned) * CNT_N));
-    SR_HIP(hipHostMalloc(&host_sc, sizeof(double) * SC_N));
+    SR_HIP(hipMalloc(&part_true, sizeof(double) * 4 * nbe));
+    SR_HIP(hipMalloc(&part_pq, sizeof(double) * ((w + NT - 1) / NT) * h));
+    SR_HIP(hipMalloc(&part_rr, sizeof(double) * nbe));
+    SR_HIP(hipMalloc(&part_dots, sizeof(double) * 2 * 2 * nbd_max));
+    SR_HIP(hipMemset(part_dots, 0, sizeof(double) * 2 * 2 * nbd_max));
+    SR_HIP(hipHostMalloc(&host_part, sizeof(double) * 4 * nbe));
     SR_HIP(hipEventCreate(&ev0));
     SR_HIP(hipEventCreate(&ev1));
+    SR_HIP(hipFuncSetAttribute((const void *)k_coarse, hipFuncAttributeMaxDynamicSharedMemorySize, COARSE_LDS));
     return SR_OK;
 }
 
@@ -618,29 +730,49 @@ int FldSolver::rebuild(bool faces, hipStream_t s) {
             hipLaunchKernelGGL(k_coarsen_faces, dim3((nf + 255) / 256), dim3(256), 0, s, fl, c.ax, c.ay, c.w, c.h);
         }
     }
+    const FldLevel &lc = lv.back();
+    if (lc.w * lc.h <= DENSE_MAX)
+        hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(64), 0, s, lc.w, lc.h, lc.own, lc.ax, lc.ay, ainv);
     SR_HIP(hipGetLastError());
     return SR_OK;
 }
 
-static int grid1(size_t n) { return (int)std::min<size_t>(4096, (n + NT * 4 - 1) / (NT * 4)); }
+template <int TX, int TY>
+static void launch_pre(int nu, dim3 grid, hipStream_t s, Lv L, const float *b, float *z, float *bc, int wc) {
+    if (nu == 1)
+        hipLaunchKernelGGL((k_pre<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
+    else if (nu == 2)
+        hipLaunchKernelGGL((k_pre<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
+    else
+        hipLaunchKernelGGL((k_pre<32, TY, 3>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
+}
 
-// z = M^-1 r on level 0 (r, z level-0 vectors); with `dots`, the last kernel sums r.z and r.zold, forms beta
-int FldSolver::vcycle(const float *r, float *z, const float *zold, bool dots, hipStream_t s) {
+template <int TX, int TY>
+static void launch_post(int nu, dim3 grid, hipStream_t s, Lv L, const float *b, const float *zin, const float *zc,
+                        int wc, float *zout, const float *zold, double *part) {
+    if (nu == 1)
+        hipLaunchKernelGGL((k_post<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, zout, zold, part);
+    else if (nu == 2)
+        hipLaunchKernelGGL((k_post<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, zout, zold, part);
+    else
+        hipLaunchKernelGGL((k_post<32, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, zout, zold, part);
+}
+
+// z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
+int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s) {
     const int nl = (int)lv.size();
-    // device levels above the single-block part
+    const int tx = tile_x, ty = PTY;
     const float *bl = r;
     for (int l = 0; l < coarse_from; l++) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
         Lv L = {f.w, f.h, f.own, f.ax, f.ay};
-        dim3 grid((f.w + TILE - 1) / TILE, (f.h + TILE - 1) / TILE);
+        dim3 grid((f.w + tx - 1) / tx, (f.h + ty - 1) / ty);
         const bool t0 = timing && l == 0;
         if (t0) SR_HIP(hipEventRecord(ev0, s));
-        if (nu1 == 1)
-            hipLaunchKernelGGL(k_pre<1>, grid, dim3(NT), 0, s, L, bl, f.z, c.b, c.w);
-        else if (nu1 == 2)
-            hipLaunchKernelGGL(k_pre<2>, grid, dim3(NT), 0, s, L, bl, f.z, c.b, c.w);
+        if (tx == PTX)
+            launch_pre<PTX, PTY>(nu1, grid, s, L, bl, f.z, c.b, c.w);
         else
-            hipLaunchKernelGGL(k_pre<3>, grid, dim3(NT), 0, s, L, bl, f.z, c.b, c.w);
+            launch_pre<32, PTY>(nu1, grid, s, L, bl, f.z, c.b, c.w);
         if (t0) {
             SR_HIP(hipEventRecord(ev1, s));
             SR_HIP(hipEventSynchronize(ev1));
@@ -654,36 +786,50 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, bool dots, hi
     {
         CoarseSet C;
         C.n = nl - coarse_from;
-        size_t bytes = 0;
+        C.pack_floats = pack_floats;
+        int o = 0;
         for (int l = coarse_from; l < nl; l++) {
-            const int k = l - coarse_from;
-            C.w[k] = lv[l].w;
-            C.h[k] = lv[l].h;
-            C.own[k] = lv[l].own;
-            C.ax[k] = lv[l].ax;
-            C.ay[k] = lv[l].ay;
-            bytes += sizeof(float) * (3 * lv[l].w * lv[l].h + (lv[l].w + 1) * lv[l].h + lv[l].w * (lv[l].h + 1));
+            const int k = l - coarse_from, w = lv[l].w, h = lv[l].h;
+            C.w[k] = w;
+            C.h[k] = h;
+            C.offo[k] = o;
+            o += w * h;
+            C.offx[k] = o;
+            o += (w + 1) * h;
+            C.offy[k] = o;
+            o += w * (h + 1);
         }
+        C.offinv = o;
+        o = pack_floats;
+        for (int k = 0; k < C.n; k++) {
+            C.offz[k] = o;
+            o += C.w[k] * C.h[k];
+            C.offb[k] = o;
+            o += C.w[k] * C.h[k];
+        }
+        C.total = o;
+        const size_t floats = o;
         float *zc = coarse_from == 0 ? z : lv[coarse_from].zp;
-        hipLaunchKernelGGL(k_coarse, dim3(1), dim3(COARSE_MAX), bytes, s, C, bl, zc, nu1, nu2, 0);
+        hipLaunchKernelGGL(k_coarse, dim3(1), dim3(COARSE_NT), sizeof(float) * floats, s, C, (const float *)pack, bl,
+                           zc, nu1, nu2);
         if (coarse_from == 0 && dots) {
             const int n = w0 * h0;
-            hipLaunchKernelGGL(k_dots, dim3(grid1(n)), dim3(NT), 0, s, n, r, z, zold, sc, part, counter);
+            hipLaunchKernelGGL(k_dots, dim3(nbe), dim3(NT), 0, s, n, r, z, zold, dots);
+            *nbd = nbe;
         }
     }
     for (int l = coarse_from - 1; l >= 0; l--) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
         Lv L = {f.w, f.h, f.own, f.ax, f.ay};
-        dim3 grid((f.w + TILE - 1) / TILE, (f.h + TILE - 1) / TILE);
+        dim3 grid((f.w + tx - 1) / tx, (f.h + ty - 1) / ty);
         const float *b = l == 0 ? r : f.b;
         float *zo = l == 0 ? z : f.zp;
-        const int d = (l == 0 && dots) ? 1 : 0;
-        if (nu2 == 1)
-            hipLaunchKernelGGL(k_post<1>, grid, dim3(NT), 0, s, L, b, f.z, c.zp, c.w, zo, zold, sc, part, counter, d);
-        else if (nu2 == 2)
-            hipLaunchKernelGGL(k_post<2>, grid, dim3(NT), 0, s, L, b, f.z, c.zp, c.w, zo, zold, sc, part, counter, d);
+        double *pp = (l == 0 && dots) ? dots : nullptr;
+        if (tx == PTX)
+            launch_post<PTX, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
         else
-            hipLaunchKernelGGL(k_post<3>, grid, dim3(NT), 0, s, L, b, f.z, c.zp, c.w, zo, zold, sc, part, counter, d);
+            launch_post<32, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
+        if (l == 0 && dots) *nbd = grid.x * grid.y;
     }
     SR_HIP(hipGetLastError());
     return SR_OK;
@@ -691,14 +837,13 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, bool dots, hi
 
 int FldSolver::apply_precond(const float *r, float *z, int precond, hipStream_t s) {
     L0 a = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
-    const size_t n = (size_t)w0 * h0;
     if (precond == 0) {
-        hipLaunchKernelGGL(k_jacobi, dim3(grid1(n)), dim3(NT), 0, s, a, r, z, (const float *)nullptr, sc, part,
-                           counter, 0);
+        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, r, z, (const float *)nullptr, (double *)nullptr);
         SR_HIP(hipGetLastError());
         return SR_OK;
     }
-    return vcycle(r, z, nullptr, false, s);
+    int nbd = 0;
+    return vcycle(r, z, nullptr, nullptr, &nbd, s);
 }
 
 int FldSolver::matvec64(const double *v, double *out, hipStream_t s) {
@@ -709,29 +854,45 @@ int FldSolver::matvec64(const double *v, double *out, hipStream_t s) {
     return SR_OK;
 }
 
+int FldSolver::read_partials(const double *part, int k, int nb, double *out, hipStream_t s) {
+    SR_HIP(hipMemcpyAsync(host_part, part, sizeof(double) * k * nb, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < k; j++) {
+        double t = 0.0;
+        for (int i = 0; i < nb; i++) t += host_part[j * nb + i];
+        out[j] = t;
+    }
+    return SR_OK;
+}
+
 int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int precond, const RefStop *ref,
                      FldStats *st, hipStream_t s) {
     L0 a = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
     const size_t n = (size_t)w0 * h0;
-    const int g = grid1(n);
     const bool mg = precond != 0 && lv.size() > 1;
     *st = FldStats();
     const double inner_rtol = 1e-4;
     int it = 0;
+    double *dots[2] = {part_dots, part_dots + 2 * nbd_max};
+    // rows per matvec block: about 2048 blocks, 1 to 16 rows each
+    const int gxm = (w0 + NT - 1) / NT;
+    const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + 2047) / 2048));
+    const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
+    const int nmv = (int)(mv_grid.x * mv_grid.y);
     for (int outer = 0; outer < 50; outer++) {
-        hipLaunchKernelGGL(k_true_residual, dim3(g), dim3(NT), 0, s, a, b, x, r32, ref ? ref->heat : nullptr,
-                           ref ? ref->slope : nullptr, sc, part, counter);
-        SR_HIP(hipMemcpyAsync(host_sc, sc, sizeof(double) * SC_N, hipMemcpyDeviceToHost, s));
-        SR_HIP(hipStreamSynchronize(s));
-        const double rr = host_sc[SC_R64], bb = host_sc[SC_BB] > 0 ? host_sc[SC_BB] : 1.0;
+        hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, x, r32, ref ? ref->heat : nullptr,
+                           ref ? ref->slope : nullptr, part_true);
+        double tr[4];
+        SR_TRY(read_partials(part_true, 4, nbe, tr, s));
+        const double rr = tr[0], bb = tr[1] > 0 ? tr[1] : 1.0;
         st->rel = std::sqrt(rr / bb);
         st->refinements = outer;
-        if (!std::isfinite(rr) || (ref && !(std::isfinite(host_sc[SC_UMEAS]) && std::isfinite(host_sc[SC_HMEAS])))) {
+        if (!std::isfinite(rr) || (ref && !(std::isfinite(tr[2]) && std::isfinite(tr[3])))) {
             st->finite = false;
             break;
         }
-        const bool done = ref ? (host_sc[SC_UMEAS] <= ref->u_bound && host_sc[SC_HMEAS] <= ref->t_bound)
-                              : (std::sqrt(rr) <= tol * std::sqrt(bb));
+        const bool done =
+            ref ? (tr[2] <= ref->u_bound && tr[3] <= ref->t_bound) : (std::sqrt(rr) <= tol * std::sqrt(bb));
         if (done) {
             st->converged = true;
             break;
@@ -743,29 +904,46 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         float *zc = za, *zo = zb;
         float *pold = p, *pnew = p + n;
         SR_HIP(hipMemsetAsync(zo, 0, sizeof(float) * n, s));
-        // z = M r, rz = r.z (restart: beta = 0)
+        int k = 0, nbd = nbe, nbp = nbe;
+        // z = M r, partials of r.z (restart: beta = 0)
         if (mg)
-            SR_TRY(vcycle(r32, zc, zo, true, s));
+            SR_TRY(vcycle(r32, zc, zo, dots[k & 1], &nbd, s));
         else
-            hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(NT), 0, s, a, r32, zc, zo, sc, part, counter, 1);
+            hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, r32, zc, zo, dots[k & 1]);
+        int last_check = it, interval = 1;
+        double rr_last = rr;
         while (it < max_iter) {
-            hipLaunchKernelGGL(k_matvec_p, dim3(g), dim3(NT), 0, s, a, zc, pold, pnew, q, sc, part, counter);
-            hipLaunchKernelGGL(k_update, dim3(g), dim3(NT), 0, s, (int)n, pnew, q, x, r32, sc, part, counter);
+            hipLaunchKernelGGL(k_matvec_p, mv_grid, dim3(NT), 0, s, a, zc, pold, pnew, q, dots[k & 1],
+                               dots[(k + 1) & 1], nbd, nbp, k == 0 ? 1 : 0, mv_rows, part_pq);
+            hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, x, r32, dots[k & 1], nbd,
+                               part_pq, nmv, part_rr);
             it++;
-            SR_HIP(hipMemcpyAsync(host_sc, sc, sizeof(double) * SC_N, hipMemcpyDeviceToHost, s));
-            SR_HIP(hipStreamSynchronize(s));
-            const double rri = host_sc[SC_RR];
-            if (!std::isfinite(rri) || !std::isfinite(host_sc[SC_PQ])) {
-                st->finite = false;
-                break;
+            if (it - last_check >= interval || it >= max_iter) {
+                double rri;
+                SR_TRY(read_partials(part_rr, 1, nbe, &rri, s));
+                if (!std::isfinite(rri)) {
+                    st->finite = false;
+                    break;
+                }
+                if (rri <= stop2) break;
+                // iterations left at the rate seen since the last check: check again a little before then
+                const double rate =
+                    std::log(std::max(rri, 1e-300) / std::max(rr_last, 1e-300)) / (double)(it - last_check);
+                const double left = rate < 0.0 ? std::log(stop2 / rri) / rate : 1.0;
+                interval = std::max(1, std::min(8, (int)(0.7 * left)));
+                last_check = it;
+                rr_last = rri;
             }
-            if (rri <= stop2) break;
             std::swap(zc, zo);
             std::swap(pold, pnew);
+            k++;
+            nbp = nbd;
             if (mg)
-                SR_TRY(vcycle(r32, zc, zo, true, s));
-            else
-                hipLaunchKernelGGL(k_jacobi, dim3(g), dim3(NT), 0, s, a, r32, zc, zo, sc, part, counter, 1);
+                SR_TRY(vcycle(r32, zc, zo, dots[k & 1], &nbd, s));
+            else {
+                hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, r32, zc, zo, dots[k & 1]);
+                nbd = nbe;
+            }
         }
         if (!st->finite) break;
     }
@@ -774,12 +952,13 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     return SR_OK;
 }
 
-// HBM bytes one inner iteration moves at level 0 (f32 unless noted), the roofline's numerator: matvec (z, p, q,
-// own, ax, ay: 24), update (x f64 rw 16, p, r rw 8, q: 32), the V-cycle's level-0 kernels (pre: b, own, ax, ay, z
-// out, coarse b: 21; post: b, own, ax, ay, z in, z out, coarse z, z_old: 33), times 4/3 for the coarser levels.
+// HBM bytes one inner iteration moves at level 0 by its algorithm (f32 unless noted), the roofline's numerator:
+// matvec (z, p in, p out, q out, own, ax, ay: 28), update (x f64 rw 16, p, q, r rw 8: 32), the V-cycle's level-0
+// kernels (pre: b, own, ax, ay in, z out, coarse b out 1: 21; post: b, own, ax, ay, z in, coarse z 1, z out, z_old:
+// 33), the coarser levels adding a third of level 0's smoothing.
 double FldSolver::bytes_per_iteration() const {
     const double n = (double)w0 * h0;
-    return n * (24.0 + 32.0 + (21.0 + 33.0) * 4.0 / 3.0);
+    return n * (28.0 + 32.0 + (21.0 + 33.0) * 4.0 / 3.0);
 }
 
 } // namespace sr

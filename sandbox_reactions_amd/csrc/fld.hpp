@@ -57,17 +57,21 @@ class FldSolver {
 
   private:
     int w0 = 0, h0 = 0, nu1 = 2, nu2 = 2, coarse_from = 0;
+    int nbe = 1, ntiles0 = 1, nbd_max = 1; // blocks of the elementwise kernels, level-0 tiles
+    int tile_x = 32;
     std::vector<FldLevel> lv;
     uint8_t *gas = nullptr;
     // level-0 vectors
-    double *r64 = nullptr;
     float *r32 = nullptr, *za = nullptr, *zb = nullptr, *p = nullptr, *q = nullptr;
-    // device scalars, block partials and the last-block counters
-    double *sc = nullptr, *part = nullptr;
-    unsigned *counter = nullptr;
-    double *host_sc = nullptr; // pinned
+    float *pack = nullptr; // the single-block levels' own/ax/ay back to back, then the coarsest's inverse
+    float *ainv = nullptr;
+    int pack_floats = 0;
+    // per-block partial sums (summed by the kernel that consumes them, or on the host)
+    double *part_true = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_dots = nullptr;
+    double *host_part = nullptr; // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int vcycle(const float *r, float *z, const float *zold, bool dots, hipStream_t s);
+    int vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s);
+    int read_partials(const double *part, int k, int nb, double *out, hipStream_t s);
 };
 
 } // namespace sr

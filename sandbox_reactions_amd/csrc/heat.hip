@@ -18,8 +18,6 @@ namespace sr {
 __constant__ float h_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f / 20.0f,
                                 13.0f / 24.0f, 15.0f / 28.0f, 17.0f / 32.0f, 27.0f / 56.0f, 1.0f};
 
-enum { HS_UB = 0, HS_TB, HS_HEAT0, HS_MOVED, HS_BAD, HS_RAD, HS_GAIN, HS_HELD, HS_DELTA, HS_N = 16 };
-enum { HC_SYS = 0, HC_UPD, HC_COMMIT, HC_N = 4 };
 
 __device__ __forceinline__ double u_of(const HeatParams &P, double t) { return P.cd ? t : P.a_r * t * t * t * t; }
 __device__ __forceinline__ double t_of(const HeatParams &P, double u) { return P.cd ? u : sqrt(sqrt(u / P.a_r)); }
@@ -30,7 +28,7 @@ __device__ __forceinline__ double dt_du(const HeatParams &P, double t) {
 // the sides of every cell (heat.rs:202-219): gas flag, T with the floor, kappa, 1/mu, heat capacity C
 __global__ void k_heat_cells(HeatParams P, const double *__restrict__ sigma, const float *__restrict__ mx,
                              const float *__restrict__ my, const float *__restrict__ e, const float *__restrict__ x,
-                             HeatPlanes H, double *__restrict__ sums, double *part, unsigned *counter) {
+                             HeatPlanes H, double *part) {
     const int n = P.w * P.h;
     double v[1] = {0.0};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
@@ -69,8 +67,7 @@ __global__ void k_heat_cells(HeatParams P, const double *__restrict__ sigma, con
         H.sig[i] = (float)s;
         if (gas) v[0] += (double)H.cap[i] * t;
     }
-    double out[1];
-    if (reduce_finish<1>(v, part, gridDim.x, counter, out) && threadIdx.x == 0) sums[HS_HEAT0] = out[0];
+    block_partials<1>(v, part, gridDim.x);
 }
 
 struct SideV {
@@ -157,8 +154,7 @@ __global__ void k_heat_kvac(HeatParams P, HeatPlanes H) {
 // a Newton pass's system at the iterate t (heat.rs:474-488): own = c + k_vac, rhs = c u - (C/dt)(t - t0) + k_vac u_vac,
 // x = u; the two stopping bounds eps^2 sum c u^2 and eps^2 sum (C/dt) t^2
 __global__ void k_heat_system(HeatParams P, HeatPlanes H, const double *__restrict__ scal, float *__restrict__ own,
-                              double *__restrict__ rhs, double *__restrict__ xs, double *sums, double eps2,
-                              double *part, unsigned *counter) {
+                              double *__restrict__ rhs, double *__restrict__ xs, double *part) {
     const int n = P.w * P.h;
     const double dt = scal[0];
     const double uvac = u_of(P, P.t_floor);
@@ -182,17 +178,13 @@ __global__ void k_heat_system(HeatParams P, HeatPlanes H, const double *__restri
         v[0] += ci * u * u;
         v[1] += heat * t * t;
     }
-    double out[2];
-    if (reduce_finish<2>(v, part, gridDim.x, counter + HC_SYS, out) && threadIdx.x == 0) {
-        sums[HS_UB] = eps2 * out[0];
-        sums[HS_TB] = eps2 * out[1];
-    }
+    block_partials<2>(v, part, gridDim.x);
 }
 
 // the pass's update (heat.rs:502-515): t = clamp(T(max(x, u_vac)), t/2, 2t); sums C |dT| and counts the cells that
 // moved more than newton_cell of their T
 __global__ void k_heat_update(HeatParams P, HeatPlanes H, const double *__restrict__ xs, double newton_cell,
-                              double *sums, double *part, unsigned *counter) {
+                              double *part) {
     const int n = P.w * P.h;
     const double uvac = u_of(P, P.t_floor);
     double v[2] = {0.0, 0.0};
@@ -206,17 +198,12 @@ __global__ void k_heat_update(HeatParams P, HeatPlanes H, const double *__restri
         if (!(d <= newton_cell * t)) v[1] += 1.0;
         H.t[i] = nx;
     }
-    double out[2];
-    if (reduce_finish<2>(v, part, gridDim.x, counter + HC_UPD, out) && threadIdx.x == 0) {
-        sums[HS_MOVED] = out[0];
-        sums[HS_BAD] = out[1];
-    }
+    block_partials<2>(v, part, gridDim.x);
 }
 
 // the commit's sums (heat.rs:522-530): radiated = sum dt k_vac (u - u_vac), gained = sum C (t - t0), held = sum C t;
 // then delta = -(gained + radiated) / held
-__global__ void k_heat_commit_sums(HeatParams P, HeatPlanes H, const double *__restrict__ scal, double *sums,
-                                   double *part, unsigned *counter) {
+__global__ void k_heat_commit_sums(HeatParams P, HeatPlanes H, const double *__restrict__ scal, double *part) {
     const int n = P.w * P.h;
     const double dt = scal[0], uvac = u_of(P, P.t_floor);
     double v[3] = {0.0, 0.0, 0.0};
@@ -227,13 +214,7 @@ __global__ void k_heat_commit_sums(HeatParams P, HeatPlanes H, const double *__r
         v[1] += c * (t - (double)H.t0[i]);
         v[2] += c * t;
     }
-    double out[3];
-    if (reduce_finish<3>(v, part, gridDim.x, counter + HC_COMMIT, out) && threadIdx.x == 0) {
-        sums[HS_RAD] = out[0];
-        sums[HS_GAIN] = out[1];
-        sums[HS_HELD] = out[2];
-        sums[HS_DELTA] = out[2] != 0.0 ? -(out[1] + out[0]) / out[2] : 0.0;
-    }
+    block_partials<3>(v, part, gridDim.x);
 }
 
 // F = -k dx (v_b - v_a) at a face, v the iterate's u on gas cells, u_vac elsewhere (heat.rs:340-362)
@@ -248,8 +229,8 @@ __device__ inline double face_flux(const HeatParams &P, const HeatPlanes &H, flo
 // E += C ((1 + delta) t - t0) on the gas, and the cell-centred F the next P4 reads (heat.rs:89-100, :531-534); with
 // `limited`, the ceiling's commit instead: every gas cell's outflow scaled to the heat it holds above its floor
 // (heat.rs:544-592), applied by k_heat_limited.
-__global__ void k_heat_commit(HeatParams P, HeatPlanes H, const double *__restrict__ scal, const double *sums,
-                              float *__restrict__ e, float *__restrict__ fx, float *__restrict__ fy, int pinned) {
+__global__ void k_heat_commit(HeatParams P, HeatPlanes H, double delta, float *__restrict__ e, float *__restrict__ fx,
+                              float *__restrict__ fy, int pinned) {
     const int w = P.w, h = P.h;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w * h) return;
@@ -261,7 +242,6 @@ __global__ void k_heat_commit(HeatParams P, HeatPlanes H, const double *__restri
     fx[i] = (float)(0.5 * (fl + fr));
     fy[i] = (float)(0.5 * (fu + fd));
     if (pinned || !H.gas[i]) return;
-    const double delta = sums[HS_DELTA];
     e[i] = (float)((double)e[i] + (double)H.cap[i] * ((1.0 + delta) * (double)H.t[i] - (double)H.t0[i]));
 }
 
@@ -292,8 +272,7 @@ __device__ inline double limited_flux(const HeatParams &P, const HeatPlanes &H, 
 }
 
 __global__ void k_heat_limited(HeatParams P, HeatPlanes H, const double *__restrict__ scal, float *__restrict__ e,
-                               float *__restrict__ fx, float *__restrict__ fy, double *sums, double *part,
-                               unsigned *counter) {
+                               float *__restrict__ fx, float *__restrict__ fy, double *part) {
     const int w = P.w, h = P.h, n = w * h;
     const double dt = scal[0];
     double v[1] = {0.0};
@@ -315,8 +294,7 @@ __global__ void k_heat_limited(HeatParams P, HeatPlanes H, const double *__restr
         v[0] += em;
         e[i] = (float)((double)e[i] - dt * P.inv_dx * (o[0] + o[1] + o[2] + o[3]));
     }
-    double out[1];
-    if (reduce_finish<1>(v, part, gridDim.x, counter + HC_COMMIT, out) && threadIdx.x == 0) sums[HS_RAD] = out[0];
+    block_partials<1>(v, part, gridDim.x);
 }
 
 // ------------------------------------------------------------------------------------------------------------ host
@@ -325,7 +303,7 @@ static int grid_cells(size_t n) { return (int)std::min<size_t>(4096, (n + NT * 4
 
 HeatPass::~HeatPass() {
     void *ps[] = {H.gas, H.t0, H.t, H.kappa, H.imu, H.cap, H.sig, H.kx, H.ky, H.kvac, H.heat, H.slope, H.scale,
-                  rhs, xs, sums, part, counter};
+                  rhs, xs, part};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (host) (void)hipHostFree(host);
@@ -335,6 +313,7 @@ int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
     P = p;
     hs = hs_;
     const size_t n = (size_t)P.w * P.h, nx = (size_t)(P.w + 1) * P.h, ny = (size_t)P.w * (P.h + 1);
+    nb = grid_cells(n);
     SR_HIP(hipMalloc(&H.gas, n));
     float **fp[] = {&H.kappa, &H.imu, &H.cap, &H.sig, &H.kvac, &H.heat, &H.slope, &H.scale};
     for (float **q : fp) SR_HIP(hipMalloc(q, sizeof(float) * n));
@@ -344,77 +323,73 @@ int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
     SR_HIP(hipMalloc(&H.ky, sizeof(float) * ny));
     SR_HIP(hipMalloc(&rhs, sizeof(double) * n));
     SR_HIP(hipMalloc(&xs, sizeof(double) * n));
-    SR_HIP(hipMalloc(&sums, sizeof(double) * HS_N));
-    SR_HIP(hipMemset(sums, 0, sizeof(double) * HS_N));
-    SR_HIP(hipMalloc(&part, sizeof(double) * 4 * 4096));
-    SR_HIP(hipMalloc(&counter, sizeof(unsigned) * HC_N));
-    SR_HIP(hipMemset(counter, 0, sizeof(unsigned) * HC_N));
-    SR_HIP(hipHostMalloc(&host, sizeof(double) * HS_N));
+    SR_HIP(hipMalloc(&part, sizeof(double) * 4 * nb));
+    SR_HIP(hipHostMalloc(&host, sizeof(double) * 4 * nb));
     return solver.init(P.w, P.h, hs.levels, hs.nu1, hs.nu2);
 }
 
-int HeatPass::read_sums(hipStream_t s) {
-    SR_HIP(hipMemcpyAsync(host, sums, sizeof(double) * HS_N, hipMemcpyDeviceToHost, s));
+// the k sums of the last kernel's partials, in block order
+int HeatPass::read_sums(int k, double *out, hipStream_t s) {
+    SR_HIP(hipMemcpyAsync(host, part, sizeof(double) * k * nb, hipMemcpyDeviceToHost, s));
     SR_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < k; j++) {
+        double t = 0.0;
+        for (int i = 0; i < nb; i++) t += host[j * nb + i];
+        out[j] = t;
+    }
     return SR_OK;
 }
 
 int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, float *fx, float *fy, int precond,
                    hipStream_t s) {
     const size_t n = (size_t)P.w * P.h;
-    const int g = grid_cells(n);
+    const int g = nb;
     report = sr_heat_report{};
     report.dt = dt_host;
     report.converged = 1;
-    hipLaunchKernelGGL(k_heat_cells, dim3(g), dim3(NT), 0, s, P, st.sigma, st.mx, st.my, st.e, st.x, H, sums, part,
-                       counter + 3);
+    hipLaunchKernelGGL(k_heat_cells, dim3(g), dim3(NT), 0, s, P, st.sigma, st.mx, st.my, st.e, st.x, H, part);
+    double heat0 = 0.0;
+    SR_TRY(read_sums(1, &heat0, s));
     const int nf = std::max((P.w + 1) * P.h, P.w * (P.h + 1));
     hipLaunchKernelGGL(k_heat_law, dim3((nf + 255) / 256), dim3(256), 0, s, P, H, solver.ax0(), solver.ay0());
     hipLaunchKernelGGL(k_heat_kvac, dim3((n + 255) / 256), dim3(256), 0, s, P, H);
     SR_HIP(hipGetLastError());
-    if (P.pinned || !(dt_host > 0.0)) {
-        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal, sums, st.e, fx, fy, 1);
+    if (P.pinned || !(dt_host > 0.0) || !(heat0 > 0.0)) {
+        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, 0.0, st.e, fx, fy, 1);
         SR_HIP(hipGetLastError());
         return SR_OK;
     }
     SR_HIP(hipMemcpyAsync(solver.gas0(), H.gas, n, hipMemcpyDeviceToDevice, s));
-    SR_TRY(read_sums(s));
-    const double heat0 = host[HS_HEAT0];
-    bool any_gas = heat0 > 0.0;
-    if (!any_gas) {
-        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal, sums, st.e, fx, fy, 1);
-        SR_HIP(hipGetLastError());
-        return SR_OK;
-    }
     const double eps = hs.stop == SR_STOP_REFERENCE ? hs.solve_tol : 0.0;
     int passes = 0, rounds = 0;
     bool converged = false, faces = true;
     double last_rel = 0.0;
     while (passes < hs.max_passes && rounds < hs.max_rounds) {
         passes++;
-        hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, sums, eps * eps,
-                           part, counter);
+        hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
+        double b2[2];
+        SR_TRY(read_sums(2, b2, s));
         SR_TRY(solver.rebuild(faces, s));
         faces = false;
-        SR_TRY(read_sums(s));
-        if (!(std::isfinite(host[HS_UB]) && std::isfinite(host[HS_TB]))) {
-            passes--;
-            break;
-        }
         RefStop ref;
         ref.heat = H.heat;
         ref.slope = H.slope;
-        ref.u_bound = host[HS_UB];
-        ref.t_bound = host[HS_TB];
+        ref.u_bound = eps * eps * b2[0];
+        ref.t_bound = eps * eps * b2[1];
+        if (!(std::isfinite(ref.u_bound) && std::isfinite(ref.t_bound))) {
+            passes--;
+            break;
+        }
         FldStats fs;
         SR_TRY(solver.solve(rhs, xs, hs.solve_tol, hs.max_rounds - rounds, precond,
                             hs.stop == SR_STOP_REFERENCE ? &ref : nullptr, &fs, s));
         rounds += fs.iterations;
         last_rel = fs.rel;
         if (!fs.finite) break;
-        hipLaunchKernelGGL(k_heat_update, dim3(g), dim3(NT), 0, s, P, H, xs, hs.newton_cell, sums, part, counter);
-        SR_TRY(read_sums(s));
-        if (host[HS_MOVED] <= hs.newton_heat * heat0 && host[HS_BAD] == 0.0) {
+        hipLaunchKernelGGL(k_heat_update, dim3(g), dim3(NT), 0, s, P, H, xs, hs.newton_cell, part);
+        double mv[2];
+        SR_TRY(read_sums(2, mv, s));
+        if (mv[0] <= hs.newton_heat * heat0 && mv[1] == 0.0) {
             converged = true;
             break;
         }
@@ -423,20 +398,23 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     report.rounds = rounds;
     report.final_residual = last_rel;
     if (converged) {
-        hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, sums, part, counter);
-        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal, sums, st.e, fx, fy, 0);
+        hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, part);
+        double c3[3];
+        SR_TRY(read_sums(3, c3, s));
+        const double delta = c3[2] != 0.0 ? -(c3[1] + c3[0]) / c3[2] : 0.0;
+        hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, delta, st.e, fx, fy, 0);
         SR_HIP(hipGetLastError());
-        SR_TRY(read_sums(s));
-        report.delta = host[HS_DELTA];
-        report.radiated = host[HS_RAD];
+        report.delta = delta;
+        report.radiated = c3[0];
         report.converged = 1;
     } else {
         hipLaunchKernelGGL(k_heat_scale, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal);
-        hipLaunchKernelGGL(k_heat_limited, dim3(g), dim3(NT), 0, s, P, H, scal, st.e, fx, fy, sums, part, counter);
+        hipLaunchKernelGGL(k_heat_limited, dim3(g), dim3(NT), 0, s, P, H, scal, st.e, fx, fy, part);
         SR_HIP(hipGetLastError());
-        SR_TRY(read_sums(s));
+        double r1;
+        SR_TRY(read_sums(1, &r1, s));
         report.delta = 0.0;
-        report.radiated = host[HS_RAD];
+        report.radiated = r1;
         report.converged = 0;
     }
     return SR_OK;

@@ -52,11 +52,11 @@ class HeatPass {
     sr_heat_settings hs{};
 
   private:
-    int read_sums(hipStream_t s);
+    int read_sums(int k, double *out, hipStream_t s);
     HeatPlanes H;
-    double *rhs = nullptr, *xs = nullptr, *sums = nullptr, *part = nullptr;
-    unsigned *counter = nullptr;
+    double *rhs = nullptr, *xs = nullptr, *part = nullptr;
     double *host = nullptr;
+    int nb = 1;
 };
 
 } // namespace sr

@@ -65,8 +65,7 @@ __global__ void k_p1(double *scal) {
 // P8 floors (floors.rs:124-175, no hold) then P9's dt (hydro.rs:618-634) and the non-finite guard: the largest
 // wave-speed rate and |g| by a max reduction, the first non-finite (cell, channel) as the max of -key.
 __global__ void __launch_bounds__(NT) k_floors_dt(FloorParams P, HydroState st, const float *__restrict__ gx,
-                                                  const float *__restrict__ gy, int apply, double *scal, double *part,
-                                                  unsigned *counter) {
+                                                  const float *__restrict__ gy, int apply, double *part) {
     const int n = P.w * P.h;
     double v[3] = {0.0, 0.0, -HUGE_VAL};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
@@ -140,19 +139,34 @@ __global__ void __launch_bounds__(NT) k_floors_dt(FloorParams P, HydroState st, 
         }
         if (!isfinite(rate)) v[0] = HUGE_VAL;
     }
-    double out[3];
-    if (reduce_max_finish<3>(v, part, gridDim.x, counter, out) && threadIdx.x == 0) {
-        scal[SCAL_RATE] = out[0];
-        scal[SCAL_GMAX] = out[1];
-        scal[SCAL_NF] = out[2];
-        double dt = P.cfl * P.dx / out[0];
-        if (out[1] > 0.0) dt = fmin(dt, P.eta_g * sqrt(P.dx / out[1]));
+    block_partials<3, 1>(v, part, gridDim.x);
+}
+
+// P9's dt_{n+1} from the maxima: min(C dx / rate, eta_g sqrt(dx / |g|max)) (hydro.rs:618-634)
+__global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, int nb, double *scal) {
+    __shared__ double sh[3][4];
+    double v[3] = {0.0, 0.0, -HUGE_VAL};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        for (int k = 0; k < 3; k++) v[k] = fmax(v[k], part[(size_t)k * nb + i]);
+    for (int k = 0; k < 3; k++)
+        for (int o = 32; o > 0; o >>= 1) v[k] = fmax(v[k], __shfl_xor(v[k], o, 64));
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; k++) sh[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[3];
+        for (int k = 0; k < 3; k++) o[k] = fmax(fmax(sh[k][0], sh[k][1]), fmax(sh[k][2], sh[k][3]));
+        scal[SCAL_RATE] = o[0];
+        scal[SCAL_GMAX] = o[1];
+        scal[SCAL_NF] = o[2];
+        double dt = P.cfl * P.dx / o[0];
+        if (o[1] > 0.0) dt = fmin(dt, P.eta_g * sqrt(P.dx / o[1]));
         scal[SCAL_DT_NEXT] = dt;
     }
 }
 
 // FP64 totals: Sigma, Sigma u, Sigma v, E, Sigma X_s (hydro.rs:210-224), and the escaped sums per component
-__global__ void __launch_bounds__(NT) k_totals(HydroState st, int n, double *out, double *part, unsigned *counter) {
+__global__ void __launch_bounds__(NT) k_totals(HydroState st, int n, double *part) {
     double v[14];
     for (int k = 0; k < 14; k++) v[k] = 0.0;
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
@@ -163,9 +177,7 @@ __global__ void __launch_bounds__(NT) k_totals(HydroState st, int n, double *out
         v[3] += st.e[i];
         for (int k = 0; k < NS; k++) v[4 + k] += s * (double)st.x[(size_t)k * n + i];
     }
-    double o[14];
-    if (reduce_finish<14>(v, part, gridDim.x, counter, o) && threadIdx.x == 0)
-        for (int k = 0; k < 14; k++) out[k] = o[k];
+    block_partials<14>(v, part, gridDim.x);
 }
 
 __global__ void k_escaped_sum(const double *__restrict__ esc, int nf, double *out) {
@@ -388,9 +400,10 @@ static int p2(sr_world *wd) {
 
 static int p89(sr_world *wd, int apply) {
     HydroState &st = wd->st[wd->cur];
-    hipLaunchKernelGGL(k_floors_dt, dim3(grid_n(wd->n)), dim3(NT), 0, wd->s, wd->fp, st,
-                       wd->sw.gravity ? wd->gx : nullptr, wd->sw.gravity ? wd->gy : nullptr, apply, wd->scal,
-                       wd->rpart, wd->counters + 1);
+    const int nb = grid_n(wd->n);
+    hipLaunchKernelGGL(k_floors_dt, dim3(nb), dim3(NT), 0, wd->s, wd->fp, st, wd->sw.gravity ? wd->gx : nullptr,
+                       wd->sw.gravity ? wd->gy : nullptr, apply, wd->rpart);
+    hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(256), 0, wd->s, wd->fp, wd->rpart, nb, wd->scal);
     SR_HIP(hipGetLastError());
     return SR_OK;
 }
@@ -561,14 +574,21 @@ extern "C" int sr_read_field(sr_world *wd, int field, float *out, size_t n) {
 
 extern "C" int sr_read_totals(sr_world *wd, double *totals, double *escaped) {
     if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
-    hipLaunchKernelGGL(k_totals, dim3(grid_n(wd->n)), dim3(NT), 0, wd->s, wd->st[wd->cur], (int)wd->n, wd->totals,
-                       wd->rpart, wd->counters + 2);
+    const int nb = grid_n(wd->n);
+    hipLaunchKernelGGL(k_totals, dim3(nb), dim3(NT), 0, wd->s, wd->st[wd->cur], (int)wd->n, wd->rpart);
     hipLaunchKernelGGL(k_escaped_sum, dim3(14), dim3(256), 0, wd->s, wd->escaped, 2 * (wd->w + wd->h),
                        wd->totals + 16);
     SR_HIP(hipGetLastError());
+    std::vector<double> part((size_t)14 * nb);
     double h[32];
+    SR_HIP(hipMemcpyAsync(part.data(), wd->rpart, sizeof(double) * 14 * nb, hipMemcpyDeviceToHost, wd->s));
     SR_HIP(hipMemcpyAsync(h, wd->totals, sizeof(double) * 32, hipMemcpyDeviceToHost, wd->s));
     SR_HIP(hipStreamSynchronize(wd->s));
+    for (int k = 0; k < 14; k++) {
+        double t = 0.0;
+        for (int i = 0; i < nb; i++) t += part[(size_t)k * nb + i];
+        h[k] = t;
+    }
     if (totals) std::memcpy(totals, h, sizeof(double) * 14);
     if (escaped) std::memcpy(escaped, h + 16, sizeof(double) * 14);
     return SR_OK;

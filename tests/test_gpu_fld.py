@@ -99,3 +99,19 @@ def test_gas_mask_and_jacobi(gpu_lib):
 def test_solver_refusals(gpu_lib):
     with pytest.raises(gpu_lib.SrError):
         gpu_lib.Fld(64, 64, nu1=0)
+
+
+def test_level_caps(gpu_lib):
+    """A capped hierarchy (the 7-level V-cycle of the stiff 2048^2 workload, here 5 levels at 256^2: a 16x16
+    coarsest solved by sweeps) still matches the model; too few levels to fit one block is refused."""
+    import torch
+    n = 256
+    own, kx, ky, b = system(n)
+    f = upload(gpu_lib, n, own, kx, ky, levels=5)
+    assert f.levels == 5
+    x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    it, rel = f.solve(torch.tensor(b, device="cuda"), x, tol=1e-8)
+    xo, _, _ = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-10, precond="mg", levels=5)
+    assert rel <= 1e-8 and np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-7
+    with pytest.raises(gpu_lib.SrError):
+        gpu_lib.Fld(2048, 2048, levels=2)
