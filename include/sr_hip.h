@@ -57,7 +57,7 @@ typedef struct {
     int pin_temperature;         /* heat test flags (reference/heat.rs:62-74) */
     double constant_diffusivity; /* > 0 switches it on */
     int flux_limiter;
-    int max_padded;     /* FFT cap per padded axis; the reference's is 2048 (step/fft.rs:26-63), this build's 8192 */
+    int max_padded;     /* FFT cap per padded axis; the reference's is 2048 (step/fft.rs:26-63), this build's default 4096 */
 } sr_physics;
 
 typedef struct {

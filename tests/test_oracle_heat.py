@@ -1,0 +1,196 @@
+"""The oracle's FLD pass (oracle/heat.c restating crates/sr-physics/src/reference/heat.rs; oracle/mg.c, the f64 model
+of the multigrid preconditioner) pinned without the reference running, by what the mathematics fixes and by the checks
+of the reference's heat tests (crates/sr-engine/tests/gpu/heat.rs:343-1827):
+
+- the face-form matvec against a matrix assembled independently here (scipy.sparse), and both solvers (Jacobi-PCG,
+  MG-PCG) against a sparse direct solve;
+- the V-cycle: exact on a grid small enough for the coarsest dense solve; symmetric and positive otherwise;
+- backward Euler under constant diffusivity: the pulse's second moment grows by exactly 2 chi dt per axis (the discrete
+  Laplacian of x^2 is 2), and heat is conserved;
+- the Levermore-Pomraning law: the thick-gas limit (4/3) a_r c_sb T^3 / (kappa Sigma) dT/dx, and the free-streaming
+  bound |F| <= c_sb a_r T^4.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+
+
+def random_system(w, h, seed, holes=False, contrast=3.0):
+    rng = np.random.default_rng(seed)
+    own = np.exp(rng.normal(0.0, 1.0, (h, w)))
+    kx = np.exp(rng.normal(0.0, contrast, (h, w + 1)))
+    ky = np.exp(rng.normal(0.0, contrast, (h + 1, w)))
+    kx[:, 0] = kx[:, w] = 0.0
+    ky[0, :] = ky[h, :] = 0.0
+    gas = np.ones((h, w), np.uint8)
+    if holes:
+        gas[h // 3:h // 3 + 3, w // 4:w // 4 + 5] = 0
+        gas[rng.random((h, w)) < 0.03] = 0
+    return own, kx, ky, gas
+
+
+def assemble(own, kx, ky, gas):
+    """A = diag(own) + sum over gas-gas faces of k (e_i - e_j)(e_i - e_j)^T; non-gas rows are the identity."""
+    h, w = own.shape
+    n = w * h
+    rows, cols, vals = [], [], []
+    d = np.where(gas.ravel() > 0, own.ravel(), 1.0).astype(np.float64)
+    for y in range(h):
+        for x in range(1, w):
+            a, b = y * w + x - 1, y * w + x
+            k = kx[y, x]
+            if gas.flat[a] and gas.flat[b] and k != 0.0:
+                d[a] += k
+                d[b] += k
+                rows += [a, b]
+                cols += [b, a]
+                vals += [-k, -k]
+    for y in range(1, h):
+        for x in range(w):
+            a, b = (y - 1) * w + x, y * w + x
+            k = ky[y, x]
+            if gas.flat[a] and gas.flat[b] and k != 0.0:
+                d[a] += k
+                d[b] += k
+                rows += [a, b]
+                cols += [b, a]
+                vals += [-k, -k]
+    A = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr() + sp.diags(d)
+    return A.tocsc()
+
+
+@pytest.mark.parametrize("holes", [False, True])
+def test_matvec_is_the_assembled_matrix(holes):
+    own, kx, ky, gas = random_system(37, 29, 1, holes)
+    A = assemble(own, kx, ky, gas)
+    v = np.random.default_rng(2).normal(size=(29, 37))
+    out = oracle.fld_matvec(own, kx, ky, gas, v)
+    ref = (A @ v.ravel()).reshape(29, 37)
+    assert np.max(np.abs(out - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "mg"])
+@pytest.mark.parametrize("holes", [False, True])
+def test_pcg_reaches_the_direct_solution(precond, holes):
+    w, h = 48, 40
+    own, kx, ky, gas = random_system(w, h, 3, holes, contrast=2.0)
+    A = assemble(own, kx, ky, gas)
+    b = np.random.default_rng(4).normal(size=(h, w)) * gas
+    x, it, rel = oracle.fld_pcg(own, kx, ky, gas, b, tol=1e-13, precond=precond)
+    xd = spla.spsolve(A, b.ravel()).reshape(h, w)
+    assert rel <= 1e-13
+    assert np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
+def test_mg_pcg_needs_fewer_iterations_than_jacobi():
+    own, kx, ky, gas = random_system(128, 128, 5, contrast=2.0)
+    own *= 1e-3  # stiff: diffusion dominates the diagonal
+    b = np.random.default_rng(6).normal(size=(128, 128))
+    _, it_j, _ = oracle.fld_pcg(own, kx, ky, gas, b, tol=1e-8, precond="jacobi")
+    _, it_m, _ = oracle.fld_pcg(own, kx, ky, gas, b, tol=1e-8, precond="mg")
+    assert it_m * 4 < it_j
+
+
+def test_vcycle_on_a_coarsest_grid_is_the_exact_inverse():
+    """A 4 x 4 grid is its own coarsest level: the dense solve there makes the V-cycle A^-1."""
+    own, kx, ky, gas = random_system(4, 4, 7)
+    A = assemble(own, kx, ky, gas).toarray()
+    r = np.random.default_rng(8).normal(size=(4, 4))
+    z = oracle.mg_apply(own, kx, ky, gas, r)
+    assert np.max(np.abs(z.ravel() - np.linalg.solve(A, r.ravel()))) <= 1e-12 * np.max(np.abs(z))
+
+
+@pytest.mark.parametrize("holes", [False, True])
+def test_vcycle_is_symmetric_and_positive(holes):
+    w, h = 64, 48
+    own, kx, ky, gas = random_system(w, h, 9, holes)
+    rng = np.random.default_rng(10)
+    r1, r2 = rng.normal(size=(h, w)) * gas, rng.normal(size=(h, w)) * gas
+    z1 = oracle.mg_apply(own, kx, ky, gas, r1)
+    z2 = oracle.mg_apply(own, kx, ky, gas, r2)
+    a, b = np.sum(r2 * z1), np.sum(r1 * z2)
+    assert abs(a - b) <= 1e-10 * np.sqrt(np.sum(r1 * z1) * np.sum(r2 * z2))
+    assert np.sum(r1 * z1) > 0 and np.sum(r2 * z2) > 0
+
+
+def pulse(n=64, width=4.0, t_bg=0.05, amp=2.0):
+    p = np.zeros((14, n, n))
+    p[0] = 1.0
+    p[4] = 1.0
+    yy, xx = np.mgrid[0:n, 0:n]
+    T = t_bg + amp * np.exp(-((xx - n / 2 + 0.5) ** 2 + (yy - n / 2 + 0.5) ** 2) / (2 * width ** 2))
+    p[3] = T * 2.0  # gamma 2 and 1/mu = 2 (hydrogen): E = Sigma T / mu
+    return p, T
+
+
+def test_backward_euler_pulse_moments_are_exact():
+    chi, dt, n, t_bg = 0.2, 5.0, 64, 0.05
+    ph = oracle.reference_physics(constant_diffusivity=chi, edge=oracle.EDGE_BOUNCE)
+    p, T0 = pulse(n, t_bg=t_bg)
+    out, rep, *_ = oracle.heat_step(ph, p, dt)
+    assert rep.converged
+    T1 = out[3] / 2.0
+    d0, d1 = T0 - t_bg, T1 - t_bg
+    c = n / 2 - 0.5
+    x = np.arange(n) - c
+    m0a, m0b = d0.sum(), d1.sum()
+    assert abs(m0b - m0a) <= 1e-9 * m0a  # heat constant
+    for axis in (0, 1):
+        xs = x[:, None] if axis == 0 else x[None, :]
+        m2a, m2b = (d0 * xs ** 2).sum(), (d1 * xs ** 2).sum()
+        # (T1 - T0) = dt chi Lap T1 and sum(x^2 Lap T1) = 2 sum(T1): the second moment grows by 2 chi dt M0
+        assert abs((m2b - m2a) - 2.0 * chi * dt * m0b) <= 1e-8 * m2a
+
+
+def law_strip(T_left, T_right, sigma=1.0, w=16):
+    p = np.zeros((14, 1, w))
+    p[0] = sigma
+    p[4] = 1.0
+    T = np.linspace(T_left, T_right, w)[None, :]
+    p[3] = sigma * T * 2.0
+    return p, T
+
+
+def test_thick_gas_limit():
+    """With a gentle gradient R -> 0 and lambda -> 1/3: F = (4/3) a_r c_sb T^3 / (kappa Sigma) (-dT/dx)."""
+    ph = oracle.reference_physics(edge=oracle.EDGE_BOUNCE)
+    sigma = 50.0
+    p, T = law_strip(1000.0, 1010.0, sigma)
+    fx, _ = oracle.heat_law(ph, p)
+    i = 8
+    Tf = 0.5 * (T[0, i - 1] + T[0, i])
+    kappa = ph.kappa0 * (1.0 + 1.0)  # the reference opacity kappa0 (1 + X_H), X_H = 1
+    dTdx = (T[0, i] - T[0, i - 1]) / ph.dx
+    expect = -(4.0 / 3.0) * ph.a_r * ph.c_sb * Tf ** 3 / (kappa * sigma) * dTdx
+    assert fx[0, i] == pytest.approx(expect, rel=1e-2)
+
+
+def test_free_streaming_bound():
+    ph = oracle.reference_physics(edge=oracle.EDGE_BOUNCE)
+    p, T = law_strip(2.0e4, 10.0, 1e-4)
+    fx, _ = oracle.heat_law(ph, p)
+    cap = ph.c_sb * ph.a_r * T.max() ** 4
+    assert np.abs(fx).max() <= cap * (1 + 1e-6)
+    assert np.abs(fx).max() >= 0.2 * cap  # a steep, thin front streams near the bound
+
+
+def test_lp_limiter_formula():
+    """lambda(R) = (2 + R) / (6 + 3R + R^2): 1/3 at R = 0 and R lambda -> 1 as R grows."""
+    lam = lambda R: (2 + R) / (6 + 3 * R + R * R)
+    assert lam(0.0) == pytest.approx(1 / 3)
+    assert 1e6 * lam(1e6) == pytest.approx(1.0, rel=1e-5)
+    # through the oracle: one face between two cells at u_a, u_b gives F = c_sb lambda(R) (u_a - u_b) / (kappa Sigma dx)
+    ph = oracle.reference_physics(edge=oracle.EDGE_BOUNCE)
+    for ta, tb in [(1000.0, 900.0), (1e4, 10.0), (50.0, 40.0)]:
+        p = np.zeros((14, 1, 2))
+        p[0] = 0.01
+        p[4] = 1.0
+        p[3, 0] = [0.01 * ta * 2.0, 0.01 * tb * 2.0]
+        fx, _ = oracle.heat_law(ph, p)
+        ua, ub = ph.a_r * ta ** 4, ph.a_r * tb ** 4
+        k = ph.kappa0 * 2.0 * 0.01
+        R = abs(ub - ua) / (ph.dx * k * 0.5 * (ua + ub))
+        assert fx[0, 1] == pytest.approx(ph.c_sb * lam(R) * (ua - ub) / (k * ph.dx), rel=1e-12)
