@@ -32,6 +32,7 @@ def parse():
     ap.add_argument("--fld-n", type=int, default=N_FLD)
     ap.add_argument("--step-n", type=int, default=N_STEP)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fld", action="store_true", help="skip the standalone FLD solve (counter runs)")
     return ap.parse_args()
 
 
@@ -87,6 +88,35 @@ def fld_solve(sr, scenes, n):
     return dict(ms=float(np.median(times) * 1e3), iterations=int(its[0]), rel=float(max(rels)), levels=f.levels)
 
 
+def copy_bandwidth(mib=512, reps=10):
+    """A plain device-to-device copy of a known size: the achievable HBM rate on this box (read + write bytes / s),
+    and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(mib * 2**20 // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return 2 * src.numel() * 4 / (ms * 1e-3) / 1e9
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^2 from the committed counter summary (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs, each scaled by what the counter reads for a copy of known size), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_full_step_{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("traffic_bytes")
+
+
 def cpu_baseline(scenes, budget_s=12.0):
     """The oracle twin (f64 C, one thread) on the same workload's statistics at 128^2 (the bounded sample): steps
     until the budget is spent."""
@@ -125,7 +155,9 @@ def main():
     value = n * n * args.steps / fs["elapsed"] / 1e6
     launches, mean_ms, bytes_per = fs["kernel"]
     achieved = bytes_per / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else 0.0
-    fld = fld_solve(sr, scenes, args.fld_n)
+    fld = None if args.no_fld else fld_solve(sr, scenes, args.fld_n)
+    copy_gbs = copy_bandwidth()
+    log(f"device copy {copy_gbs:.0f} GB/s")
     cpu = None if args.no_cpu else cpu_baseline(scenes)
     cost = fs["cost"]
     out = {
@@ -150,15 +182,16 @@ def main():
                         "P8+P9": round(cost.pass_ms[8], 4)},
             "heat_last_step": {"passes": fs["heat"].passes, "rounds": fs["heat"].rounds,
                                "converged": fs["heat"].converged},
-            "fld_solve_2048": {"workload": f"stiff FLD solve alone, {args.fld_n}^2, diffusion number 1e3, "
-                                          f"MG-PCG V(2,2) to 1e-8", "plain_mg_pcg_ms": round(fld["ms"], 3),
-                               "iterations": fld["iterations"], "final_rel": fld["rel"], "levels": fld["levels"],
-                               "learned_ms": None},
+            "fld_solve_2048": None if fld is None else {
+                "workload": f"stiff FLD solve alone, {args.fld_n}^2, diffusion number 1e3, MG-PCG V(2,2) to 1e-8",
+                "plain_mg_pcg_ms": round(fld["ms"], 3), "iterations": fld["iterations"], "final_rel": fld["rel"],
+                "levels": fld["levels"], "learned_ms": None},
         },
         "roofline": {"bound": "hbm", "kernel": "hydro_update", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "launches": launches, "mean_ms": round(mean_ms, 5),
-                     "algorithmic_bytes_per_launch": bytes_per},
+                     "traffic": measured_traffic("sr::k_hydro<1>", n), "launches": launches,
+                     "mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": bytes_per,
+                     "device_copy_gbs": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -99,6 +99,7 @@ typedef struct {
 typedef struct sr_world sr_world;
 typedef struct sr_gravity sr_gravity;
 typedef struct sr_fld sr_fld;
+typedef struct sr_net sr_net;
 
 /* ------------------------------------------------------------------------------------------------ errors / device */
 const char *sr_last_error(void);
@@ -138,6 +139,35 @@ int sr_fld_apply_precond(sr_fld *f, const float *r_dev, float *z_dev, int precon
 int sr_fld_matvec(sr_fld *f, const double *v_dev, double *out_dev, void *stream);
 int sr_fld_levels(sr_fld *f, int *levels);
 void sr_fld_destroy(sr_fld *f);
+/* The learned preconditioner (SR_PRECOND_LEARNED): z = V r + N(r - A V r), V the multigrid V-cycle and N the
+ * network; after fallback_iterations iterations without convergence (0: never) the solve continues with plain
+ * multigrid from where it stands. No counterpart in the reference (its set_heat_precond, step/heat.rs:595, offers
+ * Jacobi and multigrid); n may be NULL to detach. Grid sides must be multiples of 4. */
+int sr_fld_set_network(sr_fld *f, sr_net *n, int fallback_iterations);
+/* sr_fld_solve, also reporting whether the learned preconditioner fell back to plain multigrid */
+int sr_fld_solve_stats(sr_fld *f, const double *b_dev, double *x_dev, double tol, int max_iter, int precond,
+                       int *iterations, double *final_rel, int *fell_back, void *stream);
+
+/* ----------------------------------------------------------------------------- the learned preconditioner's network */
+/* A 3-level U-Net (16/32/64 channels, 3x3 convolutions, BF16 activations and weights, FP32 accumulation on the
+ * matrix cores) mapping (residual, log coupling, diagonal share) to a correction, with its backward pass and Adam on
+ * the device. No counterpart in the reference. Parameters are FP32 in the order of the layers e1a, e1, e2a, e2,
+ * e3a, e3, d2a, d2, d1a, d1 (each weights (3, 3, cin, cout) then biases), then the 1x1 output's 16 weights and bias. */
+int sr_net_create(int max_w, int max_h, sr_net **out);
+int sr_net_param_count(sr_net *n, size_t *count);
+int sr_net_set_params(sr_net *n, const float *params, size_t count); /* host array; resets Adam's moments */
+int sr_net_get_params(sr_net *n, float *params, size_t count);
+int sr_net_get_grads(sr_net *n, float *grads, size_t count);        /* of the last sr_net_loss_grad */
+/* z = N(r) scaled to the system's units, for a w x h system (own, kx, ky as sr_fld_set_system's, f32 device) */
+int sr_net_apply(sr_net *n, int w, int h, const float *own_dev, const float *kx_dev, const float *ky_dev,
+                 const float *r_dev, float *z_dev, void *stream);
+/* the energy-norm loss (z - e)^T A (z - e) / e^T A e of z = N(r) against e = A^-1 r, and every parameter's gradient */
+int sr_net_loss_grad(sr_net *n, int w, int h, const float *own_dev, const float *kx_dev, const float *ky_dev,
+                     const float *r_dev, const float *e_dev, double *loss, void *stream);
+int sr_net_adam(sr_net *n, double lr, void *stream);   /* one Adam step with the last gradients */
+/* one activation of the last forward pass as f32 (names e1a e1 p1 e2a e2 p2 e3a e3 d2a d2 d1a d1 x), channels-last */
+int sr_net_activation(sr_net *n, const char *name, float *out, size_t count);
+void sr_net_destroy(sr_net *n);
 
 /* ------------------------------------------------------------------------------------------------ the world / step */
 /* Step::with_heat (step/mod.rs:415-527) + State::new (state.rs:139): refuses an axis not a multiple of 8 in
@@ -158,6 +188,8 @@ int sr_read_field(sr_world *wd, int field, float *out, size_t n); /* side fields
  * through the edges (reference/hydro.rs:210-224, :954-976) */
 int sr_read_totals(sr_world *wd, double *totals, double *escaped);
 int sr_set_precond(sr_world *wd, int precond);                /* set_heat_precond (step/heat.rs:595) */
+/* the network SR_PRECOND_LEARNED uses in the world's heat pass (no counterpart in the reference) */
+int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
 int sr_set_dt_cap(sr_world *wd, double cap);                  /* Twin::cap_dt_next (reference/mod.rs:225) */
 int sr_set_timing(sr_world *wd, int on);                      /* Step::set_timing (step/mod.rs:722) */
 int sr_read_cost(sr_world *wd, sr_step_cost *out);            /* Step::read_cost (step/mod.rs:753) */

@@ -132,6 +132,7 @@ class Gravity:
 
 
 from .fld import Fld  # noqa: E402
+from .net import Net  # noqa: E402
 from .step import World  # noqa: E402
 
 __all__ = ["SrError", "Gravity", "World", "Fld", "fft2", "physics", "heat_settings", "require_device", "lib",

@@ -1,4 +1,5 @@
-"""Builds libsrhip.so in-tree with hipcc for gfx950 (the package's only native artefact).
+"""Builds libsrhip.so in-tree with hipcc for gfx950 (the grid-physics library), and libsrcalib.so beside it (the
+known-size device copies the bench uses to calibrate the traffic counters).
 
     python -m sandbox_reactions_amd.build        # or sandbox_reactions_amd.build.build()
 """
@@ -14,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsrhip.so")
+CALIB_LIB = os.path.join(HERE, "libsrcalib.so")
+CALIB_SRC = "calib.hip"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -40,12 +43,14 @@ def build(verbose: bool = False) -> str:
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    calib = [o for o in objs if os.path.basename(o).startswith(CALIB_SRC)]
+    for lib, parts in [(LIB, [o for o in objs if o not in calib]), (CALIB_LIB, calib)]:
+        tmp = lib + ".tmp"
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + parts
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, lib)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     return LIB

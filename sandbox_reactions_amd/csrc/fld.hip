@@ -7,10 +7,11 @@
 // machinery is this library's own:
 //   * outer loop in FP64: r = b - A x with FP64 x and b, the stopping rule on that true residual;
 //   * inner loop in FP32: flexible (Polak-Ribiere) PCG on the residual, FP64 dot products, x updated in FP64;
-//   * the pre-smoother of a level, its residual and its restriction are one kernel: a 64x32 tile with a halo of
-//     2 nu1 + 1 cells in LDS, each half sweep recomputed redundantly on the shrinking halo, so the result is exact
-//     red-black Gauss-Seidel with one pass over HBM; the prolongation and the post-smoother are another such kernel;
-//   * every level of at most 1024 cells is handled inside one 1024-thread block (an LDS-resident V-cycle), the
+//   * the pre-smoother of a level, its residual and its restriction are one kernel: a 32x32 tile (16x16 on levels
+//     too small to fill the chip) with a halo of 2 nu1 + 1 cells in LDS, each half sweep recomputed redundantly on
+//     the shrinking halo, so the result is exact red-black Gauss-Seidel with one pass over HBM; the prolongation and
+//     the post-smoother are another such kernel;
+//   * every level of at most 4096 cells is handled inside one 1024-thread block (an LDS-resident V-cycle), the
 //     coarsest (at most 16 cells) by an exact dense solve;
 //   * dot products reduce by wave64 shuffles into per-block partials that the next kernel sums itself in a fixed
 //     order (deterministic, and no device-wide fence: a fence at agent scope writes back the whole L2 of an XCD);
@@ -22,6 +23,7 @@
 
 #include "common.hpp"
 #include "fld.hpp"
+#include "net.hpp"
 
 namespace sr {
 
@@ -29,7 +31,7 @@ namespace sr {
 #define NB_MAX 1024
 #define COARSE_SWEEPS 16
 #define COARSE_MAX 4096
-#define COARSE_NT 512
+#define COARSE_NT 1024
 #define COARSE_LDS (150 * 1024)
 #define DENSE_MAX 16
 
@@ -185,6 +187,45 @@ __global__ void __launch_bounds__(NT) k_dots(int n, const float *__restrict__ r,
     block_partials<2>(v, part, gridDim.x);
 }
 
+// r1 = r - A z in FP32 on the gas cells (0 elsewhere): what the V-cycle left, the learned correction's input
+__global__ void __launch_bounds__(NT) k_resid32(L0 a, const float *__restrict__ r, const float *__restrict__ z,
+                                                float *__restrict__ r1) {
+    const int n = a.w * a.h;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        const int xx = i % a.w, yy = i / a.w;
+        float v = 0.0f;
+        if (is_gas(a, i)) {
+            const float zi = z[i];
+            const float al = a.ax[(size_t)yy * (a.w + 1) + xx], ar = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
+            const float au = a.ay[(size_t)yy * a.w + xx], ad = a.ay[(size_t)(yy + 1) * a.w + xx];
+            float s = a.own[i] * zi;
+            if (al != 0.0f) s += al * (zi - z[i - 1]);
+            if (ar != 0.0f) s += ar * (zi - z[i + 1]);
+            if (au != 0.0f) s += au * (zi - z[i - a.w]);
+            if (ad != 0.0f) s += ad * (zi - z[i + a.w]);
+            v = r[i] - s;
+        }
+        r1[i] = v;
+    }
+}
+
+// z = 0 off the gas; partials of r.z and r.z_old
+__global__ void __launch_bounds__(NT) k_dots_gas(L0 a, const float *__restrict__ r, float *__restrict__ z,
+                                                 const float *__restrict__ zold, double *part) {
+    const int n = a.w * a.h;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        float zi = z[i];
+        if (!is_gas(a, i)) {
+            zi = 0.0f;
+            z[i] = 0.0f;
+        }
+        v[0] += (double)r[i] * zi;
+        if (zold) v[1] += (double)r[i] * zold[i];
+    }
+    if (part) block_partials<2>(v, part, gridDim.x);
+}
+
 // ------------------------------------------------------------------------------------------------ tile smoothers
 
 struct Lv {
@@ -249,7 +290,7 @@ __device__ inline void half_sweep(Region<TX, TY, HW> &R, const Lv &L, int x0, in
         const int c = ly * RG::RX + lx;
         const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RG::RX];
         const float num = R.b[c] + al * R.z[c - 1] + ar * R.z[c + 1] + au * R.z[c - RG::RX] + ad * R.z[c + RG::RX];
-        R.z[c] = num / (R.own[c] + al + ar + au + ad);
+        R.z[c] = __fdividef(num, R.own[c] + al + ar + au + ad);
     }
     __syncthreads();
 }
@@ -363,38 +404,48 @@ struct CoarseSet {
     int n;
     int w[16], h[16];
     int pack_floats;   // own/ax/ay of every level, then the inverse
-    int offo[16], offx[16], offy[16], offz[16], offb[16], offinv, total;  // LDS offsets (floats)
+    int offo[16], offx[16], offy[16], offz[16], offb[16], offd[16], offinv, total;  // LDS offsets (floats)
+    int lw[16];        // log2 of the level's width when a power of two, else -1
 };
+
+// (y, x) of entry t in rows of width w: a shift when w is a power of two (lw >= 0), a division otherwise
+__device__ __forceinline__ int row_of(int t, int w, int lw) { return lw >= 0 ? t >> lw : t / w; }
 
 __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *__restrict__ pack,
                                                       const float *__restrict__ b0, float *__restrict__ z0, int nu1,
                                                       int nu2) {
     extern __shared__ float sm[];
-    const int *offo = C.offo, *offx = C.offx, *offy = C.offy, *offz = C.offz, *offb = C.offb;
+    const int *offo = C.offo, *offx = C.offx, *offy = C.offy, *offz = C.offz, *offb = C.offb, *offd = C.offd;
     const int offinv = C.offinv, total = C.total;
 #pragma unroll 8
     for (int t = threadIdx.x; t < C.pack_floats; t += COARSE_NT) sm[t] = pack[t];
     for (int t = C.pack_floats + threadIdx.x; t < total; t += COARSE_NT) sm[t] = 0.0f;
     __syncthreads();
     for (int t = threadIdx.x; t < C.w[0] * C.h[0]; t += COARSE_NT) sm[offb[0] + t] = b0[t];
+    // the inverse of every cell's diagonal (own + its couplings), once per application
+    for (int l = 0; l < C.n; l++) {
+        const int w = C.w[l], h = C.h[l];
+        const float *own = sm + offo[l], *al = sm + offx[l], *au = sm + offy[l];
+        float *rd = sm + offd[l];
+        for (int t = threadIdx.x; t < w * h; t += COARSE_NT) {
+            const int y = row_of(t, w, C.lw[l]), x = t - y * w;
+            rd[t] = 1.0f / (own[t] + al[y * (w + 1) + x] + al[y * (w + 1) + x + 1] + au[t] + au[t + w]);
+        }
+    }
     __syncthreads();
-    // one half sweep of `color` on level l, colour-compact
+    // one half sweep of `color` on level l, colour-compact. Couplings on the grid's rim are zero, so the neighbour
+    // entries read there (the adjacent LDS arrays, all finite) contribute nothing.
     auto sweep = [&](int l, int color) {
-        const int w = C.w[l], h = C.h[l], hw = (w + 1) >> 1;
-        const float *own = sm + offo[l], *al = sm + offx[l], *au = sm + offy[l], *b = sm + offb[l];
+        const int w = C.w[l], h = C.h[l], hw = (w + 1) >> 1, lh = C.lw[l] > 0 ? C.lw[l] - 1 : -1;
+        const float *al = sm + offx[l], *au = sm + offy[l], *b = sm + offb[l], *rd = sm + offd[l];
         float *z = sm + offz[l];
         for (int t = threadIdx.x; t < hw * h; t += COARSE_NT) {
-            const int y = t / hw;
+            const int y = row_of(t, hw, lh);
             const int x = 2 * (t - y * hw) + ((color + y) & 1);
             if (x >= w) continue;
             const int i = y * w + x;
             const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[i], a3 = au[i + w];
-            float num = b[i], den = own[i];
-            if (a0 != 0.0f) { num += a0 * z[i - 1]; den += a0; }
-            if (a1 != 0.0f) { num += a1 * z[i + 1]; den += a1; }
-            if (a2 != 0.0f) { num += a2 * z[i - w]; den += a2; }
-            if (a3 != 0.0f) { num += a3 * z[i + w]; den += a3; }
-            z[i] = num / den;
+            z[i] = (b[i] + a0 * z[i - 1] + a1 * z[i + 1] + a2 * z[i - w] + a3 * z[i + w]) * rd[i];
         }
         __syncthreads();
     };
@@ -408,19 +459,19 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
                     *b = sm + offb[l];
         float *bc = sm + offb[l + 1];
         for (int t = threadIdx.x; t < wc * hc; t += COARSE_NT) {
-            const int Y = t / wc, X = t - Y * wc;
+            const int Y = row_of(t, wc, C.lw[l + 1]), X = t - Y * wc;
             float s = 0.0f;
+#pragma unroll
             for (int dy = 0; dy < 2; dy++)
+#pragma unroll
                 for (int dx = 0; dx < 2; dx++) {
                     const int x = 2 * X + dx, y = 2 * Y + dy;
                     if (x >= w || y >= h) continue;
                     const int i = y * w + x;
                     const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[i], a3 = au[i + w];
-                    float az = own[i] * z[i];
-                    if (a0 != 0.0f) az += a0 * (z[i] - z[i - 1]);
-                    if (a1 != 0.0f) az += a1 * (z[i] - z[i + 1]);
-                    if (a2 != 0.0f) az += a2 * (z[i] - z[i - w]);
-                    if (a3 != 0.0f) az += a3 * (z[i] - z[i + w]);
+                    const float zi = z[i];
+                    const float az = own[i] * zi + a0 * (zi - z[i - 1]) + a1 * (zi - z[i + 1]) +
+                                     a2 * (zi - z[i - w]) + a3 * (zi - z[i + w]);
                     s += b[i] - az;
                 }
             bc[t] = s;
@@ -454,7 +505,7 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
         float *z = sm + offz[l];
         const float *zc = sm + offz[l + 1];
         for (int t = threadIdx.x; t < w * h; t += COARSE_NT) {
-            const int y = t / w, x = t - y * w;
+            const int y = row_of(t, w, C.lw[l]), x = t - y * w;
             z[t] += zc[(y >> 1) * wc + (x >> 1)];
         }
         __syncthreads();
@@ -635,13 +686,13 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_) {
         const int c = coarse_from - 1;
         if (lv[c].w * lv[c].h > COARSE_MAX || (int)lv.size() - c > 16) break;
         size_t floats = DENSE_MAX * DENSE_MAX;
-        for (int l = c; l < (int)lv.size(); l++) floats += level_floats(lv[l].w, lv[l].h) + 2 * lv[l].w * lv[l].h;
+        for (int l = c; l < (int)lv.size(); l++) floats += level_floats(lv[l].w, lv[l].h) + 3 * lv[l].w * lv[l].h;
         if (floats * sizeof(float) > COARSE_LDS) break;
         coarse_from = c;
     }
     {
         const FldLevel &c = lv.back();
-        size_t floats = DENSE_MAX * DENSE_MAX + level_floats(c.w, c.h) + 2 * c.w * c.h;
+        size_t floats = DENSE_MAX * DENSE_MAX + level_floats(c.w, c.h) + 3 * c.w * c.h;
         if (c.w * c.h > COARSE_MAX || floats * sizeof(float) > COARSE_LDS)
             return fail(SR_ERR_ARGUMENT, "too few multigrid levels: the coarsest level must fit one block (" +
                                              std::to_string(COARSE_MAX) + " cells)");
@@ -758,6 +809,11 @@ static void launch_post(int nu, dim3 grid, hipStream_t s, Lv L, const float *b, 
         hipLaunchKernelGGL((k_post<32, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, zout, zold, part);
 }
 
+// Levels below level 0 that would launch fewer than 512 blocks of 32x32 take 16x16 tiles: four times the blocks
+// and a quarter of the serial work per thread, where the launch is latency-bound anyway. Level 0 keeps the wide
+// tile (its partials count is sized for it).
+static bool small_tiles(int l, int w, int h) { return l > 0 && ((w + 31) / 32) * ((h + 31) / 32) < 512; }
+
 // z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
 int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s) {
     const int nl = (int)lv.size();
@@ -766,10 +822,14 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
     for (int l = 0; l < coarse_from; l++) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
         Lv L = {f.w, f.h, f.own, f.ax, f.ay};
-        dim3 grid((f.w + tx - 1) / tx, (f.h + ty - 1) / ty);
+        const bool small = small_tiles(l, f.w, f.h);
+        const int sx = small ? 16 : tx, sy = small ? 16 : ty;
+        dim3 grid((f.w + sx - 1) / sx, (f.h + sy - 1) / sy);
         const bool t0 = timing && l == 0;
         if (t0) SR_HIP(hipEventRecord(ev0, s));
-        if (tx == PTX)
+        if (small)
+            launch_pre<16, 16>(nu1, grid, s, L, bl, f.z, c.b, c.w);
+        else if (tx == PTX)
             launch_pre<PTX, PTY>(nu1, grid, s, L, bl, f.z, c.b, c.w);
         else
             launch_pre<32, PTY>(nu1, grid, s, L, bl, f.z, c.b, c.w);
@@ -806,6 +866,9 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
             o += C.w[k] * C.h[k];
             C.offb[k] = o;
             o += C.w[k] * C.h[k];
+            C.offd[k] = o;
+            o += C.w[k] * C.h[k];
+            C.lw[k] = is_pow2(C.w[k]) ? ilog2(C.w[k]) : -1;
         }
         C.total = o;
         const size_t floats = o;
@@ -821,16 +884,41 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
     for (int l = coarse_from - 1; l >= 0; l--) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
         Lv L = {f.w, f.h, f.own, f.ax, f.ay};
-        dim3 grid((f.w + tx - 1) / tx, (f.h + ty - 1) / ty);
+        const bool small = small_tiles(l, f.w, f.h);
+        const int sx = small ? 16 : tx, sy = small ? 16 : ty;
+        dim3 grid((f.w + sx - 1) / sx, (f.h + sy - 1) / sy);
         const float *b = l == 0 ? r : f.b;
         float *zo = l == 0 ? z : f.zp;
         double *pp = (l == 0 && dots) ? dots : nullptr;
-        if (tx == PTX)
+        if (small)
+            launch_post<16, 16>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
+        else if (tx == PTX)
             launch_post<PTX, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
         else
             launch_post<32, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
         if (l == 0 && dots) *nbd = grid.x * grid.y;
     }
+    SR_HIP(hipGetLastError());
+    return SR_OK;
+}
+
+int FldSolver::set_network(Net *n, int fallback) {
+    if (n && (w0 % 4 || h0 % 4))
+        return fail(SR_ERR_ARGUMENT, "the learned preconditioner needs grid sides that are multiples of 4");
+    net = n;
+    fallback_it = std::max(0, fallback);
+    return SR_OK;
+}
+
+// the learned preconditioner: z = V r + N(r - A V r), z = 0 off the gas; with `dots`, partials of r.z and r.zold
+int FldSolver::learned(const float *r, float *z, const float *zold, double *dots, hipStream_t s) {
+    if (!net) return fail(SR_ERR_ARGUMENT, "the learned preconditioner has no network");
+    L0 a = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
+    int nbd = 0;
+    SR_TRY(vcycle(r, z, nullptr, nullptr, &nbd, s));
+    hipLaunchKernelGGL(k_resid32, dim3(nbe), dim3(NT), 0, s, a, r, (const float *)z, q);
+    SR_TRY(net->apply(w0, h0, lv[0].own, lv[0].ax, lv[0].ay, q, z, z, s));
+    hipLaunchKernelGGL(k_dots_gas, dim3(nbe), dim3(NT), 0, s, a, r, z, zold, dots);
     SR_HIP(hipGetLastError());
     return SR_OK;
 }
@@ -842,6 +930,7 @@ int FldSolver::apply_precond(const float *r, float *z, int precond, hipStream_t 
         SR_HIP(hipGetLastError());
         return SR_OK;
     }
+    if (precond == SR_PRECOND_LEARNED) return learned(r, z, nullptr, nullptr, s);
     int nbd = 0;
     return vcycle(r, z, nullptr, nullptr, &nbd, s);
 }
@@ -869,8 +958,24 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                      FldStats *st, hipStream_t s) {
     L0 a = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
     const size_t n = (size_t)w0 * h0;
+    if (precond == SR_PRECOND_LEARNED && !net)
+        return fail(SR_ERR_ARGUMENT, "the learned preconditioner has no network (sr_fld_set_network)");
+    bool use_net = precond == SR_PRECOND_LEARNED;
     const bool mg = precond != 0 && lv.size() > 1;
     *st = FldStats();
+    // z = M r with partials of r.z and r.z_old into `dots` (their count into nbd)
+    auto precondition = [&](float *zc, const float *zo, double *dots, int *nbd) -> int {
+        if (use_net) {
+            SR_TRY(learned(r32, zc, zo, dots, s));
+            *nbd = nbe;
+            return SR_OK;
+        }
+        if (mg) return vcycle(r32, zc, zo, dots, nbd, s);
+        L0 aa = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
+        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, aa, r32, zc, zo, dots);
+        *nbd = nbe;
+        return SR_OK;
+    };
     const double inner_rtol = 1e-4;
     int it = 0;
     double *dots[2] = {part_dots, part_dots + 2 * nbd_max};
@@ -906,10 +1011,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         SR_HIP(hipMemsetAsync(zo, 0, sizeof(float) * n, s));
         int k = 0, nbd = nbe, nbp = nbe;
         // z = M r, partials of r.z (restart: beta = 0)
-        if (mg)
-            SR_TRY(vcycle(r32, zc, zo, dots[k & 1], &nbd, s));
-        else
-            hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, r32, zc, zo, dots[k & 1]);
+        SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
         int last_check = it, interval = 1;
         double rr_last = rr;
         while (it < max_iter) {
@@ -926,6 +1028,12 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                     break;
                 }
                 if (rri <= stop2) break;
+                if (use_net && fallback_it > 0 && it >= fallback_it) {
+                    // the learned preconditioner has used plain multigrid's budget: restart with plain multigrid
+                    use_net = false;
+                    st->fell_back = true;
+                    break;
+                }
                 // iterations left at the rate seen since the last check: check again a little before then
                 const double rate =
                     std::log(std::max(rri, 1e-300) / std::max(rr_last, 1e-300)) / (double)(it - last_check);
@@ -938,12 +1046,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
             std::swap(pold, pnew);
             k++;
             nbp = nbd;
-            if (mg)
-                SR_TRY(vcycle(r32, zc, zo, dots[k & 1], &nbd, s));
-            else {
-                hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, r32, zc, zo, dots[k & 1]);
-                nbd = nbe;
-            }
+            SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
         }
         if (!st->finite) break;
     }
@@ -970,6 +1073,10 @@ struct sr_fld {
     FldSolver s;
 };
 
+namespace sr {
+Net *net_of(sr_net *p);
+}
+
 extern "C" int sr_fld_create(int w, int h, const sr_heat_settings *hs, sr_fld **out) {
     if (!out) return fail(SR_ERR_ARGUMENT, "sr_fld_create: null out");
     sr_heat_settings d;
@@ -994,7 +1101,6 @@ extern "C" int sr_fld_set_system(sr_fld *f, const float *own_dev, const float *k
 extern "C" int sr_fld_solve(sr_fld *f, const double *b_dev, double *x_dev, double tol, int max_iter, int precond,
                             int *iterations, double *final_rel, void *stream) {
     if (!f || !b_dev || !x_dev) return fail(SR_ERR_ARGUMENT, "sr_fld_solve: null pointer");
-    if (precond == SR_PRECOND_LEARNED) return fail(SR_ERR_ARGUMENT, "sr_fld_solve: the learned preconditioner is not built");
     FldStats st;
     SR_TRY(f->s.solve(b_dev, x_dev, tol, max_iter, precond, nullptr, &st, (hipStream_t)stream));
     if (iterations) *iterations = st.iterations;
@@ -1011,6 +1117,23 @@ extern "C" int sr_fld_apply_precond(sr_fld *f, const float *r_dev, float *z_dev,
 extern "C" int sr_fld_matvec(sr_fld *f, const double *v_dev, double *out_dev, void *stream) {
     if (!f || !v_dev || !out_dev) return fail(SR_ERR_ARGUMENT, "sr_fld_matvec: null pointer");
     return f->s.matvec64(v_dev, out_dev, (hipStream_t)stream);
+}
+
+extern "C" int sr_fld_set_network(sr_fld *f, sr_net *n, int fallback_iterations) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_network: null solver");
+    return f->s.set_network(net_of(n), fallback_iterations);
+}
+
+extern "C" int sr_fld_solve_stats(sr_fld *f, const double *b_dev, double *x_dev, double tol, int max_iter,
+                                  int precond, int *iterations, double *final_rel, int *fell_back, void *stream) {
+    if (!f || !b_dev || !x_dev) return fail(SR_ERR_ARGUMENT, "sr_fld_solve_stats: null pointer");
+    FldStats st;
+    SR_TRY(f->s.solve(b_dev, x_dev, tol, max_iter, precond, nullptr, &st, (hipStream_t)stream));
+    if (iterations) *iterations = st.iterations;
+    if (final_rel) *final_rel = st.rel;
+    if (fell_back) *fell_back = st.fell_back ? 1 : 0;
+    if (!st.finite) return fail(SR_ERR_NON_FINITE, "sr_fld_solve: a sum left the float's range");
+    return SR_OK;
 }
 
 extern "C" int sr_fld_levels(sr_fld *f, int *levels) {

@@ -6,6 +6,8 @@
 
 namespace sr {
 
+class Net;
+
 // One level of the hierarchy: (A v)_i = own_i v_i + sum_f a_f (v_i - v_j); ax is (w+1) x h (face x between cells
 // x-1 and x, 0 at the edges), ay is w x (h+1).
 struct FldLevel {
@@ -18,6 +20,7 @@ struct FldStats {
     double rel = 0.0;      // ||b - A x|| / ||b|| in FP64
     bool converged = false;
     bool finite = true;
+    bool fell_back = false;  // the learned preconditioner gave way to plain multigrid
 };
 
 // The reference's energy-weighted stopping rule (crates/sr-physics/src/reference/heat.rs:424-431): r^T D^-1 r <=
@@ -44,6 +47,9 @@ class FldSolver {
     int solve(const double *b, double *x, double tol, int max_iter, int precond, const RefStop *ref, FldStats *st,
               hipStream_t s);
     int apply_precond(const float *r, float *z, int precond, hipStream_t s);
+    // the network of the learned preconditioner (z = V r + N(r - A V r)); after `fallback` iterations without
+    // convergence (0: never) the solve continues with plain multigrid
+    int set_network(Net *n, int fallback);
     int matvec64(const double *v, double *out, hipStream_t s);
     int nlevels() const { return (int)lv.size(); }
     int width() const { return w0; }
@@ -70,7 +76,10 @@ class FldSolver {
     double *part_true = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_dots = nullptr;
     double *host_part = nullptr; // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Net *net = nullptr;
+    int fallback_it = 0;
     int vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s);
+    int learned(const float *r, float *z, const float *zold, double *dots, hipStream_t s);
     int read_partials(const double *part, int k, int nb, double *out, hipStream_t s);
 };
 

@@ -202,6 +202,10 @@ __global__ void k_extract_box(const float *__restrict__ ringed, int w, int h, fl
 
 using namespace sr;
 
+namespace sr {
+Net *net_of(sr_net *p);
+}
+
 struct KernelClock {
     unsigned long long launches = 0;
     double ms = 0.0;
@@ -215,6 +219,7 @@ struct sr_world {
     sr_switches sw{};
     sr_heat_settings hs{};
     int precond = SR_PRECOND_MULTIGRID;
+    sr_net *net = nullptr;
     hipStream_t s = nullptr;
     HydroState st[2]{};
     int cur = 0;
@@ -596,10 +601,20 @@ extern "C" int sr_read_totals(sr_world *wd, double *totals, double *escaped) {
 
 extern "C" int sr_set_precond(sr_world *wd, int precond) {
     if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
-    if (precond == SR_PRECOND_LEARNED) return fail(SR_ERR_ARGUMENT, "the learned preconditioner is not built");
-    if (precond != SR_PRECOND_JACOBI && precond != SR_PRECOND_MULTIGRID)
+    if (precond != SR_PRECOND_JACOBI && precond != SR_PRECOND_MULTIGRID && precond != SR_PRECOND_LEARNED)
         return fail(SR_ERR_ARGUMENT, "unknown preconditioner");
+    if (precond == SR_PRECOND_LEARNED && !wd->net)
+        return fail(SR_ERR_ARGUMENT, "the learned preconditioner needs a network (sr_set_network)");
     wd->precond = precond;
+    return SR_OK;
+}
+
+extern "C" int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (!wd->sw.heat) return fail(SR_ERR_ARGUMENT, "sr_set_network: the world has no heat pass");
+    SR_TRY(wd->heat.solver.set_network(net_of(n), fallback_iterations));
+    wd->net = n;
+    if (!n && wd->precond == SR_PRECOND_LEARNED) wd->precond = SR_PRECOND_MULTIGRID;
     return SR_OK;
 }
 

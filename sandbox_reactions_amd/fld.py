@@ -42,6 +42,21 @@ class Fld:
                                   precond, C.byref(it), C.byref(rel), self._s(stream)))
         return it.value, rel.value
 
+    def solve_stats(self, b, x, tol=1e-8, max_iter=10000, precond=_lib.PRECOND_MULTIGRID, stream=None):
+        """As solve(), also returning whether the learned preconditioner fell back to plain multigrid."""
+        from . import _check, lib
+        it, rel, fb = C.c_int(), C.c_double(), C.c_int()
+        _check(lib().sr_fld_solve_stats(self._p, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), tol,
+                                        max_iter, precond, C.byref(it), C.byref(rel), C.byref(fb),
+                                        self._s(stream)))
+        return it.value, rel.value, bool(fb.value)
+
+    def set_network(self, net, fallback_iterations: int = 0):
+        """The network of the learned preconditioner (None detaches it)."""
+        from . import _check, lib
+        self._net = net  # keep it alive while attached
+        _check(lib().sr_fld_set_network(self._p, net._p if net is not None else None, fallback_iterations))
+
     def apply_precond(self, r, z, precond=_lib.PRECOND_MULTIGRID, stream=None):
         from . import _check, lib
         _check(lib().sr_fld_apply_precond(self._p, C.c_void_p(r.data_ptr()), C.c_void_p(z.data_ptr()), precond,

@@ -1,0 +1,98 @@
+"""HBM traffic per kernel launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).
+
+Each counter is scaled by what it reads for a device copy of known size (the 512 MiB copy `bench.py` makes: the
+largest `copyBuffer` dispatch reads and writes 512 MiB), so the figures are bytes as the copy kernel would count
+them. Usage (after the two runs, on any machine):
+
+    python -m sandbox_reactions_amd.pmc FETCH_DIR WRITE_DIR OUT_JSON [--workload TEXT] [--command TEXT]
+
+OUT_JSON gets {"copy_scale": {...}, "kernels": {name: {"launches", "fetch_bytes", "write_bytes", "traffic_bytes",
+"grid"}}} for the dispatch grid with the most total traffic per kernel; a Markdown table goes beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+
+COPY_BYTES = 512 * 2**20
+
+
+def read_counters(d):
+    paths = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not paths:
+        raise FileNotFoundError(f"no counter_collection.csv under {d}")
+    rows = []
+    for p in paths:
+        with open(p) as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def short(name):
+    return name.split("(")[0].replace("void ", "").strip()
+
+
+def per_kernel(rows):
+    agg = collections.defaultdict(list)
+    for r in rows:
+        agg[(short(r["Kernel_Name"]), int(r["Grid_Size"]))].append(float(r["Counter_Value"]) * 1024.0)
+    return agg
+
+
+def copy_reading(agg):
+    """The counter's bytes for the largest copyBuffer dispatch (the 512 MiB calibration copy)."""
+    cands = [(g, v) for (k, g), v in agg.items() if "copyBuffer" in k]
+    if not cands:
+        raise ValueError("no copyBuffer dispatch in the counter run: the calibration copy is missing")
+    g, v = max(cands, key=lambda c: c[0])
+    return max(v)
+
+
+def summarize(fetch_dir, write_dir):
+    fa, wa = per_kernel(read_counters(fetch_dir)), per_kernel(read_counters(write_dir))
+    sf = COPY_BYTES / copy_reading(fa)
+    sw = COPY_BYTES / copy_reading(wa)
+    kernels = {}
+    for key in fa.keys() & wa.keys():
+        name, grid = key
+        if "copyBuffer" in name or "fillBuffer" in name:
+            continue
+        f = sum(fa[key]) / len(fa[key]) * sf
+        w = sum(wa[key]) / len(wa[key]) * sw
+        cur = kernels.get(name)
+        if cur is None or (f + w) * len(fa[key]) > cur["traffic_bytes"] * cur["launches"]:
+            kernels[name] = {"grid": grid, "launches": len(fa[key]), "fetch_bytes": f, "write_bytes": w,
+                             "traffic_bytes": f + w}
+    return {"copy_scale": {"fetch": sf, "write": sw}, "kernels": kernels}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("fetch_dir")
+    ap.add_argument("write_dir")
+    ap.add_argument("out_json")
+    ap.add_argument("--workload", default="")
+    ap.add_argument("--command", default="")
+    a = ap.parse_args()
+    s = summarize(a.fetch_dir, a.write_dir)
+    s["workload"] = a.workload
+    s["command"] = a.command
+    with open(a.out_json, "w") as f:
+        json.dump(s, f, indent=1, sort_keys=True)
+    md = os.path.splitext(a.out_json)[0] + ".md"
+    with open(md, "w") as f:
+        f.write(f"# HBM traffic per launch: {a.workload}\n\nCommands: `{a.command}`\n\n")
+        f.write(f"Counters scaled by the 512 MiB calibration copy: FETCH x{s['copy_scale']['fetch']:.4f}, "
+                f"WRITE x{s['copy_scale']['write']:.4f}.\n\n")
+        f.write("| kernel | grid | launches | fetch MB | write MB | traffic MB |\n|---|---|---|---|---|---|\n")
+        for k, v in sorted(s["kernels"].items(), key=lambda kv: -kv[1]["traffic_bytes"] * kv[1]["launches"]):
+            f.write(f"| {k} | {v['grid']} | {v['launches']} | {v['fetch_bytes'] / 1e6:.2f} | "
+                    f"{v['write_bytes'] / 1e6:.2f} | {v['traffic_bytes'] / 1e6:.2f} |\n")
+
+
+if __name__ == "__main__":
+    main()

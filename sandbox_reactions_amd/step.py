@@ -75,6 +75,11 @@ class World:
     def set_precond(self, precond: int):
         self._c(self.lib.sr_set_precond(self._p, precond))
 
+    def set_network(self, net, fallback_iterations: int = 0):
+        """The network the learned preconditioner uses in the heat pass (None detaches it)."""
+        self._net = net
+        self._c(self.lib.sr_set_network(self._p, net._p if net is not None else None, fallback_iterations))
+
     def set_dt_cap(self, cap: float):
         self._c(self.lib.sr_set_dt_cap(self._p, cap))
 

@@ -1,0 +1,226 @@
+"""The learned preconditioner's network on the GPU against the f64 numpy oracle (oracle/unet.py): every layer from
+the GPU's own inputs (which pins the matrix-core lane layout and the implicit-GEMM indexing), the whole forward pass,
+every parameter's gradient, Adam, and the network inside flexible CG (convergence, fallback to plain multigrid).
+
+Tolerances: activations and weights are BF16 (8 significant bits), so a layer from its exact input agrees to about
+2^-8 of its scale; gradients flow through ten BF16 layers and are compared by relative L2 and direction.
+"""
+import numpy as np
+import pytest
+
+from oracle import unet
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """Round to BF16 (nearest even), returned as float64."""
+    b = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def system(n, seed, contrast=2.0):
+    rng = np.random.default_rng(seed)
+    own = np.exp(rng.normal(0.0, 1.0, (n, n))).astype(np.float32)
+    kx = np.exp(rng.normal(0.0, contrast, (n, n + 1))).astype(np.float32)
+    ky = np.exp(rng.normal(0.0, contrast, (n + 1, n))).astype(np.float32)
+    kx[:, 0] = kx[:, n] = 0.0
+    ky[0, :] = ky[n, :] = 0.0
+    return own, kx, ky
+
+
+def cuda(*arrs):
+    import torch
+    return [torch.tensor(np.ascontiguousarray(a, np.float32), device="cuda") for a in arrs]
+
+
+def random_params(seed):
+    from sandbox_reactions_amd import net as N
+    p = N.init_params(seed)
+    rng = np.random.default_rng(seed + 100)
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = (rng.normal(size=p[k].shape) * 0.1).astype(np.float32)
+    p["out.w"] = (rng.normal(size=16) * 0.3).astype(np.float32)
+    return p
+
+
+def close(gpu, ref, frac):
+    scale = max(np.abs(ref).max(), 1e-30)
+    return np.abs(gpu - ref).max() / scale <= frac
+
+
+def test_every_layer_matches_the_oracle(gpu_lib):
+    import torch
+    import sandbox_reactions_amd as sr
+    print("net layers 64x48", flush=True)
+    w, h = 64, 48
+    p = random_params(1)
+    net = sr.Net(w, h, p)
+    own, kx, ky = system(64, 2)
+    own, kx, ky = own[:h, :w].copy(), kx[:h, :w + 1].copy(), ky[:h + 1, :w].copy()
+    kx[:, w] = 0.0
+    ky[h, :] = 0.0
+    r = np.random.default_rng(3).normal(size=(h, w)).astype(np.float32)
+    t_own, t_kx, t_ky, t_r = cuda(own, kx, ky, r)
+    z = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    net.apply(t_own, t_kx, t_ky, t_r, z)
+    torch.cuda.synchronize()
+    A = {k: net.activation(k, w, h).astype(np.float64) for k in
+         ["x", "e1a", "e1", "p1", "e2a", "e2", "p2", "e3a", "e3", "d2a", "d2", "d1a", "d1"]}
+    # features
+    xo, s, dis = unet.features(own.astype(np.float64), kx.astype(np.float64), ky.astype(np.float64),
+                               r.astype(np.float64))
+    xo[:, :, 1] = np.clip(xo[:, :, 1], -20, 20)
+    assert np.abs(A["x"][:, :, :3] - xo).max() <= 2 ** -7 * np.abs(xo).max(axis=(0, 1)).max()
+    assert np.all(A["x"][:, :, 3:] == 0.0)
+    W = {k: bf16(v) for k, v in p.items() if k.endswith(".w") and not k.startswith("out")}
+    B = {k: np.asarray(v, np.float64) for k, v in p.items() if k.endswith(".b")}
+
+    def layer(name, inp):
+        return unet.relu(unet.conv3(inp, W[name + ".w"], B[name + ".b"]))
+
+    checks = [("e1a", A["x"][:, :, :3]), ("e1", A["e1a"]), ("e2a", A["p1"]), ("e2", A["e2a"]), ("e3a", A["p2"]),
+              ("e3", A["e3a"]), ("d2a", np.concatenate([unet.up(A["e3"]), A["e2"]], axis=2)), ("d2", A["d2a"]),
+              ("d1a", np.concatenate([unet.up(A["d2"]), A["e1"]], axis=2)), ("d1", A["d1a"])]
+    for name, inp in checks:
+        ref = layer(name, inp)
+        err = np.abs(A[name] - ref).max() / max(np.abs(ref).max(), 1e-30)
+        assert err <= 2 ** -7, (name, err)
+    assert close(A["p1"], unet.pool(A["e1"]), 2 ** -7)
+    assert close(A["p2"], unet.pool(A["e2"]), 2 ** -7)
+    zref = (A["d1"] @ p["out.w"].astype(np.float64) + p["out.b"][0]) * s * dis
+    assert close(z.cpu().numpy(), zref, 1e-4)
+
+
+def test_forward_matches_the_f64_network(gpu_lib):
+    import torch
+    import sandbox_reactions_amd as sr
+    print("net forward 64x64", flush=True)
+    n = 64
+    p = random_params(4)
+    net = sr.Net(n, n, p)
+    own, kx, ky = system(n, 5)
+    r = np.random.default_rng(6).normal(size=(n, n)).astype(np.float32)
+    t = cuda(own, kx, ky, r)
+    z = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    net.apply(*t, z)
+    pd = {k: np.asarray(v, np.float64) for k, v in p.items()}
+    zref = unet.apply(pd, own.astype(np.float64), kx.astype(np.float64), ky.astype(np.float64),
+                      r.astype(np.float64))
+    rel = np.linalg.norm(z.cpu().numpy() - zref) / np.linalg.norm(zref)
+    assert rel <= 3e-2, rel
+
+
+def dense_solve(own, kx, ky, r):
+    """e = A^-1 r by a sparse direct solve of the face-form operator (interior faces)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    h, w = own.shape
+    idx = np.arange(w * h).reshape(h, w)
+    kxi, kyi = kx[:, 1:w], ky[1:h, :]
+    a, b = idx[:, :-1].ravel(), idx[:, 1:].ravel()
+    c, d = idx[:-1, :].ravel(), idx[1:, :].ravel()
+    rows = np.concatenate([a, b, c, d])
+    cols = np.concatenate([b, a, d, c])
+    vals = -np.concatenate([kxi.ravel(), kxi.ravel(), kyi.ravel(), kyi.ravel()])
+    diag = own.ravel().astype(np.float64).copy()
+    np.add.at(diag, a, kxi.ravel())
+    np.add.at(diag, b, kxi.ravel())
+    np.add.at(diag, c, kyi.ravel())
+    np.add.at(diag, d, kyi.ravel())
+    A = sp.coo_matrix((vals, (rows, cols)), shape=(w * h, w * h)).tocsr() + sp.diags(diag)
+    return spla.spsolve(A.tocsc(), r.ravel()).reshape(h, w)
+
+
+def test_gradients_match_the_oracle(gpu_lib):
+    import sandbox_reactions_amd as sr
+    from sandbox_reactions_amd import net as N
+    print("net gradients 32x32", flush=True)
+    n = 32
+    p = random_params(7)
+    net = sr.Net(n, n, p)
+    own, kx, ky = system(n, 8, contrast=1.0)
+    r = np.random.default_rng(9).normal(size=(n, n)).astype(np.float32)
+    o64, kx64, ky64, r64 = (a.astype(np.float64) for a in (own, kx, ky, r))
+    e = dense_solve(o64, kx64, ky64, r64).astype(np.float32)
+    loss = net.loss_grad(*cuda(own, kx, ky, r, e))
+    g = N.unflatten(net.grads())
+    pd = {k: bf16(v) if (k.endswith(".w") and not k.startswith("out")) else np.asarray(v, np.float64)
+          for k, v in p.items()}
+    lref, gref = unet.energy_loss(pd, o64, kx64, ky64, r64, e.astype(np.float64))
+    assert abs(loss - lref) <= 2e-2 * lref, (loss, lref)
+    for k in gref:
+        a, b = g[k].astype(np.float64).ravel(), gref[k].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+        assert rel <= 0.15 and cos >= 0.99, (k, rel, cos)
+
+
+def test_adam_matches_the_algorithm(gpu_lib):
+    import sandbox_reactions_amd as sr
+    print("net adam", flush=True)
+    n = 32
+    p = random_params(10)
+    net = sr.Net(n, n, p)
+    own, kx, ky = system(n, 11, contrast=1.0)
+    r = np.random.default_rng(12).normal(size=(n, n)).astype(np.float32)
+    e = dense_solve(*(a.astype(np.float64) for a in (own, kx, ky, r))).astype(np.float32)
+    t = cuda(own, kx, ky, r, e)
+    p0 = net.params().astype(np.float64)
+    net.loss_grad(*t)
+    g = net.grads().astype(np.float64)
+    net.adam(1e-3)
+    p1 = net.params().astype(np.float64)
+    # first Adam step: the bias-corrected moments are g and g^2, so each parameter moves by lr g / (|g| + eps)
+    expect = p0 - 1e-3 * g / (np.abs(g) + 1e-8)
+    assert np.allclose(p1, expect, atol=1e-6)
+
+
+def test_training_lowers_the_loss(gpu_lib):
+    import sandbox_reactions_amd as sr
+    print("net training 64x64", flush=True)
+    n = 64
+    net = sr.Net(n, n)
+    own, kx, ky = system(n, 13)
+    rng = np.random.default_rng(14)
+    samples = []
+    o64, kx64, ky64 = (a.astype(np.float64) for a in (own, kx, ky))
+    for _ in range(4):
+        r = rng.normal(size=(n, n)).astype(np.float32)
+        samples.append(cuda(own, kx, ky, r, dense_solve(o64, kx64, ky64, r.astype(np.float64))))
+    first = np.mean([net.loss_grad(*s) for s in samples])
+    for it in range(150):
+        net.loss_grad(*samples[it % 4])
+        net.adam(2e-3)
+    last = np.mean([net.loss_grad(*s) for s in samples])
+    assert first > 0.9 and last < 0.6 * first, (first, last)
+
+
+def test_learned_fcg_converges_and_falls_back(gpu_lib):
+    import torch
+    import sandbox_reactions_amd as sr
+    print("learned FCG 128x128", flush=True)
+    n = 128
+    own, kx, ky = system(n, 15)
+    own *= 1e-2
+    b = np.random.default_rng(16).normal(size=(n, n))
+    f = sr.Fld(n, n)
+    f.set_system(*cuda(own, kx, ky))
+    bt = torch.tensor(b, device="cuda")
+    xm = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    it_m, rel_m = f.solve(bt, xm, tol=1e-8)
+    net = sr.Net(n, n, random_params(17))
+    with pytest.raises(sr.SrError):
+        f.solve(bt, torch.zeros_like(xm), tol=1e-8, precond=sr.PRECOND_LEARNED)
+    f.set_network(net)
+    xl = torch.zeros_like(xm)
+    it_l, rel_l, fb = f.solve_stats(bt, xl, tol=1e-8, precond=sr.PRECOND_LEARNED)
+    assert rel_l <= 1e-8 and not fb
+    assert torch.linalg.norm(xl - xm) <= 1e-6 * torch.linalg.norm(xm)
+    f.set_network(net, fallback_iterations=2)
+    xf = torch.zeros_like(xm)
+    it_f, rel_f, fb = f.solve_stats(bt, xf, tol=1e-8, precond=sr.PRECOND_LEARNED)
+    assert fb and rel_f <= 1e-8
+    assert torch.linalg.norm(xf - xm) <= 1e-6 * torch.linalg.norm(xm)
