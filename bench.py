@@ -65,44 +65,89 @@ def full_step(sr, scenes, n, steps, warmup):
 
 
 def fld_solve(sr, scenes, n):
+    """The stiff FLD system alone: plain MG-PCG, then the learned preconditioner (the committed trained network)
+    with its fallback to plain multigrid, then the learned preconditioner without fallback (one solve, for the
+    record), and one network application timed alone."""
     import torch
+    from sandbox_reactions_amd import train
     log(f"FLD system {n}^2: building")
     c, kx, ky, b, dt = scenes.fld_system(n)
     t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    own, tkx, tky = t(c), t(kx), t(ky)
     f = sr.Fld(n, n)
-    f.set_system(t(c), t(kx), t(ky))
+    f.set_system(own, tkx, tky)
     bt = torch.tensor(b, device="cuda")
-    times, its, rels = [], [], []
-    for rep in range(4):
-        x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        it, rel = f.solve(bt, x, tol=1e-8)
-        torch.cuda.synchronize()
-        if rep > 0:
-            times.append(time.perf_counter() - t0)
-            its.append(it)
-            rels.append(rel)
-    log(f"FLD solve {n}^2: {np.median(times) * 1e3:.2f} ms, {its[0]} iterations, rel {rels[0]:.2e}, "
-        f"{f.levels} levels")
-    return dict(ms=float(np.median(times) * 1e3), iterations=int(its[0]), rel=float(max(rels)), levels=f.levels)
+
+    def timed(precond, reps):
+        out = []
+        for rep in range(reps):
+            x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            it, rel, li, fb = f.solve_stats(bt, x, tol=1e-8, precond=precond)
+            torch.cuda.synchronize()
+            out.append((time.perf_counter() - t0, it, rel, li, fb))
+        return out
+
+    plain = timed(sr.PRECOND_MULTIGRID, 4)[1:]
+    ms = float(np.median([p[0] for p in plain]) * 1e3)
+    res = dict(ms=ms, iterations=int(plain[0][1]), rel=float(max(p[2] for p in plain)), levels=f.levels)
+    log(f"FLD solve {n}^2: {ms:.2f} ms, {plain[0][1]} iterations, rel {res['rel']:.2e}, {f.levels} levels")
+    if not os.path.exists(train.DEFAULT_WEIGHTS):
+        log("no trained network: the learned preconditioner is not measured")
+        return res
+    net = sr.Net(n, n, train.load())
+    f.set_network(net, fallback_iterations=plain[0][1])
+    learned = timed(sr.PRECOND_LEARNED, 3)[1:]
+    f.set_network(net, fallback_iterations=0)
+    pure = timed(sr.PRECOND_LEARNED, 1)[0] if plain[0][1] > 0 else None
+    r = torch.randn((n, n), device="cuda")
+    z = torch.empty_like(r)
+    net.apply(own, tkx, tky, r, z)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        net.apply(own, tkx, tky, r, z)
+    e1.record()
+    torch.cuda.synchronize()
+    apply_ms = e0.elapsed_time(e1) / 5
+    lms = float(np.median([p[0] for p in learned]) * 1e3)
+    res.update(learned_ms=lms, learned_iterations=int(learned[0][1]), learned_net_iterations=int(learned[0][3]),
+               learned_fell_back=bool(learned[0][4]), learned_rel=float(max(p[2] for p in learned)),
+               net_apply_ms=apply_ms,
+               learned_without_fallback={"ms": pure[0] * 1e3, "iterations": pure[1], "rel": pure[2]}
+               if pure else None)
+    log(f"FLD learned {n}^2: {lms:.2f} ms, {learned[0][1]} iterations ({learned[0][3]} with the network, "
+        f"fell back {learned[0][4]}); without fallback {pure[1]} iterations in {pure[0] * 1e3:.1f} ms; "
+        f"network apply {apply_ms:.3f} ms")
+    return res
 
 
 def copy_bandwidth(mib=512, reps=10):
-    """A plain device-to-device copy of a known size: the achievable HBM rate on this box (read + write bytes / s),
-    and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    """Device copies of a known size (libsrcalib.so: one float per lane, one float4 per lane) and the runtime's
+    own copy: the achievable HBM rates on this box (read + write bytes / s), and the calibration kernels of the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
     import torch
+    from sandbox_reactions_amd import build
+    cal = ctypes.CDLL(build.CALIB_LIB)
+    cal.sr_calib_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
     src = torch.ones(mib * 2**20 // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    return 2 * src.numel() * 4 / (ms * 1e-3) / 1e9
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for name, fn in [("copy_float", lambda: cal.sr_calib_copy(src.data_ptr(), dst.data_ptr(), src.numel(), 1, stream)),
+                     ("copy_float4", lambda: cal.sr_calib_copy(src.data_ptr(), dst.data_ptr(), src.numel(), 4, stream)),
+                     ("copy_runtime", lambda: dst.copy_(src))]:
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = round(2 * src.numel() * 4 / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9, 1)
+    return out
 
 
 def measured_traffic(kernel, n):
@@ -128,7 +173,7 @@ def cpu_baseline(scenes, budget_s=12.0):
     twin = oracle.Twin(ph, planes, gravity=True, hydro=True, heat=True)
     steps = 0
     t0 = time.perf_counter()
-    while steps < 1 or (time.perf_counter() - t0 < budget_s and steps < 50):
+    while steps < 1 or time.perf_counter() - t0 < budget_s:
         twin.step()
         steps += 1
     el = time.perf_counter() - t0
@@ -157,7 +202,7 @@ def main():
     achieved = bytes_per / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else 0.0
     fld = None if args.no_fld else fld_solve(sr, scenes, args.fld_n)
     copy_gbs = copy_bandwidth()
-    log(f"device copy {copy_gbs:.0f} GB/s")
+    log(f"device copies (GB/s): {copy_gbs}")
     cpu = None if args.no_cpu else cpu_baseline(scenes)
     cost = fs["cost"]
     out = {
@@ -185,13 +230,19 @@ def main():
             "fld_solve_2048": None if fld is None else {
                 "workload": f"stiff FLD solve alone, {args.fld_n}^2, diffusion number 1e3, MG-PCG V(2,2) to 1e-8",
                 "plain_mg_pcg_ms": round(fld["ms"], 3), "iterations": fld["iterations"], "final_rel": fld["rel"],
-                "levels": fld["levels"], "learned_ms": None},
+                "levels": fld["levels"], "learned_ms": fld.get("learned_ms"),
+                "learned_vs_plain": round(fld["ms"] / fld["learned_ms"], 4) if fld.get("learned_ms") else None,
+                "learned_iterations": fld.get("learned_iterations"),
+                "learned_net_iterations": fld.get("learned_net_iterations"),
+                "learned_fell_back": fld.get("learned_fell_back"),
+                "learned_without_fallback": fld.get("learned_without_fallback"),
+                "net_apply_ms": fld.get("net_apply_ms")},
         },
         "roofline": {"bound": "hbm", "kernel": "hydro_update", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("sr::k_hydro<1>", n), "launches": launches,
                      "mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": bytes_per,
-                     "device_copy_gbs": round(copy_gbs, 1)},
+                     "device_copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -140,13 +140,16 @@ int sr_fld_matvec(sr_fld *f, const double *v_dev, double *out_dev, void *stream)
 int sr_fld_levels(sr_fld *f, int *levels);
 void sr_fld_destroy(sr_fld *f);
 /* The learned preconditioner (SR_PRECOND_LEARNED): z = V r + N(r - A V r), V the multigrid V-cycle and N the
- * network; after fallback_iterations iterations without convergence (0: never) the solve continues with plain
- * multigrid from where it stands. No counterpart in the reference (its set_heat_precond, step/heat.rs:595, offers
+ * network. With fallback_iterations > 0 the solve continues with plain multigrid from where it stands once the
+ * learned preconditioner has used that many iterations, or earlier when after 8 iterations it converges more slowly
+ * per iteration than the last plain multigrid solve of the same system; 0: never. No counterpart in the reference (its set_heat_precond, step/heat.rs:595, offers
  * Jacobi and multigrid); n may be NULL to detach. Grid sides must be multiples of 4. */
 int sr_fld_set_network(sr_fld *f, sr_net *n, int fallback_iterations);
-/* sr_fld_solve, also reporting whether the learned preconditioner fell back to plain multigrid */
+/* sr_fld_solve, also reporting the iterations made with the learned preconditioner and whether it gave way to plain
+ * multigrid (after fallback_iterations, or when after 8 iterations it converges more slowly per iteration than the
+ * last plain multigrid solve of the same system did) */
 int sr_fld_solve_stats(sr_fld *f, const double *b_dev, double *x_dev, double tol, int max_iter, int precond,
-                       int *iterations, double *final_rel, int *fell_back, void *stream);
+                       int *iterations, double *final_rel, int *learned_iterations, int *fell_back, void *stream);
 
 /* ----------------------------------------------------------------------------- the learned preconditioner's network */
 /* A 3-level U-Net (16/32/64 channels, 3x3 convolutions, BF16 activations and weights, FP32 accumulation on the

@@ -94,7 +94,8 @@ SIGNATURES = [
     ("sr_destroy", None, [_vp]),
     ("sr_kernel_time", _i, [_vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_d), C.POINTER(_d)]),
     ("sr_fld_set_network", _i, [_vp, _vp, _i]),
-    ("sr_fld_solve_stats", _i, [_vp, _vp, _vp, _d, _i, _i, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i), _vp]),
+    ("sr_fld_solve_stats", _i, [_vp, _vp, _vp, _d, _i, _i, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i),
+                                C.POINTER(_i), _vp]),
     ("sr_set_network", _i, [_vp, _vp, _i]),
     ("sr_net_create", _i, [_i, _i, _pp]),
     ("sr_net_param_count", _i, [_vp, C.POINTER(_sz)]),

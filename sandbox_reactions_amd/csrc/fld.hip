@@ -756,6 +756,7 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_) {
 }
 
 int FldSolver::set_system(const float *own, const float *ax, const float *ay, const uint8_t *g, hipStream_t s) {
+    plain_rate = 0.0;
     const size_t n = (size_t)w0 * h0;
     SR_HIP(hipMemcpyAsync(lv[0].own, own, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     SR_HIP(hipMemcpyAsync(lv[0].ax, ax, sizeof(float) * (w0 + 1) * h0, hipMemcpyDeviceToDevice, s));
@@ -977,7 +978,8 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         return SR_OK;
     };
     const double inner_rtol = 1e-4;
-    int it = 0;
+    int it = 0, net_it = 0;
+    double rel0 = -1.0, rr_net0 = 0.0;
     double *dots[2] = {part_dots, part_dots + 2 * nbd_max};
     // rows per matvec block: about 2048 blocks, 1 to 16 rows each
     const int gxm = (w0 + NT - 1) / NT;
@@ -992,6 +994,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         const double rr = tr[0], bb = tr[1] > 0 ? tr[1] : 1.0;
         st->rel = std::sqrt(rr / bb);
         st->refinements = outer;
+        if (rel0 < 0.0) rel0 = st->rel;
         if (!std::isfinite(rr) || (ref && !(std::isfinite(tr[2]) && std::isfinite(tr[3])))) {
             st->finite = false;
             break;
@@ -1014,12 +1017,14 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
         int last_check = it, interval = 1;
         double rr_last = rr;
+        if (use_net && net_it == 0) rr_net0 = rr;
         while (it < max_iter) {
             hipLaunchKernelGGL(k_matvec_p, mv_grid, dim3(NT), 0, s, a, zc, pold, pnew, q, dots[k & 1],
                                dots[(k + 1) & 1], nbd, nbp, k == 0 ? 1 : 0, mv_rows, part_pq);
             hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, x, r32, dots[k & 1], nbd,
                                part_pq, nmv, part_rr);
             it++;
+            if (use_net) net_it++;
             if (it - last_check >= interval || it >= max_iter) {
                 double rri;
                 SR_TRY(read_partials(part_rr, 1, nbe, &rri, s));
@@ -1028,8 +1033,13 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                     break;
                 }
                 if (rri <= stop2) break;
-                if (use_net && fallback_it > 0 && it >= fallback_it) {
-                    // the learned preconditioner has used plain multigrid's budget: restart with plain multigrid
+                // with a fallback budget, the learned preconditioner gives way to plain multigrid once it has used
+                // the budget, or when after 8 iterations it reduces the residual more slowly per iteration than
+                // plain multigrid did on its last solve of this system
+                const bool slower = fallback_it > 0 && plain_rate < 0.0 && net_it >= 8 &&
+                                    0.5 * std::log(std::max(rri, 1e-300) / std::max(rr_net0, 1e-300)) / net_it >
+                                        plain_rate;
+                if (use_net && ((fallback_it > 0 && it >= fallback_it) || slower)) {
                     use_net = false;
                     st->fell_back = true;
                     break;
@@ -1051,6 +1061,9 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         if (!st->finite) break;
     }
     st->iterations = it;
+    st->learned_iterations = net_it;
+    if (precond == SR_PRECOND_MULTIGRID && st->converged && it > 0 && rel0 > 0.0 && st->rel > 0.0)
+        plain_rate = std::log(st->rel / rel0) / it;
     SR_HIP(hipGetLastError());
     return SR_OK;
 }
@@ -1125,12 +1138,14 @@ extern "C" int sr_fld_set_network(sr_fld *f, sr_net *n, int fallback_iterations)
 }
 
 extern "C" int sr_fld_solve_stats(sr_fld *f, const double *b_dev, double *x_dev, double tol, int max_iter,
-                                  int precond, int *iterations, double *final_rel, int *fell_back, void *stream) {
+                                  int precond, int *iterations, double *final_rel, int *learned_iterations,
+                                  int *fell_back, void *stream) {
     if (!f || !b_dev || !x_dev) return fail(SR_ERR_ARGUMENT, "sr_fld_solve_stats: null pointer");
     FldStats st;
     SR_TRY(f->s.solve(b_dev, x_dev, tol, max_iter, precond, nullptr, &st, (hipStream_t)stream));
     if (iterations) *iterations = st.iterations;
     if (final_rel) *final_rel = st.rel;
+    if (learned_iterations) *learned_iterations = st.learned_iterations;
     if (fell_back) *fell_back = st.fell_back ? 1 : 0;
     if (!st.finite) return fail(SR_ERR_NON_FINITE, "sr_fld_solve: a sum left the float's range");
     return SR_OK;

@@ -21,6 +21,7 @@ struct FldStats {
     bool converged = false;
     bool finite = true;
     bool fell_back = false;  // the learned preconditioner gave way to plain multigrid
+    int learned_iterations = 0;
 };
 
 // The reference's energy-weighted stopping rule (crates/sr-physics/src/reference/heat.rs:424-431): r^T D^-1 r <=
@@ -78,6 +79,7 @@ class FldSolver {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Net *net = nullptr;
     int fallback_it = 0;
+    double plain_rate = 0.0;  // ln(residual reduction) per iteration of the last converged plain multigrid solve
     int vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s);
     int learned(const float *r, float *z, const float *zold, double *dots, hipStream_t s);
     int read_partials(const double *part, int k, int nb, double *out, hipStream_t s);

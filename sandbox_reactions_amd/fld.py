@@ -43,13 +43,14 @@ class Fld:
         return it.value, rel.value
 
     def solve_stats(self, b, x, tol=1e-8, max_iter=10000, precond=_lib.PRECOND_MULTIGRID, stream=None):
-        """As solve(), also returning whether the learned preconditioner fell back to plain multigrid."""
+        """As solve(): (iterations, relative residual, iterations with the learned preconditioner, whether it gave
+        way to plain multigrid)."""
         from . import _check, lib
-        it, rel, fb = C.c_int(), C.c_double(), C.c_int()
+        it, rel, li, fb = C.c_int(), C.c_double(), C.c_int(), C.c_int()
         _check(lib().sr_fld_solve_stats(self._p, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), tol,
-                                        max_iter, precond, C.byref(it), C.byref(rel), C.byref(fb),
+                                        max_iter, precond, C.byref(it), C.byref(rel), C.byref(li), C.byref(fb),
                                         self._s(stream)))
-        return it.value, rel.value, bool(fb.value)
+        return it.value, rel.value, li.value, bool(fb.value)
 
     def set_network(self, net, fallback_iterations: int = 0):
         """The network of the learned preconditioner (None detaches it)."""

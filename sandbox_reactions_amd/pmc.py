@@ -1,8 +1,8 @@
 """HBM traffic per kernel launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).
 
-Each counter is scaled by what it reads for a device copy of known size (the 512 MiB copy `bench.py` makes: the
-largest `copyBuffer` dispatch reads and writes 512 MiB), so the figures are bytes as the copy kernel would count
-them. Usage (after the two runs, on any machine):
+Each counter is scaled by what it reads for a device copy of known size: the 512 MiB copy `bench.py` makes with
+one float per lane (k_calib_copy1, the access pattern of the stencil kernels), which reads and writes 512 MiB. The
+float4 and runtime copies are reported beside it (wide requests can count differently). Usage (after the two runs, on any machine):
 
     python -m sandbox_reactions_amd.pmc FETCH_DIR WRITE_DIR OUT_JSON [--workload TEXT] [--command TEXT]
 
@@ -17,6 +17,8 @@ import csv
 import glob
 import json
 import os
+
+import numpy as np
 
 COPY_BYTES = 512 * 2**20
 
@@ -43,23 +45,28 @@ def per_kernel(rows):
     return agg
 
 
-def copy_reading(agg):
-    """The counter's bytes for the largest copyBuffer dispatch (the 512 MiB calibration copy)."""
-    cands = [(g, v) for (k, g), v in agg.items() if "copyBuffer" in k]
-    if not cands:
-        raise ValueError("no copyBuffer dispatch in the counter run: the calibration copy is missing")
-    g, v = max(cands, key=lambda c: c[0])
-    return max(v)
+def copy_reading(agg, kernel="k_calib_copy1"):
+    """The counter's bytes for the 512 MiB calibration copy (median over its dispatches)."""
+    vals = [x for (k, g), v in agg.items() if k.endswith(kernel) for x in v]
+    if not vals:
+        raise ValueError(f"no {kernel} dispatch in the counter run: the calibration copy is missing")
+    return float(np.median(vals))
 
 
 def summarize(fetch_dir, write_dir):
     fa, wa = per_kernel(read_counters(fetch_dir)), per_kernel(read_counters(write_dir))
     sf = COPY_BYTES / copy_reading(fa)
     sw = COPY_BYTES / copy_reading(wa)
+    others = {}
+    for k in ("k_calib_copy4", "copyBuffer"):
+        try:
+            others[k] = {"fetch": COPY_BYTES / copy_reading(fa, k), "write": COPY_BYTES / copy_reading(wa, k)}
+        except ValueError:
+            pass
     kernels = {}
     for key in fa.keys() & wa.keys():
         name, grid = key
-        if "copyBuffer" in name or "fillBuffer" in name:
+        if "copyBuffer" in name or "fillBuffer" in name or "k_calib" in name:
             continue
         f = sum(fa[key]) / len(fa[key]) * sf
         w = sum(wa[key]) / len(wa[key]) * sw
@@ -67,7 +74,7 @@ def summarize(fetch_dir, write_dir):
         if cur is None or (f + w) * len(fa[key]) > cur["traffic_bytes"] * cur["launches"]:
             kernels[name] = {"grid": grid, "launches": len(fa[key]), "fetch_bytes": f, "write_bytes": w,
                              "traffic_bytes": f + w}
-    return {"copy_scale": {"fetch": sf, "write": sw}, "kernels": kernels}
+    return {"copy_scale": {"fetch": sf, "write": sw}, "other_copy_scales": others, "kernels": kernels}
 
 
 def main():
@@ -86,8 +93,9 @@ def main():
     md = os.path.splitext(a.out_json)[0] + ".md"
     with open(md, "w") as f:
         f.write(f"# HBM traffic per launch: {a.workload}\n\nCommands: `{a.command}`\n\n")
-        f.write(f"Counters scaled by the 512 MiB calibration copy: FETCH x{s['copy_scale']['fetch']:.4f}, "
-                f"WRITE x{s['copy_scale']['write']:.4f}.\n\n")
+        f.write(f"Counters scaled by the 512 MiB one-float-per-lane calibration copy: FETCH "
+                f"x{s['copy_scale']['fetch']:.4f}, WRITE x{s['copy_scale']['write']:.4f} (the same counters over other "
+                f"copies of 512 MiB would need: {json.dumps(s['other_copy_scales'])}).\n\n")
         f.write("| kernel | grid | launches | fetch MB | write MB | traffic MB |\n|---|---|---|---|---|---|\n")
         for k, v in sorted(s["kernels"].items(), key=lambda kv: -kv[1]["traffic_bytes"] * kv[1]["launches"]):
             f.write(f"| {k} | {v['grid']} | {v['launches']} | {v['fetch_bytes'] / 1e6:.2f} | "

@@ -216,11 +216,49 @@ def test_learned_fcg_converges_and_falls_back(gpu_lib):
         f.solve(bt, torch.zeros_like(xm), tol=1e-8, precond=sr.PRECOND_LEARNED)
     f.set_network(net)
     xl = torch.zeros_like(xm)
-    it_l, rel_l, fb = f.solve_stats(bt, xl, tol=1e-8, precond=sr.PRECOND_LEARNED)
-    assert rel_l <= 1e-8 and not fb
+    it_l, rel_l, li, fb = f.solve_stats(bt, xl, tol=1e-8, precond=sr.PRECOND_LEARNED)
+    assert rel_l <= 1e-8 and (fb or li == it_l)
     assert torch.linalg.norm(xl - xm) <= 1e-6 * torch.linalg.norm(xm)
     f.set_network(net, fallback_iterations=2)
     xf = torch.zeros_like(xm)
-    it_f, rel_f, fb = f.solve_stats(bt, xf, tol=1e-8, precond=sr.PRECOND_LEARNED)
-    assert fb and rel_f <= 1e-8
+    it_f, rel_f, li, fb = f.solve_stats(bt, xf, tol=1e-8, precond=sr.PRECOND_LEARNED)
+    assert fb and li <= 8 and rel_f <= 1e-8
     assert torch.linalg.norm(xf - xm) <= 1e-6 * torch.linalg.norm(xm)
+
+
+def test_trained_network_gains_over_the_v_cycle_alone(gpu_lib):
+    """The committed network, trained on 256^2 / 512^2 systems, on a held-out 256^2 system drawn the same way: the
+    A-norm of what one V-cycle leaves, after the network's correction, against the V-cycle alone (1 by definition).
+    Inside flexible CG the correction does not pay (the iteration counts are printed, not asserted); the solve with
+    fallback converges within a few iterations of plain multigrid."""
+    import torch
+    import sandbox_reactions_amd as sr
+    from sandbox_reactions_amd import scenes, train
+    print("trained network 256x256", flush=True)
+    n = 256
+    net = sr.Net(n, n, train.load())
+    own, kx, ky = train.system_tensors(n, scenes.SEED_BASE + 777)
+    f = sr.Fld(n, n)
+    f.set_system(own, kx, ky)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    losses = []
+    for cg in (0, 5, 10, 20, 30, 40):
+        r1, e1 = train.make_sample(f, own, kx, ky, gen, tol=1e-8, cg_steps=cg)
+        losses.append(net.loss_grad(own, kx, ky, r1, e1))
+    print(f"error energy left by V + network / V alone: {np.round(losses, 4)}", flush=True)
+    assert np.mean(losses) <= 0.95, losses
+    c, kx64, ky64, b, dt = scenes.fld_system(n, seed=scenes.SEED_BASE + 777)
+    bt = torch.tensor(b, device="cuda")
+    x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    it_p, rel_p, _, _ = f.solve_stats(bt, x, tol=1e-8)
+    f.set_network(net, fallback_iterations=it_p)
+    x2 = torch.zeros_like(x)
+    it_l, rel_l, li, fb = f.solve_stats(bt, x2, tol=1e-8, precond=sr.PRECOND_LEARNED)
+    f.set_network(net, fallback_iterations=0)
+    x3 = torch.zeros_like(x)
+    it_n, rel_n, li_n, fb_n = f.solve_stats(bt, x3, tol=1e-8, precond=sr.PRECOND_LEARNED)
+    print(f"iterations: plain {it_p}, learned with fallback {it_l} ({li} with the network, fell back {fb}), "
+          f"learned without fallback {it_n}", flush=True)
+    assert rel_l <= 1e-8 and rel_n <= 1e-8 and not fb_n and li_n == it_n
+    assert it_l <= it_p + 20
