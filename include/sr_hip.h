@@ -74,6 +74,10 @@ typedef struct {
     int max_rounds;        /* reference 4096 */
     int levels;            /* multigrid levels, 0 = down to 4x4 */
     int nu1, nu2;          /* red-black Gauss-Seidel sweeps before / after the coarse correction */
+    double forcing;        /* 0: every Newton pass solved to solve_tol (the reference's rule). > 0 with
+                            * SR_STOP_RELATIVE: inexact Newton, a pass solved to forcing x the previous pass's
+                            * relative heat change (at most 1e-2, at least solve_tol); the pass that meets the Newton
+                            * criteria after a loose solve is followed by one solved to solve_tol */
 } sr_heat_settings;
 
 typedef struct {

@@ -364,6 +364,9 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     int passes = 0, rounds = 0;
     bool converged = false, faces = true;
     double last_rel = 0.0;
+    // inexact Newton (relative stopping only): loose early passes, the last one to solve_tol
+    const bool inexact = hs.forcing > 0.0 && hs.stop == SR_STOP_RELATIVE;
+    double eta = inexact ? std::max(hs.solve_tol, 1e-2) : hs.solve_tol;
     while (passes < hs.max_passes && rounds < hs.max_rounds) {
         passes++;
         hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
@@ -381,7 +384,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
             break;
         }
         FldStats fs;
-        SR_TRY(solver.solve(rhs, xs, hs.solve_tol, hs.max_rounds - rounds, precond,
+        SR_TRY(solver.solve(rhs, xs, eta, hs.max_rounds - rounds, precond,
                             hs.stop == SR_STOP_REFERENCE ? &ref : nullptr, &fs, s));
         rounds += fs.iterations;
         last_rel = fs.rel;
@@ -390,9 +393,14 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
         double mv[2];
         SR_TRY(read_sums(2, mv, s));
         if (mv[0] <= hs.newton_heat * heat0 && mv[1] == 0.0) {
+            if (eta > hs.solve_tol) {  // converged on a loose solve: one more pass, solved to solve_tol
+                eta = hs.solve_tol;
+                continue;
+            }
             converged = true;
             break;
         }
+        if (inexact) eta = std::min(1e-2, std::max(hs.solve_tol, hs.forcing * mv[0] / heat0));
     }
     report.passes = passes;
     report.rounds = rounds;

@@ -30,6 +30,9 @@ namespace sr {
 #define NSLOT (T * T + 4 * T) // 320
 #define NS 10
 #define HOT_FACE 2.0f
+#ifndef SR_HYDRO_SB
+#define SR_HYDRO_SB 1 // species per pass of the species phase (1, 2 or 5)
+#endif
 
 // (Z + 1) / A per species, assets/elements.json order: H He C O Ne Mg Si S Fe n (registry.rs:146-154)
 __constant__ float c_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f / 20.0f,
@@ -472,7 +475,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
             float ax = 0.0f, ay = 0.0f;
             if (!vacuum) {
                 float im = L.im[lc];
-                float kap = P.kappa0 * (1.0f + in.x[ci]);
+                float kap = P.kappa0 * (1.0f + L.x[0][lc]);
                 if (P.opacity == 1 || P.kappa_dust != 0.0f) {
                     const float s = (float)sig0, mx = in.mx[ci], my = in.my[ci];
                     const float raw = in.e[ci] / s - 0.5f * (mx * mx + my * my) / (s * s);
@@ -480,7 +483,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                     if (P.opacity == 1) {
                         kap = P.kappa0 * s * powf(t, -3.5f);
                     } else if (t < P.t_dust) {
-                        const float zm = 1.0f - in.x[ci] - in.x[n + ci] - in.x[9 * n + ci];
+                        const float zm = 1.0f - L.x[0][lc] - L.x[1][lc] - L.x[9][lc];
                         kap += P.kappa_dust * zm;
                     }
                 }
@@ -538,10 +541,15 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
         }
     }
 
-    // ---- phase 4: the species, one at a time (flux = mass flux x the upwind face's fraction)
-    for (int s = 0; s < NS; s++) {
+    // ---- phase 4: the species, SB at a time in the freed face buffer (flux = mass flux x the upwind face's
+    // fraction)
+    constexpr int SB = SR_HYDRO_SB;
+    static_assert(NS % SB == 0 && SB <= 5, "species batches fill the face buffer's five fields");
+    for (int s0 = 0; s0 < NS; s0 += SB) {
         __syncthreads(); // the face buffer is reused
-        {
+#pragma unroll
+        for (int j = 0; j < SB; j++) {
+            const int s = s0 + j;
             const int offs[2] = {1, PW};
             float xf[4];
             const float xc = L.x[s][lc];
@@ -551,37 +559,44 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 xf[2 * ax + 1] = xc + 0.5f * theta[ax] * central;
             }
             const float ds = hdx * ((f0_w[0] * xf[0] - f0_w[1] * xf[1]) + (f0_w[2] * xf[2] - f0_w[3] * xf[3]));
-            for (int k = 0; k < 4; k++) L.face[0][k][tid] = s_e[k] > 0.0f ? (fs_w[k] * xf[k] + ds) / s_e[k] : xf[k];
+            for (int k = 0; k < 4; k++) L.face[j][k][tid] = s_e[k] > 0.0f ? (fs_w[k] * xf[k] + ds) / s_e[k] : xf[k];
         }
         __syncthreads();
         if (live) {
-            const size_t pi = (size_t)s * n + ci;
-            if (empty) {
-                out.x[pi] = in.x[pi];
-                continue;
-            }
-            float fsx[4];
-            for (int k = 0; k < 4; k++) {
-                if (wallf[k]) {
-                    fsx[k] = 0.0f;
+#pragma unroll
+            for (int j = 0; j < SB; j++) {
+                const int s = s0 + j;
+                const size_t pi = (size_t)s * n + ci;
+                const float xs0 = L.x[s][lc];  // the cell's own fraction, staged in phase 0
+                if (empty) {
+                    out.x[pi] = xs0;
                     continue;
                 }
-                const int dir = k >> 1;
-                const bool high = k & 1;
-                const int nx = sx_ + (dir == 0 ? (high ? 1 : -1) : 0), ny = sy_ + (dir == 1 ? (high ? 1 : -1) : 0);
-                const int ns = slot_of(nx, ny);
-                const int ls = high ? tid : ns, lk = high ? k : k + 1;
-                const int hs = high ? ns : tid, hk = high ? k - 1 : k;
-                fsx[k] = F[k][0] * (upl[k] ? L.face[0][lk][ls] : L.face[0][hk][hs]);
+                float fsx[4];
+                for (int k = 0; k < 4; k++) {
+                    if (wallf[k]) {
+                        fsx[k] = 0.0f;
+                        continue;
+                    }
+                    const int dir = k >> 1;
+                    const bool high = k & 1;
+                    const int nx = sx_ + (dir == 0 ? (high ? 1 : -1) : 0),
+                              ny = sy_ + (dir == 1 ? (high ? 1 : -1) : 0);
+                    const int ns = slot_of(nx, ny);
+                    const int ls = high ? tid : ns, lk = high ? k : k + 1;
+                    const int hs = high ? ns : tid, hk = high ? k - 1 : k;
+                    fsx[k] = F[k][0] * (upl[k] ? L.face[j][lk][ls] : L.face[j][hk][hs]);
+                }
+                const double um =
+                    start * (double)xs0 - dtdx * (((double)fsx[1] - fsx[0]) + ((double)fsx[3] - fsx[2]));
+                out.x[pi] = (float)(um / u0);
+                const int nf = 2 * (w + h);
+                const int slots[4] = {gyc, h + gyc, 2 * h + gxc, 2 * h + w + gxc};
+                const bool edges[4] = {gxc == 0, gxc + 1 == w, gyc == 0, gyc + 1 == h};
+                for (int k = 0; k < 4; k++)
+                    if (edges[k])
+                        escaped[(size_t)(4 + s) * nf + slots[k]] += ((k & 1) ? 1.0 : -1.0) * dtdx * fsx[k];
             }
-            const float xs0 = vacuum ? L.x[s][lc] : in.x[pi];
-            const double um = start * (double)xs0 - dtdx * (((double)fsx[1] - fsx[0]) + ((double)fsx[3] - fsx[2]));
-            out.x[pi] = (float)(um / u0);
-            const int nf = 2 * (w + h);
-            const int slots[4] = {gyc, h + gyc, 2 * h + gxc, 2 * h + w + gxc};
-            const bool edges[4] = {gxc == 0, gxc + 1 == w, gyc == 0, gyc + 1 == h};
-            for (int k = 0; k < 4; k++)
-                if (edges[k]) escaped[(size_t)(4 + s) * nf + slots[k]] += ((k & 1) ? 1.0 : -1.0) * dtdx * fsx[k];
         }
     }
 

@@ -1,0 +1,53 @@
+"""The small plain-method configuration on the GPU against the f64 twin: a 256^2 box (L = 1, gamma 5/3) with a
+Gaussian blob (peak rho 1, sigma 0.1 L) on rho = 1e-2, T = 1 +- 1% seeded, at rest; self-gravity by the zero-padded
+512^2 convolution (G = 1); FLD with the Levermore-Pomraning limiter and Kramers opacity, solved by plain GPU MG-PCG
+to 1e-10 relative residual. Each step runs on the same dt on both sides (the twin's dt capped to the GPU's, as the
+reference's G-REF comparison does, crates/sr-engine/tests/gpu/cpu_gpu.rs:60-195).
+
+Bars: density rel-L1 <= 1e-5 per step while the fields are smooth (every step here), energy rel-L1 <= 1e-4, total
+mass and the clock as the twin's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import f32, rel_l1
+from sandbox_reactions_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_blob_256_with_gravity_and_fld_stays_on_the_twin(gpu_lib):
+    import sandbox_reactions_amd as sr
+    n, steps = 256, 10
+    print(f"config 1 blob {n}^2, {steps} steps", flush=True)
+    kw = scenes.config_physics_kwargs(n)
+    ph = sr.physics(**kw)
+    oph = oracle.reference_physics(**{k: v for k, v in kw.items() if k != "max_padded"}, max_padded=4096.0)
+    hs = sr.heat_settings(stop=sr.STOP_RELATIVE, solve_tol=1e-10, newton_heat=1e-9, newton_cell=1e-6)
+    start = f32(scenes.blob_scene(n))
+    twin = oracle.Twin(oph, start, gravity=True, hydro=True, heat=True)
+    world = sr.World(n, n, physics=ph, gravity=True, hydro=True, heat=True, heat_settings=hs)
+    world.upload(start.astype(np.float32))
+    worst_rho, worst_e = 0.0, 0.0
+    for k in range(steps):
+        dt = world.read_dt().dt_next
+        twin.cap_dt_next(dt)
+        twin.step()
+        world.set_dt_cap(dt)
+        world.run(1)
+        g, t = world.readback(), twin.state()
+        er, ee = rel_l1(g[0], t[0]), rel_l1(g[3], t[3])
+        worst_rho, worst_e = max(worst_rho, er), max(worst_e, ee)
+        heat = world.read_heat()
+        assert heat.converged and heat.final_residual <= 1e-10, (k, heat.final_residual)
+        assert er <= 1e-5, (k, er)
+        assert ee <= 1e-4, (k, ee)
+    print(f"worst per-step rel-L1: rho {worst_rho:.2e}, E {worst_e:.2e}", flush=True)
+    totals, escaped = world.totals()
+    tw = twin.state()
+    assert abs(totals[0] + escaped[0] - start[0].sum()) <= 1e-12 * start[0].sum()
+    assert abs(totals[0] - tw[0].sum()) <= 1e-6 * tw[0].sum()
+    r = world.read_dt()
+    _, _, st, tsteps = twin.times()
+    assert r.steps == tsteps == steps and abs(r.sim_time - st) <= 1e-6 * st

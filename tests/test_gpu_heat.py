@@ -101,3 +101,29 @@ def test_full_step_with_heat_matches_the_twin(gpu_lib, opacity):
         assert rel_l1(g[c], t[c]) <= 5e-5, (c, rel_l1(g[c], t[c]))
     r = world.read_heat()
     assert r.converged
+
+
+def test_inexact_newton_agrees_with_exact_newton(gpu_lib):
+    """Loose early Newton passes (forcing 0.1) against every pass solved to 1e-8: the same state to the Newton
+    tolerance, the last pass still to 1e-8, and fewer solver rounds."""
+    import sandbox_reactions_amd as sr
+    print("inexact Newton 256", flush=True)
+    n = 256
+    kw = scenes.config_physics_kwargs(n)
+    planes = scenes.turbulent_scene(n).astype(np.float32)
+    out, rounds = [], []
+    for forcing in (0.0, 0.1):
+        hs = sr.heat_settings(stop=sr.STOP_RELATIVE, solve_tol=1e-8, newton_heat=1e-6, newton_cell=1e-3,
+                              forcing=forcing)
+        world = sr.World(n, n, physics=sr.physics(**kw), gravity=True, hydro=True, heat=True, heat_settings=hs)
+        world.upload(planes)
+        world.set_dt_cap(world.read_dt().dt_next)
+        world.run(1)
+        r = world.read_heat()
+        assert r.converged and r.final_residual <= 1e-8
+        out.append(world.readback())
+        rounds.append(r.rounds)
+    print(f"rounds: every pass to 1e-8 {rounds[0]}, inexact {rounds[1]}", flush=True)
+    assert rel_l1(out[1][3], out[0][3]) <= 1e-5
+    assert rel_l1(out[1][0], out[0][0]) <= 1e-7
+    assert rounds[1] < rounds[0]
