@@ -52,12 +52,12 @@ __device__ __forceinline__ float mc(float l, float c, float r) {
 __device__ __forceinline__ float eps_floor(const HydroParams &P, float im) { return P.t_floor_g * im; }
 
 __device__ __forceinline__ float sound(const HydroParams &P, float s, float p) {
-    return sqrtf(fmaxf(P.gamma * p / s, 0.0f));
+    return sqrtf(fmaxf(__fdividef(P.gamma * p, s), 0.0f));
 }
 
 // E of a primitive state (hydro.rs:444-459 with gamma): eps = max(Pi / ((gamma-1) Sigma), floor)
 __device__ __forceinline__ float cons_e(const HydroParams &P, const Prim &q) {
-    const float eps = fmaxf(q.p / (P.gm1 * q.s), eps_floor(P, q.im));
+    const float eps = fmaxf(__fdividef(q.p, P.gm1 * q.s), eps_floor(P, q.im));
     return q.s * eps + 0.5f * q.s * (q.u * q.u + q.v * q.v);
 }
 
@@ -66,13 +66,14 @@ __device__ __forceinline__ bool prim_of(const HydroParams &P, float s, float mx,
                                         Prim &o) {
     if (!(s > 0.0f) || !isfinite(s) || !isfinite(mx) || !isfinite(my) || !isfinite(e) || !isfinite(sim))
         return false;
-    const float im = sim / s;
-    const float raw = e / s - 0.5f * (mx * mx + my * my) / (s * s);
+    const float is = 1.0f / s;
+    const float im = sim * is, u = mx * is, v = my * is;
+    const float raw = e * is - 0.5f * (u * u + v * v);
     if (!(raw > 0.0f)) return false;
     const float eps = fmaxf(raw, eps_floor(P, im));
     o.s = s;
-    o.u = mx / s;
-    o.v = my / s;
+    o.u = u;
+    o.v = v;
     o.p = P.gm1 * s * eps;
     o.im = im;
     return true;
@@ -219,12 +220,13 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
             q.u = q.v = 0.0f;
             q.p = P.gm1 * P.sigma_floor * eps_floor(P, im);
         } else {
-            const float s = (float)sig, mx = in.mx[i], my = in.my[i];
-            const float raw = in.e[i] / s - 0.5f * (mx * mx + my * my) / (s * s);
+            const float s = (float)sig, is = 1.0f / s;
+            const float u = in.mx[i] * is, v = in.my[i] * is;
+            const float raw = in.e[i] * is - 0.5f * (u * u + v * v);
             const float eps = fmaxf(raw, eps_floor(P, im));
             q.s = s;
-            q.u = mx / s;
-            q.v = my / s;
+            q.u = u;
+            q.v = v;
             q.p = P.gm1 * s * eps;
         }
         if (P.edge == 0) {
@@ -257,6 +259,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
     float theta[2];            // the species' shared limiter factor per axis
     float fs_w[4], f0_w[4];    // reconstructed faces' Sigma and mass flux
     float s_e[4];              // evolved faces' Sigma (0 when the unevolved face is kept)
+    float is_e[4];             // its reciprocal (0 when the unevolved face is kept)
     {
         const Prim c = {L.s[lc], L.u[lc], L.v[lc], L.p[lc], L.im[lc]};
         Prim wf[4];
@@ -270,11 +273,11 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
             for (int s = 0; s < NS; s++) {
                 const float xl = L.x[s][il], xc = L.x[s][lc], xr = L.x[s][ir];
                 const float central = 0.5f * (xr - xl);
-                if (central != 0.0f) th = fminf(th, mc(xl, xc, xr) / central);
+                if (central != 0.0f) th = fminf(th, __fdividef(mc(xl, xc, xr), central));
             }
             theta[ax] = th;
             const float sim = th * 0.5f * (r.im - l.im);
-            const float hottest = fmaxf(fmaxf(l.p / l.s, c.p / c.s), r.p / r.s);
+            const float hottest = fmaxf(fmaxf(__fdividef(l.p, l.s), __fdividef(c.p, c.s)), __fdividef(r.p, r.s));
             bool hot = false;
             for (int k = 0; k < 2; k++) {
                 const float f = k ? 0.5f : -0.5f;
@@ -328,10 +331,12 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
             if (prim_of(P, s0, mx, my, e, sim, o)) {
                 oe = e;
                 s_e[k] = s0;
+                is_e[k] = 1.0f / s0;
             } else {
                 o = q;
                 oe = ue[k];
                 s_e[k] = 0.0f;
+                is_e[k] = 0.0f;
             }
             L.face[0][k][tid] = o.s;
             L.face[1][k][tid] = o.u;
@@ -541,6 +546,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
         }
     }
 
+    const double inv_u0 = u0 != 0.0 ? 1.0 / u0 : 0.0;
     // ---- phase 4: the species, SB at a time in the freed face buffer (flux = mass flux x the upwind face's
     // fraction)
     constexpr int SB = SR_HYDRO_SB;
@@ -559,7 +565,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 xf[2 * ax + 1] = xc + 0.5f * theta[ax] * central;
             }
             const float ds = hdx * ((f0_w[0] * xf[0] - f0_w[1] * xf[1]) + (f0_w[2] * xf[2] - f0_w[3] * xf[3]));
-            for (int k = 0; k < 4; k++) L.face[j][k][tid] = s_e[k] > 0.0f ? (fs_w[k] * xf[k] + ds) / s_e[k] : xf[k];
+            for (int k = 0; k < 4; k++) L.face[j][k][tid] = s_e[k] > 0.0f ? (fs_w[k] * xf[k] + ds) * is_e[k] : xf[k];
         }
         __syncthreads();
         if (live) {
@@ -589,7 +595,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 }
                 const double um =
                     start * (double)xs0 - dtdx * (((double)fsx[1] - fsx[0]) + ((double)fsx[3] - fsx[2]));
-                out.x[pi] = (float)(um / u0);
+                out.x[pi] = (float)(um * inv_u0);
                 const int nf = 2 * (w + h);
                 const int slots[4] = {gyc, h + gyc, 2 * h + gxc, 2 * h + w + gxc};
                 const bool edges[4] = {gxc == 0, gxc + 1 == w, gyc == 0, gyc + 1 == h};
