@@ -1,0 +1,268 @@
+"""Scene files of the reference engine (version 1 JSON, crates/sr-engine/src/scene.rs:328-575) turned into the 14
+canonical planes this library uploads, with the world, edge rule, switches, test settings and physics overrides.
+
+Covered: `rect` and `disk` objects (Maclaurin profile sampled at cell centres, scene.rs:545-556), each with `t` or
+`pressure`, a `composition` of mass fractions summing to 1 (within 1e-6, scene.rs:46, :215-245) and a velocity `v`;
+overlapping objects add their mass, momentum, energy and species mass (scene.rs:485-497); cells no object covers
+hold the floor state, Sigma_floor of hydrogen at rest at T_floor (scene.rs:466-470). Refused with a message: anything
+the path does not carry -- `preset` objects (they need the calibration file), the reactions / neutrinos / sinks
+switches, and cold pressure: the reference's shipped constants turn it on (k1e, k2e, k1n, k2n), so a scene must
+override all four to 0, as every one of the reference's parity scenes does.
+
+The energy follows the EOS with this library's adiabatic index gamma (the reference's gamma = 2 by default):
+eps = T / (mu (gamma - 1)) for `t`, eps = Pi / ((gamma - 1) Sigma) for `pressure`, plus the kinetic |v|^2 / 2.
+"""
+from __future__ import annotations
+
+import json
+import math
+
+import numpy as np
+
+# assets/elements.json: key, A, Z (the order of the species planes)
+ELEMENTS = [("H", 1, 1), ("He", 4, 2), ("C", 12, 6), ("O", 16, 8), ("Ne", 20, 10), ("Mg", 24, 12), ("Si", 28, 14),
+            ("S", 32, 16), ("Fe", 56, 26), ("n", 1, 0)]
+INDEX = {k: i for i, (k, _, _) in enumerate(ELEMENTS)}
+COMPOSITION_TOLERANCE = 1e-6
+# the assets/physics.json constants this path reads, and their shipped values
+SHIPPED = {"sigma_floor": 1e-8, "sigma_vac": 1e-6, "t_floor": 0.05, "a_r": 6.25e-7, "c_sb": 12800.0, "kappa0": 1.0,
+           "k_cond": 0.0, "cfl": 0.4, "eta_g": 0.3, "kappa_dust": 0.0, "t_dust": 0.0}
+COLD = ("k1e", "k2e", "k1n", "k2n")
+PHYSICS_KEYS = {"t_h", "t_he", "t_c", "t_ne", "t_o", "t_si", "q_h", "a_h_burn", "a_he_burn", "a_c_alpha", "a_c_burn",
+                "a_ne_burn", "a_o_burn", "a_mg_burn", "a_si_burn", "a_s_burn", "a_n_fe", "nu_c_burn", "nu_ne_burn",
+                "nu_o_burn", "nu_mg_burn", "nu_si_burn", "nu_s_burn", "sigma_n", "burn_dx", "burn_dt", "burn_k",
+                "sigma_floor", "sigma_vac", "t_floor", "sigma_obj", "paint_sigma", "paint_t", "heat_factor",
+                "cool_factor", "k1e", "k2e", "k1n", "k2n", "a_r", "c_sb", "kappa0", "k_cond", "a_nu", "t_nu", "m_nu",
+                "f_dep", "sigma_bh", "cfl", "eta_g", "kappa_dust", "t_dust", "block", "sigma_hold", "eta_hold",
+                "r_hold"}
+
+
+class SceneError(ValueError):
+    pass
+
+
+def inv_mu(comp):
+    """1/mu = sum_s X_s (Z_s + 1) / A_s (crates/sr-physics/src/registry.rs:146-154)."""
+    return sum(x * (z + 1.0) / a for x, (_, a, z) in zip(comp, ELEMENTS))
+
+
+def _fields(obj, at, allowed):
+    if not isinstance(obj, dict):
+        raise SceneError(f"{at}: must be an object")
+    for k in obj:
+        if k not in allowed:
+            raise SceneError(f"{at}.{k}: unknown key")
+    return obj
+
+
+def _num(f, key, at, positive=False, required=True):
+    if key not in f:
+        if required:
+            raise SceneError(f"{at}.{key}: missing")
+        return None
+    v = f[key]
+    if isinstance(v, bool) or not isinstance(v, (int, float)) or not math.isfinite(v):
+        raise SceneError(f"{at}.{key}: must be a number")
+    if positive and not v > 0:
+        raise SceneError(f"{at}.{key}: must be > 0, got {v}")
+    return float(v)
+
+
+def _whole(f, key, at):
+    v = f.get(key)
+    if isinstance(v, bool) or not isinstance(v, int):
+        raise SceneError(f"{at}.{key}: must be a whole number")
+    return v
+
+
+def _gas(f, at):
+    has_t, has_p = "t" in f, "pressure" in f
+    if has_t and has_p:
+        raise SceneError(f"{at}: takes t or pressure, not both")
+    if not (has_t or has_p):
+        raise SceneError(f"{at}: needs t or pressure")
+    heat = ("t", _num(f, "t", at, True)) if has_t else ("pressure", _num(f, "pressure", at, True))
+    comp_map = f.get("composition")
+    if not isinstance(comp_map, dict):
+        raise SceneError(f"{at}.composition: must be an object of mass fractions")
+    comp = [0.0] * len(ELEMENTS)
+    for k, v in comp_map.items():
+        if k not in INDEX:
+            raise SceneError(f"{at}.composition.{k}: unknown key")
+        if isinstance(v, bool) or not isinstance(v, (int, float)) or not (math.isfinite(v) and v >= 0):
+            raise SceneError(f"{at}.composition.{k}: must be a number >= 0")
+        comp[INDEX[k]] = float(v)
+    if abs(sum(comp) - 1.0) > COMPOSITION_TOLERANCE:
+        raise SceneError(f"{at}.composition: the fractions sum to {sum(comp)}, not 1")
+    v = f.get("v", [0.0, 0.0])
+    if not (isinstance(v, list) and len(v) == 2 and all(isinstance(c, (int, float)) and math.isfinite(c) for c in v)):
+        raise SceneError(f"{at}.v: must be [vx, vy], two finite numbers")
+    return heat, comp, (float(v[0]), float(v[1]))
+
+
+def _flag(f, key, at, default):
+    v = f.get(key, default)
+    if not isinstance(v, bool):
+        raise SceneError(f"{at}.{key}: must be true or false")
+    return v
+
+
+class Scene:
+    """A parsed scene: world (w, h), edge (0 bounce / 1 leave), switches, test settings, overrides, objects."""
+
+    def __init__(self, root: dict):
+        top = _fields(root, "scene", {"version", "world", "slowdown", "switches", "test", "objects", "overrides"})
+        if top.get("version") != 1:
+            raise SceneError("version: must be 1")
+        w = _fields(top.get("world"), "world", {"width", "height", "edge"})
+        self.width, self.height = _whole(w, "width", "world"), _whole(w, "height", "world")
+        if not (8 <= self.width <= 4096 and 8 <= self.height <= 4096 and self.width % 8 == 0 and self.height % 8 == 0):
+            raise SceneError(f"world: {self.width} x {self.height} is not a world size (multiples of 8 in [8, 4096])")
+        edge = w.get("edge", "leave")
+        if edge not in ("leave", "bounce"):
+            raise SceneError(f"world.edge: must be \"leave\" or \"bounce\", got {edge!r}")
+        self.edge = 1 if edge == "leave" else 0
+        self.slowdown = _flag(top, "slowdown", "scene", True)
+        sw = _fields(top.get("switches", {}), "switches", {"gravity", "hydro", "heat", "reactions", "neutrinos",
+                                                             "sinks"})
+        self.switches = {k: _flag(sw, k, "switches", True) for k in ("gravity", "hydro", "heat", "reactions",
+                                                                    "neutrinos", "sinks")}
+        t = _fields(top.get("test", {}), "test", {"pin_temperature", "constant_diffusivity", "flux_limiter"})
+        cd = t.get("constant_diffusivity")
+        if cd is not None:
+            cd = _num(t, "constant_diffusivity", "test", True)
+        self.test = {"pin_temperature": _flag(t, "pin_temperature", "test", False),
+                     "constant_diffusivity": cd, "flux_limiter": _flag(t, "flux_limiter", "test", True)}
+        ov = top.get("overrides", {})
+        if not isinstance(ov, dict):
+            raise SceneError("overrides: must be an object of physics.json keys")
+        self.overrides = {}
+        for k, v in ov.items():
+            if k not in PHYSICS_KEYS:
+                raise SceneError(f"overrides.{k}: unknown key (physics.json keys only)")
+            if isinstance(v, bool) or not isinstance(v, (int, float)):
+                raise SceneError(f"overrides.{k}: must be a number")
+            self.overrides[k] = float(v)
+        if any(self.overrides.get(k, 1.0) != 0.0 for k in COLD):
+            raise SceneError("cold pressure (k1e, k2e, k1n, k2n) is not carried by this path: the scene must "
+                             "override all four to 0")
+        objs = top.get("objects")
+        if not isinstance(objs, list):
+            raise SceneError("objects: must be a list")
+        self.objects = []
+        for i, o in enumerate(objs):
+            at = f"objects[{i}]"
+            kind = o.get("kind") if isinstance(o, dict) else None
+            if kind == "rect":
+                f = _fields(o, at, {"kind", "x0", "y0", "w", "h", "sigma", "t", "pressure", "composition", "v"})
+                rw, rh = _whole(f, "w", at), _whole(f, "h", at)
+                if rw < 1 or rh < 1:
+                    raise SceneError(f"{at}: w and h must be >= 1, got {rw} x {rh}")
+                self.objects.append(("rect", (_whole(f, "x0", at), _whole(f, "y0", at), rw, rh,
+                                              _num(f, "sigma", at, True)), _gas(f, at)))
+            elif kind == "disk":
+                f = _fields(o, at, {"kind", "x", "y", "a", "sigma_c", "t", "pressure", "composition", "v"})
+                self.objects.append(("disk", (_num(f, "x", at), _num(f, "y", at), _num(f, "a", at, True),
+                                              _num(f, "sigma_c", at, True)), _gas(f, at)))
+            elif kind == "preset":
+                raise SceneError(f"{at}: preset objects need the calibration file; not carried by this path")
+            else:
+                raise SceneError(f"{at}.kind: must be \"preset\", \"disk\" or \"rect\"")
+
+    def check_path(self):
+        """Refuses a scene that switches on a pass this path does not carry (reactions, neutrinos, sinks)."""
+        for k in ("reactions", "neutrinos", "sinks"):
+            if self.switches[k]:
+                raise SceneError(f"switches.{k}: this path carries gravity, hydro and heat only (set {k} false)")
+
+    def constants(self) -> dict:
+        """The physics constants this path reads, shipped values with the scene's overrides."""
+        c = dict(SHIPPED)
+        for k in SHIPPED:
+            if k in self.overrides:
+                c[k] = self.overrides[k]
+        return c
+
+    def cells(self, obj):
+        """(ys, xs, sigma) of the cells an object covers (scene.rs:521-563)."""
+        kind, geo, _ = obj
+        w, h = self.width, self.height
+        if kind == "rect":
+            x0, y0, rw, rh, sigma = geo
+            xs = np.arange(min(max(x0, 0), w), min(max(x0 + rw, 0), w))
+            ys = np.arange(min(max(y0, 0), h), min(max(y0 + rh, 0), h))
+            yy, xx = np.meshgrid(ys, xs, indexing="ij")
+            return yy.ravel(), xx.ravel(), np.full(yy.size, sigma)
+        x, y, a, sc = geo
+        i = np.arange(math.floor(x - a), math.ceil(x + a))
+        j = np.arange(math.floor(y - a), math.ceil(y + a))
+        jj, ii = np.meshgrid(j, i, indexing="ij")
+        r2 = ((ii + 0.5 - x) ** 2 + (jj + 0.5 - y) ** 2) / (a * a)
+        keep = (r2 < 1.0) & (ii >= 0) & (jj >= 0) & (ii < w) & (jj < h)
+        return jj[keep], ii[keep], sc * np.sqrt(1.0 - r2[keep])
+
+    def planes(self, gamma: float = 2.0) -> np.ndarray:
+        """The 14 planes (f32, (14, h, w)) as the reference's Scene::planes makes them (scene.rs:428-471)."""
+        w, h = self.width, self.height
+        n = w * h
+        sig, mx, my, en = (np.zeros(n) for _ in range(4))
+        spm = np.zeros((len(ELEMENTS), n))
+        for obj in self.objects:
+            ys, xs, s = self.cells(obj)
+            (kind, value), comp, v = obj[2]
+            idx = ys * w + xs
+            if kind == "t":
+                eps = value / ((gamma - 1.0) / inv_mu(comp)) * np.ones_like(s)
+            else:
+                eps = value / ((gamma - 1.0) * s)
+            eps = eps + 0.5 * (v[0] * v[0] + v[1] * v[1])
+            np.add.at(sig, idx, s)
+            np.add.at(mx, idx, s * v[0])
+            np.add.at(my, idx, s * v[1])
+            np.add.at(en, idx, s * eps)
+            for k, x in enumerate(comp):
+                if x:
+                    np.add.at(spm[k], idx, s * x)
+        c = self.constants()
+        floor_sigma = c["sigma_floor"]
+        floor_e = floor_sigma * c["t_floor"] / ((gamma - 1.0) / inv_mu([1.0] + [0.0] * 9))
+        out = np.zeros((14, n), np.float64)
+        empty = sig <= 0.0
+        out[0] = np.where(empty, floor_sigma, sig)
+        out[1] = np.where(empty, 0.0, mx)
+        out[2] = np.where(empty, 0.0, my)
+        out[3] = np.where(empty, floor_e, en)
+        safe = np.where(empty, 1.0, sig)
+        for k in range(len(ELEMENTS)):
+            out[4 + k] = np.where(empty, 1.0 if k == 0 else 0.0, spm[k] / safe)
+        return out.reshape(14, h, w).astype(np.float32)
+
+    def physics_kwargs(self, gamma: float = 2.0, dx: float = 1.0) -> dict:
+        """Keyword arguments of sandbox_reactions_amd.physics() for this scene (the reference opacity)."""
+        kw = dict(self.constants(), gamma=gamma, dx=dx, edge=self.edge,
+                  pin_temperature=int(self.test["pin_temperature"]),
+                  constant_diffusivity=self.test["constant_diffusivity"] or 0.0,
+                  flux_limiter=int(self.test["flux_limiter"]))
+        return kw
+
+
+def load(path: str) -> Scene:
+    with open(path) as f:
+        return Scene(json.load(f))
+
+
+def parse(text: str) -> Scene:
+    return Scene(json.loads(text))
+
+
+def world(scene: Scene, heat_settings=None, **physics_over):
+    """A World for the scene (its world size, switches and constants; gamma 2 and unit cells unless overridden),
+    with its planes uploaded. Refuses a scene whose passes this path does not carry."""
+    import sandbox_reactions_amd as sr
+    scene.check_path()
+    kw = scene.physics_kwargs()
+    kw.update(physics_over)
+    w = sr.World(scene.width, scene.height, physics=sr.physics(**kw), gravity=scene.switches["gravity"],
+                 hydro=scene.switches["hydro"], heat=scene.switches["heat"], heat_settings=heat_settings)
+    w.upload(scene.planes(kw["gamma"]))
+    return w

@@ -1,0 +1,124 @@
+"""The scene loader (sandbox_reactions_amd/scene_json.py) on the reference's own scene files (copied under
+tests/golden/scenes as input data), against the planes the reference's Scene::planes defines
+(crates/sr-engine/src/scene.rs:428-497): closed-form cell values, the Maclaurin disk's mass, overlap sums, the floor
+state, and the loader's refusals."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from sandbox_reactions_amd import scene_json, scenes
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
+
+
+def load(name):
+    return scene_json.load(os.path.join(HERE, name))
+
+
+def test_sod_is_the_shock_tube():
+    s = load("sod.json")
+    assert (s.width, s.height, s.edge) == (400, 64, 0)
+    assert s.switches["hydro"] and not s.switches["gravity"] and not s.switches["heat"]
+    p = s.planes()
+    ref = scenes.sod_scene(400, 64).astype(np.float32)
+    assert np.array_equal(p, ref)
+
+
+def test_hot_helium_slab_energy():
+    s = load("heat_hot.json")
+    p = s.planes()
+    assert np.all(p[0] == np.float32(30.0))
+    assert np.allclose(p[3], 30.0 * 18800.0 * 0.75, rtol=1e-7)  # gamma 2: E = Sigma T / mu, 1/mu(He) = 3/4
+    assert np.all(p[5] == 1.0) and np.all(p[4] == 0.0)
+
+
+def test_maclaurin_disk_samples_and_mass():
+    s = load("ref_disk.json")
+    p = s.planes().astype(np.float64)
+    a, sc = 24.0, 1.0
+    yy, xx = np.mgrid[0:64, 0:64]
+    r2 = ((xx + 0.5 - 32) ** 2 + (yy + 0.5 - 32) ** 2) / a ** 2
+    inside = r2 < 1.0
+    assert np.allclose(p[0][inside], sc * np.sqrt(1.0 - r2[inside]), rtol=1e-6)
+    assert np.all(p[0][~inside] == np.float32(1e-8))
+    # the continuous disk holds 2 pi a^2 sigma_c / 3; cell-centre sampling is within 1 % of it at a = 24
+    m = p[0][inside].sum()
+    assert abs(m - 2.0 * math.pi * a * a * sc / 3.0) <= 1e-2 * m
+    im = 0.75 * 2.0 + 0.25 * 0.75
+    assert np.allclose(p[3][inside], p[0][inside] * 0.05 * im, rtol=1e-6)
+    assert np.allclose(p[4][inside], 0.75) and np.allclose(p[5][inside], 0.25)
+
+
+def test_overlapping_objects_add():
+    s = load("heat_rim.json")
+    p = s.planes().astype(np.float64)
+    c = 32, 32
+    sig_disk = 1.0  # the disk's centre cell centre is 0.5 cells off: r^2 / a^2 = 0.5 / 400
+    sd = sig_disk * math.sqrt(1.0 - 0.5 / 400.0)
+    assert p[0][c] == pytest.approx(2e-6 + sd, rel=1e-6)
+    assert p[3][c] == pytest.approx(2e-6 * 5.0 * 2.0 + sd * 100.0 * 2.0, rel=1e-6)
+    assert p[0][0, 0] == pytest.approx(2e-6, rel=1e-6)
+
+
+def test_uncovered_cells_hold_the_floor():
+    p = scene_json.parse(json.dumps(base())).planes().astype(np.float64)
+    rest = p[:, 8:, 8:]
+    assert np.all(rest[0] == np.float32(1e-8))
+    assert np.allclose(rest[3], 1e-8 * 0.05 * 2.0, rtol=1e-6)  # Sigma_floor T_floor / mu(H), gamma 2
+    assert np.all(rest[4] == 1.0) and np.all(rest[1] == 0.0)
+    assert np.all(p[0, :8, :8] == 1.0)
+
+
+def test_hot_spot_in_a_cool_sheet():
+    s = load("heat_spot.json")
+    p = s.planes().astype(np.float64)
+    assert np.allclose(p[3, 28:36, 28:36], 1e-4 * 2.0 * 2.0, rtol=1e-6)
+    assert np.allclose(p[3, :28, :], 1e-4 * 0.05 * 2.0, rtol=1e-6)
+
+
+def test_test_settings_and_constants():
+    s = load("heat_pulse.json")
+    kw = s.physics_kwargs()
+    assert kw["constant_diffusivity"] == 0.5 and kw["edge"] == 0 and kw["flux_limiter"] == 1
+    assert kw["sigma_floor"] == 1e-8 and kw["c_sb"] == 12800.0
+
+
+def base():  # an 8 x 8 rect of hydrogen in a 64 x 64 world, cold pressure off
+    return {"version": 1, "world": {"width": 64, "height": 64}, "overrides": {k: 0 for k in scene_json.COLD},
+            "objects": [{"kind": "rect", "x0": 0, "y0": 0, "w": 8, "h": 8, "sigma": 1.0, "t": 1.0,
+                         "composition": {"H": 1.0}}]}
+
+
+@pytest.mark.parametrize("edit, message", [
+    (lambda d: d["objects"][0].update(pressure=1.0), "not both"),
+    (lambda d: d["objects"][0].update(composition={"H": 0.9}), "sum to"),
+    (lambda d: d["objects"][0].update(colour=1), "unknown key"),
+    (lambda d: d.pop("overrides"), "cold pressure"),
+    (lambda d: d["overrides"].update(k1e=21.3), "cold pressure"),
+    (lambda d: d["objects"].append({"kind": "preset", "preset": "sun", "x": 1, "y": 1}), "calibration"),
+    (lambda d: d.update(switches={"reactions": True, "neutrinos": False, "sinks": False}), "reactions"),
+    (lambda d: d["world"].update(width=60), "world size"),
+    (lambda d: d["overrides"].update(warp=1.0), "unknown key"),
+    (lambda d: d.update(version=2), "version"),
+])
+def test_refusals(edit, message):
+    d = base()
+    edit(d)
+    with pytest.raises(scene_json.SceneError, match=message):
+        scene_json.parse(json.dumps(d)).check_path()
+
+
+def test_every_fixture_parses():
+    """Every fixture but asym_blob.json, which keeps the shipped cold pressure (refused: not carried)."""
+    with pytest.raises(scene_json.SceneError, match="cold pressure"):
+        load("asym_blob.json")
+    for name in sorted(os.listdir(HERE)):
+        if name.endswith(".json") and name != "asym_blob.json":
+            s = load(name)
+            p = s.planes()
+            assert p.shape == (14, s.height, s.width) and np.all(np.isfinite(p))
+            frac = p[4:].sum(axis=0)
+            assert np.allclose(frac, 1.0, atol=1e-6)
