@@ -1,0 +1,79 @@
+"""SRDUMP01 state dumps (sandbox_reactions_amd/dump.py) on the reference's own dumps (tests/golden/dumps, input
+data), and the heat pass's ceiling on the first of them with the oracle: the case the reference's heat tests build
+from it (crates/sr-engine/tests/gpu/heat.rs:1413-1500). A heat step that reaches its round ceiling commits by limited
+fluxes (crates/sr-physics/src/reference/heat.rs:544-592), and its bounds pin that commit: (a) no gas cell loses more
+than its heat above the floor, (b) the grid loses the heat it books as radiated, (c) the radiated heat is at most what
+the cells held above the floor."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from sandbox_reactions_amd import dump
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dumps")
+IM = np.array([2.0, 0.75, 7 / 12, 9 / 16, 11 / 20, 13 / 24, 15 / 28, 17 / 32, 27 / 56, 1.0])  # (Z + 1) / A
+
+
+def ceiling_cut(name="heat_ceiling.bin"):
+    """The cut's state, the constants its life ran on (a_r 4e-5, c_sb 200; the reference's heat.rs:1428-1437) and the
+    step's dt (the header's resume.dt_next)."""
+    d = dump.read(os.path.join(HERE, name))
+    ph = oracle.reference_physics(a_r=4e-5, c_sb=200.0)
+    return d["planes"].astype(np.float64), ph, float(d["dt_next"])
+
+
+def held_above_floor(ph, p):
+    """C (T0 - T_floor) per gas cell (gamma 2: C = Sigma / mu, T = mu eps floored at T_floor), 0 off the gas."""
+    sigma = p[0]
+    im = np.tensordot(IM, p[4:], axes=1)
+    eps = p[3] / np.maximum(sigma, 1e-300) - 0.5 * (p[1] ** 2 + p[2] ** 2) / np.maximum(sigma, 1e-300) ** 2
+    t = np.maximum(eps / im, ph.t_floor)
+    return np.where(sigma >= ph.sigma_vac, sigma * im * (t - ph.t_floor), 0.0)
+
+
+def assert_bounded(ph, before, after, radiated, rel, abs_):
+    held = held_above_floor(ph, before)
+    lost = before[3] - after[3]
+    gas = before[0] >= ph.sigma_vac
+    assert np.all(lost[gas] <= held[gas] * (1 + rel) + abs_ * np.abs(before[3][gas]))
+    assert abs(lost.sum() - radiated) <= rel * held.sum()
+    assert radiated <= held.sum() * (1 + rel)
+
+
+def test_reads_the_reference_dumps():
+    for name, step in [("heat_ceiling.bin", 2517), ("heat_overflow.bin", 16127)]:
+        d = dump.read(os.path.join(HERE, name))
+        assert (d["width"], d["height"], d["step"]) == (64, 64, step)
+        assert d["planes"].shape == (14, 64, 64) and np.all(np.isfinite(d["planes"]))
+        assert d["flux"] is None and d["rate"] is None and d["held"] is None
+        assert np.allclose(d["planes"][4:].sum(axis=0), 1.0, atol=1e-5)
+    # the thin hot cell the reference describes at (32, 32): Sigma 2.1e-5 beside a vacuum cell
+    p = dump.read(os.path.join(HERE, "heat_ceiling.bin"))["planes"]
+    assert p[0, 32, 32] == pytest.approx(2.1e-5, rel=0.05)
+
+
+def test_round_trip_is_bitwise():
+    d = dump.read(os.path.join(HERE, "heat_overflow.bin"))
+    b = dump.to_bytes(d["planes"], step=d["step"], sim_time=d["sim_time"], dt=d["dt"], dt_next=d["dt_next"])
+    e = dump.from_bytes(b)
+    assert np.array_equal(e["planes"].view(np.uint32), d["planes"].view(np.uint32))
+    assert e["step"] == d["step"] and e["dt_next"] == d["dt_next"]
+
+
+@pytest.mark.parametrize("cut, message", [(lambda b: b"SRDUMP02" + b[8:], "magic"), (lambda b: b[:-4], "planes hold"),
+                                          (lambda b: b + b"\0\0\0\0", "planes hold"), (lambda b: b[:20], "header")])
+def test_refusals(cut, message):
+    with open(os.path.join(HERE, "heat_ceiling.bin"), "rb") as f:
+        b = f.read()
+    with pytest.raises(dump.DumpError, match=message):
+        dump.from_bytes(cut(b))
+
+
+def test_the_twin_at_the_ceiling_moves_no_more_heat_than_its_cells_hold():
+    before, ph, dt = ceiling_cut()
+    after, rep, *_ = oracle.heat_step(ph, before, dt)
+    assert not rep.converged, (rep.passes, rep.rounds)
+    assert rep.delta == 0.0
+    assert_bounded(ph, before, after, rep.radiated, 1e-9, 1e-12)

@@ -127,3 +127,22 @@ def test_inexact_newton_agrees_with_exact_newton(gpu_lib):
     assert rel_l1(out[1][3], out[0][3]) <= 1e-5
     assert rel_l1(out[1][0], out[0][0]) <= 1e-7
     assert rounds[1] < rounds[0]
+
+
+def test_ceiling_commit_moves_no_more_heat_than_the_cells_hold(gpu_lib):
+    """The reference's ceiling cut (heat_ceiling.bin) with the round ceiling forced from the host (64 rounds): the
+    step commits by limited fluxes with delta 0 and keeps the reference's bounds to f32's 1e-5."""
+    import sandbox_reactions_amd as sr
+    from test_dump import assert_bounded, ceiling_cut
+    print("ceiling cut 64x64", flush=True)
+    before, oph, dt = ceiling_cut()
+    _, gp = both(a_r=4e-5, c_sb=200.0)
+    hs = sr.heat_settings(max_rounds=64)
+    world = sr.World(64, 64, physics=gp, gravity=False, hydro=False, heat=True, heat_settings=hs)
+    world.upload(before.astype(np.float32))
+    world.set_dt_cap(dt)
+    world.run(1)
+    r = world.read_heat()
+    assert not r.converged and r.delta == 0.0
+    after = world.readback().astype(np.float64)
+    assert_bounded(oph, before, after, r.radiated, 1e-5, 2.0 * float(np.finfo(np.float32).eps))
