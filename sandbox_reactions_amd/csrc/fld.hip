@@ -231,21 +231,24 @@ __global__ void __launch_bounds__(NT) k_dots_gas(L0 a, const float *__restrict__
 struct Lv {
     int w, h;
     const float *own, *ax, *ay;
+    const float4 *cf;  // the per-cell record (tile smoothers), or null
 };
 
 // The region of a TX x TY tile plus a halo of HW cells: (TX + 2 HW) x (TY + 2 HW) entries per array. A cell at
 // least one cell inside the region reads its right and bottom faces from its neighbours' left and top entries.
 template <int TX, int TY, int HW> struct Region {
     static constexpr int RX = TX + 2 * HW, RY = TY + 2 * HW, N = RX * RY;
-    float z[N], b[N], own[N], al[N], au[N];
+    float z[N], b[N], own[N], al[N], au[N], rd[N];
 };
 
 template <int TX, int TY, int HW>
 __device__ inline void load_region(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0, const float *__restrict__ b) {
     using RG = Region<TX, TY, HW>;
     constexpr int IT = (RG::N + NT - 1) / NT;
-    // every load of the region issued before any is used: one memory round trip per block
-    float vo[IT], vb[IT], vl[IT], vu[IT];
+    // every load of the region issued before any is used: one memory round trip per block; the coefficients are one
+    // aligned 16-byte record per cell
+    float4 vc[IT];
+    float vb[IT];
 #pragma unroll
     for (int k = 0; k < IT; k++) {
         const int t = threadIdx.x + k * NT;
@@ -253,19 +256,18 @@ __device__ inline void load_region(Region<TX, TY, HW> &R, const Lv &L, int x0, i
         const int gx = x0 + lx - HW, gy = y0 + ly - HW;
         const bool in = t < RG::N && gx >= 0 && gx < L.w && gy >= 0 && gy < L.h;
         const size_t i = (size_t)gy * L.w + gx;
-        vo[k] = in ? L.own[i] : 1.0f;
+        vc[k] = in ? L.cf[i] : make_float4(1.0f, 0.0f, 0.0f, 1.0f);
         vb[k] = in && b ? b[i] : 0.0f;
-        vl[k] = in ? L.ax[(size_t)gy * (L.w + 1) + gx] : 0.0f;
-        vu[k] = in ? L.ay[i] : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < IT; k++) {
         const int t = threadIdx.x + k * NT;
         if (t < RG::N) {
-            R.own[t] = vo[k];
+            R.own[t] = vc[k].x;
+            R.al[t] = vc[k].y;
+            R.au[t] = vc[k].z;
+            R.rd[t] = vc[k].w;
             R.b[t] = vb[k];
-            R.al[t] = vl[k];
-            R.au[t] = vu[k];
         }
     }
 }
@@ -290,7 +292,7 @@ __device__ inline void half_sweep(Region<TX, TY, HW> &R, const Lv &L, int x0, in
         const int c = ly * RG::RX + lx;
         const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RG::RX];
         const float num = R.b[c] + al * R.z[c - 1] + ar * R.z[c + 1] + au * R.z[c - RG::RX] + ad * R.z[c + RG::RX];
-        R.z[c] = __fdividef(num, R.own[c] + al + ar + au + ad);
+        R.z[c] = num * R.rd[c];
     }
     __syncthreads();
 }
@@ -519,6 +521,17 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
 
 // ------------------------------------------------------------------------------------------------ hierarchy build
 
+// the tile smoothers' per-cell record {own, left face, up face, 1 / (own + all four faces)}
+__global__ void k_pack_cf(Lv L, float4 *__restrict__ cf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.w * L.h) return;
+    const int x = i % L.w, y = i / L.w;
+    const float al = L.ax[(size_t)y * (L.w + 1) + x], ar = L.ax[(size_t)y * (L.w + 1) + x + 1];
+    const float au = L.ay[i], ad = L.ay[(size_t)i + L.w];
+    const float own = L.own[i];
+    cf[i] = make_float4(own, al, au, 1.0f / (own + al + ar + au + ad));
+}
+
 // coarse own = the sum over the aggregate (gas cells only at level 0), 1 where the aggregate holds none
 __global__ void k_coarsen_own(Lv f, const uint8_t *__restrict__ gas, float *__restrict__ oc, int wc, int hc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -649,6 +662,7 @@ FldSolver::~FldSolver() {
         (void)hipFree(lv[l].z);
         (void)hipFree(lv[l].b);
         (void)hipFree(lv[l].zp);
+        if (lv[l].cf) (void)hipFree(lv[l].cf);
     }
     void *ps[] = {gas, r32, za, zb, p, q, pack, part_true, part_pq, part_rr, part_dots};
     for (void *x : ps)
@@ -724,6 +738,7 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_) {
         SR_HIP(hipMalloc(&L.z, sizeof(float) * L.w * L.h));
         SR_HIP(hipMalloc(&L.b, sizeof(float) * L.w * L.h));
         SR_HIP(hipMalloc(&L.zp, sizeof(float) * L.w * L.h));
+        if (l < coarse_from) SR_HIP(hipMalloc(&L.cf, sizeof(float4) * L.w * L.h));
     }
     const size_t n = (size_t)w * h;
     nbe = (int)std::max<size_t>(1, std::min<size_t>(NB_MAX, (n + NT * UNROLL - 1) / (NT * UNROLL)));
@@ -773,7 +788,7 @@ int FldSolver::set_system(const float *own, const float *ax, const float *ay, co
 int FldSolver::rebuild(bool faces, hipStream_t s) {
     for (size_t l = 0; l + 1 < lv.size(); l++) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
-        Lv fl = {f.w, f.h, f.own, f.ax, f.ay};
+        Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
         const int nc = c.w * c.h;
         hipLaunchKernelGGL(k_coarsen_own, dim3((nc + 255) / 256), dim3(256), 0, s, fl, l == 0 ? gas : nullptr,
                            c.own, c.w, c.h);
@@ -781,6 +796,11 @@ int FldSolver::rebuild(bool faces, hipStream_t s) {
             const int nf = std::max((c.w + 1) * c.h, c.w * (c.h + 1));
             hipLaunchKernelGGL(k_coarsen_faces, dim3((nf + 255) / 256), dim3(256), 0, s, fl, c.ax, c.ay, c.w, c.h);
         }
+    }
+    for (int l = 0; l < coarse_from; l++) {
+        const FldLevel &f = lv[l];
+        Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
+        hipLaunchKernelGGL(k_pack_cf, dim3((f.w * f.h + 255) / 256), dim3(256), 0, s, fl, f.cf);
     }
     const FldLevel &lc = lv.back();
     if (lc.w * lc.h <= DENSE_MAX)
@@ -822,7 +842,7 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
     const float *bl = r;
     for (int l = 0; l < coarse_from; l++) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
-        Lv L = {f.w, f.h, f.own, f.ax, f.ay};
+        Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
         const bool small = small_tiles(l, f.w, f.h);
         const int sx = small ? 16 : tx, sy = small ? 16 : ty;
         dim3 grid((f.w + sx - 1) / sx, (f.h + sy - 1) / sy);
@@ -884,7 +904,7 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
     }
     for (int l = coarse_from - 1; l >= 0; l--) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
-        Lv L = {f.w, f.h, f.own, f.ax, f.ay};
+        Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
         const bool small = small_tiles(l, f.w, f.h);
         const int sx = small ? 16 : tx, sy = small ? 16 : ty;
         dim3 grid((f.w + sx - 1) / sx, (f.h + sy - 1) / sy);

@@ -13,6 +13,9 @@ class Net;
 struct FldLevel {
     int w = 0, h = 0;
     float *own = nullptr, *ax = nullptr, *ay = nullptr, *z = nullptr, *b = nullptr, *zp = nullptr;
+    // the tile smoothers' per-cell record {own, left face, up face, 1 / (own + all four faces)}, levels handled
+    // by the tile kernels only
+    float4 *cf = nullptr;
 };
 
 struct FldStats {
