@@ -26,12 +26,16 @@ DEVICE_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++
                 "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-munsafe-fp-atomics"]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", f"--offload-arch={ARCH}"]
+# Per-source additions. The hydro kernel's f32 divisions and square roots take the hardware reciprocal and square
+# root (within 2.5 ulp) instead of the correctly rounded sequences: measured 0.194 against 0.213 ms per update at
+# 1024^2, the states equal to the printed digits, every parity bar held; NaN and infinity handling is unchanged.
+SOURCE_FLAGS = {"hydro.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
     flags = DEVICE_FLAGS if src.endswith(".hip") else HOST_FLAGS
-    cmd = [HIPCC] + flags + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + flags + SOURCE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")

@@ -7,10 +7,10 @@
 //
 //   * activations are channels-last BF16 (pixel-major, channels contiguous), so a lane's eight consecutive K values
 //     of the implicit GEMM are one 16-byte load;
-//   * k_conv is the implicit-GEMM 3x3 convolution on the matrix cores: M = pixels (16 per MFMA tile, MT tiles per
-//     wave), N = output channels, K = 9 taps x input channels in chunks of 32 (v_mfma_f32_16x16x32_bf16). The same
-//     kernel computes the backward-data pass with mirrored, transposed weights and the ReLU mask applied on load;
-//     upsample + concat of the decoder is folded into its loads;
+//   * k_conv is the implicit-GEMM 3x3 convolution on the matrix cores: M = pixels (16 per MFMA tile, a 32 x 4
+//     pixel tile per block with its input staged in LDS), N = output channels, K = 9 taps x input channels in chunks
+//     of 32 (v_mfma_f32_16x16x32_bf16). The same kernel computes the backward-data pass with mirrored, transposed
+//     weights and the ReLU mask applied while staging; upsample + concat of the decoder is folded into the staging;
 //   * the weight gradient (k_dw) is a reduction over pixels: FP32 on the vector units from LDS tiles, per-block
 //     partials summed in a fixed order by k_dw_reduce (deterministic, no atomics);
 //   * the output 1x1 layer, the features and the energy-norm loss are elementwise / stencil kernels.
@@ -98,83 +98,88 @@ struct ConvArgs {
     int H, W, cup, relu;
 };
 
-// Each wave computes MT tiles of 16 pixels x all COUT channels; 4 waves per block. Lane l of an MFMA holds
-// A[m = l % 16][k = 8 (l / 16) + j], B[k = 8 (l / 16) + j][n = l % 16] and D[m = 4 (l / 16) + j][n = l % 16].
-template <int CIN, int COUT, int MT>
+// A block computes a tile of TW x TH pixels (a row of TW per wave, two 16-pixel MFMA tiles) x all COUT channels. Its
+// input -- rows y0 - 1 .. y0 + TH, columns x0 - 1 .. x0 + TW, every input channel, upsampled and concatenated or
+// masked as the layer reads it -- is staged once in LDS, so the nine taps read LDS instead of global memory. Lane l
+// of an MFMA holds A[m = l % 16][k = 8 (l / 16) + j], B[k = 8 (l / 16) + j][n = l % 16] and D[m = 4 (l / 16) + j]
+// [n = l % 16]; K runs over 9 taps x CIN channels in chunks of 32, zero past 9 CIN.
+#define TW 32
+#define TH 4
+template <int CIN, int COUT>
 __global__ void __launch_bounds__(NTH) k_conv(ConvArgs a) {
     constexpr int KP = (9 * CIN + 31) / 32 * 32;
     constexpr int NT = COUT / 16;
+    constexpr int CP = CIN + 8;                    // pixel stride in LDS (bf16), staggering the banks
+    constexpr int RW = TW + 2, RH = TH + 2, OCT = CIN / 8;
     static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+    __shared__ __attribute__((aligned(16))) bf16 tile[RH * RW * CP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
-    const int H = a.H, W = a.W, npx = H * W, cup = a.cup, cs = CIN - cup;
-    const int pw = (blockIdx.x * 4 + wave) * 16 * MT;
-    int ya[MT], xa[MT];
-    bool va[MT];
+    const int H = a.H, W = a.W, cup = a.cup, cs = CIN - cup;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    // stage the input tile (zero outside the image)
+    for (int t = threadIdx.x; t < RH * RW * OCT; t += NTH) {
+        const int o = t % OCT, px = t / OCT, lx = px % RW, ly = px / RW;
+        const int xx = x0 + lx - 1, yy = y0 + ly - 1, ci = 8 * o;
+        bf16x8 v = {};
+        if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+            if (ci < cup)
+                v = ld8(a.up + ((size_t)(yy >> 1) * (W >> 1) + (xx >> 1)) * cup + ci);
+            else
+                v = ld8(a.in + ((size_t)yy * W + xx) * cs + (ci - cup));
+            if (a.mask) {
+                const bf16x8 mv = ld8(a.mask + ((size_t)yy * W + xx) * CIN + ci);
 #pragma unroll
-    for (int m = 0; m < MT; m++) {
-        const int p = pw + 16 * m + c16;
-        va[m] = p < npx;
-        ya[m] = p / W;
-        xa[m] = p - ya[m] * W;
+                for (int j = 0; j < 8; j++)
+                    if (!((float)mv[j] > 0.0f)) v[j] = (bf16)0.0f;
+            }
+        }
+        st8(tile + (size_t)px * CP + ci, v);
     }
-    f32x4 acc[MT][NT];
+    __syncthreads();
+    f32x4 acc[2][NT];
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+    for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int kc = 0; kc < KP; kc += 32) {
         const int k0 = kc + 8 * g;
         const int tap = k0 / CIN, ci = k0 - tap * CIN;
-        const int ty = tap / 3;
-        const int dy = ty - 1, dx = tap - 3 * ty - 1;
+        const int ty = tap / 3, tx = tap - 3 * ty;
         bf16x8 bv[NT];
 #pragma unroll
         for (int t = 0; t < NT; t++) bv[t] = ld8(a.wt + (size_t)(t * 16 + c16) * KP + k0);
 #pragma unroll
-        for (int m = 0; m < MT; m++) {
+        for (int m = 0; m < 2; m++) {
             bf16x8 av = {};
-            const int yy = ya[m] + dy, xx = xa[m] + dx;
-            if (tap < 9 && va[m] && yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                if (ci < cup)
-                    av = ld8(a.up + ((size_t)(yy >> 1) * (W >> 1) + (xx >> 1)) * cup + ci);
-                else
-                    av = ld8(a.in + ((size_t)yy * W + xx) * cs + (ci - cup));
-                if (a.mask) {
-                    const bf16x8 mv = ld8(a.mask + ((size_t)yy * W + xx) * CIN + ci);
-#pragma unroll
-                    for (int j = 0; j < 8; j++)
-                        if (!((float)mv[j] > 0.0f)) av[j] = (bf16)0.0f;
-                }
-            }
+            if (tap < 9) av = ld8(tile + (size_t)((wave + ty) * RW + 16 * m + c16 + tx) * CP + ci);
 #pragma unroll
             for (int t = 0; t < NT; t++)
                 acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[t], acc[m][t], 0, 0, 0);
         }
     }
+    const int y = y0 + wave;
+    if (y >= H) return;
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+    for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             const int co = t * 16 + c16;
             const float b = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int p = pw + 16 * m + 4 * g + j;
-                if (p < npx) {
+                const int x = x0 + 16 * m + 4 * g + j;
+                if (x < W) {
                     float v = acc[m][t][j] + b;
                     if (a.relu) v = fmaxf(v, 0.0f);
-                    a.out[(size_t)p * COUT + co] = (bf16)v;
+                    a.out[((size_t)y * W + x) * COUT + co] = (bf16)v;
                 }
             }
         }
 }
 
-#define CONV_MT 2
-
 template <int CIN, int COUT> static void conv(hipStream_t s, const ConvArgs &a) {
-    const int npx = a.H * a.W;
-    const int blocks = (npx + 64 * CONV_MT - 1) / (64 * CONV_MT);
-    hipLaunchKernelGGL((k_conv<CIN, COUT, CONV_MT>), dim3(blocks), dim3(NTH), 0, s, a);
+    const dim3 grid((a.W + TW - 1) / TW, (a.H + TH - 1) / TH);
+    hipLaunchKernelGGL((k_conv<CIN, COUT>), grid, dim3(NTH), 0, s, a);
 }
 
 // 2x2 mean, one thread per (output pixel, channel octet)

@@ -113,3 +113,41 @@ class Net:
         if getattr(self, "_p", None) and _LIB is not None:
             _LIB.sr_net_destroy(self._p)
             self._p = None
+
+
+def _time_main():
+    """python -m sandbox_reactions_amd.net --time N [--reps R]: the committed network on the N^2 benchmark FLD system,
+    R applications timed with HIP events (the kernels a counter run attributes to the network)."""
+    import argparse
+    import sys
+    import torch
+    import sandbox_reactions_amd as sr
+    from . import scenes, train
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--time", type=int, default=2048)
+    ap.add_argument("--reps", type=int, default=10)
+    a = ap.parse_args()
+    sr.require_device()
+    n = a.time
+    c, kx, ky, _, _ = scenes.fld_system(n)
+    t = lambda v: torch.tensor(np.ascontiguousarray(v, np.float32), device="cuda")
+    own, tkx, tky = t(c), t(kx), t(ky)
+    net = Net(n, n, train.load())
+    r = torch.randn((n, n), device="cuda")
+    z = torch.empty_like(r)
+    net.apply(own, tkx, tky, r, z)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.reps):
+        net.apply(own, tkx, tky, r, z)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.reps
+    flops = 2.0 * n * n * (sum(9 * ci * co * f for (_, ci, co), f in
+                               zip(LAYERS, (1, 1, .25, .25, 1 / 16, 1 / 16, .25, .25, 1, 1))) + 16)
+    print(f"network {n}^2: {ms:.3f} ms per application, {flops / ms / 1e9:.1f} TFLOP/s "
+          f"({flops / ms / 1e9 / 2500.0 * 100:.1f} % of 2.5 PFLOP/s dense BF16)", file=sys.stderr, flush=True)
+
+
+if __name__ == "__main__":
+    _time_main()
