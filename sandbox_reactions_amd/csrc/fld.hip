@@ -668,8 +668,7 @@ FldSolver::~FldSolver() {
     for (void *x : ps)
         if (x) (void)hipFree(x);
     if (host_part) (void)hipHostFree(host_part);
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
 }
 
 static size_t level_floats(int w, int h) { return (size_t)w * h + (size_t)(w + 1) * h + (size_t)w * (h + 1); }
@@ -764,8 +763,6 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_) {
     SR_HIP(hipMalloc(&part_dots, sizeof(double) * 2 * 2 * nbd_max));
     SR_HIP(hipMemset(part_dots, 0, sizeof(double) * 2 * 2 * nbd_max));
     SR_HIP(hipHostMalloc(&host_part, sizeof(double) * 4 * nbe));
-    SR_HIP(hipEventCreate(&ev0));
-    SR_HIP(hipEventCreate(&ev1));
     SR_HIP(hipFuncSetAttribute((const void *)k_coarse, hipFuncAttributeMaxDynamicSharedMemorySize, COARSE_LDS));
     return SR_OK;
 }
@@ -847,7 +844,25 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         const int sx = small ? 16 : tx, sy = small ? 16 : ty;
         dim3 grid((f.w + sx - 1) / sx, (f.h + sy - 1) / sy);
         const bool t0 = timing && l == 0;
-        if (t0) SR_HIP(hipEventRecord(ev0, s));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (t0) {
+            if (ev_used + 2 > 8192) {  // a bounded pool: fold what is recorded into the sums first
+                double ms;
+                unsigned long long nl;
+                SR_TRY(smooth_time(&ms, &nl));
+            }
+            if (ev_used + 2 > evs.size()) {  // grow the pool; recorded pairs are summed when read
+                for (int k = 0; k < 64; k++) {
+                    hipEvent_t e;
+                    SR_HIP(hipEventCreate(&e));
+                    evs.push_back(e);
+                }
+            }
+            e0 = evs[ev_used];
+            e1 = evs[ev_used + 1];
+            ev_used += 2;
+            SR_HIP(hipEventRecord(e0, s));
+        }
         if (small)
             launch_pre<16, 16>(nu1, grid, s, L, bl, f.z, c.b, c.w);
         else if (tx == PTX)
@@ -855,12 +870,7 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         else
             launch_pre<32, PTY>(nu1, grid, s, L, bl, f.z, c.b, c.w);
         if (t0) {
-            SR_HIP(hipEventRecord(ev1, s));
-            SR_HIP(hipEventSynchronize(ev1));
-            float ms = 0.f;
-            SR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-            smooth_ms += ms;
-            smooth_launches++;
+            SR_HIP(hipEventRecord(e1, s));
         }
         bl = c.b;
     }
@@ -921,6 +931,26 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
     }
     SR_HIP(hipGetLastError());
     return SR_OK;
+}
+
+int FldSolver::smooth_time(double *ms, unsigned long long *launches) {
+    for (size_t k = 0; k + 1 < ev_used; k += 2) {
+        SR_HIP(hipEventSynchronize(evs[k + 1]));
+        float t = 0.f;
+        SR_HIP(hipEventElapsedTime(&t, evs[k], evs[k + 1]));
+        smooth_ms += t;
+        smooth_launches++;
+    }
+    ev_used = 0;
+    *ms = smooth_ms;
+    *launches = smooth_launches;
+    return SR_OK;
+}
+
+void FldSolver::reset_smooth_time() {
+    ev_used = 0;
+    smooth_ms = 0.0;
+    smooth_launches = 0;
 }
 
 int FldSolver::set_network(Net *n, int fallback) {

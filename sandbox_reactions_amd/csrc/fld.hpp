@@ -59,11 +59,12 @@ class FldSolver {
     int width() const { return w0; }
     int height() const { return h0; }
     double bytes_per_iteration() const;
-    // kernel timing of the level-0 smoother (the solver's dominant kernel)
+    // kernel timing of the level-0 pre-smoother: event pairs recorded on the stream without waiting, summed when
+    // read (smooth_time), so timing does not stall the solve
     void set_timing(bool on) { timing = on; }
+    int smooth_time(double *ms, unsigned long long *launches);
+    void reset_smooth_time();
     bool timing = false;
-    double smooth_ms = 0.0;
-    unsigned long long smooth_launches = 0;
 
   private:
     int w0 = 0, h0 = 0, nu1 = 2, nu2 = 2, coarse_from = 0;
@@ -79,7 +80,10 @@ class FldSolver {
     // per-block partial sums (summed by the kernel that consumes them, or on the host)
     double *part_true = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_dots = nullptr;
     double *host_part = nullptr; // pinned
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evs;   // pairs: start, end
+    size_t ev_used = 0;
+    double smooth_ms = 0.0;
+    unsigned long long smooth_launches = 0;
     Net *net = nullptr;
     int fallback_it = 0;
     double plain_rate = 0.0;  // ln(residual reduction) per iteration of the last converged plain multigrid solve

@@ -633,8 +633,7 @@ extern "C" int sr_set_timing(sr_world *wd, int on) {
     wd->k_update = wd->k_mid = wd->k_grav = wd->k_pre0 = KernelClock{};
     if (wd->sw.heat) {
         wd->heat.solver.set_timing(wd->timing);
-        wd->heat.solver.smooth_ms = 0.0;
-        wd->heat.solver.smooth_launches = 0;
+        wd->heat.solver.reset_smooth_time();
     }
     return SR_OK;
 }
@@ -665,8 +664,7 @@ extern "C" int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launch
         bytes = n * (60.0 + 8.0 + 4.0);
     } else if (k == "mg_pre0") {
         if (!wd->sw.heat) return fail(SR_ERR_ARGUMENT, "no heat pass");
-        c.ms = wd->heat.solver.smooth_ms;
-        c.launches = wd->heat.solver.smooth_launches;
+        SR_TRY(wd->heat.solver.smooth_time(&c.ms, &c.launches));
         bytes = n * 21.0; // b, own, ax, ay in; z and the coarse b out
     } else {
         return fail(SR_ERR_ARGUMENT, "unknown kernel " + k);
