@@ -88,11 +88,16 @@ def main():
     s = summarize(a.fetch_dir, a.write_dir)
     s["workload"] = a.workload
     s["command"] = a.command
-    with open(a.out_json, "w") as f:
+    write_outputs(s, a.out_json)
+
+
+def write_outputs(s, out_json):
+    """The summary as JSON and, beside it, a Markdown table."""
+    with open(out_json, "w") as f:
         json.dump(s, f, indent=1, sort_keys=True)
-    md = os.path.splitext(a.out_json)[0] + ".md"
+    md = os.path.splitext(out_json)[0] + ".md"
     with open(md, "w") as f:
-        f.write(f"# HBM traffic per launch: {a.workload}\n\nCommands: `{a.command}`\n\n")
+        f.write(f"# HBM traffic per launch: {s['workload']}\n\nCommands: `{s['command']}`\n\n")
         f.write(f"Counters scaled by the 512 MiB one-float-per-lane calibration copy: FETCH "
                 f"x{s['copy_scale']['fetch']:.4f}, WRITE x{s['copy_scale']['write']:.4f} (the same counters over other "
                 f"copies of 512 MiB would need: {json.dumps(s['other_copy_scales'])}).\n\n")
