@@ -300,8 +300,10 @@ extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_swit
     int c;
     if (hipStreamCreateWithFlags(&wd->s, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(SR_ERR_DEVICE, "hipStreamCreate"));
+    // the pass clocks' events only time: no system-scope fence (an L2 write-back) at each mark
     for (auto &e : wd->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(SR_ERR_DEVICE, "hipEventCreate"));
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
+            return bail(fail(SR_ERR_DEVICE, "hipEventCreate"));
     const size_t n = wd->n;
     if ((c = hydro_state_alloc(wd, wd->st[0])) || (c = hydro_state_alloc(wd, wd->st[1]))) return bail(c);
     if ((c = world_alloc(wd, (void **)&wd->scal, sizeof(double) * SCAL_N))) return bail(c);
@@ -665,7 +667,9 @@ extern "C" int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launch
     } else if (k == "mg_pre0") {
         if (!wd->sw.heat) return fail(SR_ERR_ARGUMENT, "no heat pass");
         SR_TRY(wd->heat.solver.smooth_time(&c.ms, &c.launches));
-        bytes = n * 21.0; // b, own, ax, ay in; z and the coarse b out
+        // b and the operator (own, left and up faces) in; z and the coarse b out. The stored inverse diagonal is a
+        // derived copy of the operator and is not counted.
+        bytes = n * 21.0;
     } else {
         return fail(SR_ERR_ARGUMENT, "unknown kernel " + k);
     }
