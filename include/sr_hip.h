@@ -57,7 +57,7 @@ typedef struct {
     int pin_temperature;         /* heat test flags (reference/heat.rs:62-74) */
     double constant_diffusivity; /* > 0 switches it on */
     int flux_limiter;
-    int max_padded;     /* FFT cap per padded axis; the reference's is 2048 (step/fft.rs:26-63), this build's default 4096 */
+    int max_padded;     /* FFT cap per padded axis; the reference's is 2048 (step/fft.rs:26-63), the library's default 4096 */
 } sr_physics;
 
 typedef struct {
@@ -78,6 +78,9 @@ typedef struct {
                             * SR_STOP_RELATIVE: inexact Newton, a pass solved to forcing x the previous pass's
                             * relative heat change (at most 1e-2, at least solve_tol); the pass that meets the Newton
                             * criteria after a loose solve is followed by one solved to solve_tol */
+    double coarse_scale;   /* omega, the scale of every coarse-grid correction: 1 (or 0) is the reference's plain
+                            * prolongation; in (0, 2) the V-cycle stays an SPD preconditioner, and about 1.2 cuts the
+                            * iterations of the stiff lognormal systems by a tenth */
 } sr_heat_settings;
 
 typedef struct {

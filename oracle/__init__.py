@@ -87,6 +87,8 @@ lib.sro_fld_matvec.argtypes = [_sz, _sz, _dp, _dp, _dp, C.c_void_p, _dp, _dp]
 lib.sro_fld_pcg.argtypes = [_sz, _sz, _dp, _dp, _dp, C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int,
                             C.c_int, C.c_int, C.POINTER(C.c_double)]
 lib.sro_mg_apply.argtypes = [_sz, _sz, _dp, _dp, _dp, C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_int]
+lib.sro_mg_set_coarse_scale.argtypes = [C.c_double]
+lib.sro_mg_set_coarse_scale.restype = None
 lib.sro_floors_apply.argtypes = [_PP, _dp, _sz, _sz, C.POINTER(C.c_double), C.POINTER(C.c_double)]
 lib.sro_twin_new.argtypes = [_PP, _sz, _sz, _dp, C.c_int, C.c_int, C.c_int]
 lib.sro_twin_new.restype = C.c_void_p
@@ -218,8 +220,10 @@ def fld_matvec(own, kx, ky, gas, v):
     return out
 
 
-def fld_pcg(own, kx, ky, gas, b, x0=None, tol=1e-8, max_iter=10000, precond="mg", levels=0, nu1=2, nu2=2):
+def fld_pcg(own, kx, ky, gas, b, x0=None, tol=1e-8, max_iter=10000, precond="mg", levels=0, nu1=2, nu2=2,
+            coarse_scale=1.0):
     h, w = own.shape
+    lib.sro_mg_set_coarse_scale(coarse_scale)
     x = np.zeros((h, w)) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
     g = None if gas is None else np.ascontiguousarray(gas, np.uint8)
     rel = C.c_double()
@@ -229,8 +233,9 @@ def fld_pcg(own, kx, ky, gas, b, x0=None, tol=1e-8, max_iter=10000, precond="mg"
     return x, it, rel.value
 
 
-def mg_apply(own, kx, ky, gas, r, levels=0, nu1=2, nu2=2):
+def mg_apply(own, kx, ky, gas, r, levels=0, nu1=2, nu2=2, coarse_scale=1.0):
     h, w = own.shape
+    lib.sro_mg_set_coarse_scale(coarse_scale)
     z = np.zeros((h, w))
     g = None if gas is None else np.ascontiguousarray(gas, np.uint8)
     lib.sro_mg_apply(w, h, np.ascontiguousarray(own), np.ascontiguousarray(kx), np.ascontiguousarray(ky),

@@ -10,14 +10,20 @@
  * builds the same aggregation), coarsened until a level holds at most 16 cells. Smoothing is exact red-black
  * Gauss-Seidel, red first on the way down and black first on the way up (the adjoint order, so the preconditioner
  * is symmetric); the coarsest level is solved exactly (dense Cholesky) when it holds at most 16 cells, else (a
- * capped level count) by 16 forward and 16 backward sweeps. The CG is the flexible (Polak-Ribiere) form. The device solver applies the same
- * operator in f32; this model pins it.
+ * capped level count) by 16 forward and 16 backward sweeps. The coarse correction may be scaled by omega (1 is the
+ * reference's plain prolongation; the preconditioner stays SPD for 0 < omega < 2, sro_mg_set_coarse_scale). The CG is
+ * the flexible (Polak-Ribiere) form. The device solver applies the same operator in f32; this model pins it.
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "sro.h"
+
+/* the coarse correction's scale (1: the reference's plain prolongation) */
+static double g_omega = 1.0;
+
+void sro_mg_set_coarse_scale(double omega) { g_omega = omega; }
 
 #define COARSE_SWEEPS 16
 
@@ -213,7 +219,7 @@ static void vcycle(hier_t *H, int L, int nu1, int nu2) {
         for (size_t x = 0; x < l->w; x++) {
             size_t i = y * l->w + x;
             if (L == 0 && H->gas && !H->gas[i]) continue;
-            l->z[i] += c->z[(y / 2) * c->w + x / 2];
+            l->z[i] += g_omega * c->z[(y / 2) * c->w + x / 2];
         }
     for (int k = 0; k < nu2; k++) {
         half_sweep(l, 1);

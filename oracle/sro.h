@@ -105,6 +105,7 @@ int sro_fld_pcg(size_t w, size_t h, const double *own, const double *kx, const d
                 const double *b, double *x, double tol, int max_iter, int precond, int levels, int nu1, int nu2,
                 double *final_rel);
 /* one application z = M^-1 r of the multigrid preconditioner (f64 model) */
+void sro_mg_set_coarse_scale(double omega); /* the coarse correction's scale for both calls; 1 by default */
 int sro_mg_apply(size_t w, size_t h, const double *own, const double *kx, const double *ky, const uint8_t *gas,
                  const double *r, double *z, int levels, int nu1, int nu2);
 

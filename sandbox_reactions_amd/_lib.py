@@ -43,7 +43,8 @@ class Switches(C.Structure):
 class HeatSettings(C.Structure):
     _fields_ = [("stop", C.c_int), ("solve_tol", C.c_double), ("newton_heat", C.c_double),
                 ("newton_cell", C.c_double), ("max_passes", C.c_int), ("max_rounds", C.c_int),
-                ("levels", C.c_int), ("nu1", C.c_int), ("nu2", C.c_int), ("forcing", C.c_double)]
+                ("levels", C.c_int), ("nu1", C.c_int), ("nu2", C.c_int), ("forcing", C.c_double),
+                ("coarse_scale", C.c_double)]
 
 
 class DtReport(C.Structure):

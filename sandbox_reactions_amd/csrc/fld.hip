@@ -352,12 +352,12 @@ __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, f
     }
 }
 
-// Prolongation of the coarse correction and post-smoothing of level L (nu sweeps, black first). Halo 2 nu. With
-// `part` (level 0), partials of r.z_new and r.z_old.
+// Prolongation of the coarse correction (scaled by omega) and post-smoothing of level L (nu sweeps, black first).
+// Halo 2 nu. With `part` (level 0), partials of r.z_new and r.z_old.
 template <int TX, int TY, int NU>
 __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, const float *__restrict__ zin,
-                                             const float *__restrict__ zc, int wc, float *__restrict__ zout,
-                                             const float *__restrict__ zold, double *part) {
+                                             const float *__restrict__ zc, int wc, float omega,
+                                             float *__restrict__ zout, const float *__restrict__ zold, double *part) {
     constexpr int HW = 2 * NU;
     using RG = Region<TX, TY, HW>;
     __shared__ RG R;
@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, 
             const int lx = t % RG::RX, ly = t / RG::RX;
             const int gx = x0 + lx - HW, gy = y0 + ly - HW;
             const bool in = t < RG::N && gx >= 0 && gx < L.w && gy >= 0 && gy < L.h;
-            zv[k] = in ? zin[(size_t)gy * L.w + gx] + zc[(size_t)(gy >> 1) * wc + (gx >> 1)] : 0.0f;
+            zv[k] = in ? zin[(size_t)gy * L.w + gx] + omega * zc[(size_t)(gy >> 1) * wc + (gx >> 1)] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < IT; k++) {
@@ -418,7 +418,7 @@ __device__ __forceinline__ int row_of(int t, int w, int lw) { return lw >= 0 ? t
 
 __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *__restrict__ pack,
                                                       const float *__restrict__ b0, float *__restrict__ z0, int nu1,
-                                                      int nu2) {
+                                                      int nu2, float omega) {
     extern __shared__ float sm[];
     const int *offo = C.offo, *offx = C.offx, *offy = C.offy, *offz = C.offz, *offb = C.offb, *offd = C.offd;
     const int offinv = C.offinv, total = C.total;
@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
         const float *zc = sm + offz[l + 1];
         for (int t = threadIdx.x; t < w * h; t += COARSE_NT) {
             const int y = row_of(t, w, C.lw[l]), x = t - y * w;
-            z[t] += zc[(y >> 1) * wc + (x >> 1)];
+            z[t] += omega * zc[(y >> 1) * wc + (x >> 1)];
         }
         __syncthreads();
         for (int k = 0; k < nu2; k++) {
@@ -676,9 +676,13 @@ FldSolver::~FldSolver() {
 
 static size_t level_floats(int w, int h) { return (size_t)w * h + (size_t)(w + 1) * h + (size_t)w * (h + 1); }
 
-int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_) {
+int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_) {
     if (w <= 0 || h <= 0) return fail(SR_ERR_ARGUMENT, "an empty FLD grid");
     if (nu1_ < 1 || nu1_ > 3 || nu2_ < 1 || nu2_ > 3) return fail(SR_ERR_ARGUMENT, "nu1, nu2 must be in 1..3");
+    if (omega_ == 0.0) omega_ = 1.0;
+    if (!(omega_ > 0.0 && omega_ < 2.0))
+        return fail(SR_ERR_ARGUMENT, "the coarse correction scale must lie in (0, 2)");
+    omega = (float)omega_;
     w0 = w;
     h0 = h;
     nu1 = nu1_;
@@ -821,13 +825,13 @@ static void launch_pre(int nu, dim3 grid, hipStream_t s, Lv L, const float *b, f
 
 template <int TX, int TY>
 static void launch_post(int nu, dim3 grid, hipStream_t s, Lv L, const float *b, const float *zin, const float *zc,
-                        int wc, float *zout, const float *zold, double *part) {
+                        int wc, float omega, float *zout, const float *zold, double *part) {
     if (nu == 1)
-        hipLaunchKernelGGL((k_post<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, zout, zold, part);
+        hipLaunchKernelGGL((k_post<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
     else if (nu == 2)
-        hipLaunchKernelGGL((k_post<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, zout, zold, part);
+        hipLaunchKernelGGL((k_post<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
     else
-        hipLaunchKernelGGL((k_post<32, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, zout, zold, part);
+        hipLaunchKernelGGL((k_post<32, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
 }
 
 // Levels below level 0 that would launch fewer than 512 blocks of 32x32 take 16x16 tiles: four times the blocks
@@ -909,7 +913,7 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         const size_t floats = o;
         float *zc = coarse_from == 0 ? z : lv[coarse_from].zp;
         hipLaunchKernelGGL(k_coarse, dim3(1), dim3(COARSE_NT), sizeof(float) * floats, s, C, (const float *)pack, bl,
-                           zc, nu1, nu2);
+                           zc, nu1, nu2, omega);
         if (coarse_from == 0 && dots) {
             const int n = w0 * h0;
             hipLaunchKernelGGL(k_dots, dim3(nbe), dim3(NT), 0, s, n, r, z, zold, dots);
@@ -926,11 +930,11 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         float *zo = l == 0 ? z : f.zp;
         double *pp = (l == 0 && dots) ? dots : nullptr;
         if (small)
-            launch_post<16, 16>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
+            launch_post<16, 16>(nu2, grid, s, L, b, f.z, c.zp, c.w, omega, zo, zold, pp);
         else if (tx == PTX)
-            launch_post<PTX, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
+            launch_post<PTX, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, omega, zo, zold, pp);
         else
-            launch_post<32, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, zo, zold, pp);
+            launch_post<32, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, omega, zo, zold, pp);
         if (l == 0 && dots) *nbd = grid.x * grid.y;
     }
     SR_HIP(hipGetLastError());
@@ -1150,7 +1154,7 @@ extern "C" int sr_fld_create(int w, int h, const sr_heat_settings *hs, sr_fld **
     sr_heat_settings_defaults(&d);
     if (!hs) hs = &d;
     auto *f = new sr_fld();
-    int c = f->s.init(w, h, hs->levels, hs->nu1, hs->nu2);
+    int c = f->s.init(w, h, hs->levels, hs->nu1, hs->nu2, hs->coarse_scale);
     if (c != SR_OK) {
         delete f;
         return c;

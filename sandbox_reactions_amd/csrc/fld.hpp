@@ -37,7 +37,8 @@ struct RefStop {
 class FldSolver {
   public:
     ~FldSolver();
-    int init(int w, int h, int levels, int nu1, int nu2);
+    // omega scales the coarse-grid correction (1: plain prolongation; SPD for 0 < omega < 2)
+    int init(int w, int h, int levels, int nu1, int nu2, double omega = 1.0);
     // level 0's operator from device arrays (own, couplings ax/ay, gas mask for the coarsening); builds the
     // Galerkin hierarchy. own/ax/ay are copied.
     int set_system(const float *own, const float *ax, const float *ay, const uint8_t *gas, hipStream_t s);
@@ -68,6 +69,7 @@ class FldSolver {
 
   private:
     int w0 = 0, h0 = 0, nu1 = 2, nu2 = 2, coarse_from = 0;
+    float omega = 1.0f;
     int nbe = 1, ntiles0 = 1, nbd_max = 1; // blocks of the elementwise kernels, level-0 tiles
     int tile_x = 32;
     std::vector<FldLevel> lv;

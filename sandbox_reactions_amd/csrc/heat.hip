@@ -325,7 +325,7 @@ int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
     SR_HIP(hipMalloc(&xs, sizeof(double) * n));
     SR_HIP(hipMalloc(&part, sizeof(double) * 4 * nb));
     SR_HIP(hipHostMalloc(&host, sizeof(double) * 4 * nb));
-    return solver.init(P.w, P.h, hs.levels, hs.nu1, hs.nu2);
+    return solver.init(P.w, P.h, hs.levels, hs.nu1, hs.nu2, hs.coarse_scale);
 }
 
 // the k sums of the last kernel's partials, in block order
@@ -370,19 +370,22 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     while (passes < hs.max_passes && rounds < hs.max_rounds) {
         passes++;
         hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
-        double b2[2];
-        SR_TRY(read_sums(2, b2, s));
+        // the reference's two bounds are needed (and read back) only under its stopping rule
+        RefStop ref;
+        if (hs.stop == SR_STOP_REFERENCE) {
+            double b2[2];
+            SR_TRY(read_sums(2, b2, s));
+            ref.heat = H.heat;
+            ref.slope = H.slope;
+            ref.u_bound = eps * eps * b2[0];
+            ref.t_bound = eps * eps * b2[1];
+            if (!(std::isfinite(ref.u_bound) && std::isfinite(ref.t_bound))) {
+                passes--;
+                break;
+            }
+        }
         SR_TRY(solver.rebuild(faces, s));
         faces = false;
-        RefStop ref;
-        ref.heat = H.heat;
-        ref.slope = H.slope;
-        ref.u_bound = eps * eps * b2[0];
-        ref.t_bound = eps * eps * b2[1];
-        if (!(std::isfinite(ref.u_bound) && std::isfinite(ref.t_bound))) {
-            passes--;
-            break;
-        }
         FldStats fs;
         SR_TRY(solver.solve(rhs, xs, eta, hs.max_rounds - rounds, precond,
                             hs.stop == SR_STOP_REFERENCE ? &ref : nullptr, &fs, s));

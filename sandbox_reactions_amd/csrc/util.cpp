@@ -76,4 +76,5 @@ extern "C" void sr_heat_settings_defaults(sr_heat_settings *hs) {
     hs->levels = 0;
     hs->nu1 = 2;
     hs->nu2 = 2;
+    hs->coarse_scale = 1.0;
 }

@@ -11,10 +11,10 @@ class Fld:
     """Solver for the 5-point SPD system own_i v_i + sum_f k_f (v_i - v_j) on a w x h grid (float32 own, kx
     (h, w+1), ky (h+1, w) couplings on CUDA tensors; gas uint8 (h, w) or None)."""
 
-    def __init__(self, w: int, h: int, levels: int = 0, nu1: int = 2, nu2: int = 2):
+    def __init__(self, w: int, h: int, levels: int = 0, nu1: int = 2, nu2: int = 2, coarse_scale: float = 1.0):
         from . import _check, heat_settings, lib
         self.w, self.h = w, h
-        hs = heat_settings(levels=levels, nu1=nu1, nu2=nu2)
+        hs = heat_settings(levels=levels, nu1=nu1, nu2=nu2, coarse_scale=coarse_scale)
         p = C.c_void_p()
         _check(lib().sr_fld_create(w, h, C.byref(hs), C.byref(p)))
         self._p = p

@@ -17,21 +17,22 @@ def system(n, seed=0x5EED0000, stiffness=1e3):
     return own, kx, ky, b
 
 
-def upload(sr, n, own, kx, ky, gas=None, levels=0):
+def upload(sr, n, own, kx, ky, gas=None, levels=0, coarse_scale=1.0):
     import torch
-    f = sr.Fld(n, n, levels=levels)
+    f = sr.Fld(n, n, levels=levels, coarse_scale=coarse_scale)
     t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
     g = None if gas is None else torch.tensor(gas.astype(np.uint8), device="cuda")
     f.set_system(t(own), t(kx), t(ky), g)
     return f
 
 
+@pytest.mark.parametrize("omega", [1.0, 1.2])
 @pytest.mark.parametrize("n", [64, 128, 256])
-def test_mg_pcg_solves_to_1e8_and_matches_the_f64_model(gpu_lib, n):
+def test_mg_pcg_solves_to_1e8_and_matches_the_f64_model(gpu_lib, n, omega):
     import torch
-    print(f"fld {n}", flush=True)
+    print(f"fld {n} coarse scale {omega}", flush=True)
     own, kx, ky, b = system(n)
-    f = upload(gpu_lib, n, own, kx, ky)
+    f = upload(gpu_lib, n, own, kx, ky, coarse_scale=omega)
     x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
     bt = torch.tensor(b, device="cuda")
     it, rel = f.solve(bt, x, tol=1e-8)
@@ -41,20 +42,21 @@ def test_mg_pcg_solves_to_1e8_and_matches_the_f64_model(gpu_lib, n):
     assert rel <= 1e-8 and true_rel <= 1.01e-8, (rel, true_rel)
     xo, ito, relo = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-10, precond="mg")
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-7
-    _, it_model, _ = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-8, precond="mg")
+    _, it_model, _ = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-8, precond="mg", coarse_scale=omega)
     print(f"iterations gpu {it} model {it_model}")
     assert it <= 2 * it_model + 6
 
 
-def test_the_v_cycle_is_the_models(gpu_lib):
+@pytest.mark.parametrize("omega", [1.0, 1.2])
+def test_the_v_cycle_is_the_models(gpu_lib, omega):
     import torch
     n = 128
     own, kx, ky, _ = system(n)
-    f = upload(gpu_lib, n, own, kx, ky)
+    f = upload(gpu_lib, n, own, kx, ky, coarse_scale=omega)
     r = np.random.default_rng(1).standard_normal((n, n))
     z = torch.empty((n, n), dtype=torch.float32, device="cuda")
     f.apply_precond(torch.tensor(r.astype(np.float32), device="cuda"), z)
-    zm = oracle.mg_apply(own, kx, ky, None, r.astype(np.float32).astype(np.float64))
+    zm = oracle.mg_apply(own, kx, ky, None, r.astype(np.float32).astype(np.float64), coarse_scale=omega)
     assert np.linalg.norm(z.cpu().numpy() - zm) / np.linalg.norm(zm) <= 1e-5
     assert f.levels == 6  # 128 -> 64 -> 32 -> 16 -> 8 -> 4
 
@@ -99,6 +101,9 @@ def test_gas_mask_and_jacobi(gpu_lib):
 def test_solver_refusals(gpu_lib):
     with pytest.raises(gpu_lib.SrError):
         gpu_lib.Fld(64, 64, nu1=0)
+    for omega in (-1.0, 2.0, 2.5):
+        with pytest.raises(gpu_lib.SrError):
+            gpu_lib.Fld(64, 64, coarse_scale=omega)
 
 
 def test_level_caps(gpu_lib):

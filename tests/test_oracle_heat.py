@@ -4,7 +4,8 @@ of the reference's heat tests (crates/sr-engine/tests/gpu/heat.rs:343-1827):
 
 - the face-form matvec against a matrix assembled independently here (scipy.sparse), and both solvers (Jacobi-PCG,
   MG-PCG) against a sparse direct solve;
-- the V-cycle: exact on a grid small enough for the coarsest dense solve; symmetric and positive otherwise;
+- the V-cycle: exact on a grid small enough for the coarsest dense solve; with no smoothing, the (scaled) Galerkin
+  coarse solve omega P (P^T A P)^-1 P^T in closed form; symmetric and positive otherwise;
 - backward Euler under constant diffusivity: the pulse's second moment grows by exactly 2 chi dt per axis (the discrete
   Laplacian of x^2 is 2), and heat is conserved;
 - the Levermore-Pomraning law: the thick-gas limit (4/3) a_r c_sb T^3 / (kappa Sigma) dT/dx, and the free-streaming
@@ -101,6 +102,39 @@ def test_vcycle_on_a_coarsest_grid_is_the_exact_inverse():
     r = np.random.default_rng(8).normal(size=(4, 4))
     z = oracle.mg_apply(own, kx, ky, gas, r)
     assert np.max(np.abs(z.ravel() - np.linalg.solve(A, r.ravel()))) <= 1e-12 * np.max(np.abs(z))
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.2])
+def test_two_level_correction_is_the_scaled_galerkin_solve(omega):
+    """With no smoothing and an exact coarsest level (8 x 8 -> 4 x 4), the cycle is omega P (P^T A P)^-1 P^T r with P
+    the piecewise-constant 2 x 2 aggregation: pins the coarse operator and where the scale enters."""
+    own, kx, ky, gas = random_system(8, 8, 11)
+    A = assemble(own, kx, ky, gas).toarray()
+    P = np.zeros((64, 16))
+    for y in range(8):
+        for x in range(8):
+            P[y * 8 + x, (y // 2) * 4 + x // 2] = 1.0
+    r = np.random.default_rng(12).normal(size=(8, 8))
+    z = oracle.mg_apply(own, kx, ky, gas, r, levels=2, nu1=0, nu2=0, coarse_scale=omega)
+    zc = omega * P @ np.linalg.solve(P.T @ A @ P, P.T @ r.ravel())
+    assert np.max(np.abs(z.ravel() - zc)) <= 1e-12 * np.max(np.abs(zc))
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.2])
+def test_scaled_vcycle_stays_symmetric_positive_and_helps_cg(omega):
+    w = h = 128
+    own, kx, ky, gas = random_system(w, h, 13, contrast=2.0)
+    own *= 1e-3
+    rng = np.random.default_rng(14)
+    r1, r2 = rng.normal(size=(2, h, w))
+    z1 = oracle.mg_apply(own, kx, ky, gas, r1, coarse_scale=omega)
+    z2 = oracle.mg_apply(own, kx, ky, gas, r2, coarse_scale=omega)
+    a, b = np.sum(r2 * z1), np.sum(r1 * z2)
+    assert abs(a - b) <= 1e-10 * np.sqrt(np.sum(r1 * z1) * np.sum(r2 * z2))
+    assert np.sum(r1 * z1) > 0 and np.sum(r2 * z2) > 0
+    _, it, rel = oracle.fld_pcg(own, kx, ky, gas, r1, tol=1e-8, coarse_scale=omega)
+    _, it1, _ = oracle.fld_pcg(own, kx, ky, gas, r1, tol=1e-8)
+    assert rel <= 1e-8 and it <= it1 + 2, (it, it1)
 
 
 @pytest.mark.parametrize("holes", [False, True])
