@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--step-n", type=int, default=N_STEP)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fld", action="store_true", help="skip the standalone FLD solve (counter runs)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-scene ensemble")
+    ap.add_argument("--ensemble-threads", type=int, default=16)
     return ap.parse_args()
 
 
@@ -159,6 +161,22 @@ def copy_bandwidth(mib=512, reps=10):
     return out
 
 
+def event_bracket_ms(reps=512):
+    """The mean time a pair of timing events reads around an empty kernel that follows another (libsrcalib.so): the
+    part of an event-timed kernel mean that a profiler's kernel trace does not count."""
+    import ctypes
+    import torch
+    from sandbox_reactions_amd import build
+    cal = ctypes.CDLL(build.CALIB_LIB)
+    cal.sr_calib_event_bracket.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ms = ctypes.c_double()
+    cal.sr_calib_event_bracket(16, stream, ctypes.byref(ms))
+    if cal.sr_calib_event_bracket(reps, stream, ctypes.byref(ms)) != 0:
+        return None
+    return round(ms.value, 5)
+
+
 def measured_traffic(kernel, n):
     """HBM bytes per launch of `kernel` at n^2 from the committed counter summary (rocprofv3 FETCH_SIZE and
     WRITE_SIZE runs, each scaled by what the counter reads for a copy of known size), or None."""
@@ -169,6 +187,46 @@ def measured_traffic(kernel, n):
         t = json.load(f)
     k = t.get("kernels", {}).get(kernel)
     return None if k is None else k.get("traffic_bytes")
+
+
+def ensemble(sr, scenes, threads, steps=2):
+    """The batched ensemble: 64 independent scenes of sides 128..1024 (log-uniform), blobs on a lognormal
+    background, the full step with the same heat settings as the single world, all worlds stepped together by
+    sr_ensemble_run over `threads` host threads; one more step with one thread for comparison."""
+    import torch
+    batch = scenes.ensemble_scenes()
+    cells = sum(s * s for s, _ in batch)
+    log(f"ensemble: {len(batch)} scenes, {cells / 1e6:.1f} Mcells; creating")
+    hs = sr.heat_settings(stop=sr.STOP_RELATIVE, solve_tol=1e-8, newton_heat=1e-6, newton_cell=1e-3, forcing=0.1,
+                          coarse_scale=COARSE_SCALE)
+    worlds = []
+    for side, planes in batch:
+        w = sr.World(side, side, physics=sr.physics(**scenes.config_physics_kwargs(side)), gravity=True, hydro=True,
+                      heat=True, heat_settings=hs)
+        w.upload(planes)
+        worlds.append(w)
+    del batch
+    sr.ensemble_run(worlds, 1, threads)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sr.ensemble_run(worlds, steps, threads)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    sr.ensemble_run(worlds, 1, 1)
+    torch.cuda.synchronize()
+    serial = time.perf_counter() - t1
+    bad = sum(1 for w in worlds if w.read_dt().non_finite)
+    res = {"workload": "64 scenes, sides 128..1024 log-uniform, 1-3 Gaussian blobs (peak rho 0.1-1, sigma "
+                       "0.03-0.15 L) on a lognormal background (sigma_ln 1.0), full hydro+gravity+FLD step",
+           "scenes": len(worlds), "cells": cells, "threads": threads, "steps": steps,
+           "ms_per_step": round(elapsed / steps * 1e3, 3),
+           "mcell_steps_per_s": round(cells * steps / elapsed / 1e6, 3),
+           "one_thread_mcell_steps_per_s": round(cells / serial / 1e6, 3), "worlds_non_finite": bad}
+    log(f"ensemble: {res['ms_per_step']} ms per step of all {len(worlds)}, {res['mcell_steps_per_s']} Mcell-steps/s "
+        f"({threads} threads; one thread {res['one_thread_mcell_steps_per_s']})")
+    del worlds
+    return res
 
 
 def cpu_baseline(scenes, budget_s=12.0):
@@ -214,8 +272,10 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(counter_name, n),
                 "launches": launches, "mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": bytes_per}
     fld = None if args.no_fld else fld_solve(sr, scenes, args.fld_n)
+    ens = None if args.no_ensemble else ensemble(sr, scenes, args.ensemble_threads)
     copy_gbs = copy_bandwidth()
-    log(f"device copies (GB/s): {copy_gbs}")
+    bracket = event_bracket_ms()
+    log(f"device copies (GB/s): {copy_gbs}; timing-event bracket {bracket} ms")
     cpu = None if args.no_cpu else cpu_baseline(scenes)
     cost = fs["cost"]
     out = {
@@ -252,11 +312,13 @@ def main():
                 "learned_fell_back": fld.get("learned_fell_back"),
                 "learned_without_fallback": fld.get("learned_without_fallback"),
                 "net_apply_ms": fld.get("net_apply_ms")},
+            "ensemble_64": ens,
         },
         # the step's dominant HBM-bound kernel: the FLD's level-0 pre-smoother (P5 is most of the step); the hydro
-        # update beside it
+        # update beside it. achieved uses the event-timed mean as read; event_bracket_ms is what the same events
+        # read around an empty kernel, the part of that mean a kernel trace does not count
         "roofline": dict(roof("mg_pre0", "sr::k_pre<32, 32, 2>"), device_copy_gbs=copy_gbs,
-                         also=[roof("hydro_update", "sr::k_hydro<1>")]),
+                         event_bracket_ms=bracket, also=[roof("hydro_update", "sr::k_hydro<1>")]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

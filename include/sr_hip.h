@@ -191,6 +191,13 @@ int sr_upload(sr_world *wd, const float *planes, size_t n);
 int sr_readback(sr_world *wd, float *planes, size_t n);
 /* Step::run (step/mod.rs:703): `steps` steps, P1 -> P2 -> P4 -> P5 -> P8 -> P9 (reference/mod.rs:264-297) */
 int sr_run(sr_world *wd, uint32_t steps);
+/* A batch of independent worlds, each `steps` steps as sr_run would take them (every world's result is the one it
+ * reaches alone). No counterpart in the reference, where a Step drives one world (step/mod.rs:367-527, :703-718) and
+ * headless runs one scene (headless.rs:1087-1260): the batched-ensemble workload of many uneven scenes. The worlds
+ * are shared out over `threads` host threads (1..256), largest first onto the least-loaded; each world is used by one
+ * thread during the call, so no world may appear twice or be used elsewhere meanwhile. The first error stops the
+ * batch and is returned. */
+int sr_ensemble_run(sr_world **worlds, int count, uint32_t steps, int threads);
 int sr_read_dt(sr_world *wd, sr_dt_report *out);              /* Step::read_dt (step/mod.rs:691) */
 int sr_read_heat(sr_world *wd, sr_heat_report *out);          /* read_heat (step/heat.rs:499) */
 int sr_read_field(sr_world *wd, int field, float *out, size_t n); /* side fields: phi, g, F (step/heat.rs:549) */

@@ -2,7 +2,7 @@
 kernels. TEST INFRASTRUCTURE ONLY: the shipped library never imports it.
 
 The reference has no learned part (its FLD preconditioners are Jacobi and the multigrid of
-crates/sr-engine/shaders/heat.wgsl:1160-1526); the network is this build's, specified by its workload: a 3-level
+crates/sr-engine/shaders/heat.wgsl:1160-1526); the network is this library's own, specified by its workload: a 3-level
 U-Net with 16/32/64 channels and 3x3 convolutions, inputs (residual, log D, diagonal), output the correction.
 
 Layout: activations are channels-last arrays (H, W, C); a 3x3 convolution's weights are (3, 3, Cin, Cout) with

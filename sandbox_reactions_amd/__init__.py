@@ -133,9 +133,9 @@ class Gravity:
 
 from .fld import Fld  # noqa: E402
 from .net import Net  # noqa: E402
-from .step import World  # noqa: E402
+from .step import World, ensemble_run  # noqa: E402
 
-__all__ = ["SrError", "Gravity", "World", "Fld", "fft2", "physics", "heat_settings", "require_device", "lib",
+__all__ = ["SrError", "Gravity", "World", "ensemble_run", "Fld", "fft2", "physics", "heat_settings", "require_device", "lib",
            "device_info", "COMPONENTS", "EDGE_BOUNCE", "EDGE_LEAVE", "OPACITY_KRAMERS", "OPACITY_REFERENCE",
            "PRECOND_JACOBI", "PRECOND_MULTIGRID", "PRECOND_LEARNED", "STOP_REFERENCE", "STOP_RELATIVE",
            "FIELD_PHI", "FIELD_GX", "FIELD_GY", "FIELD_FLUX_X", "FIELD_FLUX_Y", "FIELD_SIGMA_MID",

@@ -84,6 +84,7 @@ SIGNATURES = [
     ("sr_upload", _i, [_vp, _vp, _sz]),
     ("sr_readback", _i, [_vp, _vp, _sz]),
     ("sr_run", _i, [_vp, _u32]),
+    ("sr_ensemble_run", _i, [C.POINTER(_vp), _i, _u32, _i]),
     ("sr_read_dt", _i, [_vp, C.POINTER(DtReport)]),
     ("sr_read_heat", _i, [_vp, C.POINTER(HeatReport)]),
     ("sr_read_field", _i, [_vp, _i, _vp, _sz]),

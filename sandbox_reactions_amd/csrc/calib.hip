@@ -27,3 +27,36 @@ extern "C" int sr_calib_copy(const float *src, float *dst, size_t n, int kind, v
     }
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
+
+__global__ void k_calib_empty() {}
+
+// What a pair of timing events adds around a kernel that follows another on a stream: the mean elapsed time of
+// `reps` brackets (created as the library's timing events are, without the system-scope fence) around an empty
+// kernel, each after an empty predecessor, read after one synchronisation. A kernel's event-timed mean less this is
+// what a profiler's kernel trace reports as its duration.
+extern "C" int sr_calib_event_bracket(int reps, void *stream, double *mean_ms) {
+    if (reps <= 0 || reps > 4096 || !mean_ms) return -5;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t *ev = new hipEvent_t[2 * reps]();
+    int rc = 0;
+    for (int k = 0; k < 2 * reps && rc == 0; k++)
+        if (hipEventCreateWithFlags(&ev[k], hipEventDisableSystemFence) != hipSuccess) rc = -4;
+    for (int r = 0; r < reps && rc == 0; r++) {
+        hipLaunchKernelGGL(k_calib_empty, dim3(1), dim3(64), 0, s);
+        if (hipEventRecord(ev[2 * r], s) != hipSuccess) rc = -4;
+        hipLaunchKernelGGL(k_calib_empty, dim3(1), dim3(64), 0, s);
+        if (hipEventRecord(ev[2 * r + 1], s) != hipSuccess) rc = -4;
+    }
+    double sum = 0.0;
+    if (rc == 0 && hipStreamSynchronize(s) != hipSuccess) rc = -4;
+    for (int r = 0; r < reps && rc == 0; r++) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]) != hipSuccess) rc = -4;
+        sum += ms;
+    }
+    for (int k = 0; k < 2 * reps; k++)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+    delete[] ev;
+    if (rc == 0) *mean_ms = sum / reps;
+    return rc;
+}

@@ -8,7 +8,10 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <map>
+#include <mutex>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -232,6 +235,23 @@ static std::vector<float> kernel_spectrum(int px, int py, double G, double dx) {
     return q;
 }
 
+// The spectrum for (px, py, G, dx), built once per process: an ensemble's worlds of one size share the host work
+static std::vector<float> kernel_spectrum_cached(int px, int py, double G, double dx) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, double, double>, std::vector<float>> cache;
+    const auto key = std::make_tuple(px, py, G, dx);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    std::vector<float> q = kernel_spectrum(px, py, G, dx);
+    std::lock_guard<std::mutex> g(mu);
+    if (cache.size() >= 16) cache.clear();
+    cache[key] = q;
+    return q;
+}
+
 int padded_size(int len, int max_padded, int *out) {
     if (len <= 0) return fail(SR_ERR_BOX_AXIS, "a box axis of 0 cells");
     int n = 1;
@@ -284,7 +304,7 @@ int Gravity::init(int w_, int h_, double G_, double dx_, double sigma_vac_, int 
     }
     SR_HIP(hipMalloc(&tw, sizeof(float2) * nmax));
     SR_HIP(hipMemcpy(tw, t.data(), sizeof(float2) * nmax, hipMemcpyHostToDevice));
-    std::vector<float> q = kernel_spectrum(px, py, G, dx);
+    std::vector<float> q = kernel_spectrum_cached(px, py, G, dx);
     SR_HIP(hipMalloc(&khat, sizeof(float) * q.size()));
     SR_HIP(hipMemcpy(khat, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice));
     ld_spec = h;

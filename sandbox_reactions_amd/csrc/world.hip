@@ -4,9 +4,13 @@
 // time live in a small device array that every kernel reads, so a step with no heat pass issues its kernels
 // back to back on the world's stream without a host round trip.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -523,6 +527,69 @@ extern "C" int sr_run(sr_world *wd, uint32_t steps) {
     if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
     for (uint32_t k = 0; k < steps; k++) SR_TRY(one_step(wd));
     SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+// Independent worlds stepped together. Each world stays on one host thread for the call (its stream and its heat
+// pass's host decisions are that thread's), the worlds shared out largest first onto the least-loaded thread; a
+// thread takes its worlds one step each in turn, so while one waits on a heat-pass decision the others' kernels,
+// on their own streams, keep the device busy. Every world's result is the one it reaches stepped alone.
+extern "C" int sr_ensemble_run(sr_world **worlds, int count, uint32_t steps, int threads) {
+    if (!worlds || count <= 0) return fail(SR_ERR_ARGUMENT, "sr_ensemble_run: no worlds");
+    if (threads <= 0 || threads > 256) return fail(SR_ERR_ARGUMENT, "sr_ensemble_run: threads must be in 1..256");
+    std::vector<sr_world *> ws(worlds, worlds + count);
+    for (sr_world *w : ws)
+        if (!w) return fail(SR_ERR_ARGUMENT, "sr_ensemble_run: a null world");
+    {
+        std::vector<sr_world *> sorted = ws;
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(SR_ERR_ARGUMENT, "sr_ensemble_run: a world appears twice");
+    }
+    const int nt = std::min(threads, count);
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ws[a]->n > ws[b]->n; });
+    std::vector<std::vector<sr_world *>> mine(nt);
+    std::vector<size_t> load(nt, 0);
+    for (int i : order) {
+        const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        mine[t].push_back(ws[i]);
+        load[t] += ws[i]->n;
+    }
+    int dev = 0;
+    SR_HIP(hipGetDevice(&dev));
+    std::atomic<bool> stop{false};
+    std::mutex mu;
+    int err = SR_OK;
+    std::string msg;
+    auto body = [&](int t) {
+        auto bail = [&](int c) {
+            std::lock_guard<std::mutex> g(mu);
+            if (err == SR_OK) {
+                err = c;
+                msg = sr_last_error();
+            }
+            stop = true;
+        };
+        if (hipSetDevice(dev) != hipSuccess) return bail(fail(SR_ERR_DEVICE, "hipSetDevice"));
+        for (uint32_t k = 0; k < steps && !stop.load(); k++) {
+            for (sr_world *wd : mine[t]) {
+                const int c = one_step(wd);
+                if (c != SR_OK) return bail(c);
+            }
+            for (sr_world *wd : mine[t])
+                if (hipStreamSynchronize(wd->s) != hipSuccess) return bail(fail(SR_ERR_DEVICE, "hipStreamSynchronize"));
+        }
+    };
+    if (nt == 1) {
+        body(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; t++) pool.emplace_back(body, t);
+        for (auto &th : pool) th.join();
+    }
+    if (err != SR_OK) return fail(err, msg);
     return SR_OK;
 }
 

@@ -89,7 +89,7 @@ def config_physics_kwargs(n: int) -> dict:
 
 
 def blob_scene(n: int = 256, seed: int = SEED_BASE) -> np.ndarray:
-    """Config 1: rho = 1e-2 + exp(-r^2 / 2 sigma^2), sigma = 0.1 L, centred; T = 1 +- 1% seeded; v = 0."""
+    """The blob workload: rho = 1e-2 + exp(-r^2 / 2 sigma^2), sigma = 0.1 L, centred; T = 1 +- 1% seeded; v = 0."""
     c = (np.arange(n) + 0.5) / n - 0.5
     r2 = c[None, :] ** 2 + c[:, None] ** 2
     rho = 1e-2 + np.exp(-r2 / (2 * 0.1 ** 2))
@@ -98,7 +98,7 @@ def blob_scene(n: int = 256, seed: int = SEED_BASE) -> np.ndarray:
 
 
 def turbulent_scene(n: int = 1024, gamma: float = 5.0 / 3.0, seed: int = SEED_BASE) -> np.ndarray:
-    """Config 2: rho lognormal (sigma_ln 1.0), T lognormal (0.5), velocities from two fields at rms Mach 0.5."""
+    """The turbulent full-step workload: rho lognormal (sigma_ln 1.0), T lognormal (0.5), velocities from two fields at rms Mach 0.5."""
     rho = lognormal(n, 1.0, seed + 1)
     T = lognormal(n, 0.5, seed + 2)
     u = gaussian_random_field(n, seed + 3)
@@ -124,7 +124,7 @@ def sod_scene(w: int = 400, h: int = 8) -> np.ndarray:
 
 def fld_system(n: int, seed: int = SEED_BASE, sigma_rho: float = 1.5, sigma_t: float = 0.8, stiffness: float = 1e3,
                a_r: float = 6.25e-7, c_sb: float = 12800.0, gamma: float = 5.0 / 3.0):
-    """Config 3: the stiff FLD system alone at n x n, L = 1. rho lognormal (1.5), T lognormal (0.8), Kramers kappa
+    """The stiff FLD system alone at n x n, L = 1. rho lognormal (1.5), T lognormal (0.8), Kramers kappa
     = rho T^-3.5, Levermore-Pomraning D on the faces; dt such that median_i (sum_f k_f / c_i) = stiffness;
     b = c_i E_r with E_r = a_r T^4 (1 + 0.1 (2u - 1)). Returns (own, kx, ky, b, dt) as float64 arrays: own the
     diagonal c_i (the grid's edge faces carry the vacuum's coupling into own)."""
@@ -156,3 +156,28 @@ def fld_system(n: int, seed: int = SEED_BASE, sigma_rho: float = 1.5, sigma_t: f
     u = lcg_uniform(seed + 13, n * n).reshape(n, n)
     b = c * er * (1.0 + 0.1 * (2.0 * u - 1.0))
     return c, kx, ky, b, dt
+
+
+ENSEMBLE_SIDES = (128, 256, 512, 1024)
+
+
+def ensemble_scenes(count: int = 64, seed: int = SEED_BASE + 0x1000, sides=ENSEMBLE_SIDES):
+    """The batched-ensemble workload: `count` independent scenes, each a square of side drawn uniformly from `sides`
+    (log-uniform sizes over 128..1024), 1-3 Gaussian blobs (peak rho U[0.1, 1], sigma U[0.03, 0.15] L, centres
+    U[0.25, 0.75] L) on a lognormal background (sigma_ln 1.0, mean 1e-2), T = 1 +- 1% seeded, at rest.
+    Returns [(side, planes)] with planes float32 (14, side, side)."""
+    u = lcg_uniform(seed, count * 14).reshape(count, 14)
+    out = []
+    for k in range(count):
+        side = int(sides[min(len(sides) - 1, int(u[k, 0] * len(sides)))])
+        blobs = 1 + min(2, int(u[k, 1] * 3))
+        c = (np.arange(side) + 0.5) / side
+        rho = 1e-2 * lognormal(side, 1.0, seed + 16 + 2 * k)
+        for b in range(blobs):
+            peak, sig, cx, cy = u[k, 2 + 4 * b:6 + 4 * b]
+            peak, sig = 0.1 + 0.9 * peak, 0.03 + 0.12 * sig
+            cx, cy = 0.25 + 0.5 * cx, 0.25 + 0.5 * cy
+            rho = rho + peak * np.exp(-((c[None, :] - cx) ** 2 + (c[:, None] - cy) ** 2) / (2.0 * sig * sig))
+        T = 1.0 + 0.01 * (2.0 * lcg_uniform(seed + 17 + 2 * k, side * side).reshape(side, side) - 1.0)
+        out.append((side, planes_from(rho, T).astype(np.float32)))
+    return out

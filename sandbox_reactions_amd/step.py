@@ -108,3 +108,12 @@ class World:
             self.close()
         except Exception:
             pass
+
+
+def ensemble_run(worlds, steps: int = 1, threads: int = 8):
+    """Steps every world of `worlds` (distinct World objects) `steps` times, the batch shared out over `threads` host
+    threads (sr_ensemble_run); each world ends where it would stepped alone."""
+    from . import _check, lib
+    ws = list(worlds)
+    arr = (C.c_void_p * len(ws))(*[w._p.value for w in ws])
+    _check(lib().sr_ensemble_run(arr, len(ws), steps, threads))

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 def test_blob_256_with_gravity_and_fld_stays_on_the_twin(gpu_lib):
     import sandbox_reactions_amd as sr
     n, steps = 256, 10
-    print(f"config 1 blob {n}^2, {steps} steps", flush=True)
+    print(f"blob workload {n}^2, {steps} steps", flush=True)
     kw = scenes.config_physics_kwargs(n)
     ph = sr.physics(**kw)
     oph = oracle.reference_physics(**{k: v for k, v in kw.items() if k != "max_padded"}, max_padded=4096.0)
