@@ -102,7 +102,8 @@ def fld_solve(sr, scenes, n):
 
     plain = timed(sr.PRECOND_MULTIGRID, 4)[1:]
     ms = float(np.median([p[0] for p in plain]) * 1e3)
-    res = dict(ms=ms, iterations=int(plain[0][1]), rel=float(max(p[2] for p in plain)), levels=f.levels)
+    res = dict(ms=ms, iterations=int(plain[0][1]), rel=float(max(p[2] for p in plain)), levels=f.levels,
+               tiles=f.tiles())
     log(f"FLD solve {n}^2: {ms:.2f} ms, {plain[0][1]} iterations, rel {res['rel']:.2e}, {f.levels} levels")
     if not os.path.exists(train.DEFAULT_WEIGHTS):
         log("no trained network: the learned preconditioner is not measured")
@@ -305,7 +306,7 @@ def main():
                 "workload": f"stiff FLD solve alone, {args.fld_n}^2, diffusion number 1e3, MG-PCG V(2,2), "
                             f"coarse scale {COARSE_SCALE}, to 1e-8",
                 "plain_mg_pcg_ms": round(fld["ms"], 3), "iterations": fld["iterations"], "final_rel": fld["rel"],
-                "levels": fld["levels"], "learned_ms": fld.get("learned_ms"),
+                "levels": fld["levels"], "tiles_pre_post": fld["tiles"], "learned_ms": fld.get("learned_ms"),
                 "learned_vs_plain": round(fld["ms"] / fld["learned_ms"], 4) if fld.get("learned_ms") else None,
                 "learned_iterations": fld.get("learned_iterations"),
                 "learned_net_iterations": fld.get("learned_net_iterations"),

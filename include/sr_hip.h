@@ -39,6 +39,7 @@ enum sr_edge { SR_EDGE_BOUNCE = 0, SR_EDGE_LEAVE = 1 };
 enum sr_opacity { SR_OPACITY_REFERENCE = 0, SR_OPACITY_KRAMERS = 1 };
 enum sr_precond { SR_PRECOND_JACOBI = 0, SR_PRECOND_MULTIGRID = 1, SR_PRECOND_LEARNED = 2 };
 enum sr_stop { SR_STOP_REFERENCE = 0, SR_STOP_RELATIVE = 1 };
+enum sr_tile { SR_TILE_16 = 0, SR_TILE_32 = 1, SR_TILE_64 = 2 }; /* smoother tiles: 16x16, 32x32, 64x32 */
 enum sr_field { SR_FIELD_PHI = 0, SR_FIELD_GX = 1, SR_FIELD_GY = 2, SR_FIELD_FLUX_X = 3, SR_FIELD_FLUX_Y = 4,
                 SR_FIELD_SIGMA_MID = 5, SR_FIELD_PHI_MID = 6 };
 
@@ -145,6 +146,14 @@ int sr_fld_apply_precond(sr_fld *f, const float *r_dev, float *z_dev, int precon
 /* out = A v in FP64 */
 int sr_fld_matvec(sr_fld *f, const double *v_dev, double *out_dev, void *stream);
 int sr_fld_levels(sr_fld *f, int *levels);
+/* Tile shapes of the smoothers of the levels above the single-block ones (SR_TILE_16 16x16, SR_TILE_32 32x32,
+ * SR_TILE_64 64x32, the last for nu1, nu2 <= 2 only): the project's autotuner times every shape on the first system
+ * built and keeps the fastest per level and smoother; sr_fld_set_tiles fixes them instead (level 0's post-smoother
+ * stays 32x32, its dot partials are counted per tile). No result bit depends on the shapes (exact red-black
+ * Gauss-Seidel in every tiling). No counterpart in the reference, whose tiles are fixed by WebGPU's limits
+ * (heat.wgsl:1401-1445: 16x16 with 4 RB pairs). */
+int sr_fld_tiles(sr_fld *f, int *pre, int *post, int max_levels, int *levels);
+int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int levels);
 void sr_fld_destroy(sr_fld *f);
 /* The learned preconditioner (SR_PRECOND_LEARNED): z = V r + N(r - A V r), V the multigrid V-cycle and N the
  * network. With fallback_iterations > 0 the solve continues with plain multigrid from where it stands once the

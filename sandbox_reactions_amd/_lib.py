@@ -19,6 +19,7 @@ EDGE_BOUNCE, EDGE_LEAVE = 0, 1
 OPACITY_REFERENCE, OPACITY_KRAMERS = 0, 1
 PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 STOP_REFERENCE, STOP_RELATIVE = 0, 1
+TILE_16, TILE_32, TILE_64 = 0, 1, 2
 FIELD_PHI, FIELD_GX, FIELD_GY, FIELD_FLUX_X, FIELD_FLUX_Y, FIELD_SIGMA_MID, FIELD_PHI_MID = range(7)
 COMPONENTS = 14
 
@@ -84,6 +85,8 @@ SIGNATURES = [
     ("sr_upload", _i, [_vp, _vp, _sz]),
     ("sr_readback", _i, [_vp, _vp, _sz]),
     ("sr_run", _i, [_vp, _u32]),
+    ("sr_fld_tiles", _i, [_vp, C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i)]),
+    ("sr_fld_set_tiles", _i, [_vp, C.POINTER(_i), C.POINTER(_i), _i]),
     ("sr_ensemble_run", _i, [C.POINTER(_vp), _i, _u32, _i]),
     ("sr_read_dt", _i, [_vp, C.POINTER(DtReport)]),
     ("sr_read_heat", _i, [_vp, C.POINTER(HeatReport)]),

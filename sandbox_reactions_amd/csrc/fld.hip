@@ -236,15 +236,16 @@ struct Lv {
 
 // The region of a TX x TY tile plus a halo of HW cells: (TX + 2 HW) x (TY + 2 HW) entries per array. A cell at
 // least one cell inside the region reads its right and bottom faces from its neighbours' left and top entries. The
-// sweeps need no `own` term (the inverse diagonal carries it), so it stays out of LDS: five arrays let four 32x32
-// pre-smoothing blocks (halo 5) share a CU's 160 KB instead of three.
+// sweeps need no `own` term (the inverse diagonal carries it): only the pre-smoother's residual reads it, on the tile's
+// own cells, so it is kept for those alone, and four 32x32 pre-smoothing blocks (halo 5) share a CU's 160 KB.
 template <int TX, int TY, int HW> struct Region {
     static constexpr int RX = TX + 2 * HW, RY = TY + 2 * HW, N = RX * RY;
     float z[N], b[N], al[N], au[N], rd[N];
 };
 
 template <int TX, int TY, int HW>
-__device__ inline void load_region(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0, const float *__restrict__ b) {
+__device__ inline void load_region(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0, const float *__restrict__ b,
+                                   float *own_in = nullptr) {
     using RG = Region<TX, TY, HW>;
     constexpr int IT = (RG::N + NT - 1) / NT;
     // every load of the region issued before any is used: one memory round trip per block; the coefficients are one
@@ -265,6 +266,10 @@ __device__ inline void load_region(Region<TX, TY, HW> &R, const Lv &L, int x0, i
     for (int k = 0; k < IT; k++) {
         const int t = threadIdx.x + k * NT;
         if (t < RG::N) {
+            if (own_in) {
+                const int lx = t % RG::RX - HW, ly = t / RG::RX - HW;
+                if (lx >= 0 && lx < TX && ly >= 0 && ly < TY) own_in[ly * TX + lx] = vc[k].x;
+            }
             R.al[t] = vc[k].y;
             R.au[t] = vc[k].z;
             R.rd[t] = vc[k].w;
@@ -316,8 +321,9 @@ __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, f
     constexpr int HW = 2 * NU + 1;
     using RG = Region<TX, TY, HW>;
     __shared__ RG R;
+    __shared__ float own_in[TX * TY];
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    load_region<TX, TY, HW>(R, L, x0, y0, b);
+    load_region<TX, TY, HW>(R, L, x0, y0, b, own_in);
 #pragma unroll 4
     for (int t = threadIdx.x; t < RG::N; t += NT) R.z[t] = 0.0f;
     __syncthreads();
@@ -337,8 +343,7 @@ __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, f
                 const int c = ly * RG::RX + lx;
                 const float zc = R.z[c];
                 const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RG::RX];
-                // own of an interior cell: the record this block has just loaded, so a cache hit
-                const float own = L.cf[(size_t)(y0 + ly - HW) * L.w + (x0 + lx - HW)].x;
+                const float own = own_in[(2 * cy + dy) * TX + 2 * cx + dx];
                 const float az = own * zc + al * (zc - R.z[c - 1]) + ar * (zc - R.z[c + 1]) +
                                  au * (zc - R.z[c - RG::RX]) + ad * (zc - R.z[c + RG::RX]);
                 sum += R.b[c] - az;
@@ -748,11 +753,13 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
     }
     const size_t n = (size_t)w * h;
     nbe = (int)std::max<size_t>(1, std::min<size_t>(NB_MAX, (n + NT * UNROLL - 1) / (NT * UNROLL)));
-    // the level smoothers' tile width: 64 (less halo, 2 waves per SIMD) or 32 (4 waves per SIMD); SR_FLD_TILE picks
-    const char *env = std::getenv("SR_FLD_TILE");
-    tile_x = (env && std::atoi(env) == 64 && nu1 <= 2 && nu2 <= 2) ? PTX : 32;
-    const int tx = tile_x, ty = PTY;
-    ntiles0 = ((w + tx - 1) / tx) * ((h + ty - 1) / ty);
+    // until the autotuner runs: 32x32 tiles, 16x16 on levels below level 0 that would launch fewer than 512 blocks of
+    // 32x32 (four times the blocks and a quarter of the serial work per thread where the launch is latency-bound)
+    pre_tile.assign(coarse_from, TILE_32);
+    post_tile.assign(coarse_from, TILE_32);
+    for (int l = 1; l < coarse_from; l++)
+        if (((lv[l].w + 31) / 32) * ((lv[l].h + 31) / 32) < 512) pre_tile[l] = post_tile[l] = TILE_16;
+    ntiles0 = ((w + 31) / 32) * ((h + 31) / 32);
     nbd_max = std::max(nbe, ntiles0);
     SR_HIP(hipMalloc(&gas, n));
     SR_HIP(hipMemset(gas, 1, n));
@@ -810,46 +817,150 @@ int FldSolver::rebuild(bool faces, hipStream_t s) {
     if (lc.w * lc.h <= DENSE_MAX)
         hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(64), 0, s, lc.w, lc.h, lc.own, lc.ax, lc.ay, ainv);
     SR_HIP(hipGetLastError());
+    if (!tuned) {
+        tuned = true;
+        if (autotune) SR_TRY(tune(s));
+    }
     return SR_OK;
 }
 
+// The autotuner: on the system just built, each tile-smoothed level's pre- and post-smoother is timed in every tile
+// shape (one launch to warm, then the mean of three between two events) and keeps the fastest; a shape must beat
+// the one kept by 3 % to replace it, so near-ties keep the default order's choice. Level 0's post-smoother stays
+// 32x32. The smoothers compute exact red-black Gauss-Seidel sweeps whatever the tiling, so the choice changes no
+// result bit. The launches write only the hierarchy's own work arrays (z, the coarse b, zp), which every V-cycle
+// rewrites before it reads them.
+int FldSolver::tune(hipStream_t s) {
+    if (coarse_from == 0) return SR_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SR_HIP(hipEventCreate(&e0));
+    SR_HIP(hipEventCreate(&e1));
+    std::vector<int> shapes = {TILE_32, TILE_16};
+    if (nu1 <= 2 && nu2 <= 2) shapes.push_back(TILE_64);
+    auto time_of = [&](auto launch, float *ms) -> int {
+        SR_TRY(launch());
+        SR_HIP(hipEventRecord(e0, s));
+        for (int k = 0; k < 3; k++) SR_TRY(launch());
+        SR_HIP(hipEventRecord(e1, s));
+        SR_HIP(hipEventSynchronize(e1));
+        SR_HIP(hipEventElapsedTime(ms, e0, e1));
+        return SR_OK;
+    };
+    int rc = SR_OK;
+    for (int l = 0; l < coarse_from && rc == SR_OK; l++) {
+        const FldLevel &f = lv[l];
+        const float *b = l == 0 ? r32 : f.b;
+        for (int kind = 0; kind < 2 && rc == SR_OK; kind++) {
+            if (kind == 1 && l == 0) continue;
+            std::vector<int> &tile = kind == 0 ? pre_tile : post_tile;
+            int best = tile[l];
+            float best_ms = 0.0f;
+            for (int shape : shapes) {
+                tile[l] = shape;
+                float ms = 0.0f;
+                rc = kind == 0 ? time_of([&] { return launch_pre_level(l, b, s); }, &ms)
+                               : time_of([&] { return launch_post_level(l, b, f.zp, nullptr, nullptr, nullptr, s); },
+                                         &ms);
+                if (rc != SR_OK) break;
+                if (shape == shapes[0] || ms < 0.97f * best_ms) {
+                    best = shape;
+                    best_ms = ms;
+                }
+            }
+            tile[l] = best;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int FldSolver::tiles(int *pre, int *post, int max_levels, int *levels) const {
+    if (levels) *levels = coarse_from;
+    for (int l = 0; l < std::min(max_levels, coarse_from); l++) {
+        if (pre) pre[l] = pre_tile[l];
+        if (post) post[l] = post_tile[l];
+    }
+    return SR_OK;
+}
+
+int FldSolver::set_tiles(const int *pre, const int *post, int levels) {
+    if (!pre || !post || levels != coarse_from)
+        return fail(SR_ERR_ARGUMENT, "set_tiles: one pre and one post shape per tile-smoothed level (" +
+                                         std::to_string(coarse_from) + ")");
+    for (int l = 0; l < levels; l++) {
+        for (int v : {pre[l], post[l]})
+            if (v != TILE_16 && v != TILE_32 && !(v == TILE_64 && nu1 <= 2 && nu2 <= 2))
+                return fail(SR_ERR_ARGUMENT, "set_tiles: an unknown tile shape (or 64x32 with more than 2 sweeps)");
+    }
+    if (levels > 0 && post[0] != TILE_32)
+        return fail(SR_ERR_ARGUMENT, "set_tiles: level 0's post-smoother keeps 32x32 tiles");
+    pre_tile.assign(pre, pre + levels);
+    post_tile.assign(post, post + levels);
+    tuned = true;
+    autotune = false;
+    return SR_OK;
+}
+
+// the 64x32 tile serves nu <= 2 only (its nu = 3 region would not leave room for two blocks per CU)
 template <int TX, int TY>
-static void launch_pre(int nu, dim3 grid, hipStream_t s, Lv L, const float *b, float *z, float *bc, int wc) {
+static void launch_pre(int nu, hipStream_t s, Lv L, const float *b, float *z, float *bc, int wc) {
+    const dim3 grid((L.w + TX - 1) / TX, (L.h + TY - 1) / TY);
     if (nu == 1)
         hipLaunchKernelGGL((k_pre<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
     else if (nu == 2)
         hipLaunchKernelGGL((k_pre<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
-    else
-        hipLaunchKernelGGL((k_pre<32, TY, 3>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
+    else if constexpr (TX <= 32)
+        hipLaunchKernelGGL((k_pre<TX, TY, 3>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
 }
 
 template <int TX, int TY>
-static void launch_post(int nu, dim3 grid, hipStream_t s, Lv L, const float *b, const float *zin, const float *zc,
-                        int wc, float omega, float *zout, const float *zold, double *part) {
+static int launch_post(int nu, hipStream_t s, Lv L, const float *b, const float *zin, const float *zc, int wc,
+                       float omega, float *zout, const float *zold, double *part) {
+    const dim3 grid((L.w + TX - 1) / TX, (L.h + TY - 1) / TY);
     if (nu == 1)
         hipLaunchKernelGGL((k_post<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
     else if (nu == 2)
         hipLaunchKernelGGL((k_post<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
-    else
-        hipLaunchKernelGGL((k_post<32, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
+    else if constexpr (TX <= 32)
+        hipLaunchKernelGGL((k_post<TX, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
+    return (int)(grid.x * grid.y);
 }
 
-// Levels below level 0 that would launch fewer than 512 blocks of 32x32 take 16x16 tiles: four times the blocks
-// and a quarter of the serial work per thread, where the launch is latency-bound anyway. Level 0 keeps the wide
-// tile (its partials count is sized for it).
-static bool small_tiles(int l, int w, int h) { return l > 0 && ((w + 31) / 32) * ((h + 31) / 32) < 512; }
+// level l's pre-smoother, residual and restriction into level l + 1's b, in the level's tile shape
+int FldSolver::launch_pre_level(int l, const float *b, hipStream_t s) {
+    const FldLevel &f = lv[l], &c = lv[l + 1];
+    const Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
+    switch (pre_tile[l]) {
+    case TILE_16: launch_pre<16, 16>(nu1, s, L, b, f.z, c.b, c.w); break;
+    case TILE_64: launch_pre<PTX, PTY>(nu1, s, L, b, f.z, c.b, c.w); break;
+    default: launch_pre<32, 32>(nu1, s, L, b, f.z, c.b, c.w); break;
+    }
+    return SR_OK;
+}
+
+// level l's prolongation of level l + 1's correction and post-smoother into zout; the launch's block count (the
+// number of dot partials when part is given) into *nblocks
+int FldSolver::launch_post_level(int l, const float *b, float *zout, const float *zold, double *part, int *nblocks,
+                                 hipStream_t s) {
+    const FldLevel &f = lv[l], &c = lv[l + 1];
+    const Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
+    int nb = 0;
+    switch (post_tile[l]) {
+    case TILE_16: nb = launch_post<16, 16>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part); break;
+    case TILE_64: nb = launch_post<PTX, PTY>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part); break;
+    default: nb = launch_post<32, 32>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part); break;
+    }
+    if (nblocks) *nblocks = nb;
+    return SR_OK;
+}
 
 // z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
 int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s) {
     const int nl = (int)lv.size();
-    const int tx = tile_x, ty = PTY;
     const float *bl = r;
     for (int l = 0; l < coarse_from; l++) {
-        const FldLevel &f = lv[l], &c = lv[l + 1];
-        Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
-        const bool small = small_tiles(l, f.w, f.h);
-        const int sx = small ? 16 : tx, sy = small ? 16 : ty;
-        dim3 grid((f.w + sx - 1) / sx, (f.h + sy - 1) / sy);
+        const FldLevel &c = lv[l + 1];
         const bool t0 = timing && l == 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (t0) {
@@ -871,12 +982,7 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
             ev_used += 2;
             SR_HIP(hipEventRecord(e0, s));
         }
-        if (small)
-            launch_pre<16, 16>(nu1, grid, s, L, bl, f.z, c.b, c.w);
-        else if (tx == PTX)
-            launch_pre<PTX, PTY>(nu1, grid, s, L, bl, f.z, c.b, c.w);
-        else
-            launch_pre<32, PTY>(nu1, grid, s, L, bl, f.z, c.b, c.w);
+        SR_TRY(launch_pre_level(l, bl, s));
         if (t0) {
             SR_HIP(hipEventRecord(e1, s));
         }
@@ -921,21 +1027,13 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         }
     }
     for (int l = coarse_from - 1; l >= 0; l--) {
-        const FldLevel &f = lv[l], &c = lv[l + 1];
-        Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
-        const bool small = small_tiles(l, f.w, f.h);
-        const int sx = small ? 16 : tx, sy = small ? 16 : ty;
-        dim3 grid((f.w + sx - 1) / sx, (f.h + sy - 1) / sy);
+        const FldLevel &f = lv[l];
         const float *b = l == 0 ? r : f.b;
         float *zo = l == 0 ? z : f.zp;
         double *pp = (l == 0 && dots) ? dots : nullptr;
-        if (small)
-            launch_post<16, 16>(nu2, grid, s, L, b, f.z, c.zp, c.w, omega, zo, zold, pp);
-        else if (tx == PTX)
-            launch_post<PTX, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, omega, zo, zold, pp);
-        else
-            launch_post<32, PTY>(nu2, grid, s, L, b, f.z, c.zp, c.w, omega, zo, zold, pp);
-        if (l == 0 && dots) *nbd = grid.x * grid.y;
+        int nb = 0;
+        SR_TRY(launch_post_level(l, b, zo, zold, pp, &nb, s));
+        if (l == 0 && dots) *nbd = nb;
     }
     SR_HIP(hipGetLastError());
     return SR_OK;
@@ -1161,6 +1259,16 @@ extern "C" int sr_fld_create(int w, int h, const sr_heat_settings *hs, sr_fld **
     }
     *out = f;
     return SR_OK;
+}
+
+extern "C" int sr_fld_tiles(sr_fld *f, int *pre, int *post, int max_levels, int *levels) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_tiles: null solver");
+    return f->s.tiles(pre, post, max_levels, levels);
+}
+
+extern "C" int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int levels) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_tiles: null solver");
+    return f->s.set_tiles(pre, post, levels);
 }
 
 extern "C" int sr_fld_set_system(sr_fld *f, const float *own_dev, const float *kx_dev, const float *ky_dev,

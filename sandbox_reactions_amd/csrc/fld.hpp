@@ -4,9 +4,14 @@
 
 #include <vector>
 
+#include "../../include/sr_hip.h"
+
 namespace sr {
 
 class Net;
+
+// tile shapes of the level smoothers
+enum { TILE_16 = SR_TILE_16, TILE_32 = SR_TILE_32, TILE_64 = SR_TILE_64 };
 
 // One level of the hierarchy: (A v)_i = own_i v_i + sum_f a_f (v_i - v_j); ax is (w+1) x h (face x between cells
 // x-1 and x, 0 at the edges), ay is w x (h+1).
@@ -56,6 +61,10 @@ class FldSolver {
     // convergence (0: never) the solve continues with plain multigrid
     int set_network(Net *n, int fallback);
     int matvec64(const double *v, double *out, hipStream_t s);
+    // tile shapes per tile-smoothed level (the levels above the single-block ones): read, or fixed by the caller
+    // (which turns the autotuner off). Level 0's post-smoother keeps 32x32: its dot partials are counted per tile.
+    int tiles(int *pre, int *post, int max_levels, int *levels) const;
+    int set_tiles(const int *pre, const int *post, int levels);
     int nlevels() const { return (int)lv.size(); }
     int width() const { return w0; }
     int height() const { return h0; }
@@ -71,7 +80,14 @@ class FldSolver {
     int w0 = 0, h0 = 0, nu1 = 2, nu2 = 2, coarse_from = 0;
     float omega = 1.0f;
     int nbe = 1, ntiles0 = 1, nbd_max = 1; // blocks of the elementwise kernels, level-0 tiles
-    int tile_x = 32;
+    // the tile shape (TILE_16, TILE_32, TILE_64) of each tile-smoothed level's pre- and post-smoother: the autotuner's
+    // choice at the first rebuild, or the caller's (set_tiles)
+    std::vector<int> pre_tile, post_tile;
+    bool tuned = false, autotune = true;
+    int tune(hipStream_t s);
+    int launch_pre_level(int l, const float *b, hipStream_t s);
+    int launch_post_level(int l, const float *b, float *zout, const float *zold, double *part, int *nblocks,
+                          hipStream_t s);
     std::vector<FldLevel> lv;
     uint8_t *gas = nullptr;
     // level-0 vectors

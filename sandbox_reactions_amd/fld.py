@@ -74,6 +74,21 @@ class Fld:
         _check(lib().sr_fld_levels(self._p, C.byref(n)))
         return n.value
 
+    def tiles(self):
+        """([pre shape per level], [post shape per level]) of the tile-smoothed levels (TILE_16, TILE_32, TILE_64):
+        the autotuner's choice once a system is set, or set_tiles'."""
+        from . import _check, lib
+        pre, post, n = (C.c_int * 32)(), (C.c_int * 32)(), C.c_int()
+        _check(lib().sr_fld_tiles(self._p, pre, post, 32, C.byref(n)))
+        return list(pre[:n.value]), list(post[:n.value])
+
+    def set_tiles(self, pre, post):
+        from . import SrError, _check, lib
+        if len(pre) != len(post):
+            raise SrError(_lib.SR_ERR_ARGUMENT, "set_tiles: one pre and one post shape per level")
+        k = len(pre)
+        _check(lib().sr_fld_set_tiles(self._p, (C.c_int * max(1, k))(*pre), (C.c_int * max(1, k))(*post), k))
+
     def __del__(self):
         from . import _LIB
         if getattr(self, "_p", None) and _LIB is not None:

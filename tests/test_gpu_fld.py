@@ -120,3 +120,43 @@ def test_level_caps(gpu_lib):
     assert rel <= 1e-8 and np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-7
     with pytest.raises(gpu_lib.SrError):
         gpu_lib.Fld(2048, 2048, levels=2)
+
+
+def test_autotuned_tiles_change_no_result_bit(gpu_lib):
+    """The autotuner's per-level tile shapes against fixed other shapes: the same solve to the last bit (exact
+    red-black Gauss-Seidel in every tiling, level 0's dot partials per 32x32 tile in both)."""
+    import torch
+    sr = gpu_lib
+    n = 256
+    own, kx, ky, b = system(n)
+    bt = torch.tensor(b, device="cuda")
+    tuned = upload(sr, n, own, kx, ky, coarse_scale=1.2)
+    pre, post = tuned.tiles()
+    print("tuned tiles pre", pre, "post", post, flush=True)
+    assert len(pre) == len(post) >= 1 and post[0] == sr.TILE_32
+    assert all(v in (sr.TILE_16, sr.TILE_32, sr.TILE_64) for v in pre + post)
+    xs = []
+    for shapes in (None, ([sr.TILE_16] * len(pre), [sr.TILE_32] + [sr.TILE_16] * (len(pre) - 1)),
+                   ([sr.TILE_64] * len(pre), [sr.TILE_32] + [sr.TILE_64] * (len(pre) - 1))):
+        f = tuned if shapes is None else sr.Fld(n, n, coarse_scale=1.2)
+        if shapes is not None:
+            f.set_tiles(*shapes)
+            t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+            f.set_system(t(own), t(kx), t(ky))
+            assert f.tiles() == (list(shapes[0]), list(shapes[1]))
+        x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        it, rel = f.solve(bt, x, tol=1e-8)
+        assert rel <= 1e-8
+        xs.append((it, x.cpu().numpy()))
+    for it, x in xs[1:]:
+        assert it == xs[0][0] and np.array_equal(x, xs[0][1])
+    f = sr.Fld(n, n)
+    k = len(pre)
+    with pytest.raises(sr.SrError):
+        f.set_tiles([sr.TILE_32] * k, [sr.TILE_16] * k)  # level 0's post-smoother stays 32x32
+    with pytest.raises(sr.SrError):
+        f.set_tiles([7] * k, [sr.TILE_32] * k)
+    with pytest.raises(sr.SrError):
+        f.set_tiles([sr.TILE_32] * (k + 1), [sr.TILE_32] * (k + 1))
+    with pytest.raises(sr.SrError):
+        sr.Fld(n, n, nu1=3).set_tiles([sr.TILE_64] * k, [sr.TILE_32] * k)
