@@ -405,15 +405,17 @@ __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, 
     if (part) block_partials<2>(v, part, gridDim.x * gridDim.y);
 }
 
-// The V-cycle over every level of at most COARSE_MAX cells in one block. The levels' own, ax and ay arrays (and the
-// coarsest's inverse) lie back to back in one packed buffer, so one coalesced sweep fills the LDS; z and b of every
-// level follow it in LDS. The first level's b comes from global memory and its z goes back there. The coarsest is
-// solved exactly by the inverse formed at the rebuild when it holds at most DENSE_MAX cells, else by COARSE_SWEEPS
-// forward and backward sweeps.
+// The V-cycle over every level of at most COARSE_MAX cells in one block. The levels' own, ax and ay arrays, the
+// coarsest's inverse and every cell's inverse diagonal (both formed at the rebuild, k_coarse_prep) lie back to back in
+// one packed buffer, so one batch of 16-byte loads per thread fills the LDS; z and b of every level follow it in LDS.
+// The first level's b comes from global memory in the same batch, and its z goes back there. Every phase is swept by the whole block between block barriers (measured: one wave alone, with
+// no block barrier, is slower even on a 16x16 level, its LDS latency no longer hidden by the other waves). The
+// coarsest is solved exactly by the inverse formed at the rebuild when it holds at most DENSE_MAX cells, else by
+// COARSE_SWEEPS forward and backward sweeps.
 struct CoarseSet {
     int n;
     int w[16], h[16];
-    int pack_floats;   // own/ax/ay of every level, then the inverse
+    int pack_floats;   // own/ax/ay of every level, the coarsest's inverse, the inverse diagonals
     int offo[16], offx[16], offy[16], offz[16], offb[16], offd[16], offinv, total;  // LDS offsets (floats)
     int lw[16];        // log2 of the level's width when a power of two, else -1
 };
@@ -424,27 +426,47 @@ __device__ __forceinline__ int row_of(int t, int w, int lw) { return lw >= 0 ? t
 __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *__restrict__ pack,
                                                       const float *__restrict__ b0, float *__restrict__ z0, int nu1,
                                                       int nu2, float omega) {
-    extern __shared__ float sm[];
+    extern __shared__ float4 sm4[];
+    float *sm = (float *)sm4;
     const int *offo = C.offo, *offx = C.offx, *offy = C.offy, *offz = C.offz, *offb = C.offb, *offd = C.offd;
-    const int offinv = C.offinv, total = C.total;
-#pragma unroll 8
-    for (int t = threadIdx.x; t < C.pack_floats; t += COARSE_NT) sm[t] = pack[t];
-    for (int t = C.pack_floats + threadIdx.x; t < total; t += COARSE_NT) sm[t] = 0.0f;
-    __syncthreads();
-    for (int t = threadIdx.x; t < C.w[0] * C.h[0]; t += COARSE_NT) sm[offb[0] + t] = b0[t];
-    // the inverse of every cell's diagonal (own + its couplings), once per application
-    for (int l = 0; l < C.n; l++) {
-        const int w = C.w[l], h = C.h[l];
-        const float *own = sm + offo[l], *al = sm + offx[l], *au = sm + offy[l];
-        float *rd = sm + offd[l];
-        for (int t = threadIdx.x; t < w * h; t += COARSE_NT) {
-            const int y = row_of(t, w, C.lw[l]), x = t - y * w;
-            rd[t] = 1.0f / (own[t] + al[y * (w + 1) + x] + al[y * (w + 1) + x + 1] + au[t] + au[t + w]);
+    const int offinv = C.offinv, total = C.total, n0 = C.w[0] * C.h[0], ob0 = offb[0];
+    // one phase: every load issued before any is used (the pack as float4, level 0's b), then zeros elsewhere (every
+    // entry a sweep may read at a grid's rim is finite)
+    {
+        constexpr int V4 = (COARSE_LDS / 16 + COARSE_NT - 1) / COARSE_NT, VB = COARSE_MAX / COARSE_NT;
+        const int np4 = C.pack_floats >> 2;
+        const float4 *p4 = (const float4 *)pack;
+        float4 v[V4];
+        float vb[VB];
+#pragma unroll
+        for (int k = 0; k < V4; k++) {
+            const int t = threadIdx.x + k * COARSE_NT;
+            v[k] = t < np4 ? p4[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
+#pragma unroll
+        for (int k = 0; k < VB; k++) {
+            const int t = threadIdx.x + k * COARSE_NT;
+            vb[k] = t < n0 ? b0[t] : 0.0f;
+        }
+        const int tail = 4 * np4 + (int)threadIdx.x;
+        const float vt = tail < C.pack_floats ? pack[tail] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < V4; k++) {
+            const int t = threadIdx.x + k * COARSE_NT;
+            if (t < np4) sm4[t] = v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < VB; k++) {
+            const int t = threadIdx.x + k * COARSE_NT;
+            if (t < n0) sm[ob0 + t] = vb[k];
+        }
+        if (tail < C.pack_floats) sm[tail] = vt;
     }
+    for (int t = C.pack_floats + threadIdx.x; t < total; t += COARSE_NT)
+        if (t < ob0 || t >= ob0 + n0) sm[t] = 0.0f;
     __syncthreads();
-    // one half sweep of `color` on level l, colour-compact. Couplings on the grid's rim are zero, so the neighbour
-    // entries read there (the adjacent LDS arrays, all finite) contribute nothing.
+    // one half sweep of `color` on level l, colour-compact. Couplings on the grid's rim are
+    // zero, so the neighbour entries read there (the adjacent LDS arrays, all finite) contribute nothing.
     auto sweep = [&](int l, int color) {
         const int w = C.w[l], h = C.h[l], hw = (w + 1) >> 1, lh = C.lw[l] > 0 ? C.lw[l] - 1 : -1;
         const float *al = sm + offx[l], *au = sm + offy[l], *b = sm + offb[l], *rd = sm + offd[l];
@@ -459,7 +481,8 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
         }
         __syncthreads();
     };
-    for (int l = 0; l < C.n - 1; l++) {
+    // pre-smoothing of level l from z = 0 and the restriction of its residual into level l + 1's b
+    auto down = [&](int l) {
         for (int k = 0; k < nu1; k++) {
             sweep(l, 0);
             sweep(l, 1);
@@ -487,30 +510,9 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
             bc[t] = s;
         }
         __syncthreads();
-    }
-    const int lc = C.n - 1;
-    const int m = C.w[lc] * C.h[lc];
-    if (m <= DENSE_MAX) {
-        // the exact solve by the inverse formed at the system's rebuild
-        float *z = sm + offz[lc];
-        const float *b = sm + offb[lc], *ainv = sm + offinv;
-        if (threadIdx.x < m) {
-            float t = 0.0f;
-            for (int j = 0; j < m; j++) t += ainv[threadIdx.x * m + j] * b[j];
-            z[threadIdx.x] = t;
-        }
-        __syncthreads();
-    } else {
-        for (int k = 0; k < COARSE_SWEEPS; k++) {
-            sweep(lc, 0);
-            sweep(lc, 1);
-        }
-        for (int k = 0; k < COARSE_SWEEPS; k++) {
-            sweep(lc, 1);
-            sweep(lc, 0);
-        }
-    }
-    for (int l = C.n - 2; l >= 0; l--) {
+    };
+    // the prolongation of level l + 1's correction (scaled by omega) and the post-smoothing of level l
+    auto up = [&](int l) {
         const int w = C.w[l], h = C.h[l], wc = C.w[l + 1];
         float *z = sm + offz[l];
         const float *zc = sm + offz[l + 1];
@@ -523,8 +525,35 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
             sweep(l, 1);
             sweep(l, 0);
         }
-    }
-    for (int t = threadIdx.x; t < C.w[0] * C.h[0]; t += COARSE_NT) z0[t] = sm[offz[0] + t];
+    };
+    const int lc = C.n - 1, m = C.w[lc] * C.h[lc];
+    // the coarsest level's solve
+    auto coarsest = [&]() {
+        if (m <= DENSE_MAX) {
+            // the exact solve by the inverse formed at the system's rebuild
+            float *z = sm + offz[lc];
+            const float *b = sm + offb[lc], *ainv = sm + offinv;
+            if (threadIdx.x < m) {
+                float t = 0.0f;
+                for (int j = 0; j < m; j++) t += ainv[threadIdx.x * m + j] * b[j];
+                z[threadIdx.x] = t;
+            }
+            __syncthreads();
+        } else {
+            for (int k = 0; k < COARSE_SWEEPS; k++) {
+                sweep(lc, 0);
+                sweep(lc, 1);
+            }
+            for (int k = 0; k < COARSE_SWEEPS; k++) {
+                sweep(lc, 1);
+                sweep(lc, 0);
+            }
+        }
+    };
+    for (int l = 0; l < lc; l++) down(l);
+    coarsest();
+    for (int l = lc - 1; l >= 0; l--) up(l);
+    for (int t = threadIdx.x; t < n0; t += COARSE_NT) z0[t] = sm[offz[0] + t];
 }
 
 // ------------------------------------------------------------------------------------------------ hierarchy build
@@ -586,12 +615,24 @@ __global__ void k_coarsen_faces(Lv f, float *__restrict__ axc, float *__restrict
     }
 }
 
-// The inverse of the coarsest level's dense matrix (at most DENSE_MAX cells) by Gauss-Jordan elimination, one
-// thread per row; the matrix is SPD, so no pivoting.
-__global__ void k_coarse_inverse(int w, int h, const float *__restrict__ own, const float *__restrict__ ax,
-                                 const float *__restrict__ ay, float *__restrict__ ainv) {
+// Once per rebuild, for the single-block V-cycle: every cell's inverse diagonal 1 / (own + its four couplings) on
+// every level of the set, and, when the coarsest holds at most DENSE_MAX cells, the inverse of its dense matrix by
+// Gauss-Jordan elimination, one thread per row (the matrix is SPD, so no pivoting); both into the pack.
+__global__ void __launch_bounds__(COARSE_NT) k_coarse_prep(CoarseSet C, float *__restrict__ pack) {
+    for (int l = 0; l < C.n; l++) {
+        const int w = C.w[l], h = C.h[l];
+        const float *own = pack + C.offo[l], *al = pack + C.offx[l], *au = pack + C.offy[l];
+        float *rd = pack + C.offd[l];
+        for (int t = threadIdx.x; t < w * h; t += COARSE_NT) {
+            const int y = t / w, x = t - y * w;
+            rd[t] = 1.0f / (own[t] + al[y * (w + 1) + x] + al[y * (w + 1) + x + 1] + au[t] + au[t + w]);
+        }
+    }
+    const int lc = C.n - 1, w = C.w[lc], h = C.h[lc], m = w * h, i = threadIdx.x;
+    if (m > DENSE_MAX) return;
+    const float *own = pack + C.offo[lc], *ax = pack + C.offx[lc], *ay = pack + C.offy[lc];
+    float *ainv = pack + C.offinv;
     __shared__ float A[DENSE_MAX][2 * DENSE_MAX];
-    const int m = w * h, i = threadIdx.x;
     if (i < m) {
         for (int j = 0; j < 2 * m; j++) A[i][j] = 0.0f;
         const int x = i % w, y = i / w;
@@ -723,7 +764,8 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
                                              std::to_string(COARSE_MAX) + " cells)");
     }
     pack_floats = DENSE_MAX * DENSE_MAX;
-    for (int l = coarse_from; l < (int)lv.size(); l++) pack_floats += (int)level_floats(lv[l].w, lv[l].h);
+    for (int l = coarse_from; l < (int)lv.size(); l++)
+        pack_floats += (int)level_floats(lv[l].w, lv[l].h) + lv[l].w * lv[l].h;  // + the inverse diagonals
     SR_HIP(hipMalloc(&pack, sizeof(float) * pack_floats));
     SR_HIP(hipMemset(pack, 0, sizeof(float) * pack_floats));
     {
@@ -796,6 +838,43 @@ int FldSolver::set_system(const float *own, const float *ax, const float *ay, co
     return rebuild(true, s);
 }
 
+// the single-block levels' layout: in the pack (and at the same offsets in LDS) own, ax, ay per level, the
+// coarsest's inverse, the inverse diagonals per level; after the pack in LDS, z and b per level
+CoarseSet FldSolver::coarse_set() const {
+    const int nl = (int)lv.size();
+    CoarseSet C;
+    C.n = nl - coarse_from;
+    C.pack_floats = pack_floats;
+    int o = 0;
+    for (int l = coarse_from; l < nl; l++) {
+        const int k = l - coarse_from, w = lv[l].w, h = lv[l].h;
+        C.w[k] = w;
+        C.h[k] = h;
+        C.offo[k] = o;
+        o += w * h;
+        C.offx[k] = o;
+        o += (w + 1) * h;
+        C.offy[k] = o;
+        o += w * (h + 1);
+        C.lw[k] = is_pow2(w) ? ilog2(w) : -1;
+    }
+    C.offinv = o;
+    o += DENSE_MAX * DENSE_MAX;
+    for (int k = 0; k < C.n; k++) {
+        C.offd[k] = o;
+        o += C.w[k] * C.h[k];
+    }
+    o = pack_floats;
+    for (int k = 0; k < C.n; k++) {
+        C.offz[k] = o;
+        o += C.w[k] * C.h[k];
+        C.offb[k] = o;
+        o += C.w[k] * C.h[k];
+    }
+    C.total = o;
+    return C;
+}
+
 int FldSolver::rebuild(bool faces, hipStream_t s) {
     for (size_t l = 0; l + 1 < lv.size(); l++) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
@@ -813,9 +892,7 @@ int FldSolver::rebuild(bool faces, hipStream_t s) {
         Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
         hipLaunchKernelGGL(k_pack_cf, dim3((f.w * f.h + 255) / 256), dim3(256), 0, s, fl, f.cf);
     }
-    const FldLevel &lc = lv.back();
-    if (lc.w * lc.h <= DENSE_MAX)
-        hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(64), 0, s, lc.w, lc.h, lc.own, lc.ax, lc.ay, ainv);
+    hipLaunchKernelGGL(k_coarse_prep, dim3(1), dim3(COARSE_NT), 0, s, coarse_set(), pack);
     SR_HIP(hipGetLastError());
     if (!tuned) {
         tuned = true;
@@ -989,34 +1066,8 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         bl = c.b;
     }
     {
-        CoarseSet C;
-        C.n = nl - coarse_from;
-        C.pack_floats = pack_floats;
-        int o = 0;
-        for (int l = coarse_from; l < nl; l++) {
-            const int k = l - coarse_from, w = lv[l].w, h = lv[l].h;
-            C.w[k] = w;
-            C.h[k] = h;
-            C.offo[k] = o;
-            o += w * h;
-            C.offx[k] = o;
-            o += (w + 1) * h;
-            C.offy[k] = o;
-            o += w * (h + 1);
-        }
-        C.offinv = o;
-        o = pack_floats;
-        for (int k = 0; k < C.n; k++) {
-            C.offz[k] = o;
-            o += C.w[k] * C.h[k];
-            C.offb[k] = o;
-            o += C.w[k] * C.h[k];
-            C.offd[k] = o;
-            o += C.w[k] * C.h[k];
-            C.lw[k] = is_pow2(C.w[k]) ? ilog2(C.w[k]) : -1;
-        }
-        C.total = o;
-        const size_t floats = o;
+        const CoarseSet C = coarse_set();
+        const size_t floats = C.total;
         float *zc = coarse_from == 0 ? z : lv[coarse_from].zp;
         hipLaunchKernelGGL(k_coarse, dim3(1), dim3(COARSE_NT), sizeof(float) * floats, s, C, (const float *)pack, bl,
                            zc, nu1, nu2, omega);

@@ -9,6 +9,7 @@
 namespace sr {
 
 class Net;
+struct CoarseSet;
 
 // tile shapes of the level smoothers
 enum { TILE_16 = SR_TILE_16, TILE_32 = SR_TILE_32, TILE_64 = SR_TILE_64 };
@@ -106,6 +107,7 @@ class FldSolver {
     int fallback_it = 0;
     double plain_rate = 0.0;  // ln(residual reduction) per iteration of the last converged plain multigrid solve
     int vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s);
+    CoarseSet coarse_set() const;
     int learned(const float *r, float *z, const float *zold, double *dots, hipStream_t s);
     int read_partials(const double *part, int k, int nb, double *out, hipStream_t s);
 };

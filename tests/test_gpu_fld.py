@@ -106,17 +106,19 @@ def test_solver_refusals(gpu_lib):
             gpu_lib.Fld(64, 64, coarse_scale=omega)
 
 
-def test_level_caps(gpu_lib):
-    """A capped hierarchy (the 7-level V-cycle of the stiff 2048^2 workload, here 5 levels at 256^2: a 16x16
-    coarsest solved by sweeps) still matches the model; too few levels to fit one block is refused."""
+@pytest.mark.parametrize("n,levels", [(256, 5), (512, 4)])
+def test_level_caps(gpu_lib, n, levels):
+    """A capped hierarchy (the 7-level V-cycle of the stiff 2048^2 workload; here 5 levels at 256^2, a 16x16
+    coarsest solved by sweeps of one wave, and 4 at 512^2, a 64x64 coarsest swept by the whole block) still matches
+    the model; too few levels to fit one block is refused."""
     import torch
-    n = 256
+    print(f"capped hierarchy {n}^2, {levels} levels", flush=True)
     own, kx, ky, b = system(n)
-    f = upload(gpu_lib, n, own, kx, ky, levels=5)
-    assert f.levels == 5
+    f = upload(gpu_lib, n, own, kx, ky, levels=levels)
+    assert f.levels == levels
     x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
     it, rel = f.solve(torch.tensor(b, device="cuda"), x, tol=1e-8)
-    xo, _, _ = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-10, precond="mg", levels=5)
+    xo, _, _ = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-10, precond="mg", levels=levels)
     assert rel <= 1e-8 and np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-7
     with pytest.raises(gpu_lib.SrError):
         gpu_lib.Fld(2048, 2048, levels=2)
