@@ -162,3 +162,57 @@ def test_autotuned_tiles_change_no_result_bit(gpu_lib):
         f.set_tiles([sr.TILE_32] * (k + 1), [sr.TILE_32] * (k + 1))
     with pytest.raises(sr.SrError):
         sr.Fld(n, n, nu1=3).set_tiles([sr.TILE_64] * k, [sr.TILE_32] * k)
+
+
+def ragged_system(w, h, n=256):
+    """The stiff lognormal system cut to w x h (not multiples of any tile): the cut's edge faces carry no coupling."""
+    own, kx, ky, b = system(n)
+    own, b = own[:h, :w].copy(), b[:h, :w].copy()
+    kx, ky = kx[:h, :w + 1].copy(), ky[:h + 1, :w].copy()
+    kx[:, w] = 0.0
+    ky[h, :] = 0.0
+    return own, kx, ky, b
+
+
+@pytest.mark.parametrize("w,h", [(97, 61), (200, 120), (33, 250)])
+def test_ragged_grids_match_the_model(gpu_lib, w, h):
+    import torch
+    print(f"ragged fld {w}x{h}", flush=True)
+    own, kx, ky, b = ragged_system(w, h)
+    f = gpu_lib.Fld(w, h, coarse_scale=1.2)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    f.set_system(t(own), t(kx), t(ky))
+    x = torch.zeros((h, w), dtype=torch.float64, device="cuda")
+    it, rel = f.solve(torch.tensor(b, device="cuda"), x, tol=1e-8)
+    xg = x.cpu().numpy()
+    true_rel = np.linalg.norm(b - oracle.fld_matvec(own, kx, ky, None, xg)) / np.linalg.norm(b)
+    assert rel <= 1e-8 and true_rel <= 1.01e-8, (rel, true_rel)
+    xo, _, _ = oracle.fld_pcg(own, kx, ky, None, b, tol=1e-10, precond="mg", coarse_scale=1.2)
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-7
+    z = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    r = np.random.default_rng(3).standard_normal((h, w))
+    f.apply_precond(torch.tensor(r.astype(np.float32), device="cuda"), z)
+    zm = oracle.mg_apply(own, kx, ky, None, r.astype(np.float32).astype(np.float64), coarse_scale=1.2)
+    assert np.linalg.norm(z.cpu().numpy() - zm) / np.linalg.norm(zm) <= 1e-5
+
+
+def test_the_largest_axis_solves_to_1e8(gpu_lib):
+    """4096^2 (the largest world axis): the stiff lognormal system solved to 1e-8, the residual recomputed in FP64 by
+    the device's own matvec and again from the answer; a size-independent check, the oracle being too slow here."""
+    import torch
+    n = 4096
+    print(f"fld {n}^2", flush=True)
+    c, kx, ky, b, _ = scenes.fld_system(n)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    f = gpu_lib.Fld(n, n, coarse_scale=1.2)
+    f.set_system(t(c), t(kx), t(ky))
+    del kx, ky
+    bt = torch.tensor(b, device="cuda")
+    x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    it, rel = f.solve(bt, x, tol=1e-8)
+    ax = torch.empty_like(x)
+    f.matvec(x, ax)
+    true_rel = float(torch.linalg.norm(bt - ax) / torch.linalg.norm(bt))
+    print(f"iterations {it}, rel {rel:.3e}, recomputed {true_rel:.3e}, levels {f.levels}", flush=True)
+    assert rel <= 1e-8 and true_rel <= 1.01e-8
+    assert bool(torch.isfinite(x).all()) and float(x.min()) > 0.0  # u = a_r T^4 > 0 for a positive right-hand side

@@ -103,6 +103,31 @@ def test_full_step_with_heat_matches_the_twin(gpu_lib, opacity):
     assert r.converged
 
 
+def test_full_step_with_heat_on_a_ragged_world(gpu_lib):
+    """The full step with gravity and heat on a 120 x 72 box cut from the turbulent scene (no side a multiple of any
+    tile; the padded FFT 256 x 256): the same bars against the twin as the square case."""
+    import sandbox_reactions_amd as sr
+    w, h = 120, 72
+    print(f"ragged full step {w}x{h}", flush=True)
+    p = scenes.turbulent_scene(128)[:, :h, :w].copy()
+    kw = dict(scenes.config_physics_kwargs(w))
+    op, gp = both(**kw)
+    start = f32(p)
+    twin = oracle.Twin(op, start, gravity=True, hydro=True, heat=True)
+    world = sr.World(w, h, physics=gp, gravity=True, hydro=True, heat=True, heat_settings=tight(sr, coarse_scale=1.2))
+    world.upload(start.astype(np.float32))
+    for _ in range(3):
+        dtg = world.read_dt().dt_next
+        twin.cap_dt_next(dtg)
+        twin.step()
+        world.set_dt_cap(dtg)
+        world.run(1)
+    g, t = world.readback(), twin.state()
+    for c in range(4):
+        assert rel_l1(g[c], t[c]) <= 5e-5, (c, rel_l1(g[c], t[c]))
+    assert world.read_heat().converged
+
+
 def test_inexact_newton_agrees_with_exact_newton(gpu_lib):
     """Loose early Newton passes (forcing 0.1) against every pass solved to 1e-8: the same state to the Newton
     tolerance, the last pass still to 1e-8, and fewer solver rounds."""
