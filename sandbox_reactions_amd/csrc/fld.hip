@@ -91,7 +91,11 @@ __device__ inline float beta_of(const double *dots_cur, const double *dots_prev,
     return rzp != 0.0 ? (float)((rzn - rzo) / rzp) : 0.0f;
 }
 
-// p = z + beta p_old (into p_new), q = A p_new in FP32; partials of p.q. Blocks of 256 columns x `rows` rows.
+// p = z + beta p_old (into p_new), q = A p_new in FP32; partials of p.q. Blocks of 256 columns x `rows` rows; each
+// thread walks its column down the rows with p of the rows above, at and below in registers (and the face above),
+// and takes its left and right neighbours' p from the adjacent lanes, loading them only at a wave's two ends. Lanes
+// past the grid's right edge shadow its last column and store nothing. Every product is the one the face form
+// computes cell by cell, in the same order.
 #define MV_ROWS 16
 __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__ z, const float *__restrict__ pold,
                                                  float *__restrict__ pnew, float *__restrict__ q,
@@ -99,26 +103,37 @@ __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__
                                                  int restart, int rows, double *part) {
     const float beta = beta_of(dots_cur, dots_prev, nbd, nbp, restart);
     const int w = a.w, h = a.h;
-    const int x = blockIdx.x * NT + threadIdx.x, y0 = blockIdx.y * rows;
+    const int x = blockIdx.x * NT + threadIdx.x, y0 = blockIdx.y * rows, lane = threadIdx.x & 63;
+    const bool in = x < w;
+    const int xc = in ? x : w - 1;
     auto P = [&](size_t j) { return z[j] + beta * pold[j]; };
     double v[1] = {0.0};
-    if (x < w) {
-#pragma unroll 4
-        for (int r = 0; r < rows; r++) {
-            const int y = y0 + r;
-            if (y >= h) break;
-            const size_t i = (size_t)y * w + x;
-            const float pi = P(i);
-            const float al = a.ax[(size_t)y * (w + 1) + x], ar = a.ax[(size_t)y * (w + 1) + x + 1];
-            const float au = a.ay[i], ad = a.ay[i + w];
-            float s = a.own[i] * pi;
-            if (al != 0.0f) s += al * (pi - P(i - 1));
-            if (ar != 0.0f) s += ar * (pi - P(i + 1));
-            if (au != 0.0f) s += au * (pi - P(i - w));
-            if (ad != 0.0f) s += ad * (pi - P(i + w));
-            pnew[i] = pi;
-            q[i] = s;
-            v[0] += (double)pi * (double)s;
+    const int y1 = min(y0 + rows, h);
+    if (y0 < h) {
+        size_t i = (size_t)y0 * w + xc;
+        float pu = y0 > 0 ? P(i - w) : 0.0f;
+        float pc = P(i);
+        float au = a.ay[i];
+        for (int y = y0; y < y1; y++, i += w) {
+            const float pd = y + 1 < h ? P(i + w) : 0.0f;
+            const float ad = a.ay[i + w];
+            const float al = a.ax[(size_t)y * (w + 1) + xc], ar = a.ax[(size_t)y * (w + 1) + xc + 1];
+            float pl = __shfl_up(pc, 1, 64), pr = __shfl_down(pc, 1, 64);
+            if (lane == 0 && xc > 0) pl = P(i - 1);
+            if (lane == 63 && xc + 1 < w) pr = P(i + 1);
+            float s = a.own[i] * pc;
+            if (al != 0.0f) s += al * (pc - pl);
+            if (ar != 0.0f) s += ar * (pc - pr);
+            if (au != 0.0f) s += au * (pc - pu);
+            if (ad != 0.0f) s += ad * (pc - pd);
+            if (in) {
+                pnew[i] = pc;
+                q[i] = s;
+                v[0] += (double)pc * (double)s;
+            }
+            pu = pc;
+            pc = pd;
+            au = ad;
         }
     }
     block_partials<1>(v, part, gridDim.x * gridDim.y);
