@@ -157,12 +157,12 @@ int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int levels);
 void sr_fld_destroy(sr_fld *f);
 /* The learned preconditioner (SR_PRECOND_LEARNED): z = V r + N(r - A V r), V the multigrid V-cycle and N the
  * network. With fallback_iterations > 0 the solve continues with plain multigrid from where it stands once the
- * learned preconditioner has used that many iterations, or earlier when after 8 iterations it converges more slowly
+ * learned preconditioner has used that many iterations, or earlier when after 4 iterations it converges more slowly
  * per iteration than the last plain multigrid solve of the same system; 0: never. No counterpart in the reference (its set_heat_precond, step/heat.rs:595, offers
  * Jacobi and multigrid); n may be NULL to detach. Grid sides must be multiples of 4. */
 int sr_fld_set_network(sr_fld *f, sr_net *n, int fallback_iterations);
 /* sr_fld_solve, also reporting the iterations made with the learned preconditioner and whether it gave way to plain
- * multigrid (after fallback_iterations, or when after 8 iterations it converges more slowly per iteration than the
+ * multigrid (after fallback_iterations, or when after 4 iterations it converges more slowly per iteration than the
  * last plain multigrid solve of the same system did) */
 int sr_fld_solve_stats(sr_fld *f, const double *b_dev, double *x_dev, double tol, int max_iter, int precond,
                        int *iterations, double *final_rel, int *learned_iterations, int *fell_back, void *stream);

@@ -34,6 +34,7 @@ namespace sr {
 #define COARSE_NT 1024
 #define COARSE_LDS (150 * 1024)
 #define DENSE_MAX 16
+#define NET_RATE_AT 4 // learned iterations before its residual rate is compared with plain multigrid's
 
 // ------------------------------------------------------------------------------------------------ elementwise kernels
 
@@ -1256,9 +1257,9 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                 }
                 if (rri <= stop2) break;
                 // with a fallback budget, the learned preconditioner gives way to plain multigrid once it has used
-                // the budget, or when after 8 iterations it reduces the residual more slowly per iteration than
-                // plain multigrid did on its last solve of this system
-                const bool slower = fallback_it > 0 && plain_rate < 0.0 && net_it >= 8 &&
+                // the budget, or when from NET_RATE_AT iterations on it reduces the residual more slowly per
+                // iteration than plain multigrid did on its last solve of this system (checked at every check)
+                const bool slower = fallback_it > 0 && plain_rate < 0.0 && net_it >= NET_RATE_AT &&
                                     0.5 * std::log(std::max(rri, 1e-300) / std::max(rr_net0, 1e-300)) / net_it >
                                         plain_rate;
                 if (use_net && ((fallback_it > 0 && it >= fallback_it) || slower)) {
@@ -1271,6 +1272,8 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                     std::log(std::max(rri, 1e-300) / std::max(rr_last, 1e-300)) / (double)(it - last_check);
                 const double left = rate < 0.0 ? std::log(stop2 / rri) / rate : 1.0;
                 interval = std::max(1, std::min(8, (int)(0.7 * left)));
+                if (use_net && fallback_it > 0 && plain_rate < 0.0 && net_it < NET_RATE_AT)
+                    interval = std::min(interval, NET_RATE_AT - net_it);  // the first rate check comes on time
                 last_check = it;
                 rr_last = rri;
             }
