@@ -795,7 +795,6 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
             L.ay = o;
             o += L.w * (L.h + 1);
         }
-        ainv = o;
     }
     for (int l = 0; l < (int)lv.size(); l++) {
         FldLevel &L = lv[l];

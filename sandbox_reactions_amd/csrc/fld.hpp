@@ -93,8 +93,7 @@ class FldSolver {
     uint8_t *gas = nullptr;
     // level-0 vectors
     float *r32 = nullptr, *za = nullptr, *zb = nullptr, *p = nullptr, *q = nullptr;
-    float *pack = nullptr; // the single-block levels' own/ax/ay back to back, then the coarsest's inverse
-    float *ainv = nullptr;
+    float *pack = nullptr; // the single-block levels' own/ax/ay back to back, the coarsest's inverse, 1/diag
     int pack_floats = 0;
     // per-block partial sums (summed by the kernel that consumes them, or on the host)
     double *part_true = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_dots = nullptr;
