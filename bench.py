@@ -247,7 +247,20 @@ def cpu_baseline(scenes, budget_s=12.0):
     el = time.perf_counter() - t0
     return dict(value=n * n * steps / el / 1e6, unit="Mcell-steps/s", cores=1, kind="port",
                 sample=f"oracle twin (f64 C, 1 thread), full step with Jacobi-PCG FLD at the reference's 1e-12 "
-                       f"tolerances, {n}^2 sample of the same fields, {steps} steps in {el:.1f} s")
+                       f"tolerances, {n}^2 sample of the same fields, {steps} steps in {el:.1f} s",
+                cpu=host_cpu(), host_threads=len(os.sched_getaffinity(0)))
+
+
+def host_cpu():
+    """The host CPU's model name, as /proc/cpuinfo gives it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
