@@ -178,15 +178,16 @@ def event_bracket_ms(reps=512):
     return round(ms.value, 5)
 
 
-def measured_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` at n^2 from the committed counter summary (rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs, each scaled by what the counter reads for a copy of known size), or None."""
+def measured_traffic(kernel, n, grid=None):
+    """HBM bytes per launch of `kernel` (launched with `grid` threads, when given) at n^2 from the committed counter
+    summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for a copy of known
+    size), or None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_full_step_{n}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    k = t.get("kernels", {}).get(kernel)
+    k = t.get("by_grid", {}).get(f"{kernel}|{grid}") if grid is not None else t.get("kernels", {}).get(kernel)
     return None if k is None else k.get("traffic_bytes")
 
 
@@ -279,11 +280,11 @@ def main():
     fs = full_step(sr, scenes, args.step_n, args.steps, args.warmup)
     n = args.step_n
     value = n * n * args.steps / fs["elapsed"] / 1e6
-    def roof(name, counter_name):
+    def roof(name, counter_name, grid=None):
         launches, mean_ms, bytes_per = fs["kernels"][name]
         achieved = bytes_per / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else 0.0
         return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(counter_name, n),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(counter_name, n, grid),
                 "launches": launches, "mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": bytes_per}
     fld = None if args.no_fld else fld_solve(sr, scenes, args.fld_n)
     ens = None if args.no_ensemble else ensemble(sr, scenes, args.ensemble_threads)
@@ -331,8 +332,9 @@ def main():
         # the step's dominant HBM-bound kernel: the FLD's level-0 pre-smoother (P5 is most of the step); the hydro
         # update beside it. achieved uses the event-timed mean as read; event_bracket_ms is what the same events
         # read around an empty kernel, the part of that mean a kernel trace does not count
-        "roofline": dict(roof("mg_pre0", "sr::k_pre<32, 32, 2>"), device_copy_gbs=copy_gbs,
-                         event_bracket_ms=bracket, also=[roof("hydro_update", "sr::k_hydro<1>")]),
+        "roofline": dict(roof("mg_pre0", "sr::k_pre<32, 32, 2>", ((n + 31) // 32) ** 2 * 256),
+                         device_copy_gbs=copy_gbs, event_bracket_ms=bracket,
+                         also=[roof("hydro_update", "sr::k_hydro<1>")]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

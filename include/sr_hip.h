@@ -148,8 +148,8 @@ int sr_fld_matvec(sr_fld *f, const double *v_dev, double *out_dev, void *stream)
 int sr_fld_levels(sr_fld *f, int *levels);
 /* Tile shapes of the smoothers of the levels above the single-block ones (SR_TILE_16 16x16, SR_TILE_32 32x32,
  * SR_TILE_64 64x32, the last for nu1, nu2 <= 2 only): the project's autotuner times every shape on the first system
- * built and keeps the fastest per level and smoother; sr_fld_set_tiles fixes them instead (level 0's post-smoother
- * stays 32x32, its dot partials are counted per tile). No result bit depends on the shapes (exact red-black
+ * built and keeps the fastest per level below level 0 and smoother; sr_fld_set_tiles fixes them instead. Level 0
+ * keeps 32x32 (its dot partials are counted per tile). No result bit depends on the shapes (exact red-black
  * Gauss-Seidel in every tiling). No counterpart in the reference, whose tiles are fixed by WebGPU's limits
  * (heat.wgsl:1401-1445: 16x16 with 4 RB pairs). */
 int sr_fld_tiles(sr_fld *f, int *pre, int *post, int max_levels, int *levels);

@@ -916,12 +916,14 @@ int FldSolver::rebuild(bool faces, hipStream_t s) {
     return SR_OK;
 }
 
-// The autotuner: on the system just built, each tile-smoothed level's pre- and post-smoother is timed in every tile
-// shape (one launch to warm, then the mean of three between two events) and keeps the fastest; a shape must beat
-// the one kept by 3 % to replace it, so near-ties keep the default order's choice. Level 0's post-smoother stays
-// 32x32. The smoothers compute exact red-black Gauss-Seidel sweeps whatever the tiling, so the choice changes no
-// result bit. The launches write only the hierarchy's own work arrays (z, the coarse b, zp), which every V-cycle
-// rewrites before it reads them.
+// The autotuner: on the system just built, each tile-smoothed level's pre- and post-smoother below level 0 is timed
+// in every tile shape (one launch to warm, then the best of three means of eight launches, each between two events)
+// and keeps the fastest; a shape must beat the one kept by 3 % to replace it, so near-ties keep the default order's
+// choice. Level 0 keeps 32x32 tiles: it is the HBM-bound level the shape was chosen for by measurement, its
+// post-smoother's dot partials are counted per tile, and its pre-smoother is the kernel the bench's roofline names.
+// The smoothers compute exact red-black Gauss-Seidel sweeps whatever the tiling, so the choice changes no result bit.
+// The launches write only the hierarchy's own work arrays (z, the coarse b, zp), which every V-cycle rewrites before
+// it reads them.
 int FldSolver::tune(hipStream_t s) {
     if (coarse_from == 0) return SR_OK;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -931,19 +933,23 @@ int FldSolver::tune(hipStream_t s) {
     if (nu1 <= 2 && nu2 <= 2) shapes.push_back(TILE_64);
     auto time_of = [&](auto launch, float *ms) -> int {
         SR_TRY(launch());
-        SR_HIP(hipEventRecord(e0, s));
-        for (int k = 0; k < 3; k++) SR_TRY(launch());
-        SR_HIP(hipEventRecord(e1, s));
-        SR_HIP(hipEventSynchronize(e1));
-        SR_HIP(hipEventElapsedTime(ms, e0, e1));
+        *ms = 0.0f;
+        for (int trial = 0; trial < 3; trial++) {
+            SR_HIP(hipEventRecord(e0, s));
+            for (int k = 0; k < 8; k++) SR_TRY(launch());
+            SR_HIP(hipEventRecord(e1, s));
+            SR_HIP(hipEventSynchronize(e1));
+            float t = 0.0f;
+            SR_HIP(hipEventElapsedTime(&t, e0, e1));
+            if (trial == 0 || t < *ms) *ms = t;
+        }
         return SR_OK;
     };
     int rc = SR_OK;
-    for (int l = 0; l < coarse_from && rc == SR_OK; l++) {
+    for (int l = 1; l < coarse_from && rc == SR_OK; l++) {
         const FldLevel &f = lv[l];
-        const float *b = l == 0 ? r32 : f.b;
+        const float *b = f.b;
         for (int kind = 0; kind < 2 && rc == SR_OK; kind++) {
-            if (kind == 1 && l == 0) continue;
             std::vector<int> &tile = kind == 0 ? pre_tile : post_tile;
             int best = tile[l];
             float best_ms = 0.0f;
@@ -985,8 +991,8 @@ int FldSolver::set_tiles(const int *pre, const int *post, int levels) {
             if (v != TILE_16 && v != TILE_32 && !(v == TILE_64 && nu1 <= 2 && nu2 <= 2))
                 return fail(SR_ERR_ARGUMENT, "set_tiles: an unknown tile shape (or 64x32 with more than 2 sweeps)");
     }
-    if (levels > 0 && post[0] != TILE_32)
-        return fail(SR_ERR_ARGUMENT, "set_tiles: level 0's post-smoother keeps 32x32 tiles");
+    if (levels > 0 && (pre[0] != TILE_32 || post[0] != TILE_32))
+        return fail(SR_ERR_ARGUMENT, "set_tiles: level 0 keeps 32x32 tiles");
     pre_tile.assign(pre, pre + levels);
     post_tile.assign(post, post + levels);
     tuned = true;

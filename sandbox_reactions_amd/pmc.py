@@ -7,7 +7,8 @@ float4 and runtime copies are reported beside it (wide requests can count differ
     python -m sandbox_reactions_amd.pmc FETCH_DIR WRITE_DIR OUT_JSON [--workload TEXT] [--command TEXT]
 
 OUT_JSON gets {"copy_scale": {...}, "kernels": {name: {"launches", "fetch_bytes", "write_bytes", "traffic_bytes",
-"grid"}}} for the dispatch grid with the most total traffic per kernel; a Markdown table goes beside it.
+"grid"}}} for the dispatch grid with the most total traffic per kernel, and "by_grid": {"name|grid": ...} for every
+grid a kernel was launched with; a Markdown table goes beside it.
 """
 from __future__ import annotations
 
@@ -63,18 +64,21 @@ def summarize(fetch_dir, write_dir):
             others[k] = {"fetch": COPY_BYTES / copy_reading(fa, k), "write": COPY_BYTES / copy_reading(wa, k)}
         except ValueError:
             pass
-    kernels = {}
+    kernels, by_grid = {}, {}
     for key in fa.keys() & wa.keys():
         name, grid = key
         if "copyBuffer" in name or "fillBuffer" in name or "k_calib" in name:
             continue
         f = sum(fa[key]) / len(fa[key]) * sf
         w = sum(wa[key]) / len(wa[key]) * sw
+        e = {"grid": grid, "launches": len(fa[key]), "fetch_bytes": f, "write_bytes": w, "traffic_bytes": f + w}
+        by_grid[f"{name}|{grid}"] = e
         cur = kernels.get(name)
         if cur is None or (f + w) * len(fa[key]) > cur["traffic_bytes"] * cur["launches"]:
-            kernels[name] = {"grid": grid, "launches": len(fa[key]), "fetch_bytes": f, "write_bytes": w,
-                             "traffic_bytes": f + w}
-    return {"copy_scale": {"fetch": sf, "write": sw}, "other_copy_scales": others, "kernels": kernels}
+            kernels[name] = e
+    # `kernels`: per kernel name, its grid with the most traffic over the run; `by_grid`: every (name, grid)
+    return {"copy_scale": {"fetch": sf, "write": sw}, "other_copy_scales": others, "kernels": kernels,
+            "by_grid": by_grid}
 
 
 def main():
@@ -101,9 +105,11 @@ def write_outputs(s, out_json):
         f.write(f"Counters scaled by the 512 MiB one-float-per-lane calibration copy: FETCH "
                 f"x{s['copy_scale']['fetch']:.4f}, WRITE x{s['copy_scale']['write']:.4f} (the same counters over other "
                 f"copies of 512 MiB would need: {json.dumps(s['other_copy_scales'])}).\n\n")
+        f.write("Every kernel and grid (threads) the run launched, by total traffic.\n\n")
         f.write("| kernel | grid | launches | fetch MB | write MB | traffic MB |\n|---|---|---|---|---|---|\n")
-        for k, v in sorted(s["kernels"].items(), key=lambda kv: -kv[1]["traffic_bytes"] * kv[1]["launches"]):
-            f.write(f"| {k} | {v['grid']} | {v['launches']} | {v['fetch_bytes'] / 1e6:.2f} | "
+        rows = s.get("by_grid") or {f"{k}|{v['grid']}": v for k, v in s["kernels"].items()}
+        for k, v in sorted(rows.items(), key=lambda kv: -kv[1]["traffic_bytes"] * kv[1]["launches"]):
+            f.write(f"| {k.split('|')[0]} | {v['grid']} | {v['launches']} | {v['fetch_bytes'] / 1e6:.2f} | "
                     f"{v['write_bytes'] / 1e6:.2f} | {v['traffic_bytes'] / 1e6:.2f} |\n")
 
 

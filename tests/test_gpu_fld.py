@@ -135,11 +135,12 @@ def test_autotuned_tiles_change_no_result_bit(gpu_lib):
     tuned = upload(sr, n, own, kx, ky, coarse_scale=1.2)
     pre, post = tuned.tiles()
     print("tuned tiles pre", pre, "post", post, flush=True)
-    assert len(pre) == len(post) >= 1 and post[0] == sr.TILE_32
+    assert len(pre) == len(post) >= 1 and pre[0] == post[0] == sr.TILE_32
     assert all(v in (sr.TILE_16, sr.TILE_32, sr.TILE_64) for v in pre + post)
     xs = []
-    for shapes in (None, ([sr.TILE_16] * len(pre), [sr.TILE_32] + [sr.TILE_16] * (len(pre) - 1)),
-                   ([sr.TILE_64] * len(pre), [sr.TILE_32] + [sr.TILE_64] * (len(pre) - 1))):
+    k = len(pre)
+    for shapes in (None, ([sr.TILE_32] + [sr.TILE_16] * (k - 1), [sr.TILE_32] + [sr.TILE_16] * (k - 1)),
+                   ([sr.TILE_32] + [sr.TILE_64] * (k - 1), [sr.TILE_32] + [sr.TILE_64] * (k - 1))):
         f = tuned if shapes is None else sr.Fld(n, n, coarse_scale=1.2)
         if shapes is not None:
             f.set_tiles(*shapes)
@@ -155,13 +156,15 @@ def test_autotuned_tiles_change_no_result_bit(gpu_lib):
     f = sr.Fld(n, n)
     k = len(pre)
     with pytest.raises(sr.SrError):
-        f.set_tiles([sr.TILE_32] * k, [sr.TILE_16] * k)  # level 0's post-smoother stays 32x32
+        f.set_tiles([sr.TILE_32] * k, [sr.TILE_16] * k)  # level 0 keeps 32x32
+    with pytest.raises(sr.SrError):
+        f.set_tiles([sr.TILE_16] * k, [sr.TILE_32] * k)
     with pytest.raises(sr.SrError):
         f.set_tiles([7] * k, [sr.TILE_32] * k)
     with pytest.raises(sr.SrError):
         f.set_tiles([sr.TILE_32] * (k + 1), [sr.TILE_32] * (k + 1))
     with pytest.raises(sr.SrError):
-        sr.Fld(n, n, nu1=3).set_tiles([sr.TILE_64] * k, [sr.TILE_32] * k)
+        sr.Fld(n, n, nu1=3).set_tiles([sr.TILE_32] + [sr.TILE_64] * (k - 1), [sr.TILE_32] * k)
 
 
 def ragged_system(w, h, n=256):
