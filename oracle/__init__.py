@@ -30,6 +30,7 @@ class Physics(C.Structure):
         ("opacity", C.c_int), ("mu", C.c_double), ("edge", C.c_int),
         ("pin_temperature", C.c_int), ("constant_diffusivity", C.c_double), ("flux_limiter", C.c_int),
         ("max_padded", C.c_double),
+        ("k1e", C.c_double), ("k2e", C.c_double), ("k1n", C.c_double), ("k2n", C.c_double),
     ]
 
 
@@ -90,6 +91,18 @@ lib.sro_mg_apply.argtypes = [_sz, _sz, _dp, _dp, _dp, C.c_void_p, _dp, _dp, C.c_
 lib.sro_mg_set_coarse_scale.argtypes = [C.c_double]
 lib.sro_mg_set_coarse_scale.restype = None
 lib.sro_floors_apply.argtypes = [_PP, _dp, _sz, _sz, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+for _f in ("sro_cold_pressure", "sro_cold_energy_exact"):
+    getattr(lib, _f).argtypes = [C.c_double, C.c_double, C.c_double]
+    getattr(lib, _f).restype = C.c_double
+lib.sro_cold_node_x.argtypes = [C.c_int]
+lib.sro_cold_node_x.restype = C.c_double
+lib.sro_cold_table.argtypes = [C.c_double, C.c_double, _dp]
+lib.sro_cold_table_u.argtypes = [_dp, C.c_double]
+lib.sro_cold_table_u.restype = C.c_double
+lib.sro_eps_cold.argtypes = [_PP, C.c_double, C.c_double, C.c_double]
+lib.sro_eps_cold.restype = C.c_double
+lib.sro_pi_cold.argtypes = [_PP, C.c_double, C.c_double, C.c_double]
+lib.sro_pi_cold.restype = C.c_double
 lib.sro_twin_new.argtypes = [_PP, _sz, _sz, _dp, C.c_int, C.c_int, C.c_int]
 lib.sro_twin_new.restype = C.c_void_p
 lib.sro_twin_free.argtypes = [C.c_void_p]
@@ -241,6 +254,41 @@ def mg_apply(own, kx, ky, gas, r, levels=0, nu1=2, nu2=2, coarse_scale=1.0):
     lib.sro_mg_apply(w, h, np.ascontiguousarray(own), np.ascontiguousarray(kx), np.ascontiguousarray(ky),
                      None if g is None else g.ctypes.data, np.ascontiguousarray(r), z, levels, nu1, nu2)
     return z
+
+
+COLD_POINTS = 512
+SHIPPED_COLD = dict(k1e=21.3, k2e=134.0, k1n=2.67, k2n=58.2)  # assets/physics.json
+
+
+def cold_pressure(x: float, k1: float, k2: float) -> float:
+    return lib.sro_cold_pressure(x, k1, k2)
+
+
+def cold_energy_exact(x: float, k1: float, k2: float) -> float:
+    return lib.sro_cold_energy_exact(x, k1, k2)
+
+
+def cold_node_x(i: int) -> float:
+    return lib.sro_cold_node_x(i)
+
+
+def cold_table(k1: float, k2: float) -> np.ndarray:
+    """ln u at the table's 512 nodes"""
+    t = np.zeros(COLD_POINTS)
+    lib.sro_cold_table(k1, k2, t)
+    return t
+
+
+def cold_table_u(ln_u: np.ndarray, x: float) -> float:
+    return lib.sro_cold_table_u(np.ascontiguousarray(ln_u, dtype=np.float64), x)
+
+
+def eps_cold(ph: Physics, sigma: float, y_e: float, x_n: float) -> float:
+    return lib.sro_eps_cold(C.byref(ph), sigma, y_e, x_n)
+
+
+def pi_cold(ph: Physics, sigma: float, y_e: float, x_n: float) -> float:
+    return lib.sro_pi_cold(C.byref(ph), sigma, y_e, x_n)
 
 
 def floors_apply(ph: Physics, planes: np.ndarray):

@@ -67,8 +67,10 @@ static side_t side_of(const sro_physics *ph, const double *planes, size_t n, siz
         sd.kappa = kappa_of(ph, x, ph->sigma_floor, ph->t_floor);
         return sd;
     }
-    double mx = planes[n + i], my = planes[2 * n + i];
-    double raw = planes[3 * n + i] / sigma - 0.5 * (mx * mx + my * my) / (sigma * sigma);
+    double mx = planes[n + i], my = planes[2 * n + i], y_e;
+    sro_composition(x, NULL, &y_e, NULL);
+    double raw = planes[3 * n + i] / sigma - 0.5 * (mx * mx + my * my) / (sigma * sigma) -
+                 sro_eps_cold(ph, sigma, y_e, x[NS - 1]);
     double fl = ph->t_floor * sd.inv_mu / (ph->gamma - 1.0);
     double eps = raw < fl ? fl : raw;
     sd.gas = 1;

@@ -48,6 +48,13 @@ static double inv_mu_of(const sro_physics *ph, const double *x) {
     return im;
 }
 
+/* Y_e of a cell's species (registry.rs:146-154) */
+static double y_e_of(const double *x) {
+    double ye;
+    sro_composition(x, NULL, &ye, NULL);
+    return ye;
+}
+
 /* eps floor T_floor / (mu (gamma - 1)) */
 static double eps_floor(const sro_physics *ph, double inv_mu) { return ph->t_floor * inv_mu / (ph->gamma - 1.0); }
 
@@ -123,13 +130,16 @@ typedef struct {
     const sro_physics *ph;
 } hy_t;
 
-static double pi_of(const sro_physics *ph, double sigma, double eps) { return (ph->gamma - 1.0) * sigma * eps; }
+/* Pi = Pi_th + Pi_e + Pi_n (eos.rs:233-236), Pi_th = (gamma - 1) Sigma eps_th */
+static double pi_of(const sro_physics *ph, double sigma, double eps, const double *x) {
+    return (ph->gamma - 1.0) * sigma * eps + sro_pi_cold(ph, sigma, y_e_of(x), x[NS - 1]);
+}
 
-/* thermal energy recovered from E, floored (eos.rs:251-260, cold energy 0) */
+/* thermal energy recovered from E less the kinetic and the cold energy, floored (eos.rs:251-260) */
 static double thermal(const sro_physics *ph, double E, double sigma, double mx, double my, double inv_mu,
-                      double *floor_added) {
+                      const double *x, double *floor_added) {
     double kinetic = 0.5 * (mx * mx + my * my) / (sigma * sigma);
-    double raw = E / sigma - kinetic;
+    double raw = E / sigma - kinetic - sro_eps_cold(ph, sigma, y_e_of(x), x[NS - 1]);
     double fl = eps_floor(ph, inv_mu);
     if (raw < fl) {
         if (floor_added) *floor_added = sigma * (fl - raw);
@@ -147,26 +157,26 @@ static prim_t read_cell(const sro_physics *ph, const double *planes, size_t n, s
     if (sigma < ph->sigma_vac) {
         p.sigma = ph->sigma_floor;
         p.u = p.v = 0.0;
-        p.pi = pi_of(ph, ph->sigma_floor, eps_floor(ph, inv_mu));
+        p.pi = pi_of(ph, ph->sigma_floor, eps_floor(ph, inv_mu), p.x);
         return p;
     }
     double mx = planes[n + i], my = planes[2 * n + i];
-    double eps = thermal(ph, planes[3 * n + i], sigma, mx, my, inv_mu, NULL);
+    double eps = thermal(ph, planes[3 * n + i], sigma, mx, my, inv_mu, p.x, NULL);
     p.sigma = sigma;
     p.u = mx / sigma;
     p.v = my / sigma;
-    p.pi = pi_of(ph, sigma, eps);
+    p.pi = pi_of(ph, sigma, eps, p.x);
     return p;
 }
 
 static void cons(const sro_physics *ph, const prim_t *p, double *u) {
-    double inv_mu = inv_mu_of(ph, p->x);
-    double th = p->pi / ((ph->gamma - 1.0) * p->sigma);
+    double inv_mu = inv_mu_of(ph, p->x), y_e = y_e_of(p->x), x_n = p->x[NS - 1];
+    double th = (p->pi - sro_pi_cold(ph, p->sigma, y_e, x_n)) / ((ph->gamma - 1.0) * p->sigma);
     double eps = fmax(th, eps_floor(ph, inv_mu));
     u[0] = p->sigma;
     u[1] = p->sigma * p->u;
     u[2] = p->sigma * p->v;
-    u[3] = p->sigma * eps + 0.5 * p->sigma * (p->u * p->u + p->v * p->v);
+    u[3] = p->sigma * (eps + sro_eps_cold(ph, p->sigma, y_e, x_n)) + 0.5 * p->sigma * (p->u * p->u + p->v * p->v);
     for (int s = 0; s < NS; s++) u[4 + s] = p->sigma * p->x[s];
 }
 
@@ -177,13 +187,14 @@ static int prim_of(const sro_physics *ph, const double *u, prim_t *p) {
         if (!isfinite(u[k])) return 0;
     for (int s = 0; s < NS; s++) p->x[s] = u[4 + s] / sigma;
     double inv_mu = inv_mu_of(ph, p->x);
-    double raw = u[3] / sigma - 0.5 * (u[1] * u[1] + u[2] * u[2]) / (sigma * sigma);
+    double raw = u[3] / sigma - 0.5 * (u[1] * u[1] + u[2] * u[2]) / (sigma * sigma) -
+                 sro_eps_cold(ph, sigma, y_e_of(p->x), p->x[NS - 1]);
     if (isnan(raw) || raw <= 0.0) return 0;
     double eps = fmax(raw, eps_floor(ph, inv_mu));
     p->sigma = sigma;
     p->u = u[1] / sigma;
     p->v = u[2] / sigma;
-    p->pi = pi_of(ph, sigma, eps);
+    p->pi = pi_of(ph, sigma, eps, p->x);
     return 1;
 }
 
@@ -335,7 +346,7 @@ static double push_share(const sro_physics *ph, const double *u, const double d[
     for (int s = 0; s < NS; s++) x[s] = u[4 + s] / sigma;
     double inv_mu = inv_mu_of(ph, x);
     double fa;
-    double eps = thermal(ph, u[3], sigma, u[1], u[2], inv_mu, &fa);
+    double eps = thermal(ph, u[3], sigma, u[1], u[2], inv_mu, x, &fa);
     double raw = eps - fa / sigma;
     double a = sigma * (raw - eps_floor(ph, inv_mu));
     if (isnan(a) || a <= 0.0) return 0.0;
@@ -554,7 +565,8 @@ int sro_hydro_step(const sro_physics *ph, double *planes, size_t w, size_t h, do
                     double inv_mu = inv_mu_of(ph, xs);
                     double t = 0.0;
                     if (ph->opacity == SRO_OPACITY_KRAMERS || (ph->kappa_dust != 0.0)) {
-                        double eps = thermal(ph, planes[3 * n + i], sigma, planes[n + i], planes[2 * n + i], inv_mu, NULL);
+                        double eps = thermal(ph, planes[3 * n + i], sigma, planes[n + i], planes[2 * n + i], inv_mu,
+                                             xs, NULL);
                         t = (ph->gamma - 1.0) * eps / inv_mu;
                     }
                     double kap = kappa_cell(ph, xs, sigma, t);

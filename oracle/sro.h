@@ -49,7 +49,20 @@ typedef struct {
     double constant_diffusivity; /* > 0 switches it on */
     int flux_limiter;
     double max_padded;  /* FFT cap per padded axis (reference 2048) */
+    /* cold pressure pairs (eos.rs:24-30): electrons (k1e, k2e) on Y_e Sigma, neutron matter (k1n, k2n) on X_n Sigma; a
+     * pair with a K of 0 is off (the reference's shipped 21.3 / 134 and 2.67 / 58.2) */
+    double k1e, k2e, k1n, k2n;
 } sro_physics;
+
+/* ---------------- cold pressure (reference crates/sr-physics/src/eos.rs) ---------------- */
+double sro_cold_pressure(double x, double k1, double k2);
+double sro_cold_energy_exact(double x, double k1, double k2);
+double sro_cold_node_x(int i);
+void sro_cold_table(double k1, double k2, double *ln_u); /* 512 entries */
+double sro_cold_table_u(const double *ln_u, double x);
+int sro_cold_on(const sro_physics *ph);
+double sro_pi_cold(const sro_physics *ph, double sigma, double y_e, double x_n);
+double sro_eps_cold(const sro_physics *ph, double sigma, double y_e, double x_n);
 
 /* ---------------- FFT and convolution (reference crates/sr-physics/src/reference/fft.rs) ---------------- */
 int sro_padded_size(size_t len, size_t max_padded, size_t *out);

@@ -1,7 +1,8 @@
 /* Oracle (test infrastructure only): P8 floors and the twin's step driver in f64.
  * Restates crates/sr-physics/src/reference/floors.rs:45-175 (renormalised, Floors::cell, Floors::apply; Sigma_vac and
- * Sigma_floor read as the f32 the device holds, floors.rs:110) without the hold (no held cell arises with the cold
- * pressure off), and crates/sr-physics/src/reference/mod.rs:129-297 (Twin::new, Twin::step: P1 dt, P2 gravity,
+ * Sigma_floor read as the f32 the device holds, floors.rs:110; the cold energy of eos.rs in the vacuum reset and the
+ * thermal floor) without the hold (floors.rs:200-260: no cell is held below Sigma_hold = 10, nor with the cold pressure
+ * off), and crates/sr-physics/src/reference/mod.rs:129-297 (Twin::new, Twin::step: P1 dt, P2 gravity,
  * P4 hydro with the last P5 flux as the radiation force, P5 heat, P8 floors, P9 dt_{n+1} with P2's g of the step's
  * start state).
  */
@@ -37,15 +38,13 @@ void sro_floors_apply(const sro_physics *ph, double *planes, size_t w, size_t h,
             memset(x, 0, sizeof x);
             x[0] = 1.0;
         }
-        double inv_mu;
-        if (ph->mu > 0)
-            inv_mu = 1.0 / ph->mu;
-        else
-            sro_composition(x, &inv_mu, NULL, NULL);
+        double inv_mu, y_e;
+        sro_composition(x, &inv_mu, &y_e, NULL);
+        if (ph->mu > 0) inv_mu = 1.0 / ph->mu;
         double eps_fl = ph->t_floor * inv_mu / (ph->gamma - 1.0);
         double sigma = planes[i], E = planes[3 * n + i];
         if (sigma < svac) {
-            double e = sfloor * eps_fl;
+            double e = sfloor * (eps_fl + sro_eps_cold(ph, sfloor, y_e, x[NS - 1]));
             vr += sigma - sfloor;
             fa += e - E;
             planes[i] = sfloor;
@@ -54,7 +53,8 @@ void sro_floors_apply(const sro_physics *ph, double *planes, size_t w, size_t h,
             planes[3 * n + i] = e;
         } else {
             double mx = planes[n + i], my = planes[2 * n + i];
-            double rawe = E / sigma - 0.5 * (mx * mx + my * my) / (sigma * sigma);
+            double rawe = E / sigma - 0.5 * (mx * mx + my * my) / (sigma * sigma) -
+                          sro_eps_cold(ph, sigma, y_e, x[NS - 1]);
             if (rawe < eps_fl) {
                 double add = sigma * (eps_fl - rawe);
                 planes[3 * n + i] = E + add;
