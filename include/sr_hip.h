@@ -59,6 +59,9 @@ typedef struct {
     double constant_diffusivity; /* > 0 switches it on */
     int flux_limiter;
     int max_padded;     /* FFT cap per padded axis; the reference's is 2048 (step/fft.rs:26-63), the library's default 4096 */
+    double k1e, k2e;    /* the electrons' cold pressure pair on Y_e Sigma (eos.rs:24-30; shipped 21.3, 134) */
+    double k1n, k2n;    /* neutron matter's on X_n Sigma (shipped 2.67, 58.2); a pair of zeros is off (the default
+                         * here, as the reference's Sod and heat scenes set it); one K of a pair alone is refused */
 } sr_physics;
 
 typedef struct {

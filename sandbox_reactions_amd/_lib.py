@@ -34,6 +34,7 @@ class Physics(C.Structure):
         ("opacity", C.c_int), ("mu", C.c_double), ("edge", C.c_int),
         ("pin_temperature", C.c_int), ("constant_diffusivity", C.c_double), ("flux_limiter", C.c_int),
         ("max_padded", C.c_int),
+        ("k1e", C.c_double), ("k2e", C.c_double), ("k1n", C.c_double), ("k2n", C.c_double),
     ]
 
 

@@ -33,8 +33,11 @@ __global__ void k_heat_cells(HeatParams P, const double *__restrict__ sigma, con
     double v[1] = {0.0};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
         float xs0 = x[i];
-        float im = 0.0f;
-        for (int s = 0; s < NS; s++) im += x[(size_t)s * n + i] * h_imu[s];
+        float im = 0.0f, ye = 0.0f;
+        for (int s = 0; s < NS; s++) {
+            im += x[(size_t)s * n + i] * h_imu[s];
+            ye += x[(size_t)s * n + i] * c_ye[s];
+        }
         if (P.fixed_imu > 0.0f) im = P.fixed_imu;
         const double sig = sigma[i];
         const bool gas = sig >= (double)P.sigma_vac;
@@ -45,7 +48,8 @@ __global__ void k_heat_cells(HeatParams P, const double *__restrict__ sigma, con
         } else {
             s = sig;
             const double m0 = mx[i], m1 = my[i];
-            const double raw = (double)e[i] / s - 0.5 * (m0 * m0 + m1 * m1) / (s * s);
+            const double cold = P.cold.on ? (double)eps_cold(P.cold, (float)s, ye, x[(size_t)(NS - 1) * n + i]) : 0.0;
+            const double raw = (double)e[i] / s - 0.5 * (m0 * m0 + m1 * m1) / (s * s) - cold;
             const double fl = (double)P.t_floor * im / P.gm1;
             const double eps = raw < fl ? fl : raw;
             t = P.gm1 * eps / im;

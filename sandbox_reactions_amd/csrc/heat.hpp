@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/sr_hip.h"
+#include "cold.hpp"
 #include "fld.hpp"
 
 namespace sr {
@@ -22,6 +23,7 @@ struct HeatParams {
     double chi;
     int limiter;
     int pinned;
+    ColdEos cold;           // the cold energy comes off E before the temperature is read
 };
 
 struct HeatPlanes {

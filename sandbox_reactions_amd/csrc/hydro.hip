@@ -40,6 +40,7 @@ __constant__ float c_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f /
 
 struct Prim {
     float s, u, v, p, im;
+    float ye, xn;  // Y_e and X_n, read only with the cold pressure on
 };
 
 __device__ __forceinline__ float mc(float l, float c, float r) {
@@ -55,27 +56,33 @@ __device__ __forceinline__ float sound(const HydroParams &P, float s, float p) {
     return sqrtf(fmaxf(__fdividef(P.gamma * p, s), 0.0f));
 }
 
-// E of a primitive state (hydro.rs:444-459 with gamma): eps = max(Pi / ((gamma-1) Sigma), floor)
-__device__ __forceinline__ float cons_e(const HydroParams &P, const Prim &q) {
-    const float eps = fmaxf(__fdividef(q.p, P.gm1 * q.s), eps_floor(P, q.im));
-    return q.s * eps + 0.5f * q.s * (q.u * q.u + q.v * q.v);
+// E of a primitive state (hydro.rs:444-459 with gamma): eps = max((Pi - Pi_cold) / ((gamma-1) Sigma), floor),
+// E = Sigma (eps + eps_cold) + kinetic
+template <bool COLD> __device__ __forceinline__ float cons_e(const HydroParams &P, const Prim &q) {
+    const float pc = COLD ? pi_cold(P.cold, q.s, q.ye, q.xn) : 0.0f;
+    const float eps = fmaxf(__fdividef(q.p - pc, P.gm1 * q.s), eps_floor(P, q.im));
+    const float ec = COLD ? eps_cold(P.cold, q.s, q.ye, q.xn) : 0.0f;
+    return q.s * (eps + ec) + 0.5f * q.s * (q.u * q.u + q.v * q.v);
 }
 
-// primitives of a conserved (s, mx, my, E, s*im); false when not a state (hydro.rs:462-479)
+// primitives of a conserved (s, mx, my, E, s*im, s*Y_e, s*X_n); false when not a state (hydro.rs:462-479)
+template <bool COLD>
 __device__ __forceinline__ bool prim_of(const HydroParams &P, float s, float mx, float my, float e, float sim,
-                                        Prim &o) {
+                                        float sye, float sxn, Prim &o) {
     if (!(s > 0.0f) || !isfinite(s) || !isfinite(mx) || !isfinite(my) || !isfinite(e) || !isfinite(sim))
         return false;
     const float is = 1.0f / s;
-    const float im = sim * is, u = mx * is, v = my * is;
-    const float raw = e * is - 0.5f * (u * u + v * v);
+    const float im = sim * is, u = mx * is, v = my * is, ye = sye * is, xn = sxn * is;
+    const float raw = e * is - 0.5f * (u * u + v * v) - (COLD ? eps_cold(P.cold, s, ye, xn) : 0.0f);
     if (!(raw > 0.0f)) return false;
     const float eps = fmaxf(raw, eps_floor(P, im));
     o.s = s;
     o.u = u;
     o.v = v;
-    o.p = P.gm1 * s * eps;
+    o.p = P.gm1 * s * eps + (COLD ? pi_cold(P.cold, s, ye, xn) : 0.0f);
     o.im = im;
+    o.ye = ye;
+    o.xn = xn;
     return true;
 }
 
@@ -162,8 +169,9 @@ __device__ __forceinline__ void slot_pos(int t, int &x, int &y) {
     }
 }
 
-struct Lds {
+template <bool COLD> struct Lds {
     float s[NPRIM], u[NPRIM], v[NPRIM], p[NPRIM], im[NPRIM];
+    float ye[COLD ? NPRIM : 1];  // Y_e, with the cold pressure on
     float x[NS][NPRIM];
     // predicted face states of every slot: [field][face][slot], fields s u v p e; reused for species faces
     float face[5][4][NSLOT];
@@ -172,7 +180,7 @@ struct Lds {
     unsigned long long redk[8];
 };
 
-template <int MODE>
+template <int MODE, bool COLD>
 __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, HydroState out,
                                                  const double *__restrict__ scal, const float *__restrict__ gx,
                                                  const float *__restrict__ gy, const float *__restrict__ sigma_mid,
@@ -180,7 +188,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                                                  const float *__restrict__ rfy, float *__restrict__ sigma_mid_out,
                                                  float *__restrict__ pushmag, double *__restrict__ escaped,
                                                  HydroPartials part) {
-    __shared__ Lds L;
+    __shared__ Lds<COLD> L;
     const int w = P.w, h = P.h;
     const int bx = blockIdx.x * T, by = blockIdx.y * T;
     const int tid = threadIdx.x;
@@ -207,10 +215,11 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
         }
         const size_t i = (size_t)gyi * w + gxi;
         float xs[NS];
-        float im = 0.0f;
+        float im = 0.0f, ye = 0.0f;
         for (int s = 0; s < NS; s++) {
             xs[s] = in.x[s * n + i];
             im += xs[s] * c_imu[s];
+            if (COLD) ye += xs[s] * c_ye[s];
         }
         if (P.fixed_imu > 0.0f) im = P.fixed_imu;
         const double sig = in.sigma[i];
@@ -218,16 +227,18 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
         if (sig < (double)P.sigma_vac) {
             q.s = P.sigma_floor;
             q.u = q.v = 0.0f;
-            q.p = P.gm1 * P.sigma_floor * eps_floor(P, im);
+            q.p = P.gm1 * P.sigma_floor * eps_floor(P, im) +
+                  (COLD ? pi_cold(P.cold, P.sigma_floor, ye, xs[NS - 1]) : 0.0f);
         } else {
             const float s = (float)sig, is = 1.0f / s;
             const float u = in.mx[i] * is, v = in.my[i] * is;
-            const float raw = in.e[i] * is - 0.5f * (u * u + v * v);
+            const float raw = in.e[i] * is - 0.5f * (u * u + v * v) -
+                              (COLD ? eps_cold(P.cold, s, ye, xs[NS - 1]) : 0.0f);
             const float eps = fmaxf(raw, eps_floor(P, im));
             q.s = s;
             q.u = u;
             q.v = v;
-            q.p = P.gm1 * s * eps;
+            q.p = P.gm1 * s * eps + (COLD ? pi_cold(P.cold, s, ye, xs[NS - 1]) : 0.0f);
         }
         if (P.edge == 0) {
             if (fx) q.u = -q.u;
@@ -242,6 +253,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
         L.v[t] = q.v;
         L.p[t] = q.p;
         L.im[t] = im;
+        if (COLD) L.ye[t] = ye;
         for (int s = 0; s < NS; s++) L.x[s][t] = xs[s];
     }
     __syncthreads();
@@ -261,13 +273,13 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
     float s_e[4];              // evolved faces' Sigma (0 when the unevolved face is kept)
     float is_e[4];             // its reciprocal (0 when the unevolved face is kept)
     {
-        const Prim c = {L.s[lc], L.u[lc], L.v[lc], L.p[lc], L.im[lc]};
+        const Prim c = {L.s[lc], L.u[lc], L.v[lc], L.p[lc], L.im[lc], COLD ? L.ye[lc] : 0.0f, L.x[NS - 1][lc]};
         Prim wf[4];
         for (int ax = 0; ax < 2; ax++) {
             const int off = ax ? PW : 1;
             const int il = lc - off, ir = lc + off;
-            const Prim l = {L.s[il], L.u[il], L.v[il], L.p[il], L.im[il]};
-            const Prim r = {L.s[ir], L.u[ir], L.v[ir], L.p[ir], L.im[ir]};
+            const Prim l = {L.s[il], L.u[il], L.v[il], L.p[il], L.im[il], COLD ? L.ye[il] : 0.0f, L.x[NS - 1][il]};
+            const Prim r = {L.s[ir], L.u[ir], L.v[ir], L.p[ir], L.im[ir], COLD ? L.ye[ir] : 0.0f, L.x[NS - 1][ir]};
             float ss = mc(l.s, c.s, r.s), su = mc(l.u, c.u, r.u), sv = mc(l.v, c.v, r.v), sp = mc(l.p, c.p, r.p);
             float th = 1.0f;
             for (int s = 0; s < NS; s++) {
@@ -276,7 +288,9 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 if (central != 0.0f) th = fminf(th, __fdividef(mc(xl, xc, xr), central));
             }
             theta[ax] = th;
+            // 1/mu, Y_e and X_n are linear in the fractions: their faces take the species' shared slope
             const float sim = th * 0.5f * (r.im - l.im);
+            const float sye = COLD ? th * 0.5f * (r.ye - l.ye) : 0.0f, sxn = COLD ? th * 0.5f * (r.xn - l.xn) : 0.0f;
             const float hottest = fmaxf(fmaxf(__fdividef(l.p, l.s), __fdividef(c.p, c.s)), __fdividef(r.p, r.s));
             bool hot = false;
             for (int k = 0; k < 2; k++) {
@@ -296,18 +310,26 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 q.v = c.v + f * sv;
                 q.p = c.p + f * sp;
                 q.im = c.im + f * sim;
+                q.ye = c.ye + f * sye;
+                q.xn = c.xn + f * sxn;
             }
         }
         float ue[4], fl[4][4];
         for (int k = 0; k < 4; k++) {
-            ue[k] = cons_e(P, wf[k]);
+            ue[k] = cons_e<COLD>(P, wf[k]);
             flux4(wf[k], ue[k], k >> 1, fl[k]);
             fs_w[k] = wf[k].s;
             f0_w[k] = fl[k][0];
         }
-        float delta[5];
+        float delta[7];
         for (int c4 = 0; c4 < 4; c4++) delta[c4] = hdx * ((fl[0][c4] - fl[1][c4]) + (fl[2][c4] - fl[3][c4]));
         delta[4] = hdx * ((fl[0][0] * wf[0].im - fl[1][0] * wf[1].im) + (fl[2][0] * wf[2].im - fl[3][0] * wf[3].im));
+        delta[5] = COLD ? hdx * ((fl[0][0] * wf[0].ye - fl[1][0] * wf[1].ye) +
+                                 (fl[2][0] * wf[2].ye - fl[3][0] * wf[3].ye))
+                        : 0.0f;
+        delta[6] = COLD ? hdx * ((fl[0][0] * wf[0].xn - fl[1][0] * wf[1].xn) +
+                                 (fl[2][0] * wf[2].xn - fl[3][0] * wf[3].xn))
+                        : 0.0f;
         float m0 = c.s + delta[0], m1 = c.s * c.u + delta[1], m2 = c.s * c.v + delta[2];
         if (grav) {
             m1 += half * c.s * gcx;
@@ -325,10 +347,10 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 d3 += half * q.s * ((q.u + 0.5f * half * gcx) * gcx + (q.v + 0.5f * half * gcy) * gcy);
             }
             const float s0 = q.s + delta[0], mx = q.s * q.u + d1, my = q.s * q.v + d2, e = ue[k] + d3;
-            const float sim = q.s * q.im + delta[4];
+            const float sim = q.s * q.im + delta[4], sye = q.s * q.ye + delta[5], sxn = q.s * q.xn + delta[6];
             Prim o;
             float oe;
-            if (prim_of(P, s0, mx, my, e, sim, o)) {
+            if (prim_of<COLD>(P, s0, mx, my, e, sim, sye, sxn, o)) {
                 oe = e;
                 s_e[k] = s0;
                 is_e[k] = 1.0f / s0;
@@ -413,10 +435,10 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
     float u1 = 0.f, u2 = 0.f, u3 = 0.f;
     if (live && !empty) {
         if (vacuum) {
-            const Prim fq = {L.s[lc], L.u[lc], L.v[lc], L.p[lc], L.im[lc]};
+            const Prim fq = {L.s[lc], L.u[lc], L.v[lc], L.p[lc], L.im[lc], COLD ? L.ye[lc] : 0.0f, L.x[NS - 1][lc]};
             u1 = 0.0f;
             u2 = 0.0f;
-            u3 = cons_e(P, fq);
+            u3 = cons_e<COLD>(P, fq);
         } else {
             u1 = in.mx[ci];
             u2 = in.my[ci];
@@ -483,7 +505,8 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 float kap = P.kappa0 * (1.0f + L.x[0][lc]);
                 if (P.opacity == 1 || P.kappa_dust != 0.0f) {
                     const float s = (float)sig0, mx = in.mx[ci], my = in.my[ci];
-                    const float raw = in.e[ci] / s - 0.5f * (mx * mx + my * my) / (s * s);
+                    const float raw = in.e[ci] / s - 0.5f * (mx * mx + my * my) / (s * s) -
+                                      (COLD ? eps_cold(P.cold, s, L.ye[lc], L.x[NS - 1][lc]) : 0.0f);
                     const float t = P.gm1 * fmaxf(raw, eps_floor(P, im)) / im;
                     if (P.opacity == 1) {
                         kap = P.kappa0 * s * powf(t, -3.5f);
@@ -500,7 +523,9 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
             float share = 1.0f;
             if (d2 != 0.0f) {
                 const float s = (float)u0;
-                const float raw = u3 / s - 0.5f * (u1 * u1 + u2 * u2) / (s * s);
+                // the cell's composition before the update, as for 1/mu
+                const float raw = u3 / s - 0.5f * (u1 * u1 + u2 * u2) / (s * s) -
+                                  (COLD ? eps_cold(P.cold, s, L.ye[lc], L.x[NS - 1][lc]) : 0.0f);
                 const float a = s * (raw - eps_floor(P, L.im[lc]));
                 if (!(a > 0.0f)) {
                     share = 0.0f;
@@ -702,12 +727,15 @@ int hydro_launch(int mode, const HydroParams &P, const HydroState &in, const Hyd
                  const float *rfy, float *sigma_mid_out, float *pushmag, double *escaped, const HydroPartials &part,
                  hipStream_t s) {
     dim3 grid((P.w + T - 1) / T, (P.h + T - 1) / T);
+    // the cold pressure's variant only where a pair is on: the plain gas keeps the kernel without it
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(NSLOT), 0, s, P, in, out, scal, gx, gy, sigma_mid, phi_mid, rfx, rfy,
+                           sigma_mid_out, pushmag, escaped, part);
+    };
     if (mode == HYDRO_MID)
-        hipLaunchKernelGGL(k_hydro<HYDRO_MID>, grid, dim3(NSLOT), 0, s, P, in, out, scal, gx, gy, sigma_mid, phi_mid,
-                           rfx, rfy, sigma_mid_out, pushmag, escaped, part);
+        P.cold.on ? go(k_hydro<HYDRO_MID, true>) : go(k_hydro<HYDRO_MID, false>);
     else
-        hipLaunchKernelGGL(k_hydro<HYDRO_UPDATE>, grid, dim3(NSLOT), 0, s, P, in, out, scal, gx, gy, sigma_mid,
-                           phi_mid, rfx, rfy, sigma_mid_out, pushmag, escaped, part);
+        P.cold.on ? go(k_hydro<HYDRO_UPDATE, true>) : go(k_hydro<HYDRO_UPDATE, false>);
     SR_HIP(hipGetLastError());
     return SR_OK;
 }

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "cold.hpp"
+
 namespace sr {
 
 enum { HYDRO_MID = 0, HYDRO_UPDATE = 1 };
@@ -17,6 +19,7 @@ struct HydroParams {
     float inv_dx;
     float kappa0, c_sb, kappa_dust, t_dust;
     int opacity;           // 0 reference, 1 Kramers
+    ColdEos cold;          // the cold pressure pairs (off: on == 0)
 };
 
 // The state on the device: Sigma in FP64, the momenta and E in FP32, ten species fractions (planes back to back).

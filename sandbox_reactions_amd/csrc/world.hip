@@ -31,6 +31,7 @@ __constant__ float w_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f /
 struct FloorParams {
     int w, h;
     float gamma, gm1, t_floor_g, fixed_imu;
+    ColdEos cold;
     double sigma_vac;          // the hydro's test (f64)
     float svac32, sfloor32;    // the floors' f32 constants (floors.rs:110)
     double cfl, eta_g, dx;
@@ -92,17 +93,22 @@ __global__ void __launch_bounds__(NT) k_floors_dt(FloorParams P, HydroState st, 
             } else {
                 for (int s = 0; s < NS; s++) xs[s] = s == 0 ? 1.0f : 0.0f;
             }
-            float im = 0.0f;
-            for (int s = 0; s < NS; s++) im += xs[s] * w_imu[s];
+            float im = 0.0f, ye = 0.0f;
+            for (int s = 0; s < NS; s++) {
+                im += xs[s] * w_imu[s];
+                ye += xs[s] * c_ye[s];
+            }
             if (P.fixed_imu > 0.0f) im = P.fixed_imu;
             const float efl = P.t_floor_g * im;
             if (sig < (double)P.svac32) {
+                // floors.rs:128-139: the floor state at rest, its thermal energy at T_floor and its cold energy
                 sig = (double)P.sfloor32;
                 mx = my = 0.0f;
-                e = P.sfloor32 * efl;
+                e = P.sfloor32 * (efl + (P.cold.on ? eps_cold(P.cold, P.sfloor32, ye, xs[NS - 1]) : 0.0f));
             } else {
                 const float s = (float)sig;
-                const float raw = e / s - 0.5f * (mx * mx + my * my) / (s * s);
+                const float raw = e / s - 0.5f * (mx * mx + my * my) / (s * s) -
+                                  (P.cold.on ? eps_cold(P.cold, s, ye, xs[NS - 1]) : 0.0f);
                 if (raw < efl) e = e + s * (efl - raw);
             }
             st.sigma[i] = sig;
@@ -112,19 +118,26 @@ __global__ void __launch_bounds__(NT) k_floors_dt(FloorParams P, HydroState st, 
             for (int s = 0; s < NS; s++) st.x[(size_t)s * n + i] = xs[s];
         }
         // the rate as P9 reads the cell (hydro.rs:424-441): the floor state below sigma_vac
-        float im = 0.0f;
-        for (int s = 0; s < NS; s++) im += xs[s] * w_imu[s];
+        float im = 0.0f, ye = 0.0f;
+        for (int s = 0; s < NS; s++) {
+            im += xs[s] * w_imu[s];
+            ye += xs[s] * c_ye[s];
+        }
         if (P.fixed_imu > 0.0f) im = P.fixed_imu;
         float u = 0.0f, vv = 0.0f, c2;
+        // c^2 = gamma Pi / Sigma with Pi = Pi_th + Pi_e + Pi_n (hydro.rs:481-484, eos.rs:233-241)
         if (sig < P.sigma_vac) {
             c2 = P.gamma * P.gm1 * P.t_floor_g * im;
+            if (P.cold.on) c2 += P.gamma * pi_cold(P.cold, P.sfloor32, ye, xs[NS - 1]) / P.sfloor32;
         } else {
             const float s = (float)sig;
             u = mx / s;
             vv = my / s;
-            const float raw = e / s - 0.5f * (u * u + vv * vv);
+            const float cold = P.cold.on ? eps_cold(P.cold, s, ye, xs[NS - 1]) : 0.0f;
+            const float raw = e / s - 0.5f * (u * u + vv * vv) - cold;
             const float eps = fmaxf(raw, P.t_floor_g * im);
             c2 = P.gamma * P.gm1 * eps;
+            if (P.cold.on) c2 += P.gamma * pi_cold(P.cold, s, ye, xs[NS - 1]) / s;
         }
         const float c = sqrtf(fmaxf(c2, 0.0f));
         const double rate = (double)((fabsf(u) + c) + (fabsf(vv) + c));
@@ -287,6 +300,12 @@ extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_swit
                                             " world: each axis a multiple of 8 in [8, 4096]");
     if (!(ph->gamma > 1.0) || !(ph->dx > 0.0) || !(ph->t_floor > 0.0))
         return fail(SR_ERR_ARGUMENT, "gamma > 1, dx > 0 and t_floor > 0 are required");
+    // a cold pressure pair is on or off as a whole (eos.rs:172-183)
+    for (const double *k : {&ph->k1e, &ph->k1n}) {
+        const double k1 = k[0], k2 = k[1];
+        if (!std::isfinite(k1) || !std::isfinite(k2) || k1 < 0.0 || k2 < 0.0 || ((k1 > 0.0) != (k2 > 0.0)))
+            return fail(SR_ERR_ARGUMENT, "a cold pressure pair (k1e, k2e or k1n, k2n) must be both > 0 or both 0");
+    }
     auto *wd = new sr_world();
     wd->w = w;
     wd->h = h;
@@ -338,7 +357,24 @@ extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_swit
         if ((c = padded_size(w, ph->max_padded > 0 ? ph->max_padded : 4096, &px))) return bail(c);
         if ((c = padded_size(h, ph->max_padded > 0 ? ph->max_padded : 4096, &px))) return bail(c);
     }
+    ColdEos cold{};
+    cold.k1e = (float)ph->k1e;
+    cold.k2e = (float)ph->k2e;
+    cold.k1n = (float)ph->k1n;
+    cold.k2n = (float)ph->k2n;
+    cold.on = ph->k1e > 0.0 || ph->k1n > 0.0;
+    if (cold.on) {
+        std::vector<float> t(2 * COLD_POINTS, 0.0f);
+        if (ph->k1e > 0.0) cold_table(ph->k1e, ph->k2e, t.data());
+        if (ph->k1n > 0.0) cold_table(ph->k1n, ph->k2n, t.data() + COLD_POINTS);
+        float *d = nullptr;
+        if ((c = world_alloc(wd, (void **)&d, sizeof(float) * t.size()))) return bail(c);
+        if (hipMemcpy(d, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(SR_ERR_DEVICE, "hipMemcpy (cold tables)"));
+        cold.lnu = d;
+    }
     HydroParams &hp = wd->hp;
+    hp.cold = cold;
     hp.w = w;
     hp.h = h;
     hp.gamma = (float)ph->gamma;
@@ -355,6 +391,7 @@ extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_swit
     hp.t_dust = (float)ph->t_dust;
     hp.opacity = ph->opacity;
     FloorParams &fp = wd->fp;
+    fp.cold = cold;
     fp.w = w;
     fp.h = h;
     fp.gamma = hp.gamma;
@@ -389,6 +426,7 @@ extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_swit
         P.chi = ph->constant_diffusivity;
         P.limiter = ph->flux_limiter;
         P.pinned = ph->pin_temperature;
+        P.cold = cold;
         if ((c = wd->heat.init(P, wd->hs))) return bail(c);
     }
     *out = wd;

@@ -41,6 +41,7 @@ def run(argv=None) -> int:
     try:
         scene = scene_json.load(a.scene)
         scene.check_path()
+        scene.planes(a.gamma)  # every cell laid down before a device is touched (a bad scene is exit 4)
         if a.steps < 0:
             raise scene_json.SceneError("--steps must be >= 0")
     except (OSError, ValueError) as e:

@@ -6,11 +6,13 @@ Covered: `rect` and `disk` objects (Maclaurin profile sampled at cell centres, s
 overlapping objects add their mass, momentum, energy and species mass (scene.rs:485-497); cells no object covers
 hold the floor state, Sigma_floor of hydrogen at rest at T_floor (scene.rs:466-470). Refused with a message: anything
 the path does not carry -- `preset` objects (they need the calibration file), the reactions / neutrinos / sinks
-switches, and cold pressure: the reference's shipped constants turn it on (k1e, k2e, k1n, k2n), so a scene must
-override all four to 0, as every one of the reference's parity scenes does.
+switches, and, with the cold pressure on, a start with a cell at or above sigma_hold (the hold of floors.rs:200-260,
+which the path does not carry, can only take such a cell).
 
-The energy follows the EOS with this library's adiabatic index gamma (the reference's gamma = 2 by default):
-eps = T / (mu (gamma - 1)) for `t`, eps = Pi / ((gamma - 1) Sigma) for `pressure`, plus the kinetic |v|^2 / 2.
+The energy follows the EOS with this library's adiabatic index gamma (the reference's gamma = 2 by default) and the
+cold pressure of the shipped constants (k1e, k2e, k1n, k2n; a scene's overrides may turn a pair off):
+eps_th = T / (mu (gamma - 1)) for `t`, eps_th = (Pi - Pi_e - Pi_n) / ((gamma - 1) Sigma) for `pressure` (refused where
+the cold pressure alone exceeds Pi, scene.rs:454-457), plus eps_cold and the kinetic |v|^2 / 2.
 """
 from __future__ import annotations
 
@@ -18,6 +20,8 @@ import json
 import math
 
 import numpy as np
+
+from . import cold as cold_eos
 
 # assets/elements.json: key, A, Z (the order of the species planes)
 ELEMENTS = [("H", 1, 1), ("He", 4, 2), ("C", 12, 6), ("O", 16, 8), ("Ne", 20, 10), ("Mg", 24, 12), ("Si", 28, 14),
@@ -143,9 +147,12 @@ class Scene:
             if isinstance(v, bool) or not isinstance(v, (int, float)):
                 raise SceneError(f"overrides.{k}: must be a number")
             self.overrides[k] = float(v)
-        if any(self.overrides.get(k, 1.0) != 0.0 for k in COLD):
-            raise SceneError("cold pressure (k1e, k2e, k1n, k2n) is not carried by this path: the scene must "
-                             "override all four to 0")
+        for a, b in (("k1e", "k2e"), ("k1n", "k2n")):
+            ka, kb = (self.overrides.get(k, cold_eos.SHIPPED[k]) for k in (a, b))
+            if not (isinstance(ka, (int, float)) and isinstance(kb, (int, float))) or ka < 0 or kb < 0 or \
+                    (ka > 0) != (kb > 0):
+                raise SceneError(f"overrides.{a}, {b}: a cold pressure pair is on or off as a whole")
+        self.sigma_hold = float(self.overrides.get("sigma_hold", 10.0))
         objs = top.get("objects")
         if not isinstance(objs, list):
             raise SceneError("objects: must be a list")
@@ -177,11 +184,15 @@ class Scene:
 
     def constants(self) -> dict:
         """The physics constants this path reads, shipped values with the scene's overrides."""
-        c = dict(SHIPPED)
-        for k in SHIPPED:
+        c = dict(SHIPPED, **cold_eos.SHIPPED)
+        for k in c:
             if k in self.overrides:
-                c[k] = self.overrides[k]
+                c[k] = float(self.overrides[k])
         return c
+
+    def cold_on(self) -> bool:
+        c = self.constants()
+        return c["k1e"] > 0.0 or c["k1n"] > 0.0
 
     def cells(self, obj):
         """(ys, xs, sigma) of the cells an object covers (scene.rs:521-563)."""
@@ -207,15 +218,22 @@ class Scene:
         n = w * h
         sig, mx, my, en = (np.zeros(n) for _ in range(4))
         spm = np.zeros((len(ELEMENTS), n))
-        for obj in self.objects:
+        c = self.constants()
+        for oi, obj in enumerate(self.objects):
             ys, xs, s = self.cells(obj)
             (kind, value), comp, v = obj[2]
             idx = ys * w + xs
+            ye, xn = float(cold_eos.y_e(comp)), float(comp[-1])
             if kind == "t":
                 eps = value / ((gamma - 1.0) / inv_mu(comp)) * np.ones_like(s)
             else:
-                eps = value / ((gamma - 1.0) * s)
-            eps = eps + 0.5 * (v[0] * v[0] + v[1] * v[1])
+                thermal = value - cold_eos.pi_cold(c, s, ye, xn)
+                if np.any(thermal <= 0.0):
+                    k = int(np.argmax(thermal <= 0.0))
+                    raise SceneError(f"objects[{oi}].pressure: {value} is below the cold pressure at cell "
+                                     f"({xs[k]}, {ys[k]})")
+                eps = thermal / ((gamma - 1.0) * s)
+            eps = eps + cold_eos.eps_cold(c, s, ye, xn) + 0.5 * (v[0] * v[0] + v[1] * v[1])
             np.add.at(sig, idx, s)
             np.add.at(mx, idx, s * v[0])
             np.add.at(my, idx, s * v[1])
@@ -223,9 +241,12 @@ class Scene:
             for k, x in enumerate(comp):
                 if x:
                     np.add.at(spm[k], idx, s * x)
-        c = self.constants()
         floor_sigma = c["sigma_floor"]
-        floor_e = floor_sigma * c["t_floor"] / ((gamma - 1.0) / inv_mu([1.0] + [0.0] * 9))
+        floor_e = floor_sigma * (c["t_floor"] / ((gamma - 1.0) / inv_mu([1.0] + [0.0] * 9)) +
+                                 float(cold_eos.eps_cold(c, floor_sigma, 1.0, 0.0)))
+        if self.cold_on() and np.max(sig, initial=0.0) >= self.sigma_hold:
+            raise SceneError(f"a cell starts at Sigma {np.max(sig):g} >= sigma_hold {self.sigma_hold:g} with the "
+                             "cold pressure on: the hold (floors.rs:200-260) is not carried by this path")
         out = np.zeros((14, n), np.float64)
         empty = sig <= 0.0
         out[0] = np.where(empty, floor_sigma, sig)

@@ -48,3 +48,47 @@ def test_heat_scenes(gpu_lib, name):
     g, t, world = run(name, 1, tight=True)
     assert world.read_heat().converged
     assert rel_l1(g[3], t[3]) <= 5e-5, (name, rel_l1(g[3], t[3]))
+
+
+def run_scene(s, steps):
+    import sandbox_reactions_amd as sr
+    kw = s.physics_kwargs()
+    op, gp = both(**kw)
+    world = scene_json.world(s)
+    start = s.planes(kw["gamma"]).astype(np.float64)
+    twin = oracle.Twin(op, start, gravity=s.switches["gravity"], hydro=s.switches["hydro"], heat=s.switches["heat"])
+    for _ in range(steps):
+        dt = world.read_dt().dt_next
+        twin.cap_dt_next(dt)
+        twin.step()
+        world.set_dt_cap(dt)
+        world.run(1)
+    return world.readback(), twin.state(), world
+
+
+PASSES_ON = {"gravity": True, "hydro": True, "heat": True, "reactions": False, "neutrinos": False, "sinks": False}
+
+
+@pytest.mark.parametrize("case", ["asym_blob", "neutron_disk"])
+def test_cold_pressure_scenes(gpu_lib, case):
+    """The shipped cold pressure on (eos.rs: electrons and neutron matter), gravity, hydro and heat: asym_blob.json
+    with its passes the path carries, and a disk of half hydrogen, half neutrons beside a `pressure` rect of helium
+    (both pairs, and a pressure object's cold share). Five steps on the twin's dt."""
+    import json
+    print(f"cold pressure scene {case}", flush=True)
+    if case == "asym_blob":
+        with open(os.path.join(HERE, "asym_blob.json")) as f:
+            d = json.load(f)
+    else:
+        d = {"version": 1, "world": {"width": 64, "height": 64, "edge": "bounce"},
+             "objects": [{"kind": "disk", "x": 30, "y": 33, "a": 12, "sigma_c": 2.0, "t": 1.0,
+                          "composition": {"H": 0.5, "n": 0.5}},
+                         {"kind": "rect", "x0": 44, "y0": 10, "w": 10, "h": 8, "sigma": 0.8, "pressure": 30.0,
+                          "composition": {"He": 1.0}}]}
+    d["switches"] = PASSES_ON
+    s = scene_json.parse(json.dumps(d))
+    assert s.cold_on()
+    g, t, world = run_scene(s, 5)
+    assert rel_l1(g[0], t[0]) <= 1e-5, (case, rel_l1(g[0], t[0]))
+    assert rel_l1(g[3], t[3]) <= 1e-4, (case, rel_l1(g[3], t[3]))
+    assert world.read_heat().converged

@@ -23,9 +23,17 @@ def test_a_missing_scene_is_exit_4():
     assert code == 4 and out.startswith("SR-ERROR")
 
 
-def test_a_scene_with_cold_pressure_is_exit_4():
+def test_a_scene_the_path_does_not_carry_is_exit_4(tmp_path):
     code, out = headless("--scene", os.path.join(SCENES, "asym_blob.json"), "--steps", "1")
-    assert code == 4 and "cold pressure" in out
+    assert code == 4 and "reactions" in out  # its default switches run P6
+    bad = {"version": 1, "world": {"width": 32, "height": 32},
+           "switches": {"reactions": False, "neutrinos": False, "sinks": False},
+           "objects": [{"kind": "rect", "x0": 0, "y0": 0, "w": 8, "h": 8, "sigma": 1.0, "pressure": 1e-3,
+                        "composition": {"H": 1.0}}]}
+    path = tmp_path / "below_cold.json"
+    path.write_text(json.dumps(bad))
+    code, out = headless("--scene", str(path), "--steps", "1")
+    assert code == 4 and "below the cold pressure" in out
 
 
 def test_no_device_is_exit_3():

@@ -86,6 +86,9 @@ def test_test_settings_and_constants():
     assert kw["sigma_floor"] == 1e-8 and kw["c_sb"] == 12800.0
 
 
+OFF = {"reactions": False, "neutrinos": False, "sinks": False}
+
+
 def base():  # an 8 x 8 rect of hydrogen in a 64 x 64 world, cold pressure off
     return {"version": 1, "world": {"width": 64, "height": 64}, "overrides": {k: 0 for k in scene_json.COLD},
             "objects": [{"kind": "rect", "x0": 0, "y0": 0, "w": 8, "h": 8, "sigma": 1.0, "t": 1.0,
@@ -96,8 +99,10 @@ def base():  # an 8 x 8 rect of hydrogen in a 64 x 64 world, cold pressure off
     (lambda d: d["objects"][0].update(pressure=1.0), "not both"),
     (lambda d: d["objects"][0].update(composition={"H": 0.9}), "sum to"),
     (lambda d: d["objects"][0].update(colour=1), "unknown key"),
-    (lambda d: d.pop("overrides"), "cold pressure"),
-    (lambda d: d["overrides"].update(k1e=21.3), "cold pressure"),
+    (lambda d: d["overrides"].update(k1e=21.3), "pair is on or off"),
+    (lambda d: (d.pop("overrides"), d.update(switches=OFF), d["objects"][0].update(sigma=12.0)), "sigma_hold"),
+    (lambda d: (d.pop("overrides"), d.update(switches=OFF), d["objects"][0].pop("t"),
+                d["objects"][0].update(pressure=1e-3)), "below the cold pressure"),
     (lambda d: d["objects"].append({"kind": "preset", "preset": "sun", "x": 1, "y": 1}), "calibration"),
     (lambda d: d.update(switches={"reactions": True, "neutrinos": False, "sinks": False}), "reactions"),
     (lambda d: d["world"].update(width=60), "world size"),
@@ -108,17 +113,56 @@ def test_refusals(edit, message):
     d = base()
     edit(d)
     with pytest.raises(scene_json.SceneError, match=message):
-        scene_json.parse(json.dumps(d)).check_path()
+        s = scene_json.parse(json.dumps(d))
+        s.check_path()
+        s.planes()
 
 
 def test_every_fixture_parses():
-    """Every fixture but asym_blob.json, which keeps the shipped cold pressure (refused: not carried)."""
-    with pytest.raises(scene_json.SceneError, match="cold pressure"):
-        load("asym_blob.json")
+    """Every fixture, asym_blob.json with the shipped cold pressure on."""
+    assert load("asym_blob.json").cold_on() and not load("sod.json").cold_on()
     for name in sorted(os.listdir(HERE)):
-        if name.endswith(".json") and name != "asym_blob.json":
+        if name.endswith(".json"):
             s = load(name)
             p = s.planes()
             assert p.shape == (14, s.height, s.width) and np.all(np.isfinite(p))
             frac = p[4:].sum(axis=0)
             assert np.allclose(frac, 1.0, atol=1e-6)
+
+
+def test_cold_energy_laid_down_with_the_gas():
+    """With the shipped pairs on (no overrides): a `t` object's cells hold Sigma (T / (mu (gamma - 1)) + eps_cold),
+    a `pressure` object's eps_th = (Pi - Pi_e - Pi_n) / ((gamma - 1) Sigma), the floor cells the floor state with its
+    cold energy (scene.rs:446-470); eps_cold and Pi_cold from the oracle's f64 restatement."""
+    import oracle
+    d = base()
+    d.pop("overrides")
+    d["objects"][0].update(composition={"H": 0.6, "He": 0.3, "n": 0.1}, t=2.0)
+    d["objects"].append({"kind": "rect", "x0": 20, "y0": 20, "w": 4, "h": 4, "sigma": 0.5, "pressure": 10.0,
+                         "composition": {"H": 1.0}})
+    s = scene_json.parse(json.dumps(d))
+    p = s.planes().astype(np.float64)
+    ph = oracle.reference_physics(**oracle.SHIPPED_COLD)
+    ye = 0.6 * 1.0 + 0.3 * 0.5
+    inv_mu = 0.6 * 2.0 + 0.3 * 0.75 + 0.1 * 1.0
+    want = 1.0 * (2.0 * inv_mu + oracle.eps_cold(ph, 1.0, ye, 0.1))
+    assert abs(p[3][3, 3] / want - 1.0) < 1e-6
+    eth = (10.0 - oracle.pi_cold(ph, 0.5, 1.0, 0.0)) / 0.5
+    assert abs(p[3][21, 21] / (0.5 * (eth + oracle.eps_cold(ph, 0.5, 1.0, 0.0))) - 1.0) < 1e-6
+    sf = 1e-8
+    assert abs(p[3][40, 40] / (sf * (0.05 * 2.0 + oracle.eps_cold(ph, sf, 1.0, 0.0))) - 1.0) < 1e-6
+    kw = s.physics_kwargs()
+    assert (kw["k1e"], kw["k2e"], kw["k1n"], kw["k2n"]) == (21.3, 134.0, 2.67, 58.2)
+
+
+def test_host_cold_eos_is_the_oracles():
+    import oracle
+    from sandbox_reactions_amd import cold
+    for k1, k2 in ((21.3, 134.0), (2.67, 58.2)):
+        assert np.max(np.abs(cold.table(k1, k2) - oracle.cold_table(k1, k2))) < 1e-12
+        xs = np.exp(np.linspace(np.log(1e-11), np.log(1e9), 500))
+        u = cold.energy(xs, k1, k2)
+        t = oracle.cold_table(k1, k2)
+        assert np.max(np.abs(u / np.array([oracle.cold_table_u(t, x) for x in xs]) - 1.0)) < 1e-12
+        assert np.max(np.abs(cold.pressure(xs, k1, k2) / np.array([oracle.cold_pressure(x, k1, k2) for x in xs])
+                             - 1.0)) < 1e-13
