@@ -94,6 +94,8 @@ lib.sro_floors_apply.argtypes = [_PP, _dp, _sz, _sz, C.POINTER(C.c_double), C.PO
 for _f in ("sro_cold_pressure", "sro_cold_energy_exact"):
     getattr(lib, _f).argtypes = [C.c_double, C.c_double, C.c_double]
     getattr(lib, _f).restype = C.c_double
+lib.sro_hydro_set_hot_face.argtypes = [C.c_int]
+lib.sro_hydro_set_hot_face.restype = None
 lib.sro_cold_node_x.argtypes = [C.c_int]
 lib.sro_cold_node_x.restype = C.c_double
 lib.sro_cold_table.argtypes = [C.c_double, C.c_double, _dp]

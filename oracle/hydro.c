@@ -99,6 +99,10 @@ static double mc(double l, double c, double r) {
     return copysign(m, a);
 }
 
+/* the HOT_FACE bracket on (1) or off (0): off only to show a test that the bracket is what it checks */
+static int g_hot_face = 1;
+void sro_hydro_set_hot_face(int on) { g_hot_face = on; }
+
 static prim_t slope(const prim_t *l, const prim_t *c, const prim_t *r) {
     prim_t s;
     s.sigma = mc(l->sigma, c->sigma, r->sigma);
@@ -119,7 +123,7 @@ static prim_t slope(const prim_t *l, const prim_t *c, const prim_t *r) {
         /* hot when pi > bound or the comparison is unordered (NaN) */
         if (!(pi <= bound)) hot = 1;
     }
-    if (hot) {
+    if (hot && g_hot_face) {
         s.sigma = 0.0;
         s.pi = 0.0;
     }
