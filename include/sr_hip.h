@@ -90,8 +90,10 @@ typedef struct {
 typedef struct {
     double dt, dt_next, sim_time; /* the step's dt, the next step's, sim time in f64 */
     uint64_t steps;
-    int non_finite;               /* 1 when the guard found a non-finite value */
-    int nf_x, nf_y, nf_channel;
+    int non_finite;               /* 1 when the guard has found a non-finite value since the upload */
+    int nf_x, nf_y, nf_channel;   /* its cell (the lowest cell index of that step's findings) and channel (0..13) */
+    uint64_t nf_step;             /* the index of the step whose P9 found it (0 also for the upload's own scan), counted
+                                   * from the upload; the first finding is kept (step/dt.rs:40-47, NonFinite) */
 } sr_dt_report;
 
 typedef struct {

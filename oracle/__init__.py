@@ -96,6 +96,8 @@ for _f in ("sro_cold_pressure", "sro_cold_energy_exact"):
     getattr(lib, _f).restype = C.c_double
 lib.sro_hydro_set_hot_face.argtypes = [C.c_int]
 lib.sro_hydro_set_hot_face.restype = None
+lib.sro_hydro_set_pair_work.argtypes = [C.c_int]
+lib.sro_hydro_set_pair_work.restype = None
 lib.sro_cold_node_x.argtypes = [C.c_int]
 lib.sro_cold_node_x.restype = C.c_double
 lib.sro_cold_table.argtypes = [C.c_double, C.c_double, _dp]

@@ -13,6 +13,9 @@
  *   commit_limited               heat.rs:544-592
  * With cell size dx a face's coefficient is k = D / dx^2 and its flux F = -D (u_b - u_a) / dx; R uses the gradient
  * |u_b - u_a| / dx. Heat capacity C = Sigma / (mu (gamma - 1)) (the reference's Sigma / mu at gamma = 2).
+ * Parity unpinned: Heat::step's Newton pass counts and delta on radiating scenes are restated only (the constant-
+ * diffusivity moments, the solve against a direct solve and commit_limited on the reference's ceiling dump are
+ * checked by the tests).
  */
 #include <math.h>
 #include <stdlib.h>

@@ -10,10 +10,13 @@
  *   dt                            hydro.rs:618-634
  *   step_held (no held cells)     hydro.rs:657-1025
  *   face_work                     hydro.rs:1048-1067
- * and the EOS it reads (crates/sr-physics/src/eos.rs:207-260) with the cold pressure off, plus the composition
- * (crates/sr-physics/src/registry.rs:146-154).
- * Held cells, sinks and the pull field are not restated (no held cell arises without cold pressure: floors.rs:207-219
- * needs eps_th <= eta_hold * eps_cold with eps_cold = 0).
+ * and the EOS it reads (crates/sr-physics/src/eos.rs:207-260; the cold pressure's terms from eos.c), plus the
+ * composition (crates/sr-physics/src/registry.rs:146-154).
+ * Held cells, sinks and the pull field are not restated (the path refuses a scene that starts at or above
+ * Sigma_hold, floors.rs:200-260).
+ * Parity unpinned: push_share and the push/kick take-backs (hydro.rs:392-410, :986-1001) are restated only; no closed
+ * form or output of the reference checks them. The other functions are checked by the tests (closed forms, the
+ * reference's own cases: the thin cell for HOT_FACE, the Jeans adiabat for face_work).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -102,6 +105,11 @@ static double mc(double l, double c, double r) {
 /* the HOT_FACE bracket on (1) or off (0): off only to show a test that the bracket is what it checks */
 static int g_hot_face = 1;
 void sro_hydro_set_hot_face(int on) { g_hot_face = on; }
+/* Test switch: 0 charges a face's negative work wholly to the two cells' energies, with no impulse between them (the
+ * split before the pairs' relative motion paid first; a planted mistake for the Jeans adiabat test). 1 is the
+ * reference's face_work. */
+static int g_pair_work = 1;
+void sro_hydro_set_pair_work(int on) { g_pair_work = on; }
 
 static prim_t slope(const prim_t *l, const prim_t *c, const prim_t *r) {
     prim_t s;
@@ -311,7 +319,7 @@ static void face_work(double q, double sl, double sh, double pl, double ph_, dou
     double w = a + b > 0.0 ? a / (a + b) : 0.5;
     double extra = dt * (q - 0.5 * (pl + ph_)) * d;
     double c = 0.0, kick = 0.0, el = 0.0, eh = 0.0;
-    if (extra < 0.0 && a > 0.0 && b > 0.0) {
+    if (g_pair_work && extra < 0.0 && a > 0.0 && b > 0.0) {
         double ul = pl / a, uh = ph_ / b;
         double du = ul - uh;
         double inv = 1.0 / a + 1.0 / b;

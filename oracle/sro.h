@@ -65,6 +65,7 @@ double sro_pi_cold(const sro_physics *ph, double sigma, double y_e, double x_n);
 double sro_eps_cold(const sro_physics *ph, double sigma, double y_e, double x_n);
 /* test switch: the hydro's HOT_FACE bracket on (default) or off */
 void sro_hydro_set_hot_face(int on);
+void sro_hydro_set_pair_work(int on);
 
 /* ---------------- FFT and convolution (reference crates/sr-physics/src/reference/fft.rs) ---------------- */
 int sro_padded_size(size_t len, size_t max_padded, size_t *out);

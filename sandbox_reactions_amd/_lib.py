@@ -51,7 +51,8 @@ class HeatSettings(C.Structure):
 
 class DtReport(C.Structure):
     _fields_ = [("dt", C.c_double), ("dt_next", C.c_double), ("sim_time", C.c_double), ("steps", C.c_uint64),
-                ("non_finite", C.c_int), ("nf_x", C.c_int), ("nf_y", C.c_int), ("nf_channel", C.c_int)]
+                ("non_finite", C.c_int), ("nf_x", C.c_int), ("nf_y", C.c_int), ("nf_channel", C.c_int),
+                ("nf_step", C.c_uint64)]
 
 
 class HeatReport(C.Structure):

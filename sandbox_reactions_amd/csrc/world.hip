@@ -23,7 +23,7 @@ namespace sr {
 #define NT 256
 #define NS 10
 
-enum { SCAL_DT = 0, SCAL_DT_NEXT, SCAL_SIM_TIME, SCAL_CAP, SCAL_RATE, SCAL_GMAX, SCAL_NF, SCAL_N = 16 };
+enum { SCAL_DT = 0, SCAL_DT_NEXT, SCAL_SIM_TIME, SCAL_CAP, SCAL_RATE, SCAL_GMAX, SCAL_NF, SCAL_NF_STEP, SCAL_N = 16 };
 
 __constant__ float w_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f / 20.0f,
                                 13.0f / 24.0f, 15.0f / 28.0f, 17.0f / 32.0f, 27.0f / 56.0f, 1.0f};
@@ -160,7 +160,12 @@ __global__ void __launch_bounds__(NT) k_floors_dt(FloorParams P, HydroState st, 
 }
 
 // P9's dt_{n+1} from the maxima: min(C dx / rate, eta_g sqrt(dx / |g|max)) (hydro.rs:618-634)
-__global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, int nb, double *scal) {
+// The device scalars at an upload: zero, and no guard finding yet
+__global__ void k_scal_reset(double *scal) {
+    if (threadIdx.x < SCAL_N) scal[threadIdx.x] = threadIdx.x == SCAL_NF ? -HUGE_VAL : 0.0;
+}
+
+__global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, int nb, double *scal, double step) {
     __shared__ double sh[3][4];
     double v[3] = {0.0, 0.0, -HUGE_VAL};
     for (int i = threadIdx.x; i < nb; i += blockDim.x)
@@ -175,7 +180,11 @@ __global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, in
         for (int k = 0; k < 3; k++) o[k] = fmax(fmax(sh[k][0], sh[k][1]), fmax(sh[k][2], sh[k][3]));
         scal[SCAL_RATE] = o[0];
         scal[SCAL_GMAX] = o[1];
-        scal[SCAL_NF] = o[2];
+        // the guard keeps its first finding and the index of the step whose P9 made it (step/dt.rs:40-47, NonFinite)
+        if (o[2] > -HUGE_VAL && !(scal[SCAL_NF] > -HUGE_VAL)) {
+            scal[SCAL_NF] = o[2];
+            scal[SCAL_NF_STEP] = step;
+        }
         double dt = P.cfl * P.dx / o[0];
         if (o[1] > 0.0) dt = fmin(dt, P.eta_g * sqrt(P.dx / o[1]));
         scal[SCAL_DT_NEXT] = dt;
@@ -452,7 +461,8 @@ static int p89(sr_world *wd, int apply) {
     const int nb = grid_n(wd->n);
     hipLaunchKernelGGL(k_floors_dt, dim3(nb), dim3(NT), 0, wd->s, wd->fp, st, wd->sw.gravity ? wd->gx : nullptr,
                        wd->sw.gravity ? wd->gy : nullptr, apply, wd->rpart);
-    hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(256), 0, wd->s, wd->fp, wd->rpart, nb, wd->scal);
+    hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(256), 0, wd->s, wd->fp, wd->rpart, nb, wd->scal,
+                       (double)wd->steps);
     SR_HIP(hipGetLastError());
     return SR_OK;
 }
@@ -466,7 +476,7 @@ extern "C" int sr_upload(sr_world *wd, const float *planes, size_t n) {
     wd->cur = 0;
     hipLaunchKernelGGL(k_upload, dim3((wd->n + 255) / 256), dim3(256), 0, wd->s, tmp, wd->st[0], (int)wd->n);
     SR_HIP(hipFreeAsync(tmp, wd->s));
-    SR_HIP(hipMemsetAsync(wd->scal, 0, sizeof(double) * SCAL_N, wd->s));
+    hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, wd->s, wd->scal);
     SR_HIP(hipMemsetAsync(wd->escaped, 0, sizeof(double) * 14 * 2 * (wd->w + wd->h), wd->s));
     wd->have_flux = false;
     wd->steps = 0;
@@ -648,6 +658,7 @@ extern "C" int sr_read_dt(sr_world *wd, sr_dt_report *out) {
         out->nf_x = (int)(cell % wd->w);
         out->nf_y = (int)(cell / wd->w);
         out->nf_channel = (int)(key % 16);
+        out->nf_step = (uint64_t)h[SCAL_NF_STEP];
     }
     return SR_OK;
 }

@@ -7,14 +7,18 @@ for the passes this path carries: gravity, hydro, heat).
 stdout: the first line `SR-ADAPTER name=... backend=hip driver=...`; the last, on success, `SR-HEADLESS DONE
 steps=<n> sim_time=<t> until=none`; errors start `SR-ERROR`. Exit codes as the reference's: 0 done, 3 no device or a
 device error, 4 bad arguments or scene (also a scene this path does not carry), 5 a non-finite value in the state
-(the message names the step, cell and channel).
+(the reference's line, naming the step, cell and channel, headless.rs:620-625). The guard scans the state at every
+step's P9 here (fused into the floors pass, so it costs nothing), where the reference's scans steps 0, 64, 128, ... and
+the state after a run's last step: the step named is the first at whose end the value was in the state, which the
+reference names only when that step is one it scans.
 
 With --out, DIR/summary.json holds the reference's run-summary fields this path has -- version, tool, adapter, scene,
 world {width, height, edge}, steps, sim_time, wall_s and, with --timing, `timing` {per_pass_ms by the reference's pass
 labels, step_ms_mean, source "gpu-timestamp"} (headless.rs:497-531, :1356-1381) -- and DIR/dumps/s_<step>.bin the
 SRDUMP01 dumps (every K steps with --dump-every, and the last). The floors and the step's reductions run fused here,
-so their time is reported under "P8 floors". --load-dump continues from a dump's cells and step index; the clock and
-the next dt restart from the cells (P9 on the loaded state), and no F plane is read (this library writes none).
+so their time is reported under "P8 floors". --load-dump continues from a dump's cells, step index and sim time, and
+--steps N is N more steps (the DONE line counts them, the dumps' indices count from the scene); the next dt restarts
+from the cells (P9 on the loaded state), and no F plane is read (this library writes none).
 """
 from __future__ import annotations
 
@@ -25,6 +29,9 @@ import sys
 import time
 
 PASS_LABELS = {1: "P1 dt", 2: "P2 gravity", 4: "P4 hydro", 5: "P5 heat", 8: "P8 floors"}
+# the state's channel names, as the guard's line names them (crates/sr-engine/src/state.rs:29-31)
+STATE_CHANNELS = ["sigma", "mom_x", "mom_y", "energy", "x_H", "x_He", "x_C", "x_O", "x_Ne", "x_Mg", "x_Si", "x_S",
+                  "x_Fe", "x_n"]
 
 
 def run(argv=None) -> int:
@@ -57,14 +64,14 @@ def run(argv=None) -> int:
     print(f"SR-ADAPTER name={name} backend=hip driver=libsrhip cus={cus}", flush=True)
     try:
         world = scene_json.world(scene, gamma=a.gamma)
-        step0 = 0
+        step0, clock0 = 0, 0.0
         if a.load_dump:
             d = dump.read(a.load_dump)
             if (d["width"], d["height"]) != (scene.width, scene.height):
                 print(f"SR-ERROR the dump's world {d['width']} x {d['height']} is not the scene's", flush=True)
                 return 4
             world.upload(d["planes"])
-            step0 = int(d["step"])
+            step0, clock0 = int(d["step"]), float(d["sim_time"])
         if a.timing:
             world.set_timing(True)
         dumps = os.path.join(a.out, "dumps") if a.out else None
@@ -78,12 +85,12 @@ def run(argv=None) -> int:
             done += k
             r = world.read_dt()
             if r.non_finite:
-                print(f"SR-ERROR a non-finite value at step {step0 + done}: cell ({r.nf_x}, {r.nf_y}) channel "
-                      f"{r.nf_channel}", flush=True)
+                print(f"SR-ERROR non-finite value in the state at step {step0 + r.nf_step}, cell ({r.nf_x}, "
+                      f"{r.nf_y}), channel {STATE_CHANNELS[r.nf_channel]}", flush=True)
                 return 5
             if dumps and (a.dump_every > 0 or done == a.steps):
                 dump.write(os.path.join(dumps, f"s_{step0 + done}.bin"), world.readback(), step=step0 + done,
-                           sim_time=r.sim_time, dt=r.dt, dt_next=r.dt_next)
+                           sim_time=clock0 + r.sim_time, dt=r.dt, dt_next=r.dt_next)
         wall = time.perf_counter() - t0
         r = world.read_dt()
     except sr.SrError as e:
@@ -94,14 +101,15 @@ def run(argv=None) -> int:
                                                                             "driver": "libsrhip"},
                    "scene": a.scene, "world": {"width": scene.width, "height": scene.height,
                                                "edge": "leave" if scene.edge == 1 else "bounce"},
-                   "steps": step0 + done, "sim_time": r.sim_time, "wall_s": wall}
+                   "steps": step0 + done, "sim_time": clock0 + r.sim_time, "wall_s": wall}
         if a.timing:
             c = world.read_cost()
             summary["timing"] = {"per_pass_ms": {PASS_LABELS[k]: c.pass_ms[k] for k in sorted(PASS_LABELS)},
                                  "step_ms_mean": c.step_ms, "source": "gpu-timestamp"}
         with open(os.path.join(a.out, "summary.json"), "w") as f:
             f.write(json.dumps(summary, indent=2) + "\n")
-    print(f"SR-HEADLESS DONE steps={step0 + done} sim_time={r.sim_time} until=none", flush=True)
+    # the steps of this run and the clock to 6 decimals (headless.rs:1396)
+    print(f"SR-HEADLESS DONE steps={done} sim_time={clock0 + r.sim_time:.6f} until=none", flush=True)
     return 0
 
 

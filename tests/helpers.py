@@ -28,3 +28,26 @@ def rel_l1(a, b):
 def f32(planes):
     """Planes as the device holds them (f32 everywhere; Sigma starts from the same f32 value)."""
     return np.asarray(planes, dtype=np.float32).astype(np.float64)
+
+
+def mass_radius(sigma, vac=1e-6, fraction=0.9):
+    """The radius about the centre of mass holding `fraction` of the mass, each cell of Sigma >= vac split into 4 x 4
+    points of its area, the crossing interpolated between sorted radii (the reference's tracker,
+    crates/sr-engine/src/calibrate.rs:333-368, and crates/sr-engine/tests/gpu/collapse.rs:81-113's r50, restated);
+    and the mass."""
+    ys, xs = np.nonzero(sigma >= vac)
+    m = sigma[ys, xs].astype(np.float64)
+    mass = m.sum()
+    cx, cy = ((xs + 0.5) * m).sum() / mass, ((ys + 0.5) * m).sum() / mass
+    off = (np.arange(4) + 0.5) / 4 - 0.5
+    a, b = np.meshgrid(off, off, indexing="ij")
+    px = (xs[:, None] + 0.5 + a.ravel()[None, :]).ravel()
+    py = (ys[:, None] + 0.5 + b.ravel()[None, :]).ravel()
+    pm = np.repeat(m / 16.0, 16)
+    r = np.hypot(px - cx, py - cy)
+    o = np.argsort(r, kind="stable")
+    r, pm = r[o], pm[o]
+    c = np.cumsum(pm)
+    k = int(np.searchsorted(c, fraction * mass))
+    below, rp = (c[k - 1], r[k - 1]) if k > 0 else (0.0, 0.0)
+    return rp + (r[k] - rp) * (fraction * mass - below) / pm[k], mass

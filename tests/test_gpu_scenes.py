@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import both, rel_l1
+from helpers import both, mass_radius as r90, rel_l1
 from sandbox_reactions_amd import scene_json
 
 pytestmark = pytest.mark.gpu
@@ -92,27 +92,6 @@ def test_cold_pressure_scenes(gpu_lib, case):
     assert rel_l1(g[0], t[0]) <= 1e-5, (case, rel_l1(g[0], t[0]))
     assert rel_l1(g[3], t[3]) <= 1e-4, (case, rel_l1(g[3], t[3]))
     assert world.read_heat().converged
-
-
-def r90(sigma, vac=1e-6, fraction=0.9):
-    """The radius about the centre of mass holding `fraction` of the mass, each cell of Sigma >= vac split into 4 x 4
-    points (the reference's tracker, crates/sr-engine/src/calibrate.rs:333-368, restated); and the mass."""
-    ys, xs = np.nonzero(sigma >= vac)
-    m = sigma[ys, xs].astype(np.float64)
-    mass = m.sum()
-    cx, cy = ((xs + 0.5) * m).sum() / mass, ((ys + 0.5) * m).sum() / mass
-    off = (np.arange(4) + 0.5) / 4 - 0.5
-    a, b = np.meshgrid(off, off, indexing="ij")
-    px = (xs[:, None] + 0.5 + a.ravel()[None, :]).ravel()
-    py = (ys[:, None] + 0.5 + b.ravel()[None, :]).ravel()
-    pm = np.repeat(m / 16.0, 16)
-    r = np.hypot(px - cx, py - cy)
-    o = np.argsort(r, kind="stable")
-    r, pm = r[o], pm[o]
-    c = np.cumsum(pm)
-    k = int(np.searchsorted(c, fraction * mass))
-    below, rp = (c[k - 1], r[k - 1]) if k > 0 else (0.0, 0.0)
-    return rp + (r[k] - rp) * (fraction * mass - below) / pm[k], mass
 
 
 def test_the_grid_holds_a_resolved_cold_disk(gpu_lib):
