@@ -13,9 +13,9 @@
  *   commit_limited               heat.rs:544-592
  * With cell size dx a face's coefficient is k = D / dx^2 and its flux F = -D (u_b - u_a) / dx; R uses the gradient
  * |u_b - u_a| / dx. Heat capacity C = Sigma / (mu (gamma - 1)) (the reference's Sigma / mu at gamma = 2).
- * Parity unpinned: Heat::step's Newton pass counts and delta on radiating scenes are restated only (the constant-
- * diffusivity moments, the solve against a direct solve and commit_limited on the reference's ceiling dump are
- * checked by the tests).
+ * Parity unpinned: the exact Newton pass counts and delta of Heat::step are restated only; its converged result is
+ * checked by the tests (the nonlinear balance cell by cell on the reference's radiating slab, the constant-diffusivity
+ * moments, the solve against a direct solve, commit_limited on the reference's ceiling dump).
  */
 #include <math.h>
 #include <stdlib.h>

@@ -14,9 +14,10 @@
  * composition (crates/sr-physics/src/registry.rs:146-154).
  * Held cells, sinks and the pull field are not restated (the path refuses a scene that starts at or above
  * Sigma_hold, floors.rs:200-260).
- * Parity unpinned: push_share and the push/kick take-backs (hydro.rs:392-410, :986-1001) are restated only; no closed
- * form or output of the reference checks them. The other functions are checked by the tests (closed forms, the
- * reference's own cases: the thin cell for HOT_FACE, the Jeans adiabat for face_work).
+ * Parity unpinned: gravity's kick take-back to the densest free cell (hydro.rs:997-1001) is restated only; no closed
+ * form or output of the reference checks it. The other functions are checked by the tests (closed forms, the
+ * reference's own cases: the thin cell for HOT_FACE, the Jeans adiabat for face_work, the radiation-force slab for
+ * push_share and the push's take-back).
  */
 #include <math.h>
 #include <stdlib.h>
