@@ -125,14 +125,26 @@ __device__ inline double block_sum(double v, double *scratch) {
     return s;
 }
 
+// The tile a block of a 2D tiled launch takes. The dispatcher deals consecutive block ids to the 8 XCDs in turn, so
+// the ids are remapped to give each XCD one contiguous run of tiles in row order: neighbouring tiles, whose regions
+// overlap by their halos, then read the shared lines through the same XCD's L2 instead of each from HBM. A bijection on
+// [0, nb) for any nb (an XCD holding one more block when nb is not a multiple of 8).
+__device__ __forceinline__ int xcd_tile() {
+    const int nb = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+}
+
 // ------------------------------------------------------------------------------------------- deterministic reduction
-// Block sums (OP = 0) or maxima (OP = 1) of K values into part[k * nb + block]. No fence and no counter: the partials
+// Block sums (OP = 0) or maxima (OP = 1) of K values into part[k * nb + block] (block: the linear block id, or `slot`
+// when a kernel orders its partials itself). No fence and no counter: the partials
 // are read by a later kernel or by the host, which sum them in block order (so the result is deterministic). A
 // device-wide fence here would write back the whole L2 of an XCD in every block.
-template <int K, int OP = 0> __device__ inline void block_partials(double (&v)[K], double *part, int nb) {
+template <int K, int OP = 0>
+__device__ inline void block_partials(double (&v)[K], double *part, int nb, int slot = -1) {
     __shared__ double sh[K][16];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const int bid = slot >= 0 ? slot : blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
     for (int k = 0; k < K; k++)
 #pragma unroll

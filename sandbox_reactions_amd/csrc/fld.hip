@@ -104,7 +104,8 @@ __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__
                                                  int restart, int rows, double *part) {
     const float beta = beta_of(dots_cur, dots_prev, nbd, nbp, restart);
     const int w = a.w, h = a.h;
-    const int x = blockIdx.x * NT + threadIdx.x, y0 = blockIdx.y * rows, lane = threadIdx.x & 63;
+    const int tile = xcd_tile();
+    const int x = (tile % gridDim.x) * NT + threadIdx.x, y0 = (tile / gridDim.x) * rows, lane = threadIdx.x & 63;
     const bool in = x < w;
     const int xc = in ? x : w - 1;
     auto P = [&](size_t j) { return z[j] + beta * pold[j]; };
@@ -137,7 +138,7 @@ __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__
             au = ad;
         }
     }
-    block_partials<1>(v, part, gridDim.x * gridDim.y);
+    block_partials<1>(v, part, gridDim.x * gridDim.y, tile);  // partials in strip order
 }
 
 // x += alpha p (FP64), r -= alpha q, alpha = r.z / p.q from the partials; partials of ||r||^2
@@ -338,7 +339,8 @@ __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, f
     using RG = Region<TX, TY, HW>;
     __shared__ RG R;
     __shared__ float own_in[TX * TY];
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int tile = xcd_tile();
+    const int x0 = (tile % gridDim.x) * TX, y0 = (tile / gridDim.x) * TY;
     load_region<TX, TY, HW>(R, L, x0, y0, b, own_in);
 #pragma unroll 4
     for (int t = threadIdx.x; t < RG::N; t += NT) R.z[t] = 0.0f;
@@ -382,7 +384,8 @@ __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, 
     constexpr int HW = 2 * NU;
     using RG = Region<TX, TY, HW>;
     __shared__ RG R;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int tile = xcd_tile();
+    const int x0 = (tile % gridDim.x) * TX, y0 = (tile / gridDim.x) * TY;
     load_region<TX, TY, HW>(R, L, x0, y0, b);
     {
         constexpr int IT = (RG::N + NT - 1) / NT;
@@ -418,7 +421,7 @@ __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, 
             }
         }
     }
-    if (part) block_partials<2>(v, part, gridDim.x * gridDim.y);
+    if (part) block_partials<2>(v, part, gridDim.x * gridDim.y, tile);  // partials in tile order
 }
 
 // The V-cycle over every level of at most COARSE_MAX cells in one block. The levels' own, ax and ay arrays, the

@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                                                  HydroPartials part) {
     __shared__ Lds<COLD> L;
     const int w = P.w, h = P.h;
-    const int bx = blockIdx.x * T, by = blockIdx.y * T;
+    const int tile = xcd_tile();
+    const int bx = (tile % gridDim.x) * T, by = (tile / gridDim.x) * T;
     const int tid = threadIdx.x;
     const double dt = scal[0];
     const float dtf = (float)dt, half = 0.5f * dtf, hdx = half * P.inv_dx;
@@ -654,7 +655,7 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
                 for (int k = 0; k < 5; k++) acc[k] += L.red[wv][k];
                 best = L.redk[wv] > best ? L.redk[wv] : best;
             }
-            const int b = blockIdx.y * gridDim.x + blockIdx.x;
+            const int b = tile;  // partials in tile order
             for (int k = 0; k < 5; k++) part.sums[(size_t)b * 5 + k] = acc[k];
             part.keys[b] = best;
         }

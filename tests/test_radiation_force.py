@@ -96,26 +96,38 @@ def test_the_force_is_the_capped_push_less_the_net(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("temps", [(1.0, 10.0), (0.06, 1000.0)])
-def test_the_gpu_force_from_its_heat_pass_matches_the_twin(gpu_lib, temps):
-    """The slab with hydro and heat on, temperatures pinned (radforce.rs:7-8), 5 steps on the twin's dt: momenta within
-    1e-5 (rel-L1) of the twin's, Sigma 1e-6, and E as the twin's (the force moves no energy)."""
+@pytest.mark.parametrize("temps, bar", [((1.0, 10.0), 2e-3), ((0.06, 1000.0), 1e-5)])
+def test_the_gpu_force_from_its_heat_pass_matches_the_twin(gpu_lib, temps, bar):
+    """The slab with hydro and heat on, temperatures pinned (radforce.rs:7-8: P5 computes F and leaves E alone), 5
+    steps, on the GPU and on the twin, each beside a force-free run (heat off) on the same dt sequence: the momentum
+    the force added (forced - free, radforce.rs:14-18's measure) against the twin's; Sigma within 1e-6 and E within
+    1e-5 of the twin's. The GPU's F itself matches the twin's to 1e-8; what bounds the comparison is the f32 flow:
+    at the contact both sides hold Pi = 2 only to an ulp, and HLL turns that into momentum of order dt ulp(Pi) per face
+    per step (about 1e-5 over the 128 contact faces and 5 steps), which the forced and the free runs, their states
+    apart, do not share. Against the weak force of (1, 10) (a gain of 0.016) that is 7e-4, hence 2e-3 there; the strong
+    force of (0.06, 1000) (a gain of 5,200, capped at the slab's edges) is held to 1e-5."""
     import sandbox_reactions_amd as sr
     print(f"radiation force slab {temps}", flush=True)
     op, gp = both(edge=0, pin_temperature=1)
     st = slab_state(*temps)
-    world = sr.World(N, N, physics=gp, gravity=False, hydro=True, heat=True)
-    world.upload(st.astype(np.float32))
-    twin = oracle.Twin(op, st, gravity=False, hydro=True, heat=True)
+    runs = []
+    for heat in (True, False):
+        w = sr.World(N, N, physics=gp, gravity=False, hydro=True, heat=heat)
+        w.upload(st.astype(np.float32))
+        runs.append((w, oracle.Twin(op, st, gravity=False, hydro=True, heat=heat)))
     for _ in range(5):
-        dt = world.read_dt().dt_next
-        twin.cap_dt_next(dt)
-        twin.step()
-        world.set_dt_cap(dt)
-        world.run(1)
-    g, t = world.readback().astype(np.float64), twin.state()
-    assert rel_l1(g[0], t[0]) <= 1e-6
-    mom_g, mom_t = np.hypot(g[1], g[2]), np.hypot(t[1], t[2])
-    assert np.abs(mom_t).sum() > 0
-    assert np.abs(g[1:3] - t[1:3]).sum() <= 1e-5 * mom_t.sum(), np.abs(g[1:3] - t[1:3]).sum() / mom_t.sum()
-    assert rel_l1(g[3], t[3]) <= 1e-5
+        dt = min(w.read_dt().dt_next for w, _ in runs)  # the step both runs can take
+        for w, t in runs:
+            t.cap_dt_next(dt)
+            t.step()
+            w.set_dt_cap(dt)
+            w.run(1)
+            assert w.read_dt().dt == dt
+    (gf, tf), (g0, t0) = [(w.readback().astype(np.float64), t.state()) for w, t in runs]
+    gain_g, gain_t = gf[1:3] - g0[1:3], tf[1:3] - t0[1:3]
+    scale = np.abs(gain_t).sum()
+    err = np.abs(gain_g - gain_t).sum() / scale
+    print(f"force's momentum {scale:.6g}, GPU against the twin {err:.2e}", flush=True)
+    assert scale > 0 and err <= bar
+    assert rel_l1(gf[0], tf[0]) <= 1e-6
+    assert rel_l1(gf[3], tf[3]) <= 1e-5
