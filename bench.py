@@ -334,7 +334,7 @@ def main():
         # read around an empty kernel, the part of that mean a kernel trace does not count
         "roofline": dict(roof("mg_pre0", "sr::k_pre<32, 32, 2>", ((n + 31) // 32) ** 2 * 256),
                          device_copy_gbs=copy_gbs, event_bracket_ms=bracket,
-                         also=[roof("hydro_update", "sr::k_hydro<1>")]),
+                         also=[roof("hydro_update", "sr::k_hydro<1, false>", ((n + 15) // 16) ** 2 * 320)]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
