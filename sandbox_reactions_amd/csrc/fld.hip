@@ -48,29 +48,25 @@ __device__ __forceinline__ bool is_gas(const L0 &a, size_t i) { return a.gas == 
 
 #define UNROLL 4
 
-// x_out = x + dx (the inner solve's f32 correction folded in FP64; dx null: none), zeroed off the gas; r = b - A x_out
-// in FP64, r32 = (float) r; partials of ||r||^2, ||b||^2 and the reference's two measures. x_out is another buffer
-// than x: every thread reads its neighbours' x + dx, which no thread writes.
-__global__ void __launch_bounds__(NT) k_true_residual(L0 a, const double *__restrict__ b, const double *__restrict__ x,
-                                                      const float *__restrict__ dx, double *__restrict__ x_out,
+// r = b - A x in FP64 (x zeroed off the gas), r32 = (float) r; partials of ||r||^2, ||b||^2 and the reference's two
+// measures
+__global__ void __launch_bounds__(NT) k_true_residual(L0 a, const double *__restrict__ b, double *__restrict__ x,
                                                       float *__restrict__ r32, const float *__restrict__ heat,
                                                       const float *__restrict__ slope, double *part) {
     const int n = a.w * a.h;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    auto X = [&](int j) { return dx ? x[j] + (double)dx[j] : x[j]; };
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
         const int xx = i % a.w, yy = i / a.w;
         double r = 0.0;
         if (is_gas(a, i)) {
-            const double xi = X(i);
-            x_out[i] = xi;
+            const double xi = x[i];
             const double al = a.ax[(size_t)yy * (a.w + 1) + xx], ar = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
             const double au = a.ay[(size_t)yy * a.w + xx], ad = a.ay[(size_t)(yy + 1) * a.w + xx];
             double ax = (double)a.own[i] * xi;
-            if (al != 0.0) ax += al * (xi - X(i - 1));
-            if (ar != 0.0) ax += ar * (xi - X(i + 1));
-            if (au != 0.0) ax += au * (xi - X(i - a.w));
-            if (ad != 0.0) ax += ad * (xi - X(i + a.w));
+            if (al != 0.0) ax += al * (xi - x[i - 1]);
+            if (ar != 0.0) ax += ar * (xi - x[i + 1]);
+            if (au != 0.0) ax += au * (xi - x[i - a.w]);
+            if (ad != 0.0) ax += ad * (xi - x[i + a.w]);
             r = b[i] - ax;
             v[1] += b[i] * b[i];
             if (heat) {
@@ -80,7 +76,7 @@ __global__ void __launch_bounds__(NT) k_true_residual(L0 a, const double *__rest
                 v[3] += (double)heat[i] * e * e;
             }
         } else {
-            x_out[i] = 0.0;
+            x[i] = 0.0;
         }
         r32[i] = (float)r;
         v[0] += r * r;
@@ -145,20 +141,17 @@ __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__
     block_partials<1>(v, part, gridDim.x * gridDim.y, tile);  // partials in strip order
 }
 
-// dx += alpha p (dx = alpha p on an inner solve's first iteration), r -= alpha q, alpha = r.z / p.q from the
-// partials; partials of ||r||^2. The correction stays f32 within one inner solve (its residual falls by 1e-4 at
-// most before the FP64 true residual folds it into the iterate), so its rounding is far below what the outer
-// refinement corrects.
+// x += alpha p (FP64), r -= alpha q, alpha = r.z / p.q from the partials; partials of ||r||^2
 __global__ void __launch_bounds__(NT) k_update(int n, const float *__restrict__ p, const float *__restrict__ q,
-                                               float *__restrict__ dx, int first, float *__restrict__ r,
-                                               const double *dots_cur, int nbd, const double *pq_part, int nbq,
-                                               double *part) {
+                                               double *__restrict__ x, float *__restrict__ r, const double *dots_cur,
+                                               int nbd, const double *pq_part, int nbq, double *part) {
     const double rz = sum_partials(dots_cur, nbd), pq = sum_partials(pq_part, nbq);
     const double alpha = pq != 0.0 ? rz / pq : 0.0;
     const float af = (float)alpha;
     double v[1] = {0.0};
     for (int base = blockIdx.x * NT * UNROLL + threadIdx.x; base < n; base += gridDim.x * NT * UNROLL) {
-        float pv[UNROLL], qv[UNROLL], rv[UNROLL], xv[UNROLL];
+        float pv[UNROLL], qv[UNROLL], rv[UNROLL];
+        double xv[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
             const int i = base + u * NT;
@@ -166,14 +159,14 @@ __global__ void __launch_bounds__(NT) k_update(int n, const float *__restrict__ 
                 pv[u] = p[i];
                 qv[u] = q[i];
                 rv[u] = r[i];
-                xv[u] = first ? 0.0f : dx[i];
+                xv[u] = x[i];
             }
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
             const int i = base + u * NT;
             if (i < n) {
-                dx[i] = xv[u] + af * pv[u];
+                x[i] = xv[u] + alpha * (double)pv[u];
                 const float ri = rv[u] - af * qv[u];
                 r[i] = ri;
                 v[0] += (double)ri * ri;
@@ -739,7 +732,7 @@ FldSolver::~FldSolver() {
         (void)hipFree(lv[l].zp);
         if (lv[l].cf) (void)hipFree(lv[l].cf);
     }
-    void *ps[] = {gas, r32, za, zb, p, q, dx, xalt, pack, part_true, part_pq, part_rr, part_dots};
+    void *ps[] = {gas, r32, za, zb, p, q, pack, part_true, part_pq, part_rr, part_dots};
     for (void *x : ps)
         if (x) (void)hipFree(x);
     if (host_part) (void)hipHostFree(host_part);
@@ -835,8 +828,6 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
     SR_HIP(hipMalloc(&zb, sizeof(float) * n));
     SR_HIP(hipMalloc(&p, sizeof(float) * 2 * n));
     SR_HIP(hipMalloc(&q, sizeof(float) * n));
-    SR_HIP(hipMalloc(&dx, sizeof(float) * n));
-    SR_HIP(hipMalloc(&xalt, sizeof(double) * n));
     SR_HIP(hipMemset(za, 0, sizeof(float) * n));
     SR_HIP(hipMemset(zb, 0, sizeof(float) * n));
     SR_HIP(hipMemset(p, 0, sizeof(float) * 2 * n));
@@ -1226,16 +1217,9 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + 2047) / 2048));
     const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
     const int nmv = (int)(mv_grid.x * mv_grid.y);
-    // the FP64 iterate lives in xc, the other of x and xalt takes the next fold; `pending`: dx holds an inner
-    // solve's correction not yet folded in
-    double *xc = x, *xn = xalt;
-    bool pending = false;
     for (int outer = 0; outer < 50; outer++) {
-        hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, (const double *)xc,
-                           pending ? (const float *)dx : nullptr, xn, r32, ref ? ref->heat : nullptr,
+        hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, x, r32, ref ? ref->heat : nullptr,
                            ref ? ref->slope : nullptr, part_true);
-        std::swap(xc, xn);
-        pending = false;
         double tr[4];
         SR_TRY(read_partials(part_true, 4, nbe, tr, s));
         const double rr = tr[0], bb = tr[1] > 0 ? tr[1] : 1.0;
@@ -1268,9 +1252,8 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         while (it < max_iter) {
             hipLaunchKernelGGL(k_matvec_p, mv_grid, dim3(NT), 0, s, a, zc, pold, pnew, q, dots[k & 1],
                                dots[(k + 1) & 1], nbd, nbp, k == 0 ? 1 : 0, mv_rows, part_pq);
-            hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, dx, k == 0 ? 1 : 0, r32,
-                               dots[k & 1], nbd, part_pq, nmv, part_rr);
-            pending = true;
+            hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, x, r32, dots[k & 1], nbd,
+                               part_pq, nmv, part_rr);
             it++;
             if (use_net) net_it++;
             if (it - last_check >= interval || it >= max_iter) {
@@ -1310,12 +1293,6 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         }
         if (!st->finite) break;
     }
-    if (pending) {  // the last inner solve's correction (the outer loop's bound ended it unfolded)
-        hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, (const double *)xc, (const float *)dx,
-                           xn, r32, ref ? ref->heat : nullptr, ref ? ref->slope : nullptr, part_true);
-        std::swap(xc, xn);
-    }
-    if (xc != x) SR_HIP(hipMemcpyAsync(x, xc, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     st->iterations = it;
     st->learned_iterations = net_it;
     if (precond == SR_PRECOND_MULTIGRID && st->converged && it > 0 && rel0 > 0.0 && st->rel > 0.0)
@@ -1325,12 +1302,12 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
 }
 
 // HBM bytes one inner iteration moves at level 0 by its algorithm (f32 unless noted), the roofline's numerator:
-// matvec (z, p in, p out, q out, own, ax, ay: 28), update (dx rw, p, q, r rw: 24), the V-cycle's level-0
+// matvec (z, p in, p out, q out, own, ax, ay: 28), update (x f64 rw 16, p, q, r rw 8: 32), the V-cycle's level-0
 // kernels (pre: b, own, ax, ay in, z out, coarse b out 1: 21; post: b, own, ax, ay, z in, coarse z 1, z out, z_old:
 // 33), the coarser levels adding a third of level 0's smoothing.
 double FldSolver::bytes_per_iteration() const {
     const double n = (double)w0 * h0;
-    return n * (28.0 + 24.0 + (21.0 + 33.0) * 4.0 / 3.0);
+    return n * (28.0 + 32.0 + (21.0 + 33.0) * 4.0 / 3.0);
 }
 
 } // namespace sr

@@ -93,10 +93,6 @@ class FldSolver {
     uint8_t *gas = nullptr;
     // level-0 vectors
     float *r32 = nullptr, *za = nullptr, *zb = nullptr, *p = nullptr, *q = nullptr;
-    // the inner solve's correction (f32), folded into the FP64 iterate by the next true residual, which writes
-    // the folded iterate into the other of the caller's x and xalt
-    float *dx = nullptr;
-    double *xalt = nullptr;
     float *pack = nullptr; // the single-block levels' own/ax/ay back to back, the coarsest's inverse, 1/diag
     int pack_floats = 0;
     // per-block partial sums (summed by the kernel that consumes them, or on the host)
