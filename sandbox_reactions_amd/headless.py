@@ -1,19 +1,22 @@
 """A headless run of a scene on this library (the reference's `headless` driver, crates/sr-engine/src/headless.rs,
 for the passes this path carries: gravity, hydro, heat).
 
-    python -m sandbox_reactions_amd.headless --scene FILE --steps N [--out DIR] [--timing] [--dump-every K]
-                                             [--load-dump FILE] [--gamma G]
+    python -m sandbox_reactions_amd.headless --scene FILE (--steps N | --until sim_time:T) [--max-steps M]
+                                             [--out DIR] [--timing] [--dump-every K] [--load-dump FILE] [--gamma G]
 
-stdout: the first line `SR-ADAPTER name=... backend=hip driver=...`; the last, on success, `SR-HEADLESS DONE
-steps=<n> sim_time=<t> until=none`; errors start `SR-ERROR`. Exit codes as the reference's: 0 done, 3 no device or a
-device error, 4 bad arguments or scene (also a scene this path does not carry), 5 a non-finite value in the state
+stdout: the first line `SR-ADAPTER name=... backend=hip driver=...`; the last, when the run ends without an error,
+`SR-HEADLESS DONE steps=<n> sim_time=<t> until=<none|met|unmet>`; errors start `SR-ERROR`. `--until sim_time:T` stops
+at the first step whose clock reaches T (its last steps taken one at a time); the observers' conditions (event:,
+stage:, ending) are not carried. Exit codes as the reference's: 0 done, 2 `--steps` or `--until` not reached within
+`--max-steps` (default 2,000,000), 3 no device or a device error, 4 bad arguments or scene (also a scene or condition
+this path does not carry), 5 a non-finite value in the state
 (the reference's line, naming the step, cell and channel, headless.rs:620-625). The guard scans the state at every
 step's P9 here (fused into the floors pass, so it costs nothing), where the reference's scans steps 0, 64, 128, ... and
 the state after a run's last step: the step named is the first at whose end the value was in the state, which the
 reference names only when that step is one it scans.
 
 With --out, DIR/summary.json holds the reference's run-summary fields this path has -- version, tool, adapter, scene,
-world {width, height, edge}, steps, sim_time, wall_s and, with --timing, `timing` {per_pass_ms by the reference's pass
+world {width, height, edge}, steps, sim_time, wall_s, with --until `until` {condition, met}, and with --timing, `timing` {per_pass_ms by the reference's pass
 labels, step_ms_mean, source "gpu-timestamp"} (headless.rs:497-531, :1356-1381) -- and DIR/dumps/s_<step>.bin the
 SRDUMP01 dumps (every K steps with --dump-every, and the last). The floors and the step's reductions run fused here,
 so their time is reported under "P8 floors". --load-dump continues from a dump's cells, step index and sim time, and
@@ -29,28 +32,61 @@ import sys
 import time
 
 PASS_LABELS = {1: "P1 dt", 2: "P2 gravity", 4: "P4 hydro", 5: "P5 heat", 8: "P8 floors"}
+MAX_STEPS = 2_000_000  # --max-steps's default (headless.rs:77)
 # the state's channel names, as the guard's line names them (crates/sr-engine/src/state.rs:29-31)
 STATE_CHANNELS = ["sigma", "mom_x", "mom_y", "energy", "x_H", "x_He", "x_C", "x_O", "x_Ne", "x_Mg", "x_Si", "x_S",
                   "x_Fe", "x_n"]
 
 
+class _Args(argparse.ArgumentParser):
+    """Bad arguments are exit 4, as the reference's (exit 2 is a run that did not reach its target)."""
+
+    def error(self, message):
+        raise ValueError(f"bad arguments: {message}")
+
+
+def parse_until(text):
+    """`--until sim_time:<t>` (t finite, >= 0), the one condition this path reads (watch.rs:115-135); the
+    observers' conditions (event:, stage:, ending) are refused as not carried."""
+    kind, _, value = text.partition(":")
+    if kind == "sim_time" and value:
+        try:
+            t = float(value)
+        except ValueError:
+            t = float("nan")
+        if not (t >= 0.0 and t != float("inf")):
+            raise ValueError(f"--until {text!r}: sim_time takes a time of 0 or more, got {value!r}")
+        return t
+    if kind in ("event", "stage") or text == "ending":
+        raise ValueError(f"--until {text!r}: the observers' conditions are not carried by this path (sim_time:<t> is)")
+    raise ValueError(f"--until {text!r}: expected sim_time:<t>, event:<kind>, stage:<id> or ending")
+
+
 def run(argv=None) -> int:
-    ap = argparse.ArgumentParser(prog="sandbox_reactions_amd.headless")
+    ap = _Args(prog="sandbox_reactions_amd.headless")
     ap.add_argument("--scene", required=True)
-    ap.add_argument("--steps", type=int, required=True)
+    ap.add_argument("--steps", type=int)
+    ap.add_argument("--until")
+    ap.add_argument("--max-steps", type=int, default=MAX_STEPS)
     ap.add_argument("--out")
     ap.add_argument("--timing", action="store_true")
     ap.add_argument("--dump-every", type=int, default=0)
     ap.add_argument("--load-dump")
     ap.add_argument("--gamma", type=float, default=2.0)
-    a = ap.parse_args(argv)
     from . import dump, scene_json
     try:
+        a = ap.parse_args(argv)
+        # headless.rs:190-202: one of --steps and --until, at least one step, a positive bound
+        if (a.steps is None) == (a.until is None):
+            raise ValueError("give --steps N or --until <condition>, one of the two")
+        if a.steps is not None and a.steps < 1:
+            raise ValueError("--steps must be at least 1")
+        if a.max_steps < 1:
+            raise ValueError("--max-steps must be at least 1")
+        until = parse_until(a.until) if a.until is not None else None
         scene = scene_json.load(a.scene)
         scene.check_path()
         scene.planes(a.gamma)  # every cell laid down before a device is touched (a bad scene is exit 4)
-        if a.steps < 0:
-            raise scene_json.SceneError("--steps must be >= 0")
     except (OSError, ValueError) as e:
         print(f"SR-ERROR {e}", flush=True)
         return 4
@@ -78,17 +114,24 @@ def run(argv=None) -> int:
         if dumps:
             os.makedirs(dumps, exist_ok=True)
         t0 = time.perf_counter()
-        done = 0
-        while done < a.steps:
-            k = a.steps - done if a.dump_every <= 0 else min(a.dump_every, a.steps - done)
+        target = min(a.steps if a.steps is not None else a.max_steps, a.max_steps)
+        done, met = 0, False
+        r = world.read_dt()
+        while done < target and not met:
+            k = target - done if a.dump_every <= 0 else min(a.dump_every, target - done)
+            if until is not None and r.dt_next > 0.0:
+                # headless.rs:1206-1210: a chunk covers at most half the time left at the next step's dt, so the
+                # last steps before the condition are single ones and the run stops at the first step past it
+                k = min(k, max(1, int((until - (clock0 + r.sim_time)) / (2.0 * r.dt_next))))
             world.run(k)
             done += k
             r = world.read_dt()
+            met = until is not None and clock0 + r.sim_time >= until
             if r.non_finite:
                 print(f"SR-ERROR non-finite value in the state at step {step0 + r.nf_step}, cell ({r.nf_x}, "
                       f"{r.nf_y}), channel {STATE_CHANNELS[r.nf_channel]}", flush=True)
                 return 5
-            if dumps and (a.dump_every > 0 or done == a.steps):
+            if dumps and (a.dump_every > 0 or done == target or met):
                 dump.write(os.path.join(dumps, f"s_{step0 + done}.bin"), world.readback(), step=step0 + done,
                            sim_time=clock0 + r.sim_time, dt=r.dt, dt_next=r.dt_next)
         wall = time.perf_counter() - t0
@@ -102,15 +145,20 @@ def run(argv=None) -> int:
                    "scene": a.scene, "world": {"width": scene.width, "height": scene.height,
                                                "edge": "leave" if scene.edge == 1 else "bounce"},
                    "steps": step0 + done, "sim_time": clock0 + r.sim_time, "wall_s": wall}
+        if until is not None:
+            summary["until"] = {"condition": f"sim_time:{a.until.partition(':')[2]}", "met": met}
         if a.timing:
             c = world.read_cost()
             summary["timing"] = {"per_pass_ms": {PASS_LABELS[k]: c.pass_ms[k] for k in sorted(PASS_LABELS)},
                                  "step_ms_mean": c.step_ms, "source": "gpu-timestamp"}
         with open(os.path.join(a.out, "summary.json"), "w") as f:
             f.write(json.dumps(summary, indent=2) + "\n")
-    # the steps of this run and the clock to 6 decimals (headless.rs:1396)
-    print(f"SR-HEADLESS DONE steps={done} sim_time={clock0 + r.sim_time:.6f} until=none", flush=True)
-    return 0
+    # the steps of this run, the clock to 6 decimals and the condition's verdict: none for a run on --steps that
+    # took them all, met, or unmet when --max-steps cut the run (exit 2) (headless.rs:1389-1400)
+    reached = met if until is not None else done >= (a.steps or 0)
+    verdict = ("met" if until is not None else "none") if reached else "unmet"
+    print(f"SR-HEADLESS DONE steps={done} sim_time={clock0 + r.sim_time:.6f} until={verdict}", flush=True)
+    return 0 if reached else 2
 
 
 if __name__ == "__main__":

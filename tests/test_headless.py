@@ -36,6 +36,25 @@ def test_a_scene_the_path_does_not_carry_is_exit_4(tmp_path):
     assert code == 4 and "below the cold pressure" in out
 
 
+@pytest.mark.parametrize("args, text", [
+    (["--steps", "3", "--until", "sim_time:1"], "one of the two"),
+    ([], "one of the two"),
+    (["--steps", "0"], "at least 1"),
+    (["--until", "sim_time:-1"], "0 or more"),
+    (["--until", "sim_time:abc"], "0 or more"),
+    (["--until", "event:supernova"], "not carried"),
+    (["--until", "ending"], "not carried"),
+    (["--until", "sometime"], "expected sim_time"),
+    (["--steps", "3", "--max-steps", "0"], "at least 1"),
+    (["--steps", "x"], "bad arguments"),
+])
+def test_bad_arguments_are_exit_4(args, text):
+    """headless.rs:190-202 and watch.rs:115-135: one of --steps and --until, at least one step, sim_time of 0 or more;
+    the observers' conditions are refused here as not carried."""
+    code, out = headless("--scene", os.path.join(SCENES, "sod.json"), *args)
+    assert code == 4 and out.startswith("SR-ERROR") and text in out, out
+
+
 def test_no_device_is_exit_3():
     import torch
     if torch.cuda.is_available():
@@ -66,3 +85,22 @@ def test_a_run_writes_the_summary_and_dumps(gpu_lib, tmp_path):
     a = dump.read(str(tmp_path / "resumed" / "dumps" / "s_3.bin"))["planes"]
     b = dump.read(str(tmp_path / "dumps" / "s_3.bin"))["planes"]
     assert np.max(np.abs(a[0] - b[0])) <= 1e-6 * np.abs(b[0]).max()
+
+
+@pytest.mark.gpu
+def test_until_sim_time_and_max_steps(gpu_lib, tmp_path):
+    """--until sim_time:T stops at the first step whose clock reaches T (exit 0, until=met, the step before short of
+    T); --max-steps cutting a run on --steps is exit 2 with until=unmet."""
+    from sandbox_reactions_amd import dump
+    print("headless --until", flush=True)
+    code, out = headless("--scene", os.path.join(SCENES, "sod.json"), "--until", "sim_time:0.05", "--out",
+                         str(tmp_path))
+    lines = out.strip().split("\n")
+    assert code == 0 and lines[-1].startswith("SR-HEADLESS DONE") and lines[-1].endswith("until=met"), out
+    s = json.load(open(tmp_path / "summary.json"))
+    assert s["until"] == {"condition": "sim_time:0.05", "met": True}
+    d = dump.read(str(tmp_path / "dumps" / f"s_{s['steps']}.bin"))
+    assert d["sim_time"] >= 0.05 > d["sim_time"] - d["dt"] * (1 + 1e-6)
+    code, out = headless("--scene", os.path.join(SCENES, "sod.json"), "--steps", "50", "--max-steps", "10")
+    last = out.strip().split("\n")[-1]
+    assert code == 2 and last.startswith("SR-HEADLESS DONE steps=10 ") and last.endswith("until=unmet"), out
