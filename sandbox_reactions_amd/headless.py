@@ -2,7 +2,8 @@
 for the passes this path carries: gravity, hydro, heat).
 
     python -m sandbox_reactions_amd.headless --scene FILE (--steps N | --until sim_time:T) [--max-steps M]
-                                             [--out DIR] [--timing] [--dump-every K] [--load-dump FILE] [--gamma G]
+                                             [--world WxH] [--edge leave|bounce] [--out DIR] [--timing]
+                                             [--dump-every K] [--load-dump FILE] [--gamma G]
 
 stdout: the first line `SR-ADAPTER name=... backend=hip driver=...`; the last, when the run ends without an error,
 `SR-HEADLESS DONE steps=<n> sim_time=<t> until=<none|met|unmet>`; errors start `SR-ERROR`. `--until sim_time:T` stops
@@ -73,6 +74,8 @@ def run(argv=None) -> int:
     ap.add_argument("--dump-every", type=int, default=0)
     ap.add_argument("--load-dump")
     ap.add_argument("--gamma", type=float, default=2.0)
+    ap.add_argument("--world")
+    ap.add_argument("--edge")
     from . import dump, scene_json
     try:
         a = ap.parse_args(argv)
@@ -85,6 +88,19 @@ def run(argv=None) -> int:
             raise ValueError("--max-steps must be at least 1")
         until = parse_until(a.until) if a.until is not None else None
         scene = scene_json.load(a.scene)
+        # --world WxH and --edge leave|bounce stand over the scene's world (headless.rs:182-187)
+        if a.world is not None:
+            wh = a.world.split("x")
+            if len(wh) != 2 or not all(v.isdigit() for v in wh):
+                raise ValueError(f"--world takes WxH, got {a.world!r}")
+            ww, hh = int(wh[0]), int(wh[1])
+            if not (8 <= ww <= 4096 and 8 <= hh <= 4096 and ww % 8 == 0 and hh % 8 == 0):
+                raise ValueError(f"--world: {ww} x {hh} is not a world size (multiples of 8 in [8, 4096])")
+            scene.width, scene.height = ww, hh
+        if a.edge is not None:
+            if a.edge not in ("leave", "bounce"):
+                raise ValueError(f"--edge takes leave or bounce, got {a.edge!r}")
+            scene.edge = 1 if a.edge == "leave" else 0
         scene.check_path()
         scene.planes(a.gamma)  # every cell laid down before a device is touched (a bad scene is exit 4)
     except (OSError, ValueError) as e:

@@ -47,6 +47,9 @@ def test_a_scene_the_path_does_not_carry_is_exit_4(tmp_path):
     (["--until", "sometime"], "expected sim_time"),
     (["--steps", "3", "--max-steps", "0"], "at least 1"),
     (["--steps", "x"], "bad arguments"),
+    (["--steps", "1", "--world", "7x8"], "not a world size"),
+    (["--steps", "1", "--world", "64by64"], "WxH"),
+    (["--steps", "1", "--edge", "sideways"], "leave or bounce"),
 ])
 def test_bad_arguments_are_exit_4(args, text):
     """headless.rs:190-202 and watch.rs:115-135: one of --steps and --until, at least one step, sim_time of 0 or more;
@@ -90,7 +93,8 @@ def test_a_run_writes_the_summary_and_dumps(gpu_lib, tmp_path):
 @pytest.mark.gpu
 def test_until_sim_time_and_max_steps(gpu_lib, tmp_path):
     """--until sim_time:T stops at the first step whose clock reaches T (exit 0, until=met, the step before short of
-    T); --max-steps cutting a run on --steps is exit 2 with until=unmet."""
+    T); --world and --edge stand over the scene's world; --max-steps cutting a run on --steps is exit 2 with
+    until=unmet."""
     from sandbox_reactions_amd import dump
     print("headless --until", flush=True)
     code, out = headless("--scene", os.path.join(SCENES, "sod.json"), "--until", "sim_time:0.05", "--out",
@@ -101,6 +105,10 @@ def test_until_sim_time_and_max_steps(gpu_lib, tmp_path):
     assert s["until"] == {"condition": "sim_time:0.05", "met": True}
     d = dump.read(str(tmp_path / "dumps" / f"s_{s['steps']}.bin"))
     assert d["sim_time"] >= 0.05 > d["sim_time"] - d["dt"] * (1 + 1e-6)
+    code, out = headless("--scene", os.path.join(SCENES, "sod.json"), "--steps", "2", "--world", "400x16", "--edge",
+                         "leave", "--out", str(tmp_path / "over"))
+    assert code == 0, out
+    assert json.load(open(tmp_path / "over" / "summary.json"))["world"] == {"width": 400, "height": 16, "edge": "leave"}
     code, out = headless("--scene", os.path.join(SCENES, "sod.json"), "--steps", "50", "--max-steps", "10")
     last = out.strip().split("\n")[-1]
     assert code == 2 and last.startswith("SR-HEADLESS DONE steps=10 ") and last.endswith("until=unmet"), out
