@@ -203,6 +203,21 @@ int sr_world_create(int w, int h, const sr_physics *ph, const sr_switches *sw, c
 int sr_upload(sr_world *wd, const float *planes, size_t n);
 /* State::readback (state.rs:273) */
 int sr_readback(sr_world *wd, float *planes, size_t n);
+/* The step's records a state dump carries beyond the cells (crates/sr-engine/src/dump.rs:8-20, :55-70 `Resume`),
+ * which load_dump restores so that a resumed run goes on bit for bit: dt_n, dt_{n+1} and the sim time. */
+typedef struct {
+    double dt, dt_next, sim_time;
+} sr_resume_records;
+/* load_dump's restore after State::upload (dump.rs:8-20; Step::set_heat_rate and side_flux, step/heat.rs:567-598):
+ * the clock from `rec`; with heat on and `flux` given, F's two planes (F_x then F_y, 2 w h f32, flux_n = 2 w h) that
+ * the next P4 pushes the gas with (NULL: none, as after an upload); and `sigma` (w h f64, sigma_n = w h, may be NULL)
+ * in place of the uploaded Sigma plane: the library carries Sigma in FP64 and a dump's f32 plane is that value
+ * rounded, so resuming bit for bit needs the FP64 plane (sr_readback_sigma). The rest of the step (phi, g, the heat
+ * pass's planes) is rebuilt by the next step from these. The step count stays the upload's (0). */
+int sr_resume(sr_world *wd, const sr_resume_records *rec, const float *flux, size_t flux_n, const double *sigma,
+              size_t sigma_n);
+/* Sigma's FP64 plane (w h values), the state's own precision */
+int sr_readback_sigma(sr_world *wd, double *sigma, size_t n);
 /* Step::run (step/mod.rs:703): `steps` steps, P1 -> P2 -> P4 -> P5 -> P8 -> P9 (reference/mod.rs:264-297) */
 int sr_run(sr_world *wd, uint32_t steps);
 /* A batch of independent worlds, each `steps` steps as sr_run would take them (every world's result is the one it
@@ -219,6 +234,15 @@ int sr_read_field(sr_world *wd, int field, float *out, size_t n); /* side fields
  * through the edges (reference/hydro.rs:210-224, :954-976) */
 int sr_read_totals(sr_world *wd, double *totals, double *escaped);
 int sr_set_precond(sr_world *wd, int precond);                /* set_heat_precond (step/heat.rs:595) */
+/* heat_first_system (step/heat.rs:604-640): the system of the last step's first Newton pass (its iterate at the start
+ * temperatures T0, rebuilt from the planes and faces the step leaves, with its multigrid hierarchy), level 0 read back
+ * in sr_fld_set_system's form: own (w h; couplings to non-gas cells and the edge folded in, 1 off the gas), kx
+ * ((w+1) h) and ky (w (h+1)) the gas-gas couplings, gas (w h bytes, 1 gas). n = w h. Host arrays (any may be NULL).
+ * Needs a step run with heat since the upload (SR_ERR_ARGUMENT otherwise). */
+int sr_heat_first_system(sr_world *wd, float *own, float *kx, float *ky, uint8_t *gas, size_t n);
+/* apply_heat_precond (step/heat.rs:645-673): z = M^-1 r by the world's preconditioner (sr_set_precond) on the system
+ * sr_heat_first_system last built; r and z host arrays of w h f32, z 0 off the gas. */
+int sr_apply_heat_precond(sr_world *wd, const float *r, float *z, size_t n);
 /* the network SR_PRECOND_LEARNED uses in the world's heat pass (no counterpart in the reference) */
 int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
 int sr_set_dt_cap(sr_world *wd, double cap);                  /* Twin::cap_dt_next (reference/mod.rs:225) */

@@ -64,6 +64,10 @@ class StepCost(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("step_ms", C.c_double), ("pass_ms", C.c_double * 10)]
 
 
+class ResumeRecords(C.Structure):
+    _fields_ = [("dt", C.c_double), ("dt_next", C.c_double), ("sim_time", C.c_double)]
+
+
 # (name, restype, argtypes): every entry point include/sr_hip.h declares
 _vp, _i, _d, _sz, _u32 = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_uint32
 _pp = C.POINTER(C.c_void_p)
@@ -86,6 +90,10 @@ SIGNATURES = [
     ("sr_world_create", _i, [_i, _i, C.POINTER(Physics), C.POINTER(Switches), C.POINTER(HeatSettings), _pp]),
     ("sr_upload", _i, [_vp, _vp, _sz]),
     ("sr_readback", _i, [_vp, _vp, _sz]),
+    ("sr_resume", _i, [_vp, C.POINTER(ResumeRecords), _vp, _sz, _vp, _sz]),
+    ("sr_readback_sigma", _i, [_vp, _vp, _sz]),
+    ("sr_heat_first_system", _i, [_vp, _vp, _vp, _vp, _vp, _sz]),
+    ("sr_apply_heat_precond", _i, [_vp, _vp, _vp, _sz]),
     ("sr_run", _i, [_vp, _u32]),
     ("sr_fld_tiles", _i, [_vp, C.POINTER(_i), C.POINTER(_i), _i, C.POINTER(_i)]),
     ("sr_fld_set_tiles", _i, [_vp, C.POINTER(_i), C.POINTER(_i), _i]),

@@ -344,6 +344,15 @@ int HeatPass::read_sums(int k, double *out, hipStream_t s) {
     return SR_OK;
 }
 
+int HeatPass::first_system(const double *scal, hipStream_t s) {
+    const size_t n = (size_t)P.w * P.h;
+    SR_HIP(hipMemcpyAsync(H.t, H.t0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_heat_system, dim3(nb), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
+    SR_HIP(hipGetLastError());
+    SR_HIP(hipMemcpyAsync(solver.gas0(), H.gas, n, hipMemcpyDeviceToDevice, s));
+    return solver.rebuild(true, s);
+}
+
 int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, float *fx, float *fy, int precond,
                    hipStream_t s) {
     const size_t n = (size_t)P.w * P.h;

@@ -48,6 +48,9 @@ class HeatPass {
     // one P5 step on the state (E in place); writes the cell-centred flux the next P4 reads
     int step(const HydroLike &st, const double *scal, double dt_host, float *fx, float *fy, int precond,
              hipStream_t s);
+    // the last step's first Newton pass's system (its iterate at T0) rebuilt into the solver's level 0 and hierarchy
+    // from the planes and faces the step left (clobbers the last iterate, which no later pass reads)
+    int first_system(const double *scal, hipStream_t s);
     sr_heat_report report{};
     FldSolver solver;
     HeatParams P{};

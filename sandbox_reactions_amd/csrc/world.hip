@@ -260,6 +260,8 @@ struct sr_world {
     float *phi = nullptr, *gx = nullptr, *gy = nullptr, *phi_mid = nullptr, *sigma_mid = nullptr;
     float *pushmag = nullptr, *rfx = nullptr, *rfy = nullptr;
     bool have_flux = false;
+    bool heat_ran = false;      // a step with heat has run since the upload (its first system can be rebuilt)
+    bool first_built = false;   // the solver holds that first system (sr_heat_first_system), not a later pass's
     double *escaped = nullptr, *totals = nullptr;
     HeatPass heat;
     uint64_t steps = 0;
@@ -479,6 +481,8 @@ extern "C" int sr_upload(sr_world *wd, const float *planes, size_t n) {
     hipLaunchKernelGGL(k_scal_reset, dim3(1), dim3(64), 0, wd->s, wd->scal);
     SR_HIP(hipMemsetAsync(wd->escaped, 0, sizeof(double) * 14 * 2 * (wd->w + wd->h), wd->s));
     wd->have_flux = false;
+    wd->heat_ran = false;
+    wd->first_built = false;
     wd->steps = 0;
     wd->last_heat = sr_heat_report{};
     // the twin's start: P2 on the state and dt_0 from it (reference/mod.rs:173-175), no floors
@@ -497,6 +501,87 @@ extern "C" int sr_readback(sr_world *wd, float *planes, size_t n) {
     SR_HIP(hipMemcpyAsync(planes, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, wd->s));
     SR_HIP(hipFreeAsync(tmp, wd->s));
     SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+__global__ void k_sigma_in(const double *__restrict__ src, double *__restrict__ sigma, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sigma[i] = src[i];
+}
+
+extern "C" int sr_resume(sr_world *wd, const sr_resume_records *rec, const float *flux, size_t flux_n,
+                         const double *sigma, size_t sigma_n) {
+    if (!wd || !rec) return fail(SR_ERR_ARGUMENT, "sr_resume: null pointer");
+    if (!(rec->dt >= 0.0) || !(rec->dt_next > 0.0) || !std::isfinite(rec->dt) || !std::isfinite(rec->dt_next) ||
+        !std::isfinite(rec->sim_time))
+        return fail(SR_ERR_ARGUMENT, "sr_resume: dt >= 0, dt_next > 0 and a finite sim time are required");
+    if (flux && flux_n != 2 * wd->n)
+        return fail(SR_ERR_LENGTH, "sr_resume: F holds " + std::to_string(flux_n) + " values, expected " +
+                                       std::to_string(2 * wd->n));
+    if (sigma && sigma_n != wd->n)
+        return fail(SR_ERR_LENGTH, "sr_resume: Sigma holds " + std::to_string(sigma_n) + " values, expected " +
+                                       std::to_string(wd->n));
+    double h[3] = {rec->dt, rec->dt_next, rec->sim_time};
+    static_assert(SCAL_DT == 0 && SCAL_DT_NEXT == 1 && SCAL_SIM_TIME == 2, "the clock's slots are the first three");
+    SR_HIP(hipMemcpyAsync(wd->scal, h, sizeof h, hipMemcpyHostToDevice, wd->s));
+    if (flux && wd->sw.heat) {
+        SR_HIP(hipMemcpyAsync(wd->rfx, flux, sizeof(float) * wd->n, hipMemcpyHostToDevice, wd->s));
+        SR_HIP(hipMemcpyAsync(wd->rfy, flux + wd->n, sizeof(float) * wd->n, hipMemcpyHostToDevice, wd->s));
+        wd->have_flux = true;
+    }
+    if (sigma) {
+        double *tmp = nullptr;
+        SR_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * wd->n, wd->s));
+        SR_HIP(hipMemcpyAsync(tmp, sigma, sizeof(double) * wd->n, hipMemcpyHostToDevice, wd->s));
+        hipLaunchKernelGGL(k_sigma_in, dim3((wd->n + 255) / 256), dim3(256), 0, wd->s, (const double *)tmp,
+                           wd->st[wd->cur].sigma, (int)wd->n);
+        SR_HIP(hipGetLastError());
+        SR_HIP(hipFreeAsync(tmp, wd->s));
+    }
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+extern "C" int sr_readback_sigma(sr_world *wd, double *sigma, size_t n) {
+    if (!wd || !sigma) return fail(SR_ERR_ARGUMENT, "sr_readback_sigma: null pointer");
+    if (n != wd->n) return fail(SR_ERR_LENGTH, "sr_readback_sigma: Sigma holds w*h values");
+    SR_HIP(hipMemcpyAsync(sigma, wd->st[wd->cur].sigma, sizeof(double) * n, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+extern "C" int sr_heat_first_system(sr_world *wd, float *own, float *kx, float *ky, uint8_t *gas, size_t n) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "sr_heat_first_system: null world");
+    if (!wd->sw.heat || !wd->heat_ran)
+        return fail(SR_ERR_ARGUMENT, "sr_heat_first_system: no step with heat has run since the upload");
+    if (n != wd->n) return fail(SR_ERR_LENGTH, "sr_heat_first_system: n must be w*h");
+    FldSolver &f = wd->heat.solver;
+    SR_TRY(wd->heat.first_system(wd->scal, wd->s));
+    const size_t nx = (size_t)(wd->w + 1) * wd->h, ny = (size_t)wd->w * (wd->h + 1);
+    if (own) SR_HIP(hipMemcpyAsync(own, f.own0(), sizeof(float) * n, hipMemcpyDeviceToHost, wd->s));
+    if (kx) SR_HIP(hipMemcpyAsync(kx, f.ax0(), sizeof(float) * nx, hipMemcpyDeviceToHost, wd->s));
+    if (ky) SR_HIP(hipMemcpyAsync(ky, f.ay0(), sizeof(float) * ny, hipMemcpyDeviceToHost, wd->s));
+    if (gas) SR_HIP(hipMemcpyAsync(gas, f.gas0(), n, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    wd->first_built = true;
+    return SR_OK;
+}
+
+extern "C" int sr_apply_heat_precond(sr_world *wd, const float *r, float *z, size_t n) {
+    if (!wd || !r || !z) return fail(SR_ERR_ARGUMENT, "sr_apply_heat_precond: null pointer");
+    if (!wd->first_built) return fail(SR_ERR_ARGUMENT, "sr_apply_heat_precond: no sr_heat_first_system yet");
+    if (n != wd->n) return fail(SR_ERR_LENGTH, "sr_apply_heat_precond: n must be w*h");
+    float *d = nullptr;
+    SR_HIP(hipMallocAsync((void **)&d, sizeof(float) * 2 * n, wd->s));
+    SR_HIP(hipMemcpyAsync(d, r, sizeof(float) * n, hipMemcpyHostToDevice, wd->s));
+    SR_TRY(wd->heat.solver.apply_precond(d, d + n, wd->precond, wd->s));
+    std::vector<uint8_t> gas(n);
+    SR_HIP(hipMemcpyAsync(z, d + n, sizeof(float) * n, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipMemcpyAsync(gas.data(), wd->heat.solver.gas0(), n, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipFreeAsync(d, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    for (size_t i = 0; i < n; i++)
+        if (!gas[i]) z[i] = 0.0f;  // rows off the gas are no part of the system
     return SR_OK;
 }
 
@@ -545,6 +630,8 @@ static int one_step(sr_world *wd) {
         SR_TRY(wd->heat.step(hl, wd->scal, wd->host_scal[SCAL_DT], wd->rfx, wd->rfy, wd->precond, s));
         wd->last_heat = wd->heat.report;
         wd->have_flux = true;
+        wd->heat_ran = true;
+        wd->first_built = false;
     }
     mark(wd, 4);
     SR_TRY(p89(wd, 1));

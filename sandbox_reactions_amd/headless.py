@@ -17,12 +17,20 @@ the state after a run's last step: the step named is the first at whose end the 
 reference names only when that step is one it scans.
 
 With --out, DIR/summary.json holds the reference's run-summary fields this path has -- version, tool, adapter, scene,
-world {width, height, edge}, steps, sim_time, wall_s, with --until `until` {condition, met}, and with --timing, `timing` {per_pass_ms by the reference's pass
-labels, step_ms_mean, source "gpu-timestamp"} (headless.rs:497-531, :1356-1381) -- and DIR/dumps/s_<step>.bin the
-SRDUMP01 dumps (every K steps with --dump-every, and the last). The floors and the step's reductions run fused here,
-so their time is reported under "P8 floors". --load-dump continues from a dump's cells, step index and sim time, and
---steps N is N more steps (the DONE line counts them, the dumps' indices count from the scene); the next dt restarts
-from the cells (P9 on the loaded state), and no F plane is read (this library writes none).
+world {width, height, edge}, steps, sim_time, wall_s, with --until `until` {condition, met}, and with --timing,
+`timing` {per_pass_ms by the reference's pass labels, step_ms_mean, source "gpu-timestamp"} (headless.rs:497-531,
+:1356-1381) -- and DIR/dumps/s_<step>.bin the SRDUMP01 dumps: at every multiple K of the step index (counted from the
+scene) with --dump-every K, and after the last step (headless.rs:1211-1213, :1298-1345). A dump of a run with heat
+carries F (resume.flux); beside every dump, s_<step>.bin.sigma64 holds Sigma's FP64 plane. The floors and the step's
+reductions run fused here, so their time is reported under "P8 floors".
+
+--load-dump (read and checked before anything else: a bad dump is exit 4, `SR-ERROR bad dump: ...`) takes the world,
+the cells, the step index, the clock (sim time, dt, dt_next) and F from the dump, and Sigma's FP64 plane from its
+.sigma64 file when there is one: such a resume goes on bit for bit, one without that file from the dump's f32 Sigma.
+--world must then agree with the dump's world. --steps N is N more steps (the DONE line counts them, the dumps'
+indices count from the scene). --world resizes the scene's world, its objects shifted to keep their place relative to
+the centre (scene.rs:428-435). Refusals of the library (a world size, an FFT axis past the cap, an argument) are
+exit 4.
 """
 from __future__ import annotations
 
@@ -31,6 +39,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 PASS_LABELS = {1: "P1 dt", 2: "P2 gravity", 4: "P4 hydro", 5: "P5 heat", 8: "P8 floors"}
 MAX_STEPS = 2_000_000  # --max-steps's default (headless.rs:77)
@@ -63,6 +73,16 @@ def parse_until(text):
     raise ValueError(f"--until {text!r}: expected sim_time:<t>, event:<kind>, stage:<id> or ending")
 
 
+def _world_arg(text):
+    wh = text.split("x")
+    if len(wh) != 2 or not all(v.isdigit() for v in wh):
+        raise ValueError(f"bad arguments: --world takes WxH, got {text!r}")
+    ww, hh = int(wh[0]), int(wh[1])
+    if not (8 <= ww <= 4096 and 8 <= hh <= 4096 and ww % 8 == 0 and hh % 8 == 0):
+        raise ValueError(f"bad arguments: --world: {ww} x {hh} is not a world size (multiples of 8 in [8, 4096])")
+    return ww, hh
+
+
 def run(argv=None) -> int:
     ap = _Args(prog="sandbox_reactions_amd.headless")
     ap.add_argument("--scene", required=True)
@@ -71,7 +91,7 @@ def run(argv=None) -> int:
     ap.add_argument("--max-steps", type=int, default=MAX_STEPS)
     ap.add_argument("--out")
     ap.add_argument("--timing", action="store_true")
-    ap.add_argument("--dump-every", type=int, default=0)
+    ap.add_argument("--dump-every", type=int)
     ap.add_argument("--load-dump")
     ap.add_argument("--gamma", type=float, default=2.0)
     ap.add_argument("--world")
@@ -79,32 +99,52 @@ def run(argv=None) -> int:
     from . import dump, scene_json
     try:
         a = ap.parse_args(argv)
-        # headless.rs:190-202: one of --steps and --until, at least one step, a positive bound
+        # headless.rs:190-202: one of --steps and --until, at least one step, a positive bound, a dump cadence of at
+        # least one step and somewhere to write it
         if (a.steps is None) == (a.until is None):
-            raise ValueError("give --steps N or --until <condition>, one of the two")
+            raise ValueError("bad arguments: give --steps N or --until <condition>, one of the two")
         if a.steps is not None and a.steps < 1:
-            raise ValueError("--steps must be at least 1")
+            raise ValueError("bad arguments: --steps must be at least 1")
         if a.max_steps < 1:
-            raise ValueError("--max-steps must be at least 1")
+            raise ValueError("bad arguments: --max-steps must be at least 1")
+        if a.dump_every is not None and a.dump_every < 1:
+            raise ValueError("bad arguments: --dump-every must be at least 1")
+        if a.dump_every is not None and not a.out:
+            raise ValueError("bad arguments: --dump-every needs --out (dumps go to DIR/dumps)")
         until = parse_until(a.until) if a.until is not None else None
+        world_over = _world_arg(a.world) if a.world is not None else None
+        if a.edge is not None and a.edge not in ("leave", "bounce"):
+            raise ValueError(f"bad arguments: --edge takes leave or bounce, got {a.edge!r}")
+    except ValueError as e:
+        msg = str(e)
+        print(f"SR-ERROR {msg if msg.startswith('bad arguments') else 'bad arguments: ' + msg}", flush=True)
+        return 4
+    # a dump to start from, read and checked before anything else (headless.rs:943-957); it gives the world
+    d = None
+    if a.load_dump:
+        try:
+            d = dump.read(a.load_dump)
+        except (OSError, ValueError) as e:
+            print(f"SR-ERROR bad dump: {e}", flush=True)
+            return 4
+        if world_over is not None and world_over != (d["width"], d["height"]):
+            print(f"SR-ERROR bad arguments: --world {world_over[0]}x{world_over[1]} against the dump's "
+                  f"{d['width']}x{d['height']}", flush=True)
+            return 4
+    try:
         scene = scene_json.load(a.scene)
-        # --world WxH and --edge leave|bounce stand over the scene's world (headless.rs:182-187)
-        if a.world is not None:
-            wh = a.world.split("x")
-            if len(wh) != 2 or not all(v.isdigit() for v in wh):
-                raise ValueError(f"--world takes WxH, got {a.world!r}")
-            ww, hh = int(wh[0]), int(wh[1])
-            if not (8 <= ww <= 4096 and 8 <= hh <= 4096 and ww % 8 == 0 and hh % 8 == 0):
-                raise ValueError(f"--world: {ww} x {hh} is not a world size (multiples of 8 in [8, 4096])")
-            scene.width, scene.height = ww, hh
+        if d is not None:
+            scene.resize(d["width"], d["height"])
+        elif world_over is not None:
+            scene.resize(*world_over)
+        # --edge leave|bounce stands over the scene's (headless.rs:182-187)
         if a.edge is not None:
-            if a.edge not in ("leave", "bounce"):
-                raise ValueError(f"--edge takes leave or bounce, got {a.edge!r}")
             scene.edge = 1 if a.edge == "leave" else 0
         scene.check_path()
-        scene.planes(a.gamma)  # every cell laid down before a device is touched (a bad scene is exit 4)
+        if d is None:
+            scene.planes(a.gamma)  # every cell laid down before a device is touched (a bad scene is exit 4)
     except (OSError, ValueError) as e:
-        print(f"SR-ERROR {e}", flush=True)
+        print(f"SR-ERROR bad scene: {e}", flush=True)
         return 4
     try:
         import sandbox_reactions_amd as sr
@@ -114,53 +154,73 @@ def run(argv=None) -> int:
         print(f"SR-ERROR {e}", flush=True)
         return 3
     print(f"SR-ADAPTER name={name} backend=hip driver=libsrhip cus={cus}", flush=True)
+    refused = (sr._lib.SR_ERR_WORLD_SIZE, sr._lib.SR_ERR_BOX_AXIS, sr._lib.SR_ERR_ARGUMENT, sr._lib.SR_ERR_LENGTH)
     try:
-        world = scene_json.world(scene, gamma=a.gamma)
-        step0, clock0 = 0, 0.0
-        if a.load_dump:
-            d = dump.read(a.load_dump)
-            if (d["width"], d["height"]) != (scene.width, scene.height):
-                print(f"SR-ERROR the dump's world {d['width']} x {d['height']} is not the scene's", flush=True)
+        try:
+            world = scene_json.world(scene, gamma=a.gamma, planes=d["planes"] if d is not None else None)
+        except sr.SrError as e:
+            if e.code in refused:
+                print(f"SR-ERROR bad arguments: {e}", flush=True)
                 return 4
-            world.upload(d["planes"])
-            step0, clock0 = int(d["step"]), float(d["sim_time"])
+            raise
+        step0 = 0
+        if d is not None:
+            # the dump's records: its clock and F (and Sigma's FP64 plane beside it, for a bit-for-bit resume); the
+            # world's clock goes on from the dump's sim time, the step index from its step
+            step0 = int(d["step"])
+            flux = d["flux"] if scene.switches["heat"] else None
+            world.resume(float(d["dt"]), float(d["dt_next"]), float(d["sim_time"]), flux=flux, sigma=d["sigma64"])
         if a.timing:
             world.set_timing(True)
         dumps = os.path.join(a.out, "dumps") if a.out else None
         if dumps:
             os.makedirs(dumps, exist_ok=True)
+
+        def write_dump(r):
+            # F follows the channels whenever the run runs P5 (dump.rs:16-19); every dump here follows a step
+            flux = np.stack([world.read_field(sr.FIELD_FLUX_X), world.read_field(sr.FIELD_FLUX_Y)]) \
+                if scene.switches["heat"] else None
+            dump.write(os.path.join(dumps, f"s_{step0 + done}.bin"), world.readback(), step=step0 + done,
+                       sim_time=r.sim_time, dt=r.dt, dt_next=r.dt_next, flux=flux, sigma64=world.readback_sigma())
+
         t0 = time.perf_counter()
         target = min(a.steps if a.steps is not None else a.max_steps, a.max_steps)
         done, met = 0, False
         r = world.read_dt()
         while done < target and not met:
-            k = target - done if a.dump_every <= 0 else min(a.dump_every, target - done)
+            k = target - done
             if until is not None and r.dt_next > 0.0:
                 # headless.rs:1206-1210: a chunk covers at most half the time left at the next step's dt, so the
                 # last steps before the condition are single ones and the run stops at the first step past it
-                k = min(k, max(1, int((until - (clock0 + r.sim_time)) / (2.0 * r.dt_next))))
+                k = min(k, max(1, int((until - (r.sim_time)) / (2.0 * r.dt_next))))
+            if a.dump_every is not None:
+                # headless.rs:1211-1213: chunks end on the multiples of K of the step index counted from the scene
+                k = min(k, a.dump_every - (step0 + done) % a.dump_every)
             world.run(k)
             done += k
             r = world.read_dt()
-            met = until is not None and clock0 + r.sim_time >= until
+            met = until is not None and r.sim_time >= until
             if r.non_finite:
                 print(f"SR-ERROR non-finite value in the state at step {step0 + r.nf_step}, cell ({r.nf_x}, "
                       f"{r.nf_y}), channel {STATE_CHANNELS[r.nf_channel]}", flush=True)
                 return 5
-            if dumps and (a.dump_every > 0 or done == target or met):
-                dump.write(os.path.join(dumps, f"s_{step0 + done}.bin"), world.readback(), step=step0 + done,
-                           sim_time=clock0 + r.sim_time, dt=r.dt, dt_next=r.dt_next)
+            # headless.rs:1298-1305: a dump at every multiple of K before the last step, the last always
+            if dumps and a.dump_every is not None and (step0 + done) % a.dump_every == 0 and done < target \
+                    and not met:
+                write_dump(r)
         wall = time.perf_counter() - t0
         r = world.read_dt()
+        if dumps:
+            write_dump(r)
     except sr.SrError as e:
         print(f"SR-ERROR {e}", flush=True)
-        return 5 if e.code == sr._lib.SR_ERR_NON_FINITE else 3
+        return 5 if e.code == sr._lib.SR_ERR_NON_FINITE else (4 if e.code in refused else 3)
     if a.out:
         summary = {"version": 1, "tool": "sandbox-reactions-amd", "adapter": {"name": name, "backend": "hip",
                                                                             "driver": "libsrhip"},
                    "scene": a.scene, "world": {"width": scene.width, "height": scene.height,
                                                "edge": "leave" if scene.edge == 1 else "bounce"},
-                   "steps": step0 + done, "sim_time": clock0 + r.sim_time, "wall_s": wall}
+                   "steps": step0 + done, "sim_time": r.sim_time, "wall_s": wall}
         if until is not None:
             summary["until"] = {"condition": f"sim_time:{a.until.partition(':')[2]}", "met": met}
         if a.timing:
@@ -173,7 +233,7 @@ def run(argv=None) -> int:
     # took them all, met, or unmet when --max-steps cut the run (exit 2) (headless.rs:1389-1400)
     reached = met if until is not None else done >= (a.steps or 0)
     verdict = ("met" if until is not None else "none") if reached else "unmet"
-    print(f"SR-HEADLESS DONE steps={done} sim_time={clock0 + r.sim_time:.6f} until={verdict}", flush=True)
+    print(f"SR-HEADLESS DONE steps={done} sim_time={r.sim_time:.6f} until={verdict}", flush=True)
     return 0 if reached else 2
 
 

@@ -122,6 +122,8 @@ class Scene:
         self.width, self.height = _whole(w, "width", "world"), _whole(w, "height", "world")
         if not (8 <= self.width <= 4096 and 8 <= self.height <= 4096 and self.width % 8 == 0 and self.height % 8 == 0):
             raise SceneError(f"world: {self.width} x {self.height} is not a world size (multiples of 8 in [8, 4096])")
+        # the file's own world: objects are placed relative to it, and a resized world shifts them (resize())
+        self.file_width, self.file_height = self.width, self.height
         edge = w.get("edge", "leave")
         if edge not in ("leave", "bounce"):
             raise SceneError(f"world.edge: must be \"leave\" or \"bounce\", got {edge!r}")
@@ -194,17 +196,32 @@ class Scene:
         c = self.constants()
         return c["k1e"] > 0.0 or c["k1n"] > 0.0
 
+    def resize(self, width: int, height: int):
+        """The world `--world` stands over the scene's (headless.rs:182-187). The objects keep their place relative to
+        the world's centre: each is shifted by ((W - W_file) / 2, (H - H_file) / 2) cells (scene.rs:428-435); both
+        sizes are multiples of 8, so the shift is whole."""
+        if not (8 <= width <= 4096 and 8 <= height <= 4096 and width % 8 == 0 and height % 8 == 0):
+            raise SceneError(f"world: {width} x {height} is not a world size (multiples of 8 in [8, 4096])")
+        self.width, self.height = width, height
+
+    def shift(self):
+        return (self.width - self.file_width) // 2, (self.height - self.file_height) // 2
+
     def cells(self, obj):
-        """(ys, xs, sigma) of the cells an object covers (scene.rs:521-563)."""
+        """(ys, xs, sigma) of the cells an object covers in the current world, shifted as resize() says
+        (scene.rs:521-563)."""
         kind, geo, _ = obj
         w, h = self.width, self.height
+        sx, sy = self.shift()
         if kind == "rect":
             x0, y0, rw, rh, sigma = geo
+            x0, y0 = x0 + sx, y0 + sy
             xs = np.arange(min(max(x0, 0), w), min(max(x0 + rw, 0), w))
             ys = np.arange(min(max(y0, 0), h), min(max(y0 + rh, 0), h))
             yy, xx = np.meshgrid(ys, xs, indexing="ij")
             return yy.ravel(), xx.ravel(), np.full(yy.size, sigma)
         x, y, a, sc = geo
+        x, y = x + sx, y + sy
         i = np.arange(math.floor(x - a), math.ceil(x + a))
         j = np.arange(math.floor(y - a), math.ceil(y + a))
         jj, ii = np.meshgrid(j, i, indexing="ij")
@@ -276,14 +293,15 @@ def parse(text: str) -> Scene:
     return Scene(json.loads(text))
 
 
-def world(scene: Scene, heat_settings=None, **physics_over):
+def world(scene: Scene, heat_settings=None, planes=None, **physics_over):
     """A World for the scene (its world size, switches and constants; gamma 2 and unit cells unless overridden),
-    with its planes uploaded. Refuses a scene whose passes this path does not carry."""
+    with its planes uploaded (`planes`, (14, h, w), in place of the scene's own: a dump's cells). Refuses a scene whose
+    passes this path does not carry."""
     import sandbox_reactions_amd as sr
     scene.check_path()
     kw = scene.physics_kwargs()
     kw.update(physics_over)
     w = sr.World(scene.width, scene.height, physics=sr.physics(**kw), gravity=scene.switches["gravity"],
                  hydro=scene.switches["hydro"], heat=scene.switches["heat"], heat_settings=heat_settings)
-    w.upload(scene.planes(kw["gamma"]))
+    w.upload(scene.planes(kw["gamma"]) if planes is None else planes)
     return w

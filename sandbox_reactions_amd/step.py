@@ -48,6 +48,50 @@ class World:
         self._c(self.lib.sr_readback(self._p, out.ctypes.data, out.size))
         return out
 
+    def resume(self, dt: float, dt_next: float, sim_time: float, flux: np.ndarray | None = None,
+               sigma: np.ndarray | None = None):
+        """load_dump's restore after `upload` (crates/sr-engine/src/dump.rs:8-20): the clock, F's two planes
+        ((2, h, w) f32, the next P4's push) and Sigma's FP64 plane ((h, w)), each optional but the clock."""
+        rec = _lib.ResumeRecords(dt, dt_next, sim_time)
+        f = None if flux is None else np.ascontiguousarray(flux, dtype=np.float32)
+        s = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+        if f is not None and f.shape != (2, self.h, self.w):
+            from . import SrError
+            raise SrError(_lib.SR_ERR_LENGTH, f"flux must be (2, {self.h}, {self.w}), got {f.shape}")
+        if s is not None and s.shape != (self.h, self.w):
+            from . import SrError
+            raise SrError(_lib.SR_ERR_LENGTH, f"sigma must be ({self.h}, {self.w}), got {s.shape}")
+        self._c(self.lib.sr_resume(self._p, C.byref(rec), None if f is None else f.ctypes.data,
+                                   0 if f is None else f.size, None if s is None else s.ctypes.data,
+                                   0 if s is None else s.size))
+
+    def readback_sigma(self) -> np.ndarray:
+        """Sigma's FP64 plane, (h, w)."""
+        out = np.empty((self.h, self.w), np.float64)
+        self._c(self.lib.sr_readback_sigma(self._p, out.ctypes.data, out.size))
+        return out
+
+    def heat_first_system(self):
+        """heat_first_system (step/heat.rs:604-640): (own, kx, ky, gas) of the last step's first Newton pass, shaped
+        (h, w), (h, w + 1), (h + 1, w), (h, w) as sr_fld_set_system takes them."""
+        own = np.empty((self.h, self.w), np.float32)
+        kx = np.empty((self.h, self.w + 1), np.float32)
+        ky = np.empty((self.h + 1, self.w), np.float32)
+        gas = np.empty((self.h, self.w), np.uint8)
+        self._c(self.lib.sr_heat_first_system(self._p, own.ctypes.data, kx.ctypes.data, ky.ctypes.data,
+                                              gas.ctypes.data, own.size))
+        return own, kx, ky, gas
+
+    def apply_heat_precond(self, r: np.ndarray) -> np.ndarray:
+        """apply_heat_precond (step/heat.rs:645-673): M^-1 r on the system heat_first_system built, 0 off the gas."""
+        a = np.ascontiguousarray(r, dtype=np.float32)
+        if a.shape != (self.h, self.w):
+            from . import SrError
+            raise SrError(_lib.SR_ERR_LENGTH, f"r must be ({self.h}, {self.w}), got {a.shape}")
+        z = np.empty_like(a)
+        self._c(self.lib.sr_apply_heat_precond(self._p, a.ctypes.data, z.ctypes.data, a.size))
+        return z
+
     def run(self, steps: int = 1):
         self._c(self.lib.sr_run(self._p, steps))
 

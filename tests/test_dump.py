@@ -77,3 +77,63 @@ def test_the_twin_at_the_ceiling_moves_no_more_heat_than_its_cells_hold():
     assert not rep.converged, (rep.passes, rep.rounds)
     assert rep.delta == 0.0
     assert_bounded(ph, before, after, rep.radiated, 1e-9, 1e-12)
+
+
+def test_written_channels_are_the_reference_names():
+    """A dump this library writes names the channels as the reference's dumps do (state.rs:29-31), so the
+    reference's reader, which requires that exact list (dump.rs:302-304), takes it."""
+    golden = dump.read(os.path.join(HERE, "heat_ceiling.bin"))["header"]["channels"]
+    b = dump.to_bytes(np.zeros((14, 8, 8), np.float32), step=3, sim_time=0.5, dt=0.1, dt_next=0.2)
+    assert dump.from_bytes(b)["header"]["channels"] == golden == dump.CHANNELS
+
+
+def _with_header(edit):
+    with open(os.path.join(HERE, "heat_overflow.bin"), "rb") as f:
+        b = f.read()
+    import json
+    import struct
+    n = struct.unpack("<I", b[8:12])[0]
+    h = json.loads(b[12:12 + n])
+    edit(h)
+    hb = json.dumps(h).encode()
+    return b[:8] + struct.pack("<I", len(hb)) + hb + b[12 + n:]
+
+
+@pytest.mark.parametrize("edit, message", [
+    (lambda h: h.__setitem__("channels", [c.lower() for c in h["channels"]]), "channels"),
+    (lambda h: h.__setitem__("channels", h["channels"][:13]), "channels"),
+    (lambda h: h.pop("channels"), "channels: missing"),
+    (lambda h: h.__setitem__("extra", 1), "extra: unknown key"),
+    (lambda h: h["resume"].pop("dt_next"), "dt_next: missing"),
+    (lambda h: h["resume"].__setitem__("rate", True), "rate: true without F"),
+    (lambda h: h["resume"].__setitem__("flux", True), "planes hold"),
+    (lambda h: h.__setitem__("version", 2), "version"),
+    (lambda h: h.__setitem__("sinks", [{"x": 1, "y": 1, "vx": 0, "vy": 0, "m": 1}] * 17), "at most 16"),
+])
+def test_header_refusals(edit, message):
+    """The reference reader's refusals (dump.rs:146-330): lowercase or missing channel names, unknown or missing keys,
+    a rate without F, a flag the planes disagree with, another version, more than 16 sinks."""
+    with pytest.raises(dump.DumpError, match=message):
+        dump.from_bytes(_with_header(edit))
+
+
+def test_flux_and_sigma64_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    planes = rng.random((14, 16, 24)).astype(np.float32)
+    flux = rng.standard_normal((2, 16, 24)).astype(np.float32)
+    sigma = rng.random((16, 24)) + 1e-3
+    path = str(tmp_path / "s_7.bin")
+    dump.write(path, planes, sigma64=sigma, step=7, sim_time=1.2345678901234567, dt=1e-3 / 3, dt_next=2e-3 / 7,
+               flux=flux)
+    d = dump.read(path)
+    assert d["header"]["resume"]["flux"] is True and d["header"]["resume"]["rate"] is False
+    assert np.array_equal(d["flux"], flux) and np.array_equal(d["planes"], planes)
+    assert np.array_equal(d["sigma64"], sigma)
+    # the FP64 clock survives the JSON header exactly
+    assert d["sim_time"] == 1.2345678901234567 and d["dt"] == 1e-3 / 3 and d["dt_next"] == 2e-3 / 7
+    dump.write(path, planes, step=7)
+    assert dump.read(path)["sigma64"] is None
+    with open(path + ".sigma64", "wb") as f:
+        f.write(b"SRSIG64\0" + b"\0" * 8)
+    with pytest.raises(dump.DumpError, match="sigma64"):
+        dump.read(path)

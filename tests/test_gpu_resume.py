@@ -1,0 +1,110 @@
+"""The world-level boundary beside the step: resuming from a dump bit for bit (crates/sr-engine/src/dump.rs:8-20, the
+step's records and F; load_dump and set_heat_rate, step/heat.rs:567-598), and the heat pass's own first system with
+its preconditioner (heat_first_system and apply_heat_precond, step/heat.rs:604-673) against the oracle's system and
+V-cycle model, symmetric and positive."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import both
+from sandbox_reactions_amd import dump, scene_json, scenes
+
+SCENES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
+
+
+def _full_world(sr, n):
+    kw = scenes.config_physics_kwargs(n)
+    hs = sr.heat_settings(stop=sr.STOP_RELATIVE, solve_tol=1e-10, newton_heat=1e-9, newton_cell=1e-6)
+    return sr.World(n, n, physics=sr.physics(**kw), gravity=True, hydro=True, heat=True, heat_settings=hs)
+
+
+@pytest.mark.gpu
+def test_a_resumed_run_goes_on_bit_for_bit(gpu_lib, tmp_path):
+    """4 steps straight equal 2 steps, a dump (with F and Sigma's FP64 plane beside it), a resume and 2 more steps, to
+    the last bit of every plane, of Sigma's FP64 plane, of F and of the clock: the dump carries everything the step
+    reads from the step before. Without F the resumed run departs: the F planes are what the next P4 pushes with."""
+    sr = gpu_lib
+    n = 64
+    print("resume: turbulent 64^2 full step, 4 straight against 2 + dump + 2", flush=True)
+    planes = scenes.turbulent_scene(n).astype(np.float32)
+    a = _full_world(sr, n)
+    a.upload(planes)
+    a.run(4)
+    b = _full_world(sr, n)
+    b.upload(planes)
+    b.run(2)
+    r = b.read_dt()
+    path = str(tmp_path / "s_2.bin")
+    flux = np.stack([b.read_field(sr.FIELD_FLUX_X), b.read_field(sr.FIELD_FLUX_Y)])
+    assert np.any(flux != 0.0)
+    dump.write(path, b.readback(), sigma64=b.readback_sigma(), step=2, sim_time=r.sim_time, dt=r.dt,
+               dt_next=r.dt_next, flux=flux)
+    d = dump.read(path)
+    c = _full_world(sr, n)
+    c.upload(d["planes"])
+    c.resume(d["dt"], d["dt_next"], d["sim_time"], flux=d["flux"], sigma=d["sigma64"])
+    c.run(2)
+    pa, pc = a.readback(), c.readback()
+    assert np.array_equal(pa.view(np.uint32), pc.view(np.uint32))
+    assert np.array_equal(a.readback_sigma().view(np.uint64), c.readback_sigma().view(np.uint64))
+    for f in (sr.FIELD_FLUX_X, sr.FIELD_FLUX_Y):
+        assert np.array_equal(a.read_field(f).view(np.uint32), c.read_field(f).view(np.uint32))
+    ra, rc = a.read_dt(), c.read_dt()
+    assert (ra.dt, ra.dt_next, ra.sim_time) == (rc.dt, rc.dt_next, rc.sim_time)
+    # the same resume without F: the next P4 has no radiation push, and the state departs
+    e = _full_world(sr, n)
+    e.upload(d["planes"])
+    e.resume(d["dt"], d["dt_next"], d["sim_time"], sigma=d["sigma64"])
+    e.run(2)
+    assert not np.array_equal(e.readback(), pa)
+
+
+def _vacuum_rim_scene():
+    """heat_rim.json with its background below sigma_vac: a hot disk in vacuum, so the system has non-gas rows."""
+    with open(os.path.join(SCENES, "heat_rim.json")) as f:
+        s = json.load(f)
+    s["objects"][0]["sigma"] = 5e-7
+    return scene_json.parse(json.dumps(s))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("omega", [1.0, 1.2])
+def test_the_world_heat_system_is_the_oracles_and_its_preconditioner_symmetric(gpu_lib, omega):
+    """sr_heat_first_system after one heat-only step is the oracle's first Newton pass system on the same cells and
+    dt (f32 against f64: 1e-6); sr_apply_heat_precond on it is the oracle's V-cycle model on the library's own f32
+    system (1e-5), 0 off the gas, symmetric (1e-5 normwise) and positive; Jacobi is r / diag."""
+    sr = gpu_lib
+    s = _vacuum_rim_scene()
+    print(f"first system: vacuum rim 64^2, omega {omega}", flush=True)
+    oph, ph = both(**s.physics_kwargs())
+    hs = sr.heat_settings(coarse_scale=omega)
+    w = sr.World(64, 64, physics=ph, gravity=False, hydro=False, heat=True, heat_settings=hs)
+    planes = s.planes().astype(np.float32)
+    w.upload(planes)
+    with pytest.raises(sr.SrError):
+        w.heat_first_system()  # no step yet
+    w.run(1)
+    dt = w.read_heat().dt
+    own, kx, ky, gas = w.heat_first_system()
+    o_own, o_kx, o_ky, _, o_gas, _ = oracle.heat_first_system(oph, planes.astype(np.float64), dt)
+    assert np.array_equal(gas, o_gas) and 0 < gas.sum() < gas.size
+    g = gas == 1
+    assert np.max(np.abs(own[g] - o_own[g]) / o_own[g]) <= 1e-6
+    assert np.max(np.abs(kx - o_kx)) <= 1e-6 * np.max(o_kx) and np.max(np.abs(ky - o_ky)) <= 1e-6 * np.max(o_ky)
+    rng = np.random.default_rng(4)
+    u, v = (np.where(g, x, 0.0).astype(np.float32) for x in rng.standard_normal((2, 64, 64)))
+    zu, zv = w.apply_heat_precond(u).astype(np.float64), w.apply_heat_precond(v).astype(np.float64)
+    assert np.all(zu[~g] == 0.0)
+    model = oracle.mg_apply(own.astype(np.float64), kx.astype(np.float64), ky.astype(np.float64), gas,
+                            u.astype(np.float64), coarse_scale=omega)
+    assert np.linalg.norm(zu - model) / np.linalg.norm(model) <= 1e-5
+    a, b = np.sum(v * zu), np.sum(u * zv)
+    assert abs(a - b) <= 1e-5 * np.sqrt(np.sum(u * zu) * np.sum(v * zv))
+    assert np.sum(u * zu) > 0 and np.sum(v * zv) > 0
+    w.set_precond(sr.PRECOND_JACOBI)
+    zj = w.apply_heat_precond(u).astype(np.float64)
+    diag = own + kx[:, :-1] + kx[:, 1:] + ky[:-1, :] + ky[1:, :]
+    assert np.allclose(zj[g], (u / diag)[g], rtol=1e-6)

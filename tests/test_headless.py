@@ -81,13 +81,17 @@ def test_a_run_writes_the_summary_and_dumps(gpu_lib, tmp_path):
     for k in (1, 2, 3):
         d = dump.read(str(tmp_path / "dumps" / f"s_{k}.bin"))
         assert d["step"] == k and np.all(np.isfinite(d["planes"]))
-    # resuming from the second dump for one step lands on the third's state
+    assert d["flux"] is not None  # a run with heat writes F (dump.rs:16-19)
+    # resuming from the second dump for one step lands on the third's state, bit for bit (dump.rs:8-20)
     code, out = headless("--scene", os.path.join(SCENES, "heat_rim.json"), "--steps", "1", "--out",
                          str(tmp_path / "resumed"), "--load-dump", str(tmp_path / "dumps" / "s_2.bin"))
     assert code == 0, out
-    a = dump.read(str(tmp_path / "resumed" / "dumps" / "s_3.bin"))["planes"]
-    b = dump.read(str(tmp_path / "dumps" / "s_3.bin"))["planes"]
-    assert np.max(np.abs(a[0] - b[0])) <= 1e-6 * np.abs(b[0]).max()
+    a = dump.read(str(tmp_path / "resumed" / "dumps" / "s_3.bin"))
+    b = dump.read(str(tmp_path / "dumps" / "s_3.bin"))
+    assert np.array_equal(a["planes"].view(np.uint32), b["planes"].view(np.uint32))
+    assert np.array_equal(a["flux"].view(np.uint32), b["flux"].view(np.uint32))
+    assert np.array_equal(a["sigma64"].view(np.uint64), b["sigma64"].view(np.uint64))
+    assert (a["sim_time"], a["dt"], a["dt_next"], a["step"]) == (b["sim_time"], b["dt"], b["dt_next"], b["step"])
 
 
 @pytest.mark.gpu
@@ -112,3 +116,28 @@ def test_until_sim_time_and_max_steps(gpu_lib, tmp_path):
     code, out = headless("--scene", os.path.join(SCENES, "sod.json"), "--steps", "50", "--max-steps", "10")
     last = out.strip().split("\n")[-1]
     assert code == 2 and last.startswith("SR-HEADLESS DONE steps=10 ") and last.endswith("until=unmet"), out
+
+
+@pytest.mark.parametrize("args, text", [
+    (["--steps", "1", "--dump-every", "0", "--out", "x"], "--dump-every must be at least 1"),
+    (["--steps", "1", "--dump-every", "2"], "needs --out"),
+])
+def test_dump_cadence_arguments_are_exit_4(args, text):
+    """headless.rs:197-202: --dump-every of at least one step, and only with --out."""
+    code, out = headless("--scene", os.path.join(SCENES, "sod.json"), *args)
+    assert code == 4 and out.startswith("SR-ERROR bad arguments") and text in out, out
+
+
+def test_bad_dumps_are_exit_4(tmp_path):
+    """headless.rs:943-957: the dump is read and checked before anything else (no device touched): a missing or cut
+    dump is exit 4 with `SR-ERROR bad dump`, and --world against the dump's world is a bad argument."""
+    scene = os.path.join(SCENES, "heat_rim.json")
+    code, out = headless("--scene", scene, "--steps", "1", "--load-dump", str(tmp_path / "none.bin"))
+    assert code == 4 and out.startswith("SR-ERROR bad dump"), out
+    src = os.path.join(ROOT, "tests", "golden", "dumps", "heat_overflow.bin")
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(open(src, "rb").read()[:-100])
+    code, out = headless("--scene", scene, "--steps", "1", "--load-dump", str(cut))
+    assert code == 4 and out.startswith("SR-ERROR bad dump") and "planes hold" in out, out
+    code, out = headless("--scene", scene, "--steps", "1", "--load-dump", src, "--world", "128x64")
+    assert code == 4 and out.startswith("SR-ERROR bad arguments") and "against the dump's 64x64" in out, out

@@ -166,3 +166,27 @@ def test_host_cold_eos_is_the_oracles():
         assert np.max(np.abs(u / np.array([oracle.cold_table_u(t, x) for x in xs]) - 1.0)) < 1e-12
         assert np.max(np.abs(cold.pressure(xs, k1, k2) / np.array([oracle.cold_pressure(x, k1, k2) for x in xs])
                              - 1.0)) < 1e-13
+
+
+def test_a_resized_world_keeps_the_objects_about_its_centre():
+    """--world over a scene shifts every object by half the change in size (scene.rs:428-435): a disk at the
+    world's centre stays at the new centre, its cells the same values, and a rect keeps its offset from the
+    centre."""
+    s = load("ref_disk.json")
+    base = s.planes().astype(np.float64)
+    s.resize(128, 96)
+    p = s.planes().astype(np.float64)
+    assert p.shape == (14, 96, 128)
+    yy, xx = np.mgrid[0:96, 0:128]
+    m = p[0] - 1e-8
+    cx, cy = (m * (xx + 0.5)).sum() / m.sum(), (m * (yy + 0.5)).sum() / m.sum()
+    assert abs(cx - 64.0) < 1e-9 and abs(cy - 48.0) < 1e-9
+    assert np.array_equal(p[:, 16:80, 32:96], base)
+    r = load("sod.json")  # 400 x 64, the left half's rect from x = 0
+    r.resize(416, 80)
+    q = r.planes()
+    assert np.array_equal(q[:, 8:72, 8:408], load("sod.json").planes())
+    r.resize(400, 64)
+    assert np.array_equal(r.planes(), load("sod.json").planes())
+    with pytest.raises(scene_json.SceneError, match="world size"):
+        r.resize(100, 64)
