@@ -98,6 +98,10 @@ lib.sro_hydro_set_hot_face.argtypes = [C.c_int]
 lib.sro_hydro_set_hot_face.restype = None
 lib.sro_hydro_set_pair_work.argtypes = [C.c_int]
 lib.sro_hydro_set_pair_work.restype = None
+lib.sro_hydro_set_field_bias.argtypes = [C.c_double, C.c_double]
+lib.sro_hydro_set_field_bias.restype = None
+lib.sro_hydro_set_kick_takeback.argtypes = [C.c_int]
+lib.sro_hydro_set_kick_takeback.restype = None
 lib.sro_cold_node_x.argtypes = [C.c_int]
 lib.sro_cold_node_x.restype = C.c_double
 lib.sro_cold_table.argtypes = [C.c_double, C.c_double, _dp]

@@ -14,10 +14,10 @@
  * composition (crates/sr-physics/src/registry.rs:146-154).
  * Held cells, sinks and the pull field are not restated (the path refuses a scene that starts at or above
  * Sigma_hold, floors.rs:200-260).
- * Parity unpinned: gravity's kick take-back to the densest free cell (hydro.rs:997-1001) is restated only; no closed
- * form or output of the reference checks it. The other functions are checked by the tests (closed forms, the
- * reference's own cases: the thin cell for HOT_FACE, the Jeans adiabat for face_work, the radiation-force slab for
- * push_share and the push's take-back).
+ * Each function is checked by the tests: closed forms, conservation, and the reference's own cases (the thin cell for
+ * HOT_FACE, the Jeans adiabat for face_work, the radiation-force slab for push_share and the push's take-back); gravity's
+ * kick take-back (hydro.rs:997-1001) by momentum conservation under a field given an external bias through a test
+ * hook, which fails with the take-back left out or mis-signed.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -111,6 +111,16 @@ void sro_hydro_set_hot_face(int on) { g_hot_face = on; }
  * reference's face_work. */
 static int g_pair_work = 1;
 void sro_hydro_set_pair_work(int on) { g_pair_work = on; }
+/* Test hooks for gravity's kick take-back: a uniform field (bx, by) added to the half step's g, which the solve's
+ * symmetric kernel never makes, so the source terms' net kick dt sum Sigma_src (g + b) is not zero; and the take-back
+ * itself: 1 the reference's (hydro.rs:997-1001), 0 left out, -1 added with the wrong sign (planted mistakes). */
+static double g_bias[2] = {0.0, 0.0};
+static int g_kick_takeback = 1;
+void sro_hydro_set_field_bias(double bx, double by) {
+    g_bias[0] = bx;
+    g_bias[1] = by;
+}
+void sro_hydro_set_kick_takeback(int mode) { g_kick_takeback = mode; }
 
 static prim_t slope(const prim_t *l, const prim_t *c, const prim_t *r) {
     prim_t s;
@@ -514,6 +524,11 @@ int sro_hydro_step(const sro_physics *ph, double *planes, size_t w, size_t h, do
         fgy = (double *)malloc(sizeof(double) * n);
         int e = sro_gravity_field(ph, src, w, h, fphi, fgx, fgy, NULL);
         if (e != SRO_OK) return e;
+        if (g_bias[0] != 0.0 || g_bias[1] != 0.0)
+            for (size_t i = 0; i < n; i++) {
+                fgx[i] += g_bias[0];
+                fgy[i] += g_bias[1];
+            }
         if (sigma_mid_out)
             for (size_t i = 0; i < n; i++) sigma_mid_out[i] = src[i];
     }
@@ -632,9 +647,9 @@ int sro_hydro_step(const sro_physics *ph, double *planes, size_t w, size_t h, do
             }
     }
     /* gravity's net kick taken back from the densest free cell (hydro.rs:997-1001) */
-    if (densest >= 0) {
-        next[n + densest] -= kick_net[0];
-        next[2 * n + densest] -= kick_net[1];
+    if (densest >= 0 && g_kick_takeback != 0) {
+        next[n + densest] -= g_kick_takeback * kick_net[0];
+        next[2 * n + densest] -= g_kick_takeback * kick_net[1];
     }
     memcpy(planes, next, sizeof(double) * NC * n);
     free(next);

@@ -66,6 +66,9 @@ double sro_eps_cold(const sro_physics *ph, double sigma, double y_e, double x_n)
 /* test switch: the hydro's HOT_FACE bracket on (default) or off */
 void sro_hydro_set_hot_face(int on);
 void sro_hydro_set_pair_work(int on);
+/* test hooks: a uniform field added to the half step's g; gravity's kick take-back 1 (on), 0 (off), -1 (mis-signed) */
+void sro_hydro_set_field_bias(double bx, double by);
+void sro_hydro_set_kick_takeback(int mode);
 
 /* ---------------- FFT and convolution (reference crates/sr-physics/src/reference/fft.rs) ---------------- */
 int sro_padded_size(size_t len, size_t max_padded, size_t *out);

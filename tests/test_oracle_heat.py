@@ -228,3 +228,21 @@ def test_lp_limiter_formula():
         k = ph.kappa0 * 2.0 * 0.01
         R = abs(ub - ua) / (ph.dx * k * 0.5 * (ua + ub))
         assert fx[0, 1] == pytest.approx(ph.c_sb * lam(R) * (ua - ub) / (k * ph.dx), rel=1e-12)
+
+
+@pytest.mark.parametrize("T0, sigma", [(1.0, 1.0), (2.0, 1.0), (1.0, 2.0), (0.5, 0.5)])
+def test_thick_gas_limit_with_kramers_opacity(T0, sigma):
+    """The Kramers law every workload runs (kappa = kappa0 Sigma T^-3.5, opacity 1): in the thick limit (R -> 0,
+    lambda -> 1/3) D_f = c_sb / (3 kappa Sigma) = c_sb / (3 kappa0 Sigma^2 T^-3.5), so
+    F = (4/3) a_r c_sb T^3 / (kappa0 Sigma^2 T^-3.5) (-dT/dx). Kramers gas is thick where it is cool and dense
+    (kappa Sigma >= 0.09 per cell here, R <= 0.03); two temperatures and two columns at 1 %: a wrong power of T (off by
+    1/2 moves F by 41 % between T = 1 and 2) or of Sigma (a factor 2 between 1 and 2) fails."""
+    ph = oracle.reference_physics(edge=oracle.EDGE_BOUNCE, opacity=1, kappa0=1.0)
+    p, T = law_strip(T0, T0 * 1.01, sigma)
+    fx, _ = oracle.heat_law(ph, p)
+    i = 8
+    Tf = 0.5 * (T[0, i - 1] + T[0, i])
+    kappa_sigma = ph.kappa0 * sigma * sigma * Tf ** -3.5
+    dTdx = (T[0, i] - T[0, i - 1]) / ph.dx
+    expect = -(4.0 / 3.0) * ph.a_r * ph.c_sb * Tf ** 3 / kappa_sigma * dTdx
+    assert fx[0, i] == pytest.approx(expect, rel=1e-2)

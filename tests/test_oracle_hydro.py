@@ -150,3 +150,130 @@ def test_leave_edge_books_escaped_mass():
     esc = tw.escaped()[0].sum()
     assert m1 < m0
     assert abs((m1 + esc) - m0) <= 1e-12 * m0
+
+
+# ---------------------------------------------------------------------------------------------------- gamma = 5/3
+# Every workload runs at gamma = 5/3 where the reference hard-wires 2 (crates/sr-physics/src/eos.rs:207-247), so the
+# oracle's gamma path is pinned on its own: the same Sod tube with E = Pi / (gamma - 1), against the exact solution at
+# gamma = 5/3. Its shock stands 9 cells behind gamma = 2's at t = 80, so a gamma of 2 left anywhere in a sound speed or
+# a conversion fails the one-cell bar.
+G53 = 5.0 / 3.0
+
+
+def strip_53(w=400, h=4):
+    p = scenes.sod_scene(w, h)
+    p[3] = p[3] / (G53 - 1.0)
+    return p
+
+
+@pytest.fixture(scope="module")
+def run_80_53():
+    ph = oracle.reference_physics(edge=oracle.EDGE_BOUNCE, gamma=G53)
+    tw = oracle.Twin(ph, strip_53(), gravity=False, hydro=True, heat=False)
+    while True:
+        _, dtn, st, _ = tw.times()
+        if st >= T_END * (1 - 1e-15):
+            break
+        if st + dtn > T_END:
+            tw.cap_dt_next(T_END - st)
+        tw.step()
+    return tw.state()
+
+
+def test_the_gammas_are_told_apart():
+    a = Solution((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 2.0).right_shock_speed() * T_END
+    b = Solution((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), G53).right_shock_speed() * T_END
+    assert a - b > 5.0
+
+
+def test_sod_at_five_thirds_shock_within_one_cell(run_80_53):
+    sol = Solution((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), G53)
+    rho = run_80_53[0, 0]
+    _, rr = sol.star_densities()
+    half = 0.5 * (rr + 0.125)
+    i = len(rho) - 1
+    while rho[i] < half:
+        i -= 1
+    x_num = i + 0.5 + (rho[i] - half) / (rho[i] - rho[i + 1])
+    assert abs(x_num - (X0 + sol.right_shock_speed() * T_END)) <= 1.0
+
+
+def test_sod_at_five_thirds_l1_and_star_state(run_80_53):
+    sol = Solution((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), G53)
+    ex = exact_cell_density(sol, 400, T_END)
+    rho = run_80_53[0, 0]
+    assert np.abs(rho - ex).sum() / ex.sum() <= 0.02
+    # on the plateau between the contact and the shock: u* and p* = (gamma - 1) (E - Sigma u^2 / 2)
+    x_c, x_s = X0 + sol.u_star * T_END, X0 + sol.right_shock_speed() * T_END
+    plateau = slice(int(x_c) + 6, int(x_s) - 4)
+    u = run_80_53[1, 0, plateau] / rho[plateau]
+    p = (G53 - 1.0) * (run_80_53[3, 0, plateau] - 0.5 * rho[plateau] * u * u)
+    assert np.max(np.abs(u - sol.u_star)) <= 0.01 * sol.u_star
+    assert np.max(np.abs(p - sol.p_star)) <= 0.01 * sol.p_star
+
+
+def test_sod_at_five_thirds_conservation_and_wall_impulse(run_80_53):
+    start = strip_53().sum(axis=(1, 2))
+    end = run_80_53.sum(axis=(1, 2))
+    assert abs(end[0] - start[0]) <= 1e-12 * start[0]
+    assert abs(end[3] - start[3]) <= 1e-12 * start[3]
+    pushed = (1.0 - 0.1) * T_END * 4  # the walls push with the initial pressures, whatever gamma
+    assert abs(end[1] - pushed) <= 1e-10 * pushed
+
+
+# ------------------------------------------------------------------------------------------ gravity's kick take-back
+# hydro.rs:860-909, :997-1001: the source terms dt Sigma_src g_mid push the grid by their sum, which a symmetric kernel
+# keeps at round-off; the step takes that net kick back from the densest free cell, so the grid's momentum is what the
+# fluxes alone make it. A uniform field added to g_mid through the oracle's test hook makes the net kick
+# dt b sum Sigma_src, far above round-off: momentum (grid + escaped) must still stand still, the two runs with and
+# without the take-back differ at the densest cell's momentum only and by exactly the net kick, and the runs with the
+# take-back left out or mis-signed fail the balance.
+def kick_scene(n=64):
+    yy, xx = np.mgrid[0:n, 0:n] + 0.5
+    # two unequal blobs, the gas well inside the box (Sigma below sigma_vac from 20 cells off the edges)
+    rho = 1e-8 + np.exp(-((xx - 28) ** 2 + (yy - 30) ** 2) / (2 * 3.0 ** 2)) \
+        + 0.5 * np.exp(-((xx - 38) ** 2 + (yy - 35) ** 2) / (2 * 2.5 ** 2))
+    return scenes.planes_from(rho, np.ones_like(rho), gamma=2.0, mu=0.5)
+
+
+def kicked_step(mode, bias=(0.3, -0.2)):
+    ph = oracle.reference_physics(edge=oracle.EDGE_LEAVE)
+    p = kick_scene()
+    _, gx, gy = oracle.gravity_field(ph, p[0])
+    phi = oracle.gravity_field(ph, p[0], ringed=True)[0]
+    dt = oracle.hydro_dt(ph, p, gx, gy)
+    esc = np.zeros((14, 2 * (64 + 64)))
+    oracle.lib.sro_hydro_set_field_bias(*bias)
+    oracle.lib.sro_hydro_set_kick_takeback(mode)
+    try:
+        q, mid = oracle.hydro_step(ph, p, dt, gravity=(phi, gx, gy), escaped=esc, want_mid=True)
+    finally:
+        oracle.lib.sro_hydro_set_field_bias(0.0, 0.0)
+        oracle.lib.sro_hydro_set_kick_takeback(1)
+    return p, q, esc, dt, mid
+
+
+def test_gravity_net_kick_is_taken_back_at_the_densest_cell():
+    p, on, esc_on, dt, mid = kicked_step(1)
+    _, off, esc_off, _, _ = kicked_step(0)
+    bias = np.array([0.3, -0.2])
+    net = dt * bias * np.where(mid >= 1e-6, mid, 0.0).sum()  # the bias's share of the net kick
+    bar = 1e-12 * (np.abs(on[1]).sum() + np.abs(on[2]).sum())  # round-off of the gross momentum after the step
+    for k in (1, 2):  # momentum: grid + escaped stands still with the take-back, to f64 round-off
+        assert abs(on[k].sum() + esc_on[k].sum() - p[k].sum()) <= bar
+        # without it, the grid gains the whole net kick, eleven orders of magnitude above that
+        gain = off[k].sum() + esc_off[k].sum() - p[k].sum()
+        assert gain == pytest.approx(net[k - 1], rel=1e-9)
+        assert abs(gain) > 1e9 * bar
+    d = on - off
+    i = np.unravel_index(np.argmax(np.where(mid >= 1e-6, mid, -1.0)), mid.shape)
+    nz = np.argwhere(d != 0.0)
+    assert {tuple(x) for x in nz} <= {(1, *i), (2, *i)}
+    assert d[1][i] == pytest.approx(-net[0], rel=1e-9) and d[2][i] == pytest.approx(-net[1], rel=1e-9)
+    assert np.array_equal(on[[0, 3]], off[[0, 3]])  # E as it is (hydro.rs:997)
+
+
+def test_a_mis_signed_take_back_fails_the_balance():
+    p, bad, esc, dt, mid = kicked_step(-1)
+    net = dt * 0.3 * np.where(mid >= 1e-6, mid, 0.0).sum()
+    assert bad[1].sum() + esc[1].sum() - p[1].sum() == pytest.approx(2.0 * net, rel=1e-9)
