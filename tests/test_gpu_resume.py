@@ -93,6 +93,13 @@ def test_the_world_heat_system_is_the_oracles_and_its_preconditioner_symmetric(g
     assert np.array_equal(gas, o_gas) and 0 < gas.sum() < gas.size
     g = gas == 1
     assert np.max(np.abs(own[g] - o_own[g]) / o_own[g]) <= 1e-6
+    # the operator's couplings are the faces between two gas cells (a vacuum face's D_f sits in own, as the
+    # reference's HeatSystem keeps fx/fy beside `vac`, step/heat.rs:620-637); the oracle's network holds every face
+    gx, gy = np.zeros_like(o_kx, bool), np.zeros_like(o_ky, bool)
+    gx[:, 1:-1] = g[:, :-1] & g[:, 1:]
+    gy[1:-1, :] = g[:-1, :] & g[1:, :]
+    o_kx, o_ky = np.where(gx, o_kx, 0.0), np.where(gy, o_ky, 0.0)
+    assert np.any(kx[~gx] == 0.0) and np.all(kx[~gx] == 0.0) and np.all(ky[~gy] == 0.0)
     assert np.max(np.abs(kx - o_kx)) <= 1e-6 * np.max(o_kx) and np.max(np.abs(ky - o_ky)) <= 1e-6 * np.max(o_ky)
     rng = np.random.default_rng(4)
     u, v = (np.where(g, x, 0.0).astype(np.float32) for x in rng.standard_normal((2, 64, 64)))
