@@ -109,6 +109,16 @@ typedef struct {
     double pass_ms[10];           /* per pass, the reference's Pass::ORDER slots P0..P9 (step/mod.rs:200-219) */
 } sr_step_cost;
 
+/* What the steps since the upload (or the last sr_read_work with reset) did, counted on the host as the launches are
+ * made, and the HBM bytes their algorithm moves (each operand plane once per kernel, f32 4 B, f64 8 B): the
+ * numerator of the full step's roofline. No counterpart in the reference (its read_cost gives times only). */
+typedef struct {
+    uint64_t steps, hydro_steps, convolutions, heat_steps, newton_passes;
+    uint64_t pcg_iterations, precond_applications, true_residuals, hierarchy_builds;
+    double algorithmic_bytes;
+    double bytes_p2, bytes_p4, bytes_p5, bytes_p89;  /* the same by pass */
+} sr_step_work;
+
 typedef struct sr_world sr_world;
 typedef struct sr_gravity sr_gravity;
 typedef struct sr_fld sr_fld;
@@ -248,6 +258,8 @@ int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
 int sr_set_dt_cap(sr_world *wd, double cap);                  /* Twin::cap_dt_next (reference/mod.rs:225) */
 int sr_set_timing(sr_world *wd, int on);                      /* Step::set_timing (step/mod.rs:722) */
 int sr_read_cost(sr_world *wd, sr_step_cost *out);            /* Step::read_cost (step/mod.rs:753) */
+/* the work counters above; reset != 0 zeroes them after reading */
+int sr_read_work(sr_world *wd, sr_step_work *out, int reset);
 void sr_destroy(sr_world *wd);
 
 /* Timing of the dominant kernel inside the last sr_run with timing on: launches and mean device ms per launch,

@@ -111,15 +111,38 @@ lib.sro_eps_cold.argtypes = [_PP, C.c_double, C.c_double, C.c_double]
 lib.sro_eps_cold.restype = C.c_double
 lib.sro_pi_cold.argtypes = [_PP, C.c_double, C.c_double, C.c_double]
 lib.sro_pi_cold.restype = C.c_double
-lib.sro_twin_new.argtypes = [_PP, _sz, _sz, _dp, C.c_int, C.c_int, C.c_int]
-lib.sro_twin_new.restype = C.c_void_p
-lib.sro_twin_free.argtypes = [C.c_void_p]
-lib.sro_twin_step.argtypes = [C.c_void_p]
-lib.sro_twin_state.argtypes = [C.c_void_p, _dp]
-lib.sro_twin_times.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_uint64)]
-lib.sro_twin_cap_dt_next.argtypes = [C.c_void_p, C.c_double]
-lib.sro_twin_last_heat.argtypes = [C.c_void_p, C.POINTER(HeatReport)]
-lib.sro_twin_escaped.argtypes = [C.c_void_p, _dp]
+
+
+def _declare_twin(l):
+    l.sro_twin_new.argtypes = [_PP, _sz, _sz, _dp, C.c_int, C.c_int, C.c_int]
+    l.sro_twin_new.restype = C.c_void_p
+    l.sro_twin_free.argtypes = [C.c_void_p]
+    l.sro_twin_step.argtypes = [C.c_void_p]
+    l.sro_twin_state.argtypes = [C.c_void_p, _dp]
+    l.sro_twin_times.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_uint64)]
+    l.sro_twin_cap_dt_next.argtypes = [C.c_void_p, C.c_double]
+    l.sro_twin_last_heat.argtypes = [C.c_void_p, C.POINTER(HeatReport)]
+    l.sro_twin_escaped.argtypes = [C.c_void_p, _dp]
+    l.sro_heat_set_mg.argtypes = [C.c_double]
+    l.sro_heat_set_mg.restype = None
+
+
+_declare_twin(lib)
+_OMP = None
+
+
+def omp_lib():
+    """liboracle_omp.so: the same restatement built with OpenMP (loops over rows and cells shared out over the host's
+    threads; sums in another order, so its results differ from liboracle.so's in the last bits). For timing only: the
+    tests check against liboracle.so."""
+    global _OMP
+    if _OMP is None:
+        path = os.path.join(HERE, "liboracle_omp.so")
+        if not os.path.exists(path):
+            _build()
+        _OMP = C.CDLL(path)
+        _declare_twin(_OMP)
+    return _OMP
 lib.sro_lcg_fill.argtypes = [C.c_uint64, _sz, _dp]
 
 
@@ -317,40 +340,41 @@ def lcg(seed: int, count: int) -> np.ndarray:
 class Twin:
     """The twin's step driver (mod.rs:129-297) over f64 planes shaped (14, h, w)."""
 
-    def __init__(self, ph: Physics, planes: np.ndarray, gravity=True, hydro=True, heat=False):
+    def __init__(self, ph: Physics, planes: np.ndarray, gravity=True, hydro=True, heat=False, library=None):
         _, self.h, self.w = planes.shape
+        self._l = library if library is not None else lib
         p = np.ascontiguousarray(planes, dtype=np.float64)
-        self._t = lib.sro_twin_new(C.byref(ph), self.w, self.h, p, int(gravity), int(hydro), int(heat))
+        self._t = self._l.sro_twin_new(C.byref(ph), self.w, self.h, p, int(gravity), int(hydro), int(heat))
 
     def step(self, n=1):
         for _ in range(n):
-            _check(lib.sro_twin_step(self._t))
+            _check(self._l.sro_twin_step(self._t))
 
     def state(self) -> np.ndarray:
         out = np.empty((COMPONENTS, self.h, self.w))
-        lib.sro_twin_state(self._t, out)
+        self._l.sro_twin_state(self._t, out)
         return out
 
     def times(self):
         dt, dtn, st = C.c_double(), C.c_double(), C.c_double()
         steps = C.c_uint64()
-        lib.sro_twin_times(self._t, C.byref(dt), C.byref(dtn), C.byref(st), C.byref(steps))
+        self._l.sro_twin_times(self._t, C.byref(dt), C.byref(dtn), C.byref(st), C.byref(steps))
         return dt.value, dtn.value, st.value, steps.value
 
     def cap_dt_next(self, cap):
-        lib.sro_twin_cap_dt_next(self._t, cap)
+        self._l.sro_twin_cap_dt_next(self._t, cap)
 
     def last_heat(self) -> HeatReport:
         r = HeatReport()
-        lib.sro_twin_last_heat(self._t, C.byref(r))
+        self._l.sro_twin_last_heat(self._t, C.byref(r))
         return r
 
     def escaped(self) -> np.ndarray:
         out = np.zeros((COMPONENTS, 2 * (self.w + self.h)))
-        lib.sro_twin_escaped(self._t, out)
+        self._l.sro_twin_escaped(self._t, out)
         return out
 
     def __del__(self):
         if getattr(self, "_t", None):
-            lib.sro_twin_free(self._t)
+            self._l.sro_twin_free(self._t)
             self._t = None

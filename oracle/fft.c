@@ -74,20 +74,25 @@ int sro_fft(double *d, size_t n, int inverse) {
 
 int sro_fft2(double *d, size_t nx, size_t ny, int inverse) {
     if (!is_pow2(nx) || !is_pow2(ny)) return SRO_ERR_POW2;
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y < ny; y++) sro_fft(d + 2 * y * nx, nx, inverse);
-    double *col = (double *)malloc(sizeof(double) * 2 * ny);
-    for (size_t x = 0; x < nx; x++) {
-        for (size_t y = 0; y < ny; y++) {
-            col[2 * y] = d[2 * (y * nx + x)];
-            col[2 * y + 1] = d[2 * (y * nx + x) + 1];
+#pragma omp parallel
+    {
+        double *col = (double *)malloc(sizeof(double) * 2 * ny);
+#pragma omp for schedule(static)
+        for (size_t x = 0; x < nx; x++) {
+            for (size_t y = 0; y < ny; y++) {
+                col[2 * y] = d[2 * (y * nx + x)];
+                col[2 * y + 1] = d[2 * (y * nx + x) + 1];
+            }
+            sro_fft(col, ny, inverse);
+            for (size_t y = 0; y < ny; y++) {
+                d[2 * (y * nx + x)] = col[2 * y];
+                d[2 * (y * nx + x) + 1] = col[2 * y + 1];
+            }
         }
-        sro_fft(col, ny, inverse);
-        for (size_t y = 0; y < ny; y++) {
-            d[2 * (y * nx + x)] = col[2 * y];
-            d[2 * (y * nx + x) + 1] = col[2 * y + 1];
-        }
+        free(col);
     }
-    free(col);
     return SRO_OK;
 }
 
@@ -106,12 +111,14 @@ int sro_convolve_ringed(const double *field, size_t w, size_t h, size_t ring, sr
         free(p);
         return SRO_ERR_ARG;
     }
+#pragma omp parallel for schedule(static)
     for (size_t j = 0; j < py; j++)
         for (size_t i = 0; i < px; i++) k[2 * (j * px + i)] = kern(signed_offset(i, px), signed_offset(j, py), ctx);
     sro_fft2(k, px, py, 0);
     for (size_t y = 0; y < h; y++)
         for (size_t x = 0; x < w; x++) p[2 * (y * px + x)] = field[y * w + x];
     sro_fft2(p, px, py, 0);
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < px * py; i++) {
         double ar = p[2 * i], ai = p[2 * i + 1], br = k[2 * i], bi = k[2 * i + 1];
         p[2 * i] = ar * br - ai * bi;

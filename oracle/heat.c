@@ -30,6 +30,12 @@
 #define NEWTON_CELL 1e-10
 #define SOLVE_TOL 1e-12
 
+/* the linear solver of Heat::step: 0 (the reference's Jacobi PCG under its energy-weighted rule, heat.rs:413-447), or
+ * > 0: multigrid PCG (sro_fld_pcg, V(2,2)) to this relative residual, so that the twin finishes on grids where Jacobi
+ * would take tens of thousands of rounds (a test-side switch; the reference has none) */
+static double g_mg_rtol = 0.0;
+void sro_heat_set_mg(double rtol) { g_mg_rtol = rtol > 0.0 ? rtol : 0.0; }
+
 typedef struct {
     int gas;
     double t, sigma, kappa, inv_mu;
@@ -137,6 +143,7 @@ static double face_coef(const sro_physics *ph, const side_t *a, const side_t *b)
 static side_t *network(const sro_physics *ph, const double *planes, size_t w, size_t h, double vac, net_t *net) {
     size_t n = w * h;
     side_t *sides = (side_t *)malloc(sizeof(side_t) * n);
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; i++) sides[i] = side_of(ph, planes, n, i);
     net->w = w;
     net->h = h;
@@ -149,10 +156,12 @@ static side_t *network(const sro_physics *ph, const double *planes, size_t w, si
         net->gas[i] = (uint8_t)sides[i].gas;
         net->c[i] = cap(ph, &sides[i]);
     }
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y < h; y++)
         for (size_t x = 0; x <= w; x++)
             net->kx[y * (w + 1) + x] =
                 face_coef(ph, x > 0 ? &sides[y * w + x - 1] : NULL, x < w ? &sides[y * w + x] : NULL);
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y <= h; y++)
         for (size_t x = 0; x < w; x++)
             net->ky[y * w + x] =
@@ -451,6 +460,7 @@ int sro_heat_step(const sro_physics *ph, double *planes, size_t w, size_t h, dou
     while (passes < MAX_PASSES && rounds < MAX_ROUNDS) {
         passes++;
         double eps2 = SOLVE_TOL * SOLVE_TOL, ub = 0, tb = 0;
+#pragma omp parallel for schedule(static) reduction(+ : ub, tb)
         for (size_t i = 0; i < n; i++) {
             if (net.gas[i]) {
                 heat[i] = net.c[i] / dt;
@@ -473,7 +483,18 @@ int sro_heat_step(const sro_physics *ph, double *planes, size_t w, size_t h, dou
             passes--;
             break;
         }
-        int r = sys_solve(&sys, x, MAX_ROUNDS - rounds);
+        int r;
+        if (g_mg_rtol > 0.0) {
+            /* the same system in face form: own = diag - the gas-gas couplings (kvac stays in own) */
+            double *own = (double *)malloc(sizeof(double) * n), rel = 0.0;
+            for (size_t i = 0; i < n; i++) own[i] = net.gas[i] ? diag[i] - (ksum[i] - kvac[i]) : 1.0;
+            int it = sro_fld_pcg(w, h, own, net.kx, net.ky, net.gas, rhs, x, g_mg_rtol, MAX_ROUNDS - rounds, 1, 0, 2,
+                                 2, &rel);
+            free(own);
+            r = (it < 0 || !isfinite(rel)) ? (it < 0 ? it : -1) : it; /* a failed iteration k counts k - 1 rounds */
+        } else {
+            r = sys_solve(&sys, x, MAX_ROUNDS - rounds);
+        }
         if (r < 0) {
             rounds += -r - 1;
             break;
@@ -481,6 +502,7 @@ int sro_heat_step(const sro_physics *ph, double *planes, size_t w, size_t h, dou
         rounds += r;
         double moved = 0;
         int every = 1;
+#pragma omp parallel for schedule(static) reduction(+ : moved) reduction(& : every)
         for (size_t i = 0; i < n; i++) {
             if (!net.gas[i]) continue;
             double nx = t_of(ph, fmax(x[i], u_vac));

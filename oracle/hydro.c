@@ -416,6 +416,7 @@ int sro_hydro_step(const sro_physics *ph, double *planes, size_t w, size_t h, do
     face_t(*faces)[4] = (face_t(*)[4])malloc(sizeof(face_t) * 4 * n);
     double(*mids)[3] = (double(*)[3])malloc(sizeof(double) * 3 * n);
     /* gravity at a ghosted index: a ghost takes its edge cell's */
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y < h; y++)
         for (size_t x = 0; x < w; x++) {
             size_t i = y * w + x;
@@ -429,6 +430,7 @@ int sro_hydro_step(const sro_physics *ph, double *planes, size_t w, size_t h, do
     /* boundary flux of an edge cell on side s */
     double(*fx)[NC] = (double(*)[NC])calloc((w + 1) * h, sizeof(double[NC]));
     double(*fy)[NC] = (double(*)[NC])calloc(w * (h + 1), sizeof(double[NC]));
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y < h; y++)
         for (size_t i = 0; i <= w; i++) {
             double *f = fx[y * (w + 1) + i];
@@ -460,6 +462,7 @@ int sro_hydro_step(const sro_physics *ph, double *planes, size_t w, size_t h, do
                 hll(ph, &faces[y * w + i - 1][1], &faces[y * w + i][0], 0, f);
             }
         }
+#pragma omp parallel for schedule(static)
     for (size_t j = 0; j <= h; j++)
         for (size_t x = 0; x < w; x++) {
             double *f = fy[j * w + x];

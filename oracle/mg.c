@@ -116,6 +116,7 @@ static void hier_free(hier_t *H) {
 
 static void level_matvec(const level_t *l, const double *v, double *out) {
     size_t w = l->w, h = l->h;
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y < h; y++)
         for (size_t x = 0; x < w; x++) {
             size_t i = y * w + x;
@@ -132,6 +133,7 @@ static void level_matvec(const level_t *l, const double *v, double *out) {
 /* one half sweep of colour c: z_i = (b_i + sum a z_j) / (own_i + sum a) */
 static void half_sweep(level_t *l, int color) {
     size_t w = l->w, h = l->h;
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y < h; y++)
         for (size_t x = (y + color) & 1; x < w; x += 2) {
             size_t i = y * w + x;
@@ -205,16 +207,21 @@ static void vcycle(hier_t *H, int L, int nu1, int nu2) {
         half_sweep(l, 1);
     }
     level_matvec(l, l->z, l->r);
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; i++) l->r[i] = l->b[i] - l->r[i];
     level_t *c = &H->lv[L + 1];
     memset(c->b, 0, sizeof(double) * c->w * c->h);
-    for (size_t y = 0; y < l->h; y++)
-        for (size_t x = 0; x < l->w; x++) {
-            size_t i = y * l->w + x;
-            if (L == 0 && H->gas && !H->gas[i]) continue;
-            c->b[(y / 2) * c->w + x / 2] += l->r[i];
-        }
+    /* by coarse rows (each coarse cell sums its fine cells in row order, whatever the threads) */
+#pragma omp parallel for schedule(static)
+    for (size_t Y = 0; Y < c->h; Y++)
+        for (size_t y = 2 * Y; y < 2 * Y + 2 && y < l->h; y++)
+            for (size_t x = 0; x < l->w; x++) {
+                size_t i = y * l->w + x;
+                if (L == 0 && H->gas && !H->gas[i]) continue;
+                c->b[Y * c->w + x / 2] += l->r[i];
+            }
     vcycle(H, L + 1, nu1, nu2);
+#pragma omp parallel for schedule(static)
     for (size_t y = 0; y < l->h; y++)
         for (size_t x = 0; x < l->w; x++) {
             size_t i = y * l->w + x;
@@ -274,6 +281,7 @@ int sro_mg_apply(size_t w, size_t h, const double *own, const double *kx, const 
 
 static double dot(const double *a, const double *b, size_t n) {
     double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
     for (size_t i = 0; i < n; i++) s += a[i] * b[i];
     return s;
 }

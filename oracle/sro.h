@@ -104,6 +104,8 @@ typedef struct {
     double delta, radiated;
 } sro_heat_report;
 /* one implicit P5 step in place; flux_x (w+1)h and flux_y w(h+1) outputs (may be NULL), emitted w*h (may be NULL) */
+/* the twin's heat solver: 0 the reference's Jacobi PCG rule, > 0 multigrid PCG to that relative residual */
+void sro_heat_set_mg(double rtol);
 int sro_heat_step(const sro_physics *ph, double *planes, size_t w, size_t h, double dt, double *flux_x,
                   double *flux_y, double *emitted, sro_heat_report *rep);
 /* the law on the start state (heat.rs:307) */

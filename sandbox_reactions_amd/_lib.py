@@ -64,6 +64,13 @@ class StepCost(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("step_ms", C.c_double), ("pass_ms", C.c_double * 10)]
 
 
+class StepWork(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("steps", "hydro_steps", "convolutions", "heat_steps", "newton_passes",
+                                          "pcg_iterations", "precond_applications", "true_residuals",
+                                          "hierarchy_builds")] + \
+               [(k, C.c_double) for k in ("algorithmic_bytes", "bytes_p2", "bytes_p4", "bytes_p5", "bytes_p89")]
+
+
 class ResumeRecords(C.Structure):
     _fields_ = [("dt", C.c_double), ("dt_next", C.c_double), ("sim_time", C.c_double)]
 
@@ -106,6 +113,7 @@ SIGNATURES = [
     ("sr_set_dt_cap", _i, [_vp, _d]),
     ("sr_set_timing", _i, [_vp, _i]),
     ("sr_read_cost", _i, [_vp, C.POINTER(StepCost)]),
+    ("sr_read_work", _i, [_vp, C.POINTER(StepWork), _i]),
     ("sr_destroy", None, [_vp]),
     ("sr_kernel_time", _i, [_vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_d), C.POINTER(_d)]),
     ("sr_fld_set_network", _i, [_vp, _vp, _i]),

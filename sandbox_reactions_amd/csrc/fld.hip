@@ -894,6 +894,10 @@ CoarseSet FldSolver::coarse_set() const {
 }
 
 int FldSolver::rebuild(bool faces, hipStream_t s) {
+    if (faces)
+        work.builds_faces++;
+    else
+        work.builds_own++;
     for (size_t l = 0; l + 1 < lv.size(); l++) {
         const FldLevel &f = lv[l], &c = lv[l + 1];
         Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
@@ -1220,6 +1224,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     for (int outer = 0; outer < 50; outer++) {
         hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, x, r32, ref ? ref->heat : nullptr,
                            ref ? ref->slope : nullptr, part_true);
+        work.true_residuals++;
         double tr[4];
         SR_TRY(read_partials(part_true, 4, nbe, tr, s));
         const double rr = tr[0], bb = tr[1] > 0 ? tr[1] : 1.0;
@@ -1246,6 +1251,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         int k = 0, nbd = nbe, nbp = nbe;
         // z = M r, partials of r.z (restart: beta = 0)
         SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
+        work.precond++;
         int last_check = it, interval = 1;
         double rr_last = rr;
         if (use_net && net_it == 0) rr_net0 = rr;
@@ -1255,6 +1261,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
             hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, x, r32, dots[k & 1], nbd,
                                part_pq, nmv, part_rr);
             it++;
+            work.iterations++;
             if (use_net) net_it++;
             if (it - last_check >= interval || it >= max_iter) {
                 double rri;
@@ -1290,6 +1297,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
             k++;
             nbp = nbd;
             SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
+            work.precond++;
         }
         if (!st->finite) break;
     }
@@ -1308,6 +1316,18 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
 double FldSolver::bytes_per_iteration() const {
     const double n = (double)w0 * h0;
     return n * (28.0 + 32.0 + (21.0 + 33.0) * 4.0 / 3.0);
+}
+
+// Per fine cell: an iteration's matvec and update (28 + 32); a preconditioner application's V-cycle, level 0's pre- and
+// post-smoother with a third more for the levels below ((21 + 33) 4/3); the FP64 true residual (b, x f64 in, r f64
+// and r32 out, own/ax/ay in: 44); a hierarchy build (each coarsening reads level l's own, ax, ay and writes level
+// l + 1's: 12 (1 + 1/4) per fine cell with the faces, own alone 5; the smoothers' packed records read own, ax, ay and
+// write the 16-byte record: 28; a third more for the levels below).
+double FldSolver::work_bytes(const FldWork &w) const {
+    const double n = (double)w0 * h0;
+    const double vcycle = (21.0 + 33.0) * 4.0 / 3.0;
+    return n * ((28.0 + 32.0) * w.iterations + vcycle * w.precond + 44.0 * w.true_residuals +
+                (12.0 * 1.25 + 28.0) * 4.0 / 3.0 * w.builds_faces + (5.0 * 1.25 + 28.0) * 4.0 / 3.0 * w.builds_own);
 }
 
 } // namespace sr

@@ -40,6 +40,11 @@ struct RefStop {
     double u_bound = 0.0, t_bound = 0.0;
 };
 
+// launches counted as they are made (the work counters behind sr_read_work)
+struct FldWork {
+    unsigned long long iterations = 0, precond = 0, true_residuals = 0, builds_faces = 0, builds_own = 0;
+};
+
 class FldSolver {
   public:
     ~FldSolver();
@@ -70,6 +75,10 @@ class FldSolver {
     int width() const { return w0; }
     int height() const { return h0; }
     double bytes_per_iteration() const;
+    // algorithmic HBM bytes of the counted work (level 0 and the hierarchy; per fine cell as the kernels' comments
+    // state)
+    double work_bytes(const FldWork &w) const;
+    FldWork work;
     // kernel timing of the level-0 pre-smoother: event pairs recorded on the stream without waiting, summed when
     // read (smooth_time), so timing does not stall the solve
     void set_timing(bool on) { timing = on; }

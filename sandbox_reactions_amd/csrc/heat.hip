@@ -344,6 +344,18 @@ int HeatPass::read_sums(int k, double *out, hipStream_t s) {
     return SR_OK;
 }
 
+// Per cell: the cells pass (the state in: Sigma f64 + 13 f32 planes; gas, T0 and T f64, kappa, 1/mu, C, Sigma out:
+// 60 + 1 + 24 + 16 = 101), the face law (both cells' T, Sigma, kappa in per face pair, kx/ky and the solver's ax/ay
+// out: 16 + 16 = 32) and the vacuum sums (gas, kx, ky in, kvac out: 13); each Newton pass's system (T, T0 f64, C,
+// kvac, gas in; own, heat, slope f32 and rhs, x f64 out: 25 + 28 = 53) and update (x, T f64, C, gas in; T out: 29);
+// the commit (its sums: T, T0 f64, C, kvac, x, gas: 33; the commit: kx, ky, T, gas, E in; E, F_x, F_y out: 29 + 12)
+// or the limited commit (the scale: 29; the limited fluxes and E: 45).
+double HeatPass::work_bytes(const HeatWork &w) const {
+    const double n = (double)P.w * P.h;
+    return n * ((101.0 + 32.0 + 13.0) * w.steps + (53.0 + 29.0) * w.passes + (33.0 + 41.0) * w.commits +
+                (29.0 + 45.0) * w.limited);
+}
+
 int HeatPass::first_system(const double *scal, hipStream_t s) {
     const size_t n = (size_t)P.w * P.h;
     SR_HIP(hipMemcpyAsync(H.t, H.t0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -361,6 +373,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     report.dt = dt_host;
     report.converged = 1;
     hipLaunchKernelGGL(k_heat_cells, dim3(g), dim3(NT), 0, s, P, st.sigma, st.mx, st.my, st.e, st.x, H, part);
+    work.steps++;
     double heat0 = 0.0;
     SR_TRY(read_sums(1, &heat0, s));
     const int nf = std::max((P.w + 1) * P.h, P.w * (P.h + 1));
@@ -382,6 +395,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     double eta = inexact ? std::max(hs.solve_tol, 1e-2) : hs.solve_tol;
     while (passes < hs.max_passes && rounds < hs.max_rounds) {
         passes++;
+        work.passes++;
         hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
         // the reference's two bounds are needed (and read back) only under its stopping rule
         RefStop ref;
@@ -423,6 +437,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     report.final_residual = last_rel;
     if (converged) {
         hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, part);
+        work.commits++;
         double c3[3];
         SR_TRY(read_sums(3, c3, s));
         const double delta = c3[2] != 0.0 ? -(c3[1] + c3[0]) / c3[2] : 0.0;
@@ -433,6 +448,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
         report.converged = 1;
     } else {
         hipLaunchKernelGGL(k_heat_scale, dim3((n + 255) / 256), dim3(256), 0, s, P, H, scal);
+        work.limited++;
         hipLaunchKernelGGL(k_heat_limited, dim3(g), dim3(NT), 0, s, P, H, scal, st.e, fx, fy, part);
         SR_HIP(hipGetLastError());
         double r1;

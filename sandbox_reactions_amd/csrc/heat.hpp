@@ -41,6 +41,11 @@ struct HydroLike {
     const float *x;
 };
 
+// the heat pass's launches, counted as they are made (the work counters behind sr_read_work)
+struct HeatWork {
+    unsigned long long steps = 0, passes = 0, commits = 0, limited = 0;
+};
+
 class HeatPass {
   public:
     ~HeatPass();
@@ -52,6 +57,9 @@ class HeatPass {
     // from the planes and faces the step left (clobbers the last iterate, which no later pass reads)
     int first_system(const double *scal, hipStream_t s);
     sr_heat_report report{};
+    HeatWork work;
+    // algorithmic HBM bytes of the counted work outside the linear solver
+    double work_bytes(const HeatWork &w) const;
     FldSolver solver;
     HeatParams P{};
     sr_heat_settings hs{};

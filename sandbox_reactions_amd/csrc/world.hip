@@ -270,6 +270,10 @@ struct sr_world {
     hipEvent_t ev[12]{};
     sr_step_cost cost{};
     KernelClock k_update, k_mid, k_grav, k_pre0;
+    // the step's launches outside the heat pass, counted as they are made (sr_read_work)
+    struct {
+        unsigned long long steps = 0, hydro = 0, mid = 0, conv_first = 0, conv_mid = 0, radiation = 0;
+    } wk;
     std::vector<void *> allocs;
 };
 
@@ -485,6 +489,11 @@ extern "C" int sr_upload(sr_world *wd, const float *planes, size_t n) {
     wd->first_built = false;
     wd->steps = 0;
     wd->last_heat = sr_heat_report{};
+    wd->wk = {};
+    if (wd->sw.heat) {
+        wd->heat.work = HeatWork{};
+        wd->heat.solver.work = FldWork{};
+    }
     // the twin's start: P2 on the state and dt_0 from it (reference/mod.rs:173-175), no floors
     SR_TRY(p2(wd));
     SR_TRY(p89(wd, 0));
@@ -601,6 +610,8 @@ static int one_step(sr_world *wd) {
     hipLaunchKernelGGL(k_p1, dim3(1), dim3(64), 0, s, wd->scal);
     mark(wd, 1);
     SR_TRY(p2(wd));
+    wd->wk.steps++;
+    if (wd->sw.gravity) wd->wk.conv_first++;
     mark(wd, 2);
     if (wd->sw.hydro) {
         HydroState &in = wd->st[wd->cur], &out = wd->st[1 - wd->cur];
@@ -611,7 +622,11 @@ static int one_step(sr_world *wd) {
                                 nullptr, wd->sigma_mid, nullptr, wd->escaped, wd->part, s));
             mark(wd, 9);
             SR_TRY(wd->grav.potential<float>(wd->sigma_mid, wd->phi_mid, s));
+            wd->wk.mid++;
+            wd->wk.conv_mid++;
         }
+        wd->wk.hydro++;
+        if (rad) wd->wk.radiation++;
         mark(wd, 10);
         SR_TRY(hydro_launch(HYDRO_UPDATE, wd->hp, in, out, wd->scal, grav ? wd->gx : nullptr,
                             grav ? wd->gy : nullptr, grav ? wd->sigma_mid : nullptr, grav ? wd->phi_mid : nullptr,
@@ -849,6 +864,52 @@ extern "C" int sr_read_cost(sr_world *wd, sr_step_cost *out) {
     if (wd->cost.steps) {
         out->step_ms = wd->cost.step_ms / wd->cost.steps;
         for (int k = 0; k < 10; k++) out->pass_ms[k] = wd->cost.pass_ms[k] / wd->cost.steps;
+    }
+    return SR_OK;
+}
+
+// Per cell: P2's convolution (the box's Sigma in, f64 for the first and f32 for the mid-step one; the half-length row
+// spectra, (P/2 + 1) h complex values, written and read; the column outputs, (P/2 + 1)(h + 2), written and read; the
+// kernel's quarter spectrum; phi with its ring out: about 40 B at P = 2n) and the gradient (phi in, g_x, g_y out: 12); P4's
+// mass-flux pass (the state in, g in, Sigma_mid out: 60 + 8 + 4) and update (the state in and out, g, Sigma_mid and
+// phi_mid in, F and |push| in with radiation: 120 + 16 + 12); P8/P9 (the state in and out, g in: 128).
+extern "C" int sr_read_work(sr_world *wd, sr_step_work *out, int reset) {
+    if (!wd || !out) return fail(SR_ERR_ARGUMENT, "sr_read_work: null pointer");
+    const double n = (double)wd->n;
+    sr_step_work w{};
+    w.steps = wd->wk.steps;
+    w.hydro_steps = wd->wk.hydro;
+    w.convolutions = wd->wk.conv_first + wd->wk.conv_mid;
+    double conv = 0.0;
+    if (wd->sw.gravity) {
+        const double P = (double)wd->grav.px, Q = (double)wd->grav.py;
+        const double fft = (2.0 * 8.0 * (P / 2.0 + 1.0) * wd->h + 2.0 * 8.0 * (P / 2.0 + 1.0) * (wd->h + 2.0) +
+                            4.0 * (P / 2.0 + 1.0) * (Q / 2.0 + 1.0) + 4.0 * (wd->w + 2.0) * (wd->h + 2.0)) /
+                           n;
+        conv = (8.0 + fft + 12.0) * wd->wk.conv_first + (4.0 + fft) * wd->wk.conv_mid;
+    }
+    w.bytes_p2 = n * conv;
+    w.bytes_p4 = n * (72.0 * wd->wk.mid + (120.0 + (wd->sw.gravity ? 16.0 : 0.0)) * wd->wk.hydro + 12.0 * wd->wk.radiation);
+    w.bytes_p89 = n * 128.0 * wd->wk.steps;
+    if (wd->sw.heat) {
+        const FldWork &f = wd->heat.solver.work;
+        const HeatWork &h = wd->heat.work;
+        w.heat_steps = h.steps;
+        w.newton_passes = h.passes;
+        w.pcg_iterations = f.iterations;
+        w.precond_applications = f.precond;
+        w.true_residuals = f.true_residuals;
+        w.hierarchy_builds = f.builds_faces + f.builds_own;
+        w.bytes_p5 = wd->heat.work_bytes(h) + wd->heat.solver.work_bytes(f);
+    }
+    w.algorithmic_bytes = w.bytes_p2 + w.bytes_p4 + w.bytes_p5 + w.bytes_p89;
+    *out = w;
+    if (reset) {
+        wd->wk = {};
+        if (wd->sw.heat) {
+            wd->heat.work = HeatWork{};
+            wd->heat.solver.work = FldWork{};
+        }
     }
     return SR_OK;
 }

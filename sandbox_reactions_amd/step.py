@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import DtReport, HeatReport, StepCost, Switches
+from ._lib import DtReport, HeatReport, StepCost, StepWork, Switches
 
 
 class World:
@@ -134,6 +134,12 @@ class World:
         c = StepCost()
         self._c(self.lib.sr_read_cost(self._p, C.byref(c)))
         return c
+
+    def read_work(self, reset: bool = False) -> StepWork:
+        """The launches' counts and the algorithmic HBM bytes of the steps since the upload or the last reset."""
+        w = StepWork()
+        self._c(self.lib.sr_read_work(self._p, C.byref(w), int(reset)))
+        return w
 
     def kernel_time(self, name: str):
         n = C.c_uint64()

@@ -1,0 +1,265 @@
+"""The five benchmark workloads on the GPU at their stated sizes, each graded against the f64 oracle or by the
+properties that do not depend on size:
+
+- the small plain method (256^2 blob, gravity by the zero-padded 512^2 convolution, FLD to 1e-10) over its 100 steps
+  against the twin on the same dt (the twin's heat pass solves by multigrid PCG to 1e-12, equal to its Jacobi rule to
+  1e-12 in E: `test_oracle_heat.py::test_the_multigrid_twin_is_the_jacobi_twin`);
+- the plain method at scale (the turbulent 1024^2 world under the benchmark's settings): the per-field error against
+  the twin after two steps on the same dt;
+- the stiff FLD solve alone at 2048^2 on its stated shape (7 levels, V(2,2) red-black Gauss-Seidel, plain
+  prolongation) and on the tuned one, each to 1e-8 by the FP64 residual recomputed from the returned x;
+- the learned preconditioner (the committed network) inside flexible CG at 1024^2 and 2048^2, with and without the
+  fallback to plain multigrid;
+- the 64-scene ensemble (sides 128..1024) for one step: every world converged, finite and holding its mass, and three
+  of them bit for bit as stepped alone;
+
+and the reference's G-REF comparison (crates/sr-engine/tests/gpu/cpu_gpu.rs:37-40, :94-170): ref_sod.json and
+ref_disk.json run 200 steps on the GPU, the twin follows to the GPU's sim time; the relative L1 of Sigma and E within
+1e-3, |dX| within 1e-4 where both hold matter, and the clocks after 200 steps within 5e-4."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import both, f32, rel_l1
+from sandbox_reactions_amd import scene_json, scenes
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
+
+
+@pytest.fixture
+def mg_twin():
+    """The twin's heat pass on multigrid PCG to 1e-12 for the test's duration."""
+    oracle.lib.sro_heat_set_mg(1e-12)
+    yield
+    oracle.lib.sro_heat_set_mg(0.0)
+
+
+def bench_heat_settings(sr):
+    return sr.heat_settings(stop=sr.STOP_RELATIVE, solve_tol=1e-8, newton_heat=1e-6, newton_cell=1e-3, forcing=0.1,
+                            coarse_scale=1.2)
+
+
+def test_blob_256_over_its_100_steps_stays_on_the_twin(gpu_lib, mg_twin):
+    """Density rel-L1 within 1e-5 and E within 1e-4 after every one of the 100 steps; mass to FP64 round-off."""
+    sr = gpu_lib
+    n, steps = 256, 100
+    print(f"blob workload {n}^2, {steps} steps", flush=True)
+    kw = scenes.config_physics_kwargs(n)
+    oph = oracle.reference_physics(**{k: v for k, v in kw.items() if k != "max_padded"}, max_padded=4096.0)
+    hs = sr.heat_settings(stop=sr.STOP_RELATIVE, solve_tol=1e-10, newton_heat=1e-9, newton_cell=1e-6)
+    start = f32(scenes.blob_scene(n))
+    twin = oracle.Twin(oph, start, gravity=True, hydro=True, heat=True)
+    world = sr.World(n, n, physics=sr.physics(**kw), gravity=True, hydro=True, heat=True, heat_settings=hs)
+    world.upload(start.astype(np.float32))
+    worst_rho = worst_e = 0.0
+    for k in range(steps):
+        dt = world.read_dt().dt_next
+        twin.cap_dt_next(dt)
+        twin.step()
+        world.set_dt_cap(dt)
+        world.run(1)
+        g, t = world.readback(), twin.state()
+        er, ee = rel_l1(g[0], t[0]), rel_l1(g[3], t[3])
+        worst_rho, worst_e = max(worst_rho, er), max(worst_e, ee)
+        heat = world.read_heat()
+        assert heat.converged and heat.final_residual <= 1e-10, (k, heat.final_residual)
+        if k % 20 == 19:
+            print(f"step {k + 1}: rel-L1 rho {er:.2e}, E {ee:.2e}", flush=True)
+        assert er <= 1e-5, (k, er)
+        assert ee <= 1e-4, (k, ee)
+    print(f"worst rel-L1 over {steps} steps: rho {worst_rho:.2e}, E {worst_e:.2e}", flush=True)
+    totals, escaped = world.totals()
+    assert abs(totals[0] + escaped[0] - start[0].sum()) <= 1e-12 * start[0].sum()
+    r = world.read_dt()
+    _, _, st, tsteps = twin.times()
+    assert r.steps == tsteps == steps and abs(r.sim_time - st) <= 1e-6 * st
+
+
+def test_turbulent_1024_per_field_error_against_the_twin(gpu_lib):
+    """The benchmark's world and settings (inexact Newton, last solve to 1e-8, coarse scale 1.2) against the twin
+    (multigrid PCG to 1e-11, its Newton tolerances 1e-12 / 1e-10) for two steps on the same dt: the relative L1 of
+    every conserved field and the largest |dX|."""
+    sr = gpu_lib
+    n, steps = 1024, 2
+    print(f"turbulent {n}^2 against the twin, {steps} steps", flush=True)
+    kw = scenes.config_physics_kwargs(n)
+    oph = oracle.reference_physics(**{k: v for k, v in kw.items() if k != "max_padded"}, max_padded=4096.0)
+    planes = scenes.turbulent_scene(n).astype(np.float32)
+    world = sr.World(n, n, physics=sr.physics(**kw), gravity=True, hydro=True, heat=True,
+                     heat_settings=bench_heat_settings(sr))
+    world.upload(planes)
+    oracle.lib.sro_heat_set_mg(1e-11)
+    try:
+        twin = oracle.Twin(oph, planes.astype(np.float64), gravity=True, hydro=True, heat=True)
+        for k in range(steps):
+            dt = world.read_dt().dt_next
+            twin.cap_dt_next(dt)
+            twin.step()
+            world.set_dt_cap(dt)
+            world.run(1)
+            assert world.read_heat().converged and twin.last_heat().converged
+    finally:
+        oracle.lib.sro_heat_set_mg(0.0)
+    g, t = world.readback(), twin.state()
+    errs = {name: rel_l1(g[c], t[c]) for c, name in enumerate(["Sigma", "Sigma_u", "Sigma_v", "E"])}
+    dx = float(np.max(np.abs(g[4:] - t[4:])))
+    print("per-field rel-L1 after 2 steps: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()) +
+          f"; max |dX| {dx:.2e}", flush=True)
+    assert errs["Sigma"] <= 1e-5
+    assert errs["E"] <= 1e-4
+    assert errs["Sigma_u"] <= 1e-4 and errs["Sigma_v"] <= 1e-4
+    assert dx <= 1e-5
+
+
+def fld_case(sr, torch, n, **solver):
+    c, kx, ky, b, dt = scenes.fld_system(n)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    own, tkx, tky = t(c), t(kx), t(ky)
+    f = sr.Fld(n, n, **solver)
+    f.set_system(own, tkx, tky)
+    return f, (own, tkx, tky), torch.tensor(b, device="cuda")
+
+
+def true_rel(f, torch, b, x):
+    ax = torch.empty_like(x)
+    f.matvec(x, ax)
+    torch.cuda.synchronize()
+    return float(torch.linalg.norm(b - ax) / torch.linalg.norm(b))
+
+
+def test_stiff_fld_2048_on_its_stated_shape_and_the_tuned_one(gpu_lib):
+    """7 levels, V(2,2), plain prolongation (the workload's own shape) and the tuned solver (all levels, coarse
+    scale 1.2): both to 1e-8 by the recomputed FP64 residual, the two solutions within 1e-7 of each other."""
+    import torch
+    sr = gpu_lib
+    n = 2048
+    print(f"stiff FLD {n}^2: 7 levels, coarse scale 1", flush=True)
+    f7, _, b = fld_case(sr, torch, n, levels=7, coarse_scale=1.0)
+    assert f7.levels == 7
+    x7 = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    it7, rel7 = f7.solve(b, x7, tol=1e-8)
+    print(f"7 levels: {it7} iterations, rel {rel7:.2e}", flush=True)
+    assert rel7 <= 1e-8 and true_rel(f7, torch, b, x7) <= 1e-8
+    ft, _, _ = fld_case(sr, torch, n, coarse_scale=1.2)
+    xt = torch.zeros_like(x7)
+    itt, relt = ft.solve(b, xt, tol=1e-8)
+    print(f"tuned ({ft.levels} levels, scale 1.2): {itt} iterations, rel {relt:.2e}", flush=True)
+    assert relt <= 1e-8 and true_rel(ft, torch, b, xt) <= 1e-8
+    assert float(torch.linalg.norm(x7 - xt) / torch.linalg.norm(xt)) <= 1e-7
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_the_committed_network_inside_fcg(gpu_lib, n):
+    """The trained network (weights/fld_unet.npz) as the preconditioner of flexible CG on the stiff system: with the
+    fallback budget set to plain multigrid's iteration count the solve reaches 1e-8 (recomputed in FP64) and x agrees
+    with plain's to 1e-7; without fallback it converges on the network alone."""
+    import torch
+    from sandbox_reactions_amd import train
+    sr = gpu_lib
+    print(f"learned FCG {n}^2", flush=True)
+    f, (own, kx, ky), b = fld_case(sr, torch, n, coarse_scale=1.2)
+    xp = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    it_p, rel_p, _, _ = f.solve_stats(b, xp, tol=1e-8)
+    net = sr.Net(n, n, train.load())
+    f.set_network(net, fallback_iterations=it_p)
+    xl = torch.zeros_like(xp)
+    it_l, rel_l, li, fb = f.solve_stats(b, xl, tol=1e-8, precond=sr.PRECOND_LEARNED)
+    print(f"plain {it_p}; learned with fallback {it_l} ({li} with the network, fell back {fb})", flush=True)
+    assert rel_l <= 1e-8 and true_rel(f, torch, b, xl) <= 1e-8
+    assert float(torch.linalg.norm(xl - xp) / torch.linalg.norm(xp)) <= 1e-7
+    assert li <= it_l and (fb or li == it_l)
+    f.set_network(net, fallback_iterations=0)
+    xn = torch.zeros_like(xp)
+    it_n, rel_n, li_n, fb_n = f.solve_stats(b, xn, tol=1e-8, precond=sr.PRECOND_LEARNED, max_iter=2000)
+    print(f"learned without fallback {it_n} iterations, rel {rel_n:.2e}", flush=True)
+    assert not fb_n and li_n == it_n and rel_n <= 1e-8 and true_rel(f, torch, b, xn) <= 1e-8
+    assert float(torch.linalg.norm(xn - xp) / torch.linalg.norm(xp)) <= 1e-7
+
+
+def test_the_64_scene_ensemble_one_step(gpu_lib):
+    """scenes.ensemble_scenes() at its defaults (64 worlds of sides 128..1024) stepped once together: every heat
+    pass converged, every value finite, every world's mass (grid + escaped) its start's to 1e-12, and the smallest,
+    a middle and the largest world bit for bit what they reach stepped alone."""
+    sr = gpu_lib
+    batch = scenes.ensemble_scenes()
+    sides = [s for s, _ in batch]
+    print(f"ensemble: {len(batch)} scenes, sides {sorted(set(sides))}, {sum(s * s for s in sides) / 1e6:.1f} "
+          "Mcells", flush=True)
+    hs = bench_heat_settings(sr)
+
+    def make(side, planes):
+        w = sr.World(side, side, physics=sr.physics(**scenes.config_physics_kwargs(side)), gravity=True,
+                     hydro=True, heat=True, heat_settings=hs)
+        w.upload(planes)
+        return w
+
+    worlds = [make(s, p) for s, p in batch]
+    m0 = [float(p[0].astype(np.float64).sum()) for _, p in batch]
+    sr.ensemble_run(worlds, 1, threads=16)
+    for k, w in enumerate(worlds):
+        r = w.read_dt()
+        assert r.steps == 1 and not r.non_finite, k
+        assert w.read_heat().converged, k
+        tot, esc = w.totals()
+        assert abs(tot[0] + esc[0] - m0[k]) <= 1e-12 * m0[k], k
+    order = np.argsort(sides, kind="stable")
+    for k in (order[0], order[len(order) // 2], order[-1]):
+        alone = make(*batch[k])
+        alone.run(1)
+        assert np.array_equal(alone.readback(), worlds[k].readback()), (k, sides[k])
+        alone.close()
+    for w in worlds:
+        w.close()
+
+
+@pytest.mark.parametrize("name", ["ref_sod.json", "ref_disk.json"])
+def test_gref_200_steps_against_the_twin(gpu_lib, name):
+    sr = gpu_lib
+    steps = 200
+    print(f"G-REF {name}: {steps} steps", flush=True)
+    s = scene_json.load(os.path.join(HERE, name))
+    kw = s.physics_kwargs()
+    op, _ = both(**kw)
+    world = scene_json.world(s)
+    start = s.planes(kw["gamma"]).astype(np.float32)
+    world.upload(start)
+    world.run(steps)
+    r = world.read_dt()
+    assert r.steps == steps and not r.non_finite
+    t_gpu = r.sim_time
+    g = world.readback()
+
+    def make_twin():
+        return oracle.Twin(op, start.astype(np.float64), gravity=s.switches["gravity"], hydro=s.switches["hydro"],
+                           heat=s.switches["heat"])
+
+    twin, clock = make_twin(), None
+    while True:  # the twin on its own dt to the GPU's sim time, the last step capped to land there
+        _, dtn, t, k = twin.times()
+        if k + 1 == steps:
+            clock = t + dtn
+        last = t + dtn >= t_gpu
+        if last:
+            twin.cap_dt_next(t_gpu - t)
+        twin.step()
+        if last:
+            break
+    landed = twin.times()[3]
+    if clock is None:
+        full = make_twin()
+        full.step(steps)
+        clock = full.times()[2]
+    tw = twin.state()
+    sig, e = rel_l1(g[0], tw[0]), rel_l1(g[3], tw[3])
+    matter = kw.get("sigma_floor", 1e-8) / 1e-4
+    live = (g[0] >= matter) & (tw[0] >= matter)
+    dx = max(float(np.max(np.abs(g[c][live] - tw[c][live]))) for c in range(4, 14))
+    off = abs(t_gpu / clock - 1.0)
+    print(f"{name}: sim time GPU {t_gpu:.5f} / twin {clock:.5f} ({100 * (t_gpu / clock - 1):+.4f} %); twin in "
+          f"{landed} steps; rel-L1 Sigma {sig:.3e}, E {e:.3e}; max |dX| {dx:.3e}", flush=True)
+    assert off <= 5e-4
+    assert sig <= 1e-3 and e <= 1e-3
+    assert dx <= 1e-4

@@ -246,3 +246,33 @@ def test_thick_gas_limit_with_kramers_opacity(T0, sigma):
     dTdx = (T[0, i] - T[0, i - 1]) / ph.dx
     expect = -(4.0 / 3.0) * ph.a_r * ph.c_sb * Tf ** 3 / kappa_sigma * dTdx
     assert fx[0, i] == pytest.approx(expect, rel=1e-2)
+
+
+def test_the_multigrid_twin_is_the_jacobi_twin():
+    """The twin's heat pass solved by multigrid PCG to 1e-12 (the test-side switch for large grids) lands where the
+    reference's Jacobi PCG rule lands: the blob workload at 64^2 with gravity, hydro and heat for two steps, E within
+    1e-10 and Sigma within 1e-13 (rel-L1), the same Newton pass counts and far fewer rounds; with the switch left at
+    the Jacobi rule the state is bitwise the default's."""
+    from sandbox_reactions_amd import scenes
+    n = 64
+    kw = scenes.config_physics_kwargs(n)
+    oph = oracle.reference_physics(**{k: v for k, v in kw.items() if k != "max_padded"}, max_padded=4096.0)
+    start = np.asarray(scenes.blob_scene(n), np.float32).astype(np.float64)
+    out = []
+    for rtol in (0.0, 1e-12, 0.0):
+        oracle.lib.sro_heat_set_mg(rtol)
+        try:
+            tw = oracle.Twin(oph, start, gravity=True, hydro=True, heat=True)
+            reps = []
+            for _ in range(2):
+                tw.step()
+                reps.append(tw.last_heat())
+        finally:
+            oracle.lib.sro_heat_set_mg(0.0)
+        out.append((tw.state(), reps))
+    (jac, rj), (mg, rm), (again, _) = out
+    assert np.array_equal(jac, again)
+    rel = lambda a, b: np.abs(a - b).sum() / np.abs(b).sum()
+    assert rel(mg[3], jac[3]) <= 1e-10 and rel(mg[0], jac[0]) <= 1e-13
+    for a, b in zip(rj, rm):
+        assert a.converged and b.converged and a.passes == b.passes and b.rounds < a.rounds
