@@ -134,6 +134,13 @@ void sr_heat_settings_defaults(sr_heat_settings *hs);
 /* Fft::transform (crates/sr-engine/src/step/fft.rs:116-134): an in-place complex 2D transform of an nx x ny
  * row-major interleaved (re, im) f32 array in device memory; forward unnormalised, inverse divided by nx*ny. */
 int sr_fft2(float *data_dev, int nx, int ny, int inverse, void *stream);
+/* sr_fft2 without allocating: no counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops). The twiddle table lives in a library-owned device array of 8192
+ * complex f32 values (64 KB, held by each GPU the library runs on until the process ends): axes longer than 8192 are
+ * refused (SR_ERR_LENGTH). Every call on one GPU shares that array, so calls that overlap on one GPU (on different
+ * streams) race. Returns once the work is enqueued on `stream`, without waiting for it; results bitwise equal to
+ * sr_fft2's. */
+int sr_fft2_noalloc(float *data_dev, int nx, int ny, int inverse, void *stream);
 
 /* Convolution::new + Gravity::new (step/fft.rs:136-182, step/grav.rs:133-200; twin reference/grav.rs:99-121):
  * a gravity solver for a w x h box, its kernel's spectrum built once (f64 on the host, stored f32). */

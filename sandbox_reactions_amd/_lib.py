@@ -84,6 +84,7 @@ SIGNATURES = [
     ("sr_physics_defaults", None, [C.POINTER(Physics)]),
     ("sr_heat_settings_defaults", None, [C.POINTER(HeatSettings)]),
     ("sr_fft2", _i, [_vp, _i, _i, _i, _vp]),
+    ("sr_fft2_noalloc", _i, [_vp, _i, _i, _i, _vp]),
     ("sr_gravity_create", _i, [_i, _i, _d, _d, _d, _i, _pp]),
     ("sr_gravity_field", _i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("sr_gravity_destroy", None, [_vp]),

@@ -381,29 +381,80 @@ extern "C" int sr_gravity_field(sr_gravity *g, const float *sigma_dev, float *ph
 
 extern "C" void sr_gravity_destroy(sr_gravity *g) { delete g; }
 
-extern "C" int sr_fft2(float *data_dev, int nx, int ny, int inverse, void *stream) {
+// The twiddles exp(-2 pi i k / n), k < n, of the complex transforms: formed in f64 on the device, stored f32.
+__global__ void k_twiddles(float2 *__restrict__ tw, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        double sn, cs;
+        sincospi(-2.0 * (double)k / (double)n, &sn, &cs);
+        tw[k] = make_float2((float)cs, (float)sn);
+    }
+}
+
+static int fft2_check(const float *data_dev, int nx, int ny) {
     if (!data_dev) return fail(SR_ERR_ARGUMENT, "sr_fft2: null pointer");
     if (!is_pow2(nx) || !is_pow2(ny)) return fail(SR_ERR_NOT_POW2, "sr_fft2: axes must be powers of two");
     if ((size_t)std::max(nx, ny) * 2 * sizeof(float2) > (size_t)max_lds_bytes())
         return fail(SR_ERR_LENGTH, "sr_fft2: axis too long for the LDS");
-    hipStream_t s = (hipStream_t)stream;
+    return SR_OK;
+}
+
+// the twiddle table of the longer axis (tw, nmax entries), then the rows in place and the columns in place
+static int fft2_launch(float2 *d, int nx, int ny, int inverse, float2 *tw, hipStream_t s) {
     const int nmax = std::max(nx, ny);
-    std::vector<float2> t(nmax);
-    for (int i = 0; i < nmax; i++) {
-        const double a = -2.0 * M_PI * (double)i / (double)nmax;
-        t[i] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    float2 *tw = nullptr;
-    SR_HIP(hipMallocAsync((void **)&tw, sizeof(float2) * nmax, s));
-    SR_HIP(hipMemcpyAsync(tw, t.data(), sizeof(float2) * nmax, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_twiddles, dim3((nmax + 255) / 256), dim3(256), 0, s, tw, nmax);
     const int rb = std::max(1, std::min(8, 4096 / nx));
-    float2 *d = (float2 *)data_dev;
     hipLaunchKernelGGL(k_c2c_rows, dim3((ny + rb - 1) / rb), dim3(256), sizeof(float2) * 2 * rb * nx, s, d, nx, ny,
-                       ilog2(nx), rb, tw, nmax / nx, inverse, inverse ? 1.0f / nx : 1.0f);
-    hipLaunchKernelGGL(k_c2c_cols, dim3(nx), dim3(256), sizeof(float2) * 2 * ny, s, d, nx, ny, ilog2(ny), tw,
-                       nmax / ny, inverse, inverse ? 1.0f / ny : 1.0f);
+                       ilog2(nx), rb, (const float2 *)tw, nmax / nx, inverse, inverse ? 1.0f / nx : 1.0f);
+    hipLaunchKernelGGL(k_c2c_cols, dim3(nx), dim3(256), sizeof(float2) * 2 * ny, s, d, nx, ny, ilog2(ny),
+                       (const float2 *)tw, nmax / ny, inverse, inverse ? 1.0f / ny : 1.0f);
     SR_HIP(hipGetLastError());
+    return SR_OK;
+}
+
+extern "C" int sr_fft2(float *data_dev, int nx, int ny, int inverse, void *stream) {
+    SR_TRY(fft2_check(data_dev, nx, ny));
+    hipStream_t s = (hipStream_t)stream;
+    float2 *tw = nullptr;
+    SR_HIP(hipMallocAsync((void **)&tw, sizeof(float2) * std::max(nx, ny), s));
+    const int c = fft2_launch((float2 *)data_dev, nx, ny, inverse, tw, s);
     SR_HIP(hipFreeAsync(tw, s));
+    SR_TRY(c);
     SR_HIP(hipStreamSynchronize(s));
     return SR_OK;
+}
+
+// The allocation-free variant's twiddle table: a file-scope device array, one copy per GPU at its own address. Host
+// code never passes the array's name to a kernel (on the host it is a stand-in, not a device address): the first call
+// on each device looks the device's address up with hipGetSymbolAddress and keeps it.
+constexpr int FFT_SCRATCH_MAX = 8192;
+static __device__ __align__(256) float2 scratch[FFT_SCRATCH_MAX];
+constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static float2 *scratch_at[MAX_DEVICES] = {};
+
+static int scratch_of_current_device(float2 **out) {
+    int dev = -1;
+    SR_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(SR_ERR_DEVICE, "sr_fft2_noalloc: device index out of range");
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_at[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SR_ERR_DEVICE, std::string("sr_fft2_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        scratch_at[dev] = (float2 *)p;
+    }
+    *out = scratch_at[dev];
+    return SR_OK;
+}
+
+extern "C" int sr_fft2_noalloc(float *data_dev, int nx, int ny, int inverse, void *stream) {
+    if (std::max(nx, ny) > FFT_SCRATCH_MAX)
+        return fail(SR_ERR_LENGTH, "sr_fft2_noalloc: an axis longer than " + std::to_string(FFT_SCRATCH_MAX) +
+                                       " (the twiddle scratch's maximum)");
+    SR_TRY(fft2_check(data_dev, nx, ny));
+    float2 *tw = nullptr;
+    SR_TRY(scratch_of_current_device(&tw));
+    return fft2_launch((float2 *)data_dev, nx, ny, inverse, tw, (hipStream_t)stream);
 }
