@@ -3,10 +3,12 @@
 The headline workload is the full hydro + gravity + FLD step at 1024^2 (a lognormal density with sigma_ln 1.0, a
 lognormal temperature with sigma_ln 0.5, random velocities at rms Mach 0.5, box L = 1, gamma = 5/3, Kramers opacity,
 FLD Newton passes solved by MG-PCG, inexactly before the last one and to a relative residual of 1e-8 on it). `value`
-is Mcell-steps/s of that step with the state resident in HBM. Beside it: the stiff FLD solve alone at 2048^2 (time to
-1e-8 relative residual by plain MG-PCG and by the learned preconditioner), the roofline of the step's dominant
-HBM-bound kernel (the FLD's level-0 pre-smoother; the hydro update beside it) from HIP events over the timed region,
-and the oracle (the f64 CPU twin restated in C) timed on a bounded sample of the same workload on one host core.
+is Mcell-steps/s of that step with the state resident in HBM, timed with the kernel clocks off. Beside it: the same
+steps one at a time (median), an instrumented window (per-pass times and per-kernel clocks, from which the dominant
+kernel per step is named), the step's algorithmic HBM bytes from the library's work counters (the full step's
+roofline), the stiff FLD solve alone at 2048^2 (plain MG-PCG on the tuned and on the stated 7-level shape, and the
+learned preconditioner; medians of 5), the 64-scene ensemble, and the oracle (the f64 CPU twin restated in C) on the
+same 1024^2 workload on one host core and on all of them.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -49,7 +51,13 @@ def log(msg):
 COARSE_SCALE = 1.2
 
 
+# the kernels the world clocks when timing is on: the P4 passes and the PCG iteration's kernels
+KERNELS = ("hydro_update", "hydro_mid", "mg_pre0", "mg_post0", "mg_coarse", "fld_matvec", "fld_update")
+
+
 def full_step(sr, scenes, n, steps, warmup):
+    """K steps timed with the clocks off (the headline), the same K one at a time (their median), then an instrumented
+    window (pass times, kernel clocks)."""
     import torch
     kw = scenes.config_physics_kwargs(n)
     ph = sr.physics(**kw)
@@ -60,59 +68,96 @@ def full_step(sr, scenes, n, steps, warmup):
     world.upload(planes)
     log(f"full step {n}^2: warmup {warmup}")
     world.run(warmup)
-    world.set_timing(True)
+    world.read_work(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     world.run(steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    cost = world.read_cost()
-    kernels = {k: world.kernel_time(k) for k in ("mg_pre0", "hydro_update")}
+    work = world.read_work(reset=True)
+    per = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        world.run(1)
+        per.append(time.perf_counter() - t1)
     heat = world.read_heat()
-    log(f"full step: {elapsed / steps * 1e3:.3f} ms/step; " +
-        "; ".join(f"{k} {v[1]:.4f} ms x {v[0]}" for k, v in kernels.items()) +
+    # the same steps with the solves decided on the device, their iterations replayed as captured graphs
+    world.set_solve_mode(sr.SOLVE_GRAPH)
+    world.upload(planes)
+    world.run(warmup)
+    world.read_work(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    world.run(steps)
+    torch.cuda.synchronize()
+    dev_elapsed = time.perf_counter() - t0
+    dev_work = world.read_work(reset=True)
+    world.set_solve_mode(sr.SOLVE_HOST)
+    # the instrumented window, host-decided (every launch does work, so the clocks' counts are the step's work)
+    inst = max(5, min(steps, 10))
+    world.set_timing(True)
+    world.run(inst)
+    cost = world.read_cost()
+    kernels = {k: world.kernel_time(k) for k in KERNELS}
+    world.set_timing(False)
+    log(f"full step: {elapsed / steps * 1e3:.3f} ms/step (median of single steps "
+        f"{np.median(per) * 1e3:.3f}; device-decided {dev_elapsed / steps * 1e3:.3f}); " +
+        "; ".join(f"{k} {v[1] * 1e3:.2f} us x {v[0] / inst:.1f}/step" for k, v in kernels.items()) +
         f"; last heat passes {heat.passes} rounds {heat.rounds}")
-    return dict(elapsed=elapsed, cost=cost, kernels=kernels, heat=heat, world=world)
+    return dict(elapsed=elapsed, work=work, per=per, cost=cost, kernels=kernels, inst=inst, heat=heat, world=world,
+                dev_elapsed=dev_elapsed, dev_work=dev_work)
 
 
-def fld_solve(sr, scenes, n):
-    """The stiff FLD system alone: plain MG-PCG, then the learned preconditioner (the committed trained network)
-    with its fallback to plain multigrid, then the learned preconditioner without fallback (one solve, for the
-    record), and one network application timed alone."""
+def fld_solve(sr, scenes, n, reps=5):
+    """The stiff FLD system alone: plain MG-PCG on the tuned shape (all levels, coarse scale 1.2) and on the
+    workload's stated shape (7 levels, V(2,2), plain prolongation), then the learned preconditioner (the committed
+    trained network) with its fallback to plain multigrid, then without fallback (one solve, for the record), and one
+    network application timed alone. Each time is the median of `reps` solves after one untimed solve."""
     import torch
     from sandbox_reactions_amd import train
     log(f"FLD system {n}^2: building")
     c, kx, ky, b, dt = scenes.fld_system(n)
     t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
     own, tkx, tky = t(c), t(kx), t(ky)
-    f = sr.Fld(n, n, coarse_scale=COARSE_SCALE)
-    f.set_system(own, tkx, tky)
     bt = torch.tensor(b, device="cuda")
 
-    def timed(precond, reps):
+    def timed(f, precond, reps):
         out = []
-        for rep in range(reps):
+        for rep in range(reps + 1):
             x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             it, rel, li, fb = f.solve_stats(bt, x, tol=1e-8, precond=precond)
             torch.cuda.synchronize()
             out.append((time.perf_counter() - t0, it, rel, li, fb))
-        return out
+        return out[1:]
 
-    plain = timed(sr.PRECOND_MULTIGRID, 4)[1:]
-    ms = float(np.median([p[0] for p in plain]) * 1e3)
-    res = dict(ms=ms, iterations=int(plain[0][1]), rel=float(max(p[2] for p in plain)), levels=f.levels,
-               tiles=f.tiles())
-    log(f"FLD solve {n}^2: {ms:.2f} ms, {plain[0][1]} iterations, rel {res['rel']:.2e}, {f.levels} levels")
+    def summary(runs):
+        return dict(ms=round(float(np.median([r[0] for r in runs]) * 1e3), 3),
+                    ms_min=round(float(min(r[0] for r in runs) * 1e3), 3), iterations=int(runs[0][1]),
+                    final_rel=float(max(r[2] for r in runs)), reps=len(runs))
+
+    f7 = sr.Fld(n, n, levels=7, coarse_scale=1.0)
+    f7.set_system(own, tkx, tky)
+    stated = summary(timed(f7, sr.PRECOND_MULTIGRID, reps))
+    stated.update(levels=f7.levels)
+    log(f"FLD solve {n}^2, 7 levels, scale 1: {stated['ms']:.2f} ms, {stated['iterations']} iterations")
+    del f7
+    f = sr.Fld(n, n, coarse_scale=COARSE_SCALE)
+    f.set_system(own, tkx, tky)
+    plain = summary(timed(f, sr.PRECOND_MULTIGRID, reps))
+    plain.update(levels=f.levels, tiles=f.tiles())
+    log(f"FLD solve {n}^2: {plain['ms']:.2f} ms, {plain['iterations']} iterations, rel {plain['final_rel']:.2e}, "
+        f"{f.levels} levels")
+    res = dict(plain=plain, stated_shape=stated)
     if not os.path.exists(train.DEFAULT_WEIGHTS):
         log("no trained network: the learned preconditioner is not measured")
         return res
     net = sr.Net(n, n, train.load())
-    f.set_network(net, fallback_iterations=plain[0][1])
-    learned = timed(sr.PRECOND_LEARNED, 3)[1:]
+    f.set_network(net, fallback_iterations=plain["iterations"])
+    learned = timed(f, sr.PRECOND_LEARNED, reps)
     f.set_network(net, fallback_iterations=0)
-    pure = timed(sr.PRECOND_LEARNED, 1)[0] if plain[0][1] > 0 else None
+    pure = timed(f, sr.PRECOND_LEARNED, 1)[0]
     r = torch.randn((n, n), device="cuda")
     z = torch.empty_like(r)
     net.apply(own, tkx, tky, r, z)
@@ -123,14 +168,12 @@ def fld_solve(sr, scenes, n):
     e1.record()
     torch.cuda.synchronize()
     apply_ms = e0.elapsed_time(e1) / 5
-    lms = float(np.median([p[0] for p in learned]) * 1e3)
-    res.update(learned_ms=lms, learned_iterations=int(learned[0][1]), learned_net_iterations=int(learned[0][3]),
-               learned_fell_back=bool(learned[0][4]), learned_rel=float(max(p[2] for p in learned)),
-               net_apply_ms=apply_ms,
-               learned_without_fallback={"ms": pure[0] * 1e3, "iterations": pure[1], "rel": pure[2]}
-               if pure else None)
-    log(f"FLD learned {n}^2: {lms:.2f} ms, {learned[0][1]} iterations ({learned[0][3]} with the network, "
-        f"fell back {learned[0][4]}); without fallback {pure[1]} iterations in {pure[0] * 1e3:.1f} ms; "
+    ls = summary(learned)
+    ls.update(net_iterations=int(learned[0][3]), fell_back=bool(learned[0][4]))
+    res.update(learned=ls, learned_vs_plain=round(plain["ms"] / ls["ms"], 4), net_apply_ms=round(apply_ms, 4),
+               learned_without_fallback={"ms": round(pure[0] * 1e3, 3), "iterations": pure[1], "rel": pure[2]})
+    log(f"FLD learned {n}^2: {ls['ms']:.2f} ms, {ls['iterations']} iterations ({ls['net_iterations']} with the "
+        f"network, fell back {ls['fell_back']}); without fallback {pure[1]} iterations in {pure[0] * 1e3:.1f} ms; "
         f"network apply {apply_ms:.3f} ms")
     return res
 
@@ -178,7 +221,7 @@ def event_bracket_ms(reps=512):
     return round(ms.value, 5)
 
 
-def measured_traffic(kernel, n, grid=None):
+def measured_traffic(kernel, grid=None, n=N_STEP):
     """HBM bytes per launch of `kernel` (launched with `grid` threads, when given) at n^2 from the committed counter
     summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter reads for a copy of known
     size), or None."""
@@ -231,25 +274,49 @@ def ensemble(sr, scenes, threads, steps=2):
     return res
 
 
-def cpu_baseline(scenes, budget_s=12.0):
-    """The oracle twin (f64 C, one thread) on the same workload's statistics at 128^2 (the bounded sample): steps
-    until the budget is spent."""
+def cpu_baseline(scenes, n=N_STEP, steps=1, omp_steps=2, ref_budget_s=4.0):
+    """The oracle twin (f64 C) on the headline workload at its full size: its heat pass solved by multigrid PCG to
+    1e-11 (the device's method; the reference twin's Jacobi rule would take thousands of rounds per pass here), one
+    step on one host thread (liboracle.so, the reference twin is single-threaded) and a few on all of the host's
+    threads (liboracle_omp.so). Beside them, the reference's own rule (Jacobi PCG at 1e-12) on a 128^2 sample of the
+    same fields for a few seconds."""
     import oracle
-    n = 128
     kw = scenes.config_physics_kwargs(n)
     ph = oracle.reference_physics(**{k: v for k, v in kw.items() if k != "max_padded"}, max_padded=4096.0)
     planes = scenes.turbulent_scene(n).astype(np.float32).astype(np.float64)
-    twin = oracle.Twin(ph, planes, gravity=True, hydro=True, heat=True)
-    steps = 0
+    out = {}
+    threads = len(os.sched_getaffinity(0))
+    for name, lib, k in (("serial", oracle.lib, steps), ("omp", oracle.omp_lib(), omp_steps)):
+        lib.sro_heat_set_mg(1e-11)
+        try:
+            twin = oracle.Twin(ph, planes, gravity=True, hydro=True, heat=True, library=lib)
+            t0 = time.perf_counter()
+            twin.step(k)
+            el = time.perf_counter() - t0
+        finally:
+            lib.sro_heat_set_mg(0.0)
+        out[name] = dict(value=n * n * k / el / 1e6, steps=k, seconds=round(el, 2))
+        log(f"CPU twin {n}^2 ({name}): {k} steps in {el:.2f} s")
+    m = 128
+    kw = scenes.config_physics_kwargs(m)
+    ph = oracle.reference_physics(**{k: v for k, v in kw.items() if k != "max_padded"}, max_padded=4096.0)
+    twin = oracle.Twin(ph, scenes.turbulent_scene(m).astype(np.float32).astype(np.float64), gravity=True, hydro=True,
+                       heat=True)
+    k = 0
     t0 = time.perf_counter()
-    while steps < 1 or time.perf_counter() - t0 < budget_s:
+    while k < 1 or time.perf_counter() - t0 < ref_budget_s:
         twin.step()
-        steps += 1
+        k += 1
     el = time.perf_counter() - t0
-    return dict(value=n * n * steps / el / 1e6, unit="Mcell-steps/s", cores=1, kind="port",
-                sample=f"oracle twin (f64 C, 1 thread), full step with Jacobi-PCG FLD at the reference's 1e-12 "
-                       f"tolerances, {n}^2 sample of the same fields, {steps} steps in {el:.1f} s",
-                cpu=host_cpu(), host_threads=len(os.sched_getaffinity(0)))
+    return dict(value=out["serial"]["value"], unit="Mcell-steps/s", cores=1, kind="port",
+                sample=f"oracle twin (f64 C), the full step on the same {n}^2 fields with multigrid-PCG heat solves to "
+                       f"1e-11: {out['serial']['steps']} step in {out['serial']['seconds']} s on one thread",
+                omp=dict(value=out["omp"]["value"], cores=threads, sample=f"liboracle_omp.so, {out['omp']['steps']} "
+                         f"steps in {out['omp']['seconds']} s on {threads} threads"),
+                reference_rule_128=dict(value=m * m * k / el / 1e6, cores=1,
+                                        sample=f"Jacobi PCG at the reference's 1e-12 rule, {m}^2 sample, {k} steps in "
+                                               f"{el:.1f} s"),
+                cpu=host_cpu(), host_threads=threads)
 
 
 def host_cpu():
@@ -280,19 +347,47 @@ def main():
     fs = full_step(sr, scenes, args.step_n, args.steps, args.warmup)
     n = args.step_n
     value = n * n * args.steps / fs["elapsed"] / 1e6
-    def roof(name, counter_name, grid=None):
+    # the clocked kernels' names and launch grids (threads) in a kernel trace, for the committed traffic summary
+    gxm = (n + 255) // 256
+    mv_rows = max(1, min(16, (n * gxm + 2047) // 2048))
+    tiles32 = ((n + 31) // 32) ** 2 * 256
+    trace = {"mg_pre0": ("sr::k_pre<32, 32, 2>", tiles32), "mg_post0": ("sr::k_post<32, 32, 2>", tiles32),
+             "fld_matvec": ("sr::k_matvec_p", gxm * 256 * ((n + mv_rows - 1) // mv_rows)),
+             "fld_update": ("sr::k_update", min(1024, (n * n + 1023) // 1024) * 256),
+             "mg_coarse": ("sr::k_coarse<true>", 1024),
+             "hydro_update": ("sr::k_hydro<1, false>", ((n + 15) // 16) ** 2 * 320),
+             "hydro_mid": ("sr::k_hydro<0, false>", ((n + 15) // 16) ** 2 * 320)}
+    per_step = {}
+    for name, (launches, mean_ms, bytes_per) in fs["kernels"].items():
+        per_step[name] = {"launches_per_step": round(launches / fs["inst"], 2), "mean_us": round(mean_ms * 1e3, 3),
+                          "ms_per_step": round(launches * mean_ms / fs["inst"], 4),
+                          "algorithmic_bytes_per_launch": bytes_per}
+
+    def roof(name):
         launches, mean_ms, bytes_per = fs["kernels"][name]
         achieved = bytes_per / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else 0.0
         return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(counter_name, n, grid),
-                "launches": launches, "mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": bytes_per}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(*trace[name], n=n),
+                "launches": launches, "mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": bytes_per,
+                "ms_per_step": per_step[name]["ms_per_step"]}
+    dominant = max(per_step, key=lambda k: per_step[k]["ms_per_step"])
     fld = None if args.no_fld else fld_solve(sr, scenes, args.fld_n)
     ens = None if args.no_ensemble else ensemble(sr, scenes, args.ensemble_threads)
     copy_gbs = copy_bandwidth()
     bracket = event_bracket_ms()
     log(f"device copies (GB/s): {copy_gbs}; timing-event bracket {bracket} ms")
     cpu = None if args.no_cpu else cpu_baseline(scenes)
-    cost = fs["cost"]
+    cost, work = fs["cost"], fs["work"]
+    k = args.steps
+    full = {"bound": "hbm", "achieved": round(work.algorithmic_bytes / fs["elapsed"] / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(work.algorithmic_bytes / fs["elapsed"] / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_step": round(work.algorithmic_bytes / k),
+            "bytes_per_step_by_pass": {"P2": round(work.bytes_p2 / k), "P4": round(work.bytes_p4 / k),
+                                       "P5": round(work.bytes_p5 / k), "P8+P9": round(work.bytes_p89 / k)},
+            "per_step": {"newton_passes": work.newton_passes / k, "pcg_iterations": work.pcg_iterations / k,
+                         "precond_applications": work.precond_applications / k,
+                         "true_residuals": work.true_residuals / k, "hierarchy_builds": work.hierarchy_builds / k,
+                         "convolutions": work.convolutions / k}}
     out = {
         "metric": "Mcell-steps/s full step + FLD solve ms to 1e-8: learned vs plain MG-PCG",
         "value": round(value, 3),
@@ -311,30 +406,35 @@ def main():
                         f"gamma 5/3, Kramers opacity, FLD by MG-PCG (coarse scale {COARSE_SCALE}) in inexact Newton "
                         f"(forcing 0.1), last pass to 1e-8",
             "grid": n, "fft_padded": 2 * n,
+            "step_ms_median": round(float(np.median(fs["per"])) * 1e3, 4),
+            "device_decided": {"ms_per_step": round(fs["dev_elapsed"] / k * 1e3, 4),
+                               "pcg_iterations_per_step": fs["dev_work"].pcg_iterations / k,
+                               "newton_passes_per_step": fs["dev_work"].newton_passes / k,
+                               "what": "the same steps from the same upload with every PCG iteration's continuation "
+                                       "decided on the device and the iterations replayed as captured graphs "
+                                       "(sr_set_solve_mode SOLVE_GRAPH); the headline decides on the host"},
+            "pcg_iterations_per_step": work.pcg_iterations / k,
+            "step_ms_single": [round(t * 1e3, 3) for t in fs["per"]],
+            "instrumented_steps": fs["inst"],
             "pass_ms": {"P1": round(cost.pass_ms[1], 4), "P2": round(cost.pass_ms[2], 4),
                         "P4": round(cost.pass_ms[4], 4), "P5": round(cost.pass_ms[5], 4),
                         "P8+P9": round(cost.pass_ms[8], 4)},
+            "kernels_per_step": per_step,
             "heat_last_step": {"passes": fs["heat"].passes, "rounds": fs["heat"].rounds,
                                "converged": fs["heat"].converged},
-            "fld_solve_2048": None if fld is None else {
-                "workload": f"stiff FLD solve alone, {args.fld_n}^2, diffusion number 1e3, MG-PCG V(2,2), "
-                            f"coarse scale {COARSE_SCALE}, to 1e-8",
-                "plain_mg_pcg_ms": round(fld["ms"], 3), "iterations": fld["iterations"], "final_rel": fld["rel"],
-                "levels": fld["levels"], "tiles_pre_post": fld["tiles"], "learned_ms": fld.get("learned_ms"),
-                "learned_vs_plain": round(fld["ms"] / fld["learned_ms"], 4) if fld.get("learned_ms") else None,
-                "learned_iterations": fld.get("learned_iterations"),
-                "learned_net_iterations": fld.get("learned_net_iterations"),
-                "learned_fell_back": fld.get("learned_fell_back"),
-                "learned_without_fallback": fld.get("learned_without_fallback"),
-                "net_apply_ms": fld.get("net_apply_ms")},
+            "fld_solve_2048": None if fld is None else dict(
+                workload=f"stiff FLD solve alone, {args.fld_n}^2, diffusion number 1e3, to 1e-8: `plain` on the "
+                         f"tuned solver (all levels, V(2,2), coarse scale {COARSE_SCALE}), `stated_shape` on 7 "
+                         f"levels, V(2,2), plain prolongation; medians of 5", **fld),
             "ensemble_64": ens,
         },
-        # the step's dominant HBM-bound kernel: the FLD's level-0 pre-smoother (P5 is most of the step); the hydro
-        # update beside it. achieved uses the event-timed mean as read; event_bracket_ms is what the same events
-        # read around an empty kernel, the part of that mean a kernel trace does not count
-        "roofline": dict(roof("mg_pre0", "sr::k_pre<32, 32, 2>", ((n + 31) // 32) ** 2 * 256),
-                         device_copy_gbs=copy_gbs, event_bracket_ms=bracket,
-                         also=[roof("hydro_update", "sr::k_hydro<1, false>", ((n + 15) // 16) ** 2 * 320)]),
+        # the kernel with the most device time per step (from the clocks of the instrumented window), with every other
+        # clocked kernel beside it; `full_step` is the whole step's algorithmic bytes over the headline window's time.
+        # achieved uses the event-timed means as read; event_bracket_ms is what the same events read around an empty
+        # kernel, the part of a mean that a kernel trace does not count
+        "roofline": dict(roof(dominant), full_step=full, device_copy_gbs=copy_gbs, event_bracket_ms=bracket,
+                         also=[roof(k) for k in sorted(per_step, key=lambda k: -per_step[k]["ms_per_step"])
+                               if k != dominant]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -119,6 +119,15 @@ typedef struct {
     double bytes_p2, bytes_p4, bytes_p5, bytes_p89;  /* the same by pass */
 } sr_step_work;
 
+/* How a heat-pass or standalone solve decides when to stop (sr_fld_set_solve_mode, sr_set_solve_mode): on the host
+ * from partial sums read back every few iterations (SR_SOLVE_HOST, the default; the learned preconditioner always
+ * does), or on the device, each PCG iteration opened by a head kernel that reads the last one's residual and decides
+ * (the FP64 true residual and restart, or the next inner iteration, or done), the later kernels of the iteration
+ * skipping themselves once it is done; the iterations launched one by one (SR_SOLVE_DEVICE) or replayed as a
+ * captured hipGraph of two (SR_SOLVE_GRAPH). The two device modes give the same bits. A world replays its step's
+ * fixed parts (P1..P4 with the take-backs, P8/P9) as captured graphs in every mode (with the clocks off). */
+enum { SR_SOLVE_HOST = 0, SR_SOLVE_DEVICE = 1, SR_SOLVE_GRAPH = 2 };
+
 typedef struct sr_world sr_world;
 typedef struct sr_gravity sr_gravity;
 typedef struct sr_fld sr_fld;
@@ -168,6 +177,9 @@ int sr_fld_apply_precond(sr_fld *f, const float *r_dev, float *z_dev, int precon
 /* out = A v in FP64 */
 int sr_fld_matvec(sr_fld *f, const double *v_dev, double *out_dev, void *stream);
 int sr_fld_levels(sr_fld *f, int *levels);
+/* the solve's stopping decisions: SR_SOLVE_HOST, SR_SOLVE_DEVICE or SR_SOLVE_GRAPH (no counterpart in the reference:
+ * its heat rounds are chained dispatches whose sizes the GPU writes, step/mod.rs:129-138) */
+int sr_fld_set_solve_mode(sr_fld *f, int mode);
 /* Tile shapes of the smoothers of the levels above the single-block ones (SR_TILE_16 16x16, SR_TILE_32 32x32,
  * SR_TILE_64 64x32, the last for nu1, nu2 <= 2 only): the project's autotuner times every shape on the first system
  * built and keeps the fastest per level below level 0 and smoother; sr_fld_set_tiles fixes them instead. Level 0
@@ -262,6 +274,8 @@ int sr_heat_first_system(sr_world *wd, float *own, float *kx, float *ky, uint8_t
 int sr_apply_heat_precond(sr_world *wd, const float *r, float *z, size_t n);
 /* the network SR_PRECOND_LEARNED uses in the world's heat pass (no counterpart in the reference) */
 int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
+/* the heat pass's solves: SR_SOLVE_HOST (the default), SR_SOLVE_DEVICE or SR_SOLVE_GRAPH */
+int sr_set_solve_mode(sr_world *wd, int mode);
 int sr_set_dt_cap(sr_world *wd, double cap);                  /* Twin::cap_dt_next (reference/mod.rs:225) */
 int sr_set_timing(sr_world *wd, int on);                      /* Step::set_timing (step/mod.rs:722) */
 int sr_read_cost(sr_world *wd, sr_step_cost *out);            /* Step::read_cost (step/mod.rs:753) */
@@ -269,8 +283,11 @@ int sr_read_cost(sr_world *wd, sr_step_cost *out);            /* Step::read_cost
 int sr_read_work(sr_world *wd, sr_step_work *out, int reset);
 void sr_destroy(sr_world *wd);
 
-/* Timing of the dominant kernel inside the last sr_run with timing on: launches and mean device ms per launch,
- * measured with HIP events on the world's stream; `kernel` names: "hydro_update", "fft_columns", "mg_smooth". */
+/* Timing of a kernel over the steps run with timing on (since sr_set_timing switched it on): launches, mean device ms
+ * per launch (HIP events on the world's stream around each launch), and the algorithmic HBM bytes of one launch.
+ * `kernel`: "hydro_update", "hydro_mid" (P4), "mg_pre0", "mg_post0" (the level-0 smoothers), "mg_coarse" (the
+ * single-block V-cycle), "fld_matvec", "fld_update" (the PCG iteration's two kernels). No counterpart in the reference
+ * (its read_cost times passes). */
 int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launches, double *mean_ms, double *bytes_per_launch);
 
 #ifdef __cplusplus

@@ -35,6 +35,10 @@ namespace sr {
 #define COARSE_LDS (150 * 1024)
 #define DENSE_MAX 16
 #define NET_RATE_AT 4 // learned iterations before its residual rate is compared with plain multigrid's
+// Restarts of the inner FP32 loop from the FP64 true residual before a solve gives up: where the FP32 inner solves
+// cannot carry the true residual below the goal (a goal at the edge of what they resolve), a Newton pass takes the
+// iterate as it stands rather than spending the pass's whole round budget
+#define MAX_REFINEMENTS 50
 
 // ------------------------------------------------------------------------------------------------ elementwise kernels
 
@@ -45,6 +49,20 @@ struct L0 {
 };
 
 __device__ __forceinline__ bool is_gas(const L0 &a, size_t i) { return a.gas == nullptr || a.gas[i]; }
+
+// The state of a device-decided solve (FldSolver::solve with the slots on): the slot head (k_slot_head) reads it,
+// decides, and writes the next copy (double-buffered by slot parity, so no block reads a copy another block of the
+// same kernel writes); the slot's other kernels read the phase (and the inner index) of the copy the head wrote.
+#define PH_TRUE 0   // the next slot opens with the FP64 true residual
+#define PH_INNER 1  // the inner iteration runs
+#define PH_DONE 2   // converged, out of iterations, or not finite: every later slot is a no-op
+struct SolveCtl {
+    int phase, it, k, refinements, converged, finite, max_iter, ref;
+    double tol, u_bound, t_bound, rel, rel0, stop2, rr, bb;
+};
+
+// a kernel of a slot runs only while the inner iteration does (gate: the phase the slot's head wrote; null: always)
+__device__ __forceinline__ bool gated_off(const int *gate) { return gate && *gate != PH_INNER; }
 
 #define UNROLL 4
 
@@ -84,6 +102,129 @@ __global__ void __launch_bounds__(NT) k_true_residual(L0 a, const double *__rest
     block_partials<4>(v, part, gridDim.x);
 }
 
+// A device-decided solve's state at its start (one thread)
+__global__ void k_ctl_init(SolveCtl *c, double tol, int max_iter, int ref, double u_bound, double t_bound) {
+    SolveCtl v{};
+    v.phase = PH_TRUE;
+    v.finite = 1;
+    v.max_iter = max_iter;
+    v.ref = ref;
+    v.tol = tol;
+    v.u_bound = u_bound;
+    v.t_bound = t_bound;
+    v.rel0 = -1.0;
+    *c = v;
+}
+
+// The head of a slot (one PCG iteration of a device-decided solve): from the state the last slot left (cin) it decides
+// what this slot does and writes that into cout for the slot's other kernels.
+//   * after an inner iteration: ||r||^2 from the update's partials (every block sums them alike); not finite: done;
+//     at or below the inner goal, or out of iterations: the true residual follows; else the next inner iteration;
+//   * at the start or after the inner loop ends: the FP64 true residual r = b - A x (as k_true_residual), and the
+//     last block to finish (a ticket counter, the only device-wide fence of the solve, taken only here) sums its
+//     partials and decides: converged (the relative or the reference's rule), out of iterations, not finite, or a
+//     restart of the inner FP32 loop from that residual (beta = 0, the inner goal 1e-4 of it or a quarter of the
+//     outer goal, as the host-decided loop).
+__global__ void __launch_bounds__(NT) k_slot_head(L0 a, const double *__restrict__ b, double *__restrict__ x,
+                                                  float *__restrict__ r32, const float *__restrict__ heat,
+                                                  const float *__restrict__ slope, double *part,
+                                                  const double *part_rr, int nrr, const SolveCtl *cin,
+                                                  SolveCtl *cout, unsigned *ticket) {
+    SolveCtl c = *cin;
+    if (c.phase == PH_DONE) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *cout = c;
+        return;
+    }
+    if (c.phase == PH_INNER) {
+        const double rr = sum_partials(part_rr, nrr);
+        c.it++;
+        c.rr = rr;
+        if (!isfinite(rr)) {
+            c.finite = 0;
+            c.phase = PH_DONE;
+        } else if (rr > c.stop2 && c.it < c.max_iter) {
+            c.k++;
+        } else {
+            c.phase = PH_TRUE;
+        }
+        if (c.phase != PH_TRUE) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) *cout = c;
+            return;
+        }
+    }
+    const int n = a.w * a.h;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        const int xx = i % a.w, yy = i / a.w;
+        double r = 0.0;
+        if (is_gas(a, i)) {
+            const double xi = x[i];
+            const double al = a.ax[(size_t)yy * (a.w + 1) + xx], ar = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
+            const double au = a.ay[(size_t)yy * a.w + xx], ad = a.ay[(size_t)(yy + 1) * a.w + xx];
+            double ax = (double)a.own[i] * xi;
+            if (al != 0.0) ax += al * (xi - x[i - 1]);
+            if (ar != 0.0) ax += ar * (xi - x[i + 1]);
+            if (au != 0.0) ax += au * (xi - x[i - a.w]);
+            if (ad != 0.0) ax += ad * (xi - x[i + a.w]);
+            r = b[i] - ax;
+            v[1] += b[i] * b[i];
+            if (heat) {
+                const double d = (double)a.own[i] + al + ar + au + ad;
+                v[2] += r * r / d;
+                const double e = r * (double)slope[i] / d;
+                v[3] += (double)heat[i] * e * e;
+            }
+        } else {
+            x[i] = 0.0;
+        }
+        r32[i] = (float)r;
+        v[0] += r * r;
+    }
+    block_partials<4>(v, part, gridDim.x);
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's partials, before its ticket
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every block's partials, after the last ticket
+    const volatile double *vp = part;
+    __shared__ double scratch[NT / 64];
+    double t[4];
+    for (int k = 0; k < 4; k++) {
+        double sum = 0.0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += NT) sum += vp[(size_t)k * gridDim.x + i];
+        t[k] = block_sum(sum, scratch);
+    }
+    if (threadIdx.x != 0) return;
+    *ticket = 0u;
+    const double rr = t[0], bb = t[1] > 0.0 ? t[1] : 1.0;
+    c.rr = rr;
+    c.bb = bb;
+    c.rel = sqrt(rr / bb);
+    if (c.rel0 < 0.0) c.rel0 = c.rel;
+    const bool fin = isfinite(rr) && (!c.ref || (isfinite(t[2]) && isfinite(t[3])));
+    const bool conv = c.ref ? (t[2] <= c.u_bound && t[3] <= c.t_bound) : (sqrt(rr) <= c.tol * sqrt(bb));
+    if (!fin) {
+        c.finite = 0;
+        c.phase = PH_DONE;
+    } else if (conv) {
+        c.converged = 1;
+        c.phase = PH_DONE;
+    } else if (c.it >= c.max_iter || c.refinements >= MAX_REFINEMENTS) {
+        c.phase = PH_DONE;
+    } else {
+        const double goal2 = c.ref ? 0.0 : c.tol * c.tol * bb;
+        c.stop2 = fmax(1e-8 * rr, 0.25 * goal2);
+        c.k = 0;
+        c.refinements++;
+        c.phase = PH_INNER;
+    }
+    *cout = c;
+}
+
 // beta from the preconditioner's partials: (r.z_new - r.z_old) / (r.z of the previous iteration), 0 on a restart
 __device__ inline float beta_of(const double *dots_cur, const double *dots_prev, int nbd, int nbp, int restart) {
     if (restart) return 0.0f;
@@ -101,7 +242,11 @@ __device__ inline float beta_of(const double *dots_cur, const double *dots_prev,
 __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__ z, const float *__restrict__ pold,
                                                  float *__restrict__ pnew, float *__restrict__ q,
                                                  const double *dots_cur, const double *dots_prev, int nbd, int nbp,
-                                                 int restart, int rows, double *part) {
+                                                 int restart, int rows, double *part, const SolveCtl *ctl) {
+    if (ctl) {  // a slot: gated by its head, a restart where the head set the inner index to 0
+        if (ctl->phase != PH_INNER) return;
+        restart = ctl->k == 0;
+    }
     const float beta = beta_of(dots_cur, dots_prev, nbd, nbp, restart);
     const int w = a.w, h = a.h;
     const int tile = xcd_tile();
@@ -144,7 +289,9 @@ __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__
 // x += alpha p (FP64), r -= alpha q, alpha = r.z / p.q from the partials; partials of ||r||^2
 __global__ void __launch_bounds__(NT) k_update(int n, const float *__restrict__ p, const float *__restrict__ q,
                                                double *__restrict__ x, float *__restrict__ r, const double *dots_cur,
-                                               int nbd, const double *pq_part, int nbq, double *part) {
+                                               int nbd, const double *pq_part, int nbq, double *part,
+                                               const int *gate) {
+    if (gated_off(gate)) return;
     const double rz = sum_partials(dots_cur, nbd), pq = sum_partials(pq_part, nbq);
     const double alpha = pq != 0.0 ? rz / pq : 0.0;
     const float af = (float)alpha;
@@ -178,7 +325,8 @@ __global__ void __launch_bounds__(NT) k_update(int n, const float *__restrict__ 
 
 // Jacobi: z = r / D, D = own + sum of couplings; partials of r.z and r.z_old
 __global__ void __launch_bounds__(NT) k_jacobi(L0 a, const float *__restrict__ r, float *__restrict__ z,
-                                               const float *__restrict__ zold, double *part) {
+                                               const float *__restrict__ zold, double *part, const int *gate) {
+    if (gated_off(gate)) return;
     const int n = a.w * a.h;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
@@ -195,7 +343,8 @@ __global__ void __launch_bounds__(NT) k_jacobi(L0 a, const float *__restrict__ r
 
 // partials of r.z and r.z_old after a preconditioner that made none
 __global__ void __launch_bounds__(NT) k_dots(int n, const float *__restrict__ r, const float *__restrict__ z,
-                                             const float *__restrict__ zold, double *part) {
+                                             const float *__restrict__ zold, double *part, const int *gate) {
+    if (gated_off(gate)) return;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
         v[0] += (double)r[i] * z[i];
@@ -334,7 +483,8 @@ __device__ inline void sweeps(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0
 // restriction into the next level's b. Halo 2 nu + 1.
 template <int TX, int TY, int NU>
 __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, float *__restrict__ z,
-                                            float *__restrict__ bc, int wc) {
+                                            float *__restrict__ bc, int wc, const int *gate) {
+    if (gated_off(gate)) return;
     constexpr int HW = 2 * NU + 1;
     using RG = Region<TX, TY, HW>;
     __shared__ RG R;
@@ -380,7 +530,9 @@ __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, f
 template <int TX, int TY, int NU>
 __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, const float *__restrict__ zin,
                                              const float *__restrict__ zc, int wc, float omega,
-                                             float *__restrict__ zout, const float *__restrict__ zold, double *part) {
+                                             float *__restrict__ zout, const float *__restrict__ zold, double *part,
+                                             const int *gate) {
+    if (gated_off(gate)) return;
     constexpr int HW = 2 * NU;
     using RG = Region<TX, TY, HW>;
     __shared__ RG R;
@@ -424,48 +576,96 @@ __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, 
     if (part) block_partials<2>(v, part, gridDim.x * gridDim.y, tile);  // partials in tile order
 }
 
-// The V-cycle over every level of at most COARSE_MAX cells in one block. The levels' own, ax and ay arrays, the
-// coarsest's inverse and every cell's inverse diagonal (both formed at the rebuild, k_coarse_prep) lie back to back in
-// one packed buffer, so one batch of 16-byte loads per thread fills the LDS; z and b of every level follow it in LDS.
-// The first level's b comes from global memory in the same batch, and its z goes back there. Every phase is swept by the whole block between block barriers (measured: one wave alone, with
-// no block barrier, is slower even on a 16x16 level, its LDS latency no longer hidden by the other waves). The
-// coarsest is solved exactly by the inverse formed at the rebuild when it holds at most DENSE_MAX cells, else by
-// COARSE_SWEEPS forward and backward sweeps.
+// The V-cycle over every level of at most COARSE_MAX cells in one block (the "set"). The levels' own, ax and ay
+// arrays, the coarsest's inverse and every cell's inverse diagonal (both formed at the rebuild, k_coarse_prep) lie in
+// one packed buffer. Every phase is swept by the whole block between block barriers (measured: one wave alone, with no
+// block barrier, is slower even on a 16x16 level, its LDS latency no longer hidden by the other waves). The coarsest is
+// solved exactly by the inverse formed at the rebuild when it holds at most DENSE_MAX cells, else by COARSE_SWEEPS
+// forward and backward sweeps.
+//
+// With a head (HEAD = true; sets of three levels or more whose first level fits the block's 2 TX x 2 TY cover) the
+// set's two largest levels run register-resident: thread (tx, ty) holds the 2x2 block of the first level at (2 tx,
+// 2 ty) (its b, z, own, four couplings and inverse diagonal per cell) and the second level's cell (tx, ty), the
+// aggregate of that block, so the restriction from the first level and the prolongation into it need no exchange;
+// a half sweep reads only the neighbours outside the thread's cells from LDS and publishes the cells it updated.
+// The remaining ("tail") levels run from LDS as without a head. Every cell's arithmetic, and the order of every sum,
+// is the one the tail levels use, so the head changes where values live, not what is computed.
 struct CoarseSet {
     int n;
     int w[16], h[16];
-    int pack_floats;   // own/ax/ay of every level, the coarsest's inverse, the inverse diagonals
-    int offo[16], offx[16], offy[16], offz[16], offb[16], offd[16], offinv, total;  // LDS offsets (floats)
+    int pack_floats;   // the tail's own/ax/ay per level, the coarsest's inverse, the tail's inverse diagonals: what
+                       // the block loads into LDS (the whole pack without a head)
+    int offo[16], offx[16], offy[16], offd[16], offinv;  // offsets in the pack (= in LDS for the tail's levels)
+    int offz[16], offb[16], total;  // LDS offsets of the tail levels' z and b; LDS floats in all
     int lw[16];        // log2 of the level's width when a power of two, else -1
+    int head, ltx;     // register-resident levels (0 or 2); log2 TX of the block's TX x (COARSE_NT / TX) layout
+    int offza, offzb, offrb;  // LDS: the head levels' published z, the second level's residuals
 };
 
 // (y, x) of entry t in rows of width w: a shift when w is a power of two (lw >= 0), a division otherwise
 __device__ __forceinline__ int row_of(int t, int w, int lw) { return lw >= 0 ? t >> lw : t / w; }
 
+template <bool HEAD>
 __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *__restrict__ pack,
                                                       const float *__restrict__ b0, float *__restrict__ z0, int nu1,
-                                                      int nu2, float omega) {
+                                                      int nu2, float omega, const int *gate) {
+    if (gated_off(gate)) return;
     extern __shared__ float4 sm4[];
     float *sm = (float *)sm4;
     const int *offo = C.offo, *offx = C.offx, *offy = C.offy, *offz = C.offz, *offb = C.offb, *offd = C.offd;
-    const int offinv = C.offinv, total = C.total, n0 = C.w[0] * C.h[0], ob0 = offb[0];
-    // one phase: every load issued before any is used (the pack as float4, level 0's b), then zeros elsewhere (every
-    // entry a sweep may read at a grid's rim is finite)
+    const int offinv = C.offinv, total = C.total, n0 = C.w[0] * C.h[0];
+    const int T = HEAD ? 2 : 0;  // the first tail level
+    const int ob0 = HEAD ? total : offb[0];
+    // the head's registers: the first level's 2x2 block (cell k = 2 j + i at (x0 + i, y0 + j)), the second level's
+    // cell (tx, ty)
+    const int tx = HEAD ? (int)(threadIdx.x & ((1u << C.ltx) - 1)) : 0, ty = HEAD ? (int)(threadIdx.x >> C.ltx) : 0;
+    const int wA = C.w[0], hA = C.h[0], wB = C.w[1], hB = C.h[1], x0 = 2 * tx, y0 = 2 * ty;
+    float ab[4], az[4], ao[4], ard[4], al[4], ar[4], au[4], ad[4];
+    float bb = 0.0f, zb = 0.0f, bo = 0.0f, brd = 0.0f, bl = 0.0f, br = 0.0f, bu = 0.0f, bd = 0.0f;
+    const bool vb = HEAD && tx < wB && ty < hB;
+    // one phase: every load issued before any is used (the pack's LDS part as float4, level 0's b or the head's
+    // registers), then zeros elsewhere (every entry a sweep may read at a grid's rim is finite)
     {
         constexpr int V4 = (COARSE_LDS / 16 + COARSE_NT - 1) / COARSE_NT, VB = COARSE_MAX / COARSE_NT;
         const int np4 = C.pack_floats >> 2;
         const float4 *p4 = (const float4 *)pack;
         float4 v[V4];
-        float vb[VB];
+        float vb0[VB];
 #pragma unroll
         for (int k = 0; k < V4; k++) {
             const int t = threadIdx.x + k * COARSE_NT;
             v[k] = t < np4 ? p4[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
+        if constexpr (HEAD) {
 #pragma unroll
-        for (int k = 0; k < VB; k++) {
-            const int t = threadIdx.x + k * COARSE_NT;
-            vb[k] = t < n0 ? b0[t] : 0.0f;
+            for (int k = 0; k < 4; k++) {
+                const int x = x0 + (k & 1), y = y0 + (k >> 1);
+                const bool in = x < wA && y < hA;
+                const int i = y * wA + x;
+                ab[k] = in ? b0[i] : 0.0f;
+                ao[k] = in ? pack[offo[0] + i] : 0.0f;
+                ard[k] = in ? pack[offd[0] + i] : 0.0f;
+                al[k] = in ? pack[offx[0] + y * (wA + 1) + x] : 0.0f;
+                ar[k] = in ? pack[offx[0] + y * (wA + 1) + x + 1] : 0.0f;
+                au[k] = in ? pack[offy[0] + i] : 0.0f;
+                ad[k] = in ? pack[offy[0] + i + wA] : 0.0f;
+                az[k] = 0.0f;
+            }
+            if (vb) {
+                const int i = ty * wB + tx;
+                bo = pack[offo[1] + i];
+                brd = pack[offd[1] + i];
+                bl = pack[offx[1] + ty * (wB + 1) + tx];
+                br = pack[offx[1] + ty * (wB + 1) + tx + 1];
+                bu = pack[offy[1] + i];
+                bd = pack[offy[1] + i + wB];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VB; k++) {
+                const int t = threadIdx.x + k * COARSE_NT;
+                vb0[k] = t < n0 ? b0[t] : 0.0f;
+            }
         }
         const int tail = 4 * np4 + (int)threadIdx.x;
         const float vt = tail < C.pack_floats ? pack[tail] : 0.0f;
@@ -474,40 +674,42 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
             const int t = threadIdx.x + k * COARSE_NT;
             if (t < np4) sm4[t] = v[k];
         }
+        if constexpr (!HEAD) {
 #pragma unroll
-        for (int k = 0; k < VB; k++) {
-            const int t = threadIdx.x + k * COARSE_NT;
-            if (t < n0) sm[ob0 + t] = vb[k];
+            for (int k = 0; k < VB; k++) {
+                const int t = threadIdx.x + k * COARSE_NT;
+                if (t < n0) sm[ob0 + t] = vb0[k];
+            }
         }
         if (tail < C.pack_floats) sm[tail] = vt;
     }
     for (int t = C.pack_floats + threadIdx.x; t < total; t += COARSE_NT)
         if (t < ob0 || t >= ob0 + n0) sm[t] = 0.0f;
     __syncthreads();
-    // one half sweep of `color` on level l, colour-compact. Couplings on the grid's rim are
-    // zero, so the neighbour entries read there (the adjacent LDS arrays, all finite) contribute nothing.
+    // one half sweep of `color` on tail level l, colour-compact. Couplings on the grid's rim are zero, so the
+    // neighbour entries read there (the adjacent LDS arrays, all finite) contribute nothing.
     auto sweep = [&](int l, int color) {
         const int w = C.w[l], h = C.h[l], hw = (w + 1) >> 1, lh = C.lw[l] > 0 ? C.lw[l] - 1 : -1;
-        const float *al = sm + offx[l], *au = sm + offy[l], *b = sm + offb[l], *rd = sm + offd[l];
+        const float *alx = sm + offx[l], *auy = sm + offy[l], *b = sm + offb[l], *rd = sm + offd[l];
         float *z = sm + offz[l];
         for (int t = threadIdx.x; t < hw * h; t += COARSE_NT) {
             const int y = row_of(t, hw, lh);
             const int x = 2 * (t - y * hw) + ((color + y) & 1);
             if (x >= w) continue;
             const int i = y * w + x;
-            const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[i], a3 = au[i + w];
+            const float a0 = alx[y * (w + 1) + x], a1 = alx[y * (w + 1) + x + 1], a2 = auy[i], a3 = auy[i + w];
             z[i] = (b[i] + a0 * z[i - 1] + a1 * z[i + 1] + a2 * z[i - w] + a3 * z[i + w]) * rd[i];
         }
         __syncthreads();
     };
-    // pre-smoothing of level l from z = 0 and the restriction of its residual into level l + 1's b
+    // pre-smoothing of tail level l from z = 0 and the restriction of its residual into level l + 1's b
     auto down = [&](int l) {
         for (int k = 0; k < nu1; k++) {
             sweep(l, 0);
             sweep(l, 1);
         }
         const int w = C.w[l], h = C.h[l], wc = C.w[l + 1], hc = C.h[l + 1];
-        const float *own = sm + offo[l], *al = sm + offx[l], *au = sm + offy[l], *z = sm + offz[l],
+        const float *own = sm + offo[l], *alx = sm + offx[l], *auy = sm + offy[l], *z = sm + offz[l],
                     *b = sm + offb[l];
         float *bc = sm + offb[l + 1];
         for (int t = threadIdx.x; t < wc * hc; t += COARSE_NT) {
@@ -520,17 +722,18 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
                     const int x = 2 * X + dx, y = 2 * Y + dy;
                     if (x >= w || y >= h) continue;
                     const int i = y * w + x;
-                    const float a0 = al[y * (w + 1) + x], a1 = al[y * (w + 1) + x + 1], a2 = au[i], a3 = au[i + w];
+                    const float a0 = alx[y * (w + 1) + x], a1 = alx[y * (w + 1) + x + 1], a2 = auy[i],
+                                a3 = auy[i + w];
                     const float zi = z[i];
-                    const float az = own[i] * zi + a0 * (zi - z[i - 1]) + a1 * (zi - z[i + 1]) +
-                                     a2 * (zi - z[i - w]) + a3 * (zi - z[i + w]);
-                    s += b[i] - az;
+                    const float azi = own[i] * zi + a0 * (zi - z[i - 1]) + a1 * (zi - z[i + 1]) +
+                                      a2 * (zi - z[i - w]) + a3 * (zi - z[i + w]);
+                    s += b[i] - azi;
                 }
             bc[t] = s;
         }
         __syncthreads();
     };
-    // the prolongation of level l + 1's correction (scaled by omega) and the post-smoothing of level l
+    // the prolongation of level l + 1's correction (scaled by omega) and the post-smoothing of tail level l
     auto up = [&](int l) {
         const int w = C.w[l], h = C.h[l], wc = C.w[l + 1];
         float *z = sm + offz[l];
@@ -569,10 +772,137 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
             }
         }
     };
-    for (int l = 0; l < lc; l++) down(l);
+    // the head's LDS: the first and second level's published z, the second level's residuals
+    float *za = sm + C.offza, *zbl = sm + C.offzb, *rb = sm + C.offrb;
+    // a first-level z outside the thread's block (clamped into the grid: a rim coupling is zero)
+    auto ZA = [&](int x, int y) {
+        return za[min(max(y, 0), hA - 1) * wA + min(max(x, 0), wA - 1)];
+    };
+    auto ZB = [&](int x, int y) {
+        return zbl[min(max(y, 0), hB - 1) * wB + min(max(x, 0), wB - 1)];
+    };
+    auto publish_a = [&](int k) {
+        const int x = x0 + (k & 1), y = y0 + (k >> 1);
+        if (x < wA && y < hA) za[y * wA + x] = az[k];
+    };
+    // a half sweep of the first level: colour 0 holds cells 0 and 3 of the block, colour 1 cells 1 and 2
+    auto sweep_a = [&](int color) {
+        if (color == 0) {
+            az[0] = (ab[0] + al[0] * ZA(x0 - 1, y0) + ar[0] * az[1] + au[0] * ZA(x0, y0 - 1) + ad[0] * az[2]) *
+                    ard[0];
+            az[3] = (ab[3] + al[3] * az[2] + ar[3] * ZA(x0 + 2, y0 + 1) + au[3] * az[1] +
+                     ad[3] * ZA(x0 + 1, y0 + 2)) *
+                    ard[3];
+            publish_a(0);
+            publish_a(3);
+        } else {
+            az[1] = (ab[1] + al[1] * az[0] + ar[1] * ZA(x0 + 2, y0) + au[1] * ZA(x0 + 1, y0 - 1) + ad[1] * az[3]) *
+                    ard[1];
+            az[2] = (ab[2] + al[2] * ZA(x0 - 1, y0 + 1) + ar[2] * az[3] + au[2] * az[0] +
+                     ad[2] * ZA(x0, y0 + 2)) *
+                    ard[2];
+            publish_a(1);
+            publish_a(2);
+        }
+        __syncthreads();
+    };
+    auto sweep_b = [&](int color) {
+        if (vb && ((tx + ty) & 1) == color) {
+            zb = (bb + bl * ZB(tx - 1, ty) + br * ZB(tx + 1, ty) + bu * ZB(tx, ty - 1) + bd * ZB(tx, ty + 1)) *
+                 brd;
+            zbl[ty * wB + tx] = zb;
+        }
+        __syncthreads();
+    };
+    if constexpr (HEAD) {
+        // the first level: pre-smoothing from z = 0, then the residual of the block restricted into the thread's
+        // second-level cell (the block's cells in row order, as the tail's restriction sums them)
+        for (int k = 0; k < nu1; k++) {
+            sweep_a(0);
+            sweep_a(1);
+        }
+        {
+            float s = 0.0f;
+            const float zn[4][4] = {{ZA(x0 - 1, y0), az[1], ZA(x0, y0 - 1), az[2]},
+                                    {az[0], ZA(x0 + 2, y0), ZA(x0 + 1, y0 - 1), az[3]},
+                                    {ZA(x0 - 1, y0 + 1), az[3], az[0], ZA(x0, y0 + 2)},
+                                    {az[2], ZA(x0 + 2, y0 + 1), az[1], ZA(x0 + 1, y0 + 2)}};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int x = x0 + (k & 1), y = y0 + (k >> 1);
+                if (x >= wA || y >= hA) continue;
+                const float zi = az[k];
+                const float azi = ao[k] * zi + al[k] * (zi - zn[k][0]) + ar[k] * (zi - zn[k][1]) +
+                                  au[k] * (zi - zn[k][2]) + ad[k] * (zi - zn[k][3]);
+                s += ab[k] - azi;
+            }
+            bb = s;
+        }
+        // the second level: pre-smoothing, its residual published for the tail's restriction
+        for (int k = 0; k < nu1; k++) {
+            sweep_b(0);
+            sweep_b(1);
+        }
+        if (vb) {
+            const float zi = zb;
+            const float azi = bo * zi + bl * (zi - ZB(tx - 1, ty)) + br * (zi - ZB(tx + 1, ty)) +
+                              bu * (zi - ZB(tx, ty - 1)) + bd * (zi - ZB(tx, ty + 1));
+            rb[ty * wB + tx] = bb - azi;
+        }
+        __syncthreads();
+        {  // the first tail level's b: the second level's residuals summed over each aggregate
+            const int wc = C.w[T], hc = C.h[T];
+            float *bc = sm + offb[T];
+            for (int t = threadIdx.x; t < wc * hc; t += COARSE_NT) {
+                const int Y = row_of(t, wc, C.lw[T]), X = t - Y * wc;
+                float s = 0.0f;
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                    for (int dx = 0; dx < 2; dx++) {
+                        const int x = 2 * X + dx, y = 2 * Y + dy;
+                        if (x >= wB || y >= hB) continue;
+                        s += rb[y * wB + x];
+                    }
+                bc[t] = s;
+            }
+            __syncthreads();
+        }
+    }
+    for (int l = T; l < lc; l++) down(l);
     coarsest();
-    for (int l = lc - 1; l >= 0; l--) up(l);
-    for (int t = threadIdx.x; t < n0; t += COARSE_NT) z0[t] = sm[offz[0] + t];
+    for (int l = lc - 1; l >= T; l--) up(l);
+    if constexpr (HEAD) {
+        // the second level: the tail's correction prolonged, published, post-smoothed
+        if (vb) {
+            zb += omega * sm[offz[T] + (ty >> 1) * C.w[T] + (tx >> 1)];
+            zbl[ty * wB + tx] = zb;
+        }
+        __syncthreads();
+        for (int k = 0; k < nu2; k++) {
+            sweep_b(1);
+            sweep_b(0);
+        }
+        // the first level: the second level's correction (the thread's own cell) prolonged, published,
+        // post-smoothed, written out
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            az[k] += omega * zb;
+            publish_a(k);
+        }
+        __syncthreads();
+        for (int k = 0; k < nu2; k++) {
+            sweep_a(1);
+            sweep_a(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int x = x0 + (k & 1), y = y0 + (k >> 1);
+            if (x < wA && y < hA) z0[y * wA + x] = az[k];
+        }
+    } else {
+        for (int t = threadIdx.x; t < n0; t += COARSE_NT) z0[t] = sm[offz[0] + t];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ hierarchy build
@@ -736,6 +1066,10 @@ FldSolver::~FldSolver() {
     for (void *x : ps)
         if (x) (void)hipFree(x);
     if (host_part) (void)hipHostFree(host_part);
+    if (host_ctl) (void)hipHostFree(host_ctl);
+    if (ctl) (void)hipFree(ctl);
+    if (ticket) (void)hipFree(ticket);
+    if (slot_exec) (void)hipGraphExecDestroy(slot_exec);
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
 }
 
@@ -782,21 +1116,31 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
             return fail(SR_ERR_ARGUMENT, "too few multigrid levels: the coarsest level must fit one block (" +
                                              std::to_string(COARSE_MAX) + " cells)");
     }
+    // the register-resident head: the set's two largest levels when the set has three or more and its first level
+    // fits the block's cover of 2 TX x 2 TY cells, TX a power of two and TX TY = COARSE_NT
+    head = 0;
+    ltx = 0;
+    if ((int)lv.size() - coarse_from >= 3) {
+        const int wa = lv[coarse_from].w, ha = lv[coarse_from].h;
+        int tx = 1;
+        while (2 * tx < wa) tx *= 2;
+        if (tx <= COARSE_NT && 2 * (COARSE_NT / tx) >= ha) {
+            head = 2;
+            ltx = ilog2(tx);
+        }
+    }
     pack_floats = DENSE_MAX * DENSE_MAX;
     for (int l = coarse_from; l < (int)lv.size(); l++)
         pack_floats += (int)level_floats(lv[l].w, lv[l].h) + lv[l].w * lv[l].h;  // + the inverse diagonals
     SR_HIP(hipMalloc(&pack, sizeof(float) * pack_floats));
     SR_HIP(hipMemset(pack, 0, sizeof(float) * pack_floats));
     {
-        float *o = pack;
-        for (int l = coarse_from; l < (int)lv.size(); l++) {
-            FldLevel &L = lv[l];
-            L.own = o;
-            o += L.w * L.h;
-            L.ax = o;
-            o += (L.w + 1) * L.h;
-            L.ay = o;
-            o += L.w * (L.h + 1);
+        const CoarseSet C = coarse_set();
+        for (int k = 0; k < C.n; k++) {
+            FldLevel &L = lv[coarse_from + k];
+            L.own = pack + C.offo[k];
+            L.ax = pack + C.offx[k];
+            L.ay = pack + C.offy[k];
         }
     }
     for (int l = 0; l < (int)lv.size(); l++) {
@@ -837,7 +1181,15 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
     SR_HIP(hipMalloc(&part_dots, sizeof(double) * 2 * 2 * nbd_max));
     SR_HIP(hipMemset(part_dots, 0, sizeof(double) * 2 * 2 * nbd_max));
     SR_HIP(hipHostMalloc(&host_part, sizeof(double) * 4 * nbe));
-    SR_HIP(hipFuncSetAttribute((const void *)k_coarse, hipFuncAttributeMaxDynamicSharedMemorySize, COARSE_LDS));
+    SR_HIP(hipMalloc(&ctl, sizeof(SolveCtl) * 2));
+    SR_HIP(hipMemset(ctl, 0, sizeof(SolveCtl) * 2));
+    SR_HIP(hipMalloc(&ticket, sizeof(unsigned)));
+    SR_HIP(hipMemset(ticket, 0, sizeof(unsigned)));
+    SR_HIP(hipHostMalloc(&host_ctl, sizeof(SolveCtl)));
+    SR_HIP(hipFuncSetAttribute((const void *)k_coarse<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               COARSE_LDS));
+    SR_HIP(hipFuncSetAttribute((const void *)k_coarse<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               COARSE_LDS));
     return SR_OK;
 }
 
@@ -856,38 +1208,60 @@ int FldSolver::set_system(const float *own, const float *ax, const float *ay, co
     return rebuild(true, s);
 }
 
-// the single-block levels' layout: in the pack (and at the same offsets in LDS) own, ax, ay per level, the
-// coarsest's inverse, the inverse diagonals per level; after the pack in LDS, z and b per level
+// The single-block levels' layout. The pack holds first the tail levels' own, ax, ay, the coarsest's inverse and the
+// tail levels' inverse diagonals (the part the block copies into LDS, at the same offsets there), then the head
+// levels' own, ax, ay and inverse diagonals (read into registers). In LDS after the tail part: z and b per tail level,
+// then the head's published z of its two levels and the second level's residuals.
 CoarseSet FldSolver::coarse_set() const {
     const int nl = (int)lv.size();
     CoarseSet C;
     C.n = nl - coarse_from;
-    C.pack_floats = pack_floats;
+    C.head = head;
+    C.ltx = ltx;
     int o = 0;
-    for (int l = coarse_from; l < nl; l++) {
-        const int k = l - coarse_from, w = lv[l].w, h = lv[l].h;
-        C.w[k] = w;
-        C.h[k] = h;
+    auto level = [&](int k) {
+        const int w = lv[coarse_from + k].w, h = lv[coarse_from + k].h;
         C.offo[k] = o;
         o += w * h;
         C.offx[k] = o;
         o += (w + 1) * h;
         C.offy[k] = o;
         o += w * (h + 1);
-        C.lw[k] = is_pow2(w) ? ilog2(w) : -1;
+    };
+    for (int k = 0; k < C.n; k++) {
+        C.w[k] = lv[coarse_from + k].w;
+        C.h[k] = lv[coarse_from + k].h;
+        C.lw[k] = is_pow2(C.w[k]) ? ilog2(C.w[k]) : -1;
     }
+    for (int k = head; k < C.n; k++) level(k);
     C.offinv = o;
     o += DENSE_MAX * DENSE_MAX;
-    for (int k = 0; k < C.n; k++) {
+    for (int k = head; k < C.n; k++) {
         C.offd[k] = o;
         o += C.w[k] * C.h[k];
     }
-    o = pack_floats;
-    for (int k = 0; k < C.n; k++) {
+    C.pack_floats = o;
+    for (int k = 0; k < head; k++) level(k);
+    for (int k = 0; k < head; k++) {
+        C.offd[k] = o;
+        o += C.w[k] * C.h[k];
+    }
+    // o is the whole pack's size here
+    o = C.pack_floats;
+    for (int k = head; k < C.n; k++) {
         C.offz[k] = o;
         o += C.w[k] * C.h[k];
         C.offb[k] = o;
         o += C.w[k] * C.h[k];
+    }
+    C.offza = C.offzb = C.offrb = o;
+    if (head) {
+        C.offza = o;
+        o += C.w[0] * C.h[0];
+        C.offzb = o;
+        o += C.w[1] * C.h[1];
+        C.offrb = o;
+        o += C.w[1] * C.h[1];
     }
     C.total = o;
     return C;
@@ -919,6 +1293,7 @@ int FldSolver::rebuild(bool faces, hipStream_t s) {
     if (!tuned) {
         tuned = true;
         if (autotune) SR_TRY(tune(s));
+        tiles_version++;
     }
     return SR_OK;
 }
@@ -1004,42 +1379,46 @@ int FldSolver::set_tiles(const int *pre, const int *post, int levels) {
     post_tile.assign(post, post + levels);
     tuned = true;
     autotune = false;
+    tiles_version++;
     return SR_OK;
 }
 
 // the 64x32 tile serves nu <= 2 only (its nu = 3 region would not leave room for two blocks per CU)
 template <int TX, int TY>
-static void launch_pre(int nu, hipStream_t s, Lv L, const float *b, float *z, float *bc, int wc) {
+static void launch_pre(int nu, hipStream_t s, Lv L, const float *b, float *z, float *bc, int wc, const int *gate) {
     const dim3 grid((L.w + TX - 1) / TX, (L.h + TY - 1) / TY);
     if (nu == 1)
-        hipLaunchKernelGGL((k_pre<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
+        hipLaunchKernelGGL((k_pre<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, z, bc, wc, gate);
     else if (nu == 2)
-        hipLaunchKernelGGL((k_pre<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
+        hipLaunchKernelGGL((k_pre<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, z, bc, wc, gate);
     else if constexpr (TX <= 32)
-        hipLaunchKernelGGL((k_pre<TX, TY, 3>), grid, dim3(NT), 0, s, L, b, z, bc, wc);
+        hipLaunchKernelGGL((k_pre<TX, TY, 3>), grid, dim3(NT), 0, s, L, b, z, bc, wc, gate);
 }
 
 template <int TX, int TY>
 static int launch_post(int nu, hipStream_t s, Lv L, const float *b, const float *zin, const float *zc, int wc,
-                       float omega, float *zout, const float *zold, double *part) {
+                       float omega, float *zout, const float *zold, double *part, const int *gate) {
     const dim3 grid((L.w + TX - 1) / TX, (L.h + TY - 1) / TY);
     if (nu == 1)
-        hipLaunchKernelGGL((k_post<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
+        hipLaunchKernelGGL((k_post<TX, TY, 1>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part,
+                           gate);
     else if (nu == 2)
-        hipLaunchKernelGGL((k_post<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
+        hipLaunchKernelGGL((k_post<TX, TY, 2>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part,
+                           gate);
     else if constexpr (TX <= 32)
-        hipLaunchKernelGGL((k_post<TX, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part);
+        hipLaunchKernelGGL((k_post<TX, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part,
+                           gate);
     return (int)(grid.x * grid.y);
 }
 
 // level l's pre-smoother, residual and restriction into level l + 1's b, in the level's tile shape
-int FldSolver::launch_pre_level(int l, const float *b, hipStream_t s) {
+int FldSolver::launch_pre_level(int l, const float *b, hipStream_t s, const int *gate) {
     const FldLevel &f = lv[l], &c = lv[l + 1];
     const Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
     switch (pre_tile[l]) {
-    case TILE_16: launch_pre<16, 16>(nu1, s, L, b, f.z, c.b, c.w); break;
-    case TILE_64: launch_pre<PTX, PTY>(nu1, s, L, b, f.z, c.b, c.w); break;
-    default: launch_pre<32, 32>(nu1, s, L, b, f.z, c.b, c.w); break;
+    case TILE_16: launch_pre<16, 16>(nu1, s, L, b, f.z, c.b, c.w, gate); break;
+    case TILE_64: launch_pre<PTX, PTY>(nu1, s, L, b, f.z, c.b, c.w, gate); break;
+    default: launch_pre<32, 32>(nu1, s, L, b, f.z, c.b, c.w, gate); break;
     }
     return SR_OK;
 }
@@ -1047,61 +1426,46 @@ int FldSolver::launch_pre_level(int l, const float *b, hipStream_t s) {
 // level l's prolongation of level l + 1's correction and post-smoother into zout; the launch's block count (the
 // number of dot partials when part is given) into *nblocks
 int FldSolver::launch_post_level(int l, const float *b, float *zout, const float *zold, double *part, int *nblocks,
-                                 hipStream_t s) {
+                                 hipStream_t s, const int *gate) {
     const FldLevel &f = lv[l], &c = lv[l + 1];
     const Lv L = {f.w, f.h, f.own, f.ax, f.ay, f.cf};
     int nb = 0;
     switch (post_tile[l]) {
-    case TILE_16: nb = launch_post<16, 16>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part); break;
-    case TILE_64: nb = launch_post<PTX, PTY>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part); break;
-    default: nb = launch_post<32, 32>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part); break;
+    case TILE_16: nb = launch_post<16, 16>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part, gate); break;
+    case TILE_64: nb = launch_post<PTX, PTY>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part, gate); break;
+    default: nb = launch_post<32, 32>(nu2, s, L, b, f.z, c.zp, c.w, omega, zout, zold, part, gate); break;
     }
     if (nblocks) *nblocks = nb;
     return SR_OK;
 }
 
 // z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
-int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s) {
+int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s,
+                      const int *gate) {
     const int nl = (int)lv.size();
     const float *bl = r;
     for (int l = 0; l < coarse_from; l++) {
         const FldLevel &c = lv[l + 1];
-        const bool t0 = timing && l == 0;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (t0) {
-            if (ev_used + 2 > 8192) {  // a bounded pool: fold what is recorded into the sums first
-                double ms;
-                unsigned long long nl;
-                SR_TRY(smooth_time(&ms, &nl));
-            }
-            if (ev_used + 2 > evs.size()) {  // grow the pool; recorded pairs are summed when read
-                for (int k = 0; k < 64; k++) {
-                    hipEvent_t e;
-                    // timing only: no system-scope fence (an L2 write-back) around the timed kernel
-                    SR_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-                    evs.push_back(e);
-                }
-            }
-            e0 = evs[ev_used];
-            e1 = evs[ev_used + 1];
-            ev_used += 2;
-            SR_HIP(hipEventRecord(e0, s));
-        }
-        SR_TRY(launch_pre_level(l, bl, s));
-        if (t0) {
-            SR_HIP(hipEventRecord(e1, s));
-        }
+        const int slot = l == 0 ? tick(CLK_PRE0, s) : -1;
+        SR_TRY(launch_pre_level(l, bl, s, gate));
+        SR_TRY(tock(slot, s));
         bl = c.b;
     }
     {
         const CoarseSet C = coarse_set();
         const size_t floats = C.total;
         float *zc = coarse_from == 0 ? z : lv[coarse_from].zp;
-        hipLaunchKernelGGL(k_coarse, dim3(1), dim3(COARSE_NT), sizeof(float) * floats, s, C, (const float *)pack, bl,
-                           zc, nu1, nu2, omega);
+        const int slot = tick(CLK_COARSE, s);
+        if (C.head)
+            hipLaunchKernelGGL(k_coarse<true>, dim3(1), dim3(COARSE_NT), sizeof(float) * floats, s, C,
+                               (const float *)pack, bl, zc, nu1, nu2, omega, gate);
+        else
+            hipLaunchKernelGGL(k_coarse<false>, dim3(1), dim3(COARSE_NT), sizeof(float) * floats, s, C,
+                               (const float *)pack, bl, zc, nu1, nu2, omega, gate);
+        SR_TRY(tock(slot, s));
         if (coarse_from == 0 && dots) {
             const int n = w0 * h0;
-            hipLaunchKernelGGL(k_dots, dim3(nbe), dim3(NT), 0, s, n, r, z, zold, dots);
+            hipLaunchKernelGGL(k_dots, dim3(nbe), dim3(NT), 0, s, n, r, z, zold, dots, gate);
             *nbd = nbe;
         }
     }
@@ -1111,31 +1475,81 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         float *zo = l == 0 ? z : f.zp;
         double *pp = (l == 0 && dots) ? dots : nullptr;
         int nb = 0;
-        SR_TRY(launch_post_level(l, b, zo, zold, pp, &nb, s));
+        const int slot = l == 0 ? tick(CLK_POST0, s) : -1;
+        SR_TRY(launch_post_level(l, b, zo, zold, pp, &nb, s, gate));
+        SR_TRY(tock(slot, s));
         if (l == 0 && dots) *nbd = nb;
     }
     SR_HIP(hipGetLastError());
     return SR_OK;
 }
 
-int FldSolver::smooth_time(double *ms, unsigned long long *launches) {
+int FldSolver::tick(int kernel, hipStream_t s) {
+    if (!timing) return -1;
+    if (ev_used + 2 > 8192) SR_TRY(fold_clocks());  // a bounded pool: fold what is recorded into the sums first
+    if (ev_used + 2 > evs.size()) {                  // grow the pool; recorded pairs are summed when read
+        for (int k = 0; k < 64; k++) {
+            hipEvent_t e;
+            // timing only: no system-scope fence (an L2 write-back) around the timed kernel
+            SR_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            evs.push_back(e);
+        }
+        ev_kernel.resize(evs.size() / 2);
+    }
+    const int slot = (int)ev_used;
+    ev_kernel[slot / 2] = kernel;
+    ev_used += 2;
+    SR_HIP(hipEventRecord(evs[slot], s));
+    return slot;
+}
+
+int FldSolver::tock(int slot, hipStream_t s) {
+    if (slot >= 0) SR_HIP(hipEventRecord(evs[slot + 1], s));
+    return SR_OK;
+}
+
+int FldSolver::fold_clocks() {
     for (size_t k = 0; k + 1 < ev_used; k += 2) {
         SR_HIP(hipEventSynchronize(evs[k + 1]));
         float t = 0.f;
         SR_HIP(hipEventElapsedTime(&t, evs[k], evs[k + 1]));
-        smooth_ms += t;
-        smooth_launches++;
+        clock_ms[ev_kernel[k / 2]] += t;
+        clock_n[ev_kernel[k / 2]]++;
     }
     ev_used = 0;
-    *ms = smooth_ms;
-    *launches = smooth_launches;
     return SR_OK;
 }
 
-void FldSolver::reset_smooth_time() {
+int FldSolver::clock(int kernel, double *ms, unsigned long long *launches) {
+    if (kernel < 0 || kernel >= CLK_N) return fail(SR_ERR_ARGUMENT, "no such solver clock");
+    SR_TRY(fold_clocks());
+    *ms = clock_ms[kernel];
+    *launches = clock_n[kernel];
+    return SR_OK;
+}
+
+void FldSolver::reset_clocks() {
     ev_used = 0;
-    smooth_ms = 0.0;
-    smooth_launches = 0;
+    for (int k = 0; k < CLK_N; k++) {
+        clock_ms[k] = 0.0;
+        clock_n[k] = 0;
+    }
+}
+
+// Per launch at level 0 (f32 unless noted): the pre-smoother reads b, own, ax, ay and writes z and the coarse b (21
+// B per fine cell); the post-smoother reads b, own, ax, ay, z, the coarse z and z_old and writes z (33); the matvec
+// reads z and p and own, ax, ay and writes p and q (28); the update reads p, q, r and writes r, and reads and writes x
+// in f64 (32); the single-block V-cycle reads the pack and its first level's b and writes that level's z.
+double FldSolver::clock_bytes(int kernel) const {
+    const double n = (double)w0 * h0;
+    switch (kernel) {
+    case CLK_PRE0: return 21.0 * n;
+    case CLK_POST0: return 33.0 * n;
+    case CLK_MATVEC: return 28.0 * n;
+    case CLK_UPDATE: return 32.0 * n;
+    case CLK_COARSE: return 4.0 * (pack_floats + 2.0 * lv[coarse_from].w * lv[coarse_from].h);
+    default: return 0.0;
+    }
 }
 
 int FldSolver::set_network(Net *n, int fallback) {
@@ -1162,7 +1576,8 @@ int FldSolver::learned(const float *r, float *z, const float *zold, double *dots
 int FldSolver::apply_precond(const float *r, float *z, int precond, hipStream_t s) {
     L0 a = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
     if (precond == 0) {
-        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, r, z, (const float *)nullptr, (double *)nullptr);
+        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, r, z, (const float *)nullptr, (double *)nullptr,
+                           (const int *)nullptr);
         SR_HIP(hipGetLastError());
         return SR_OK;
     }
@@ -1191,11 +1606,13 @@ int FldSolver::read_partials(const double *part, int k, int nb, double *out, hip
 }
 
 int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int precond, const RefStop *ref,
-                     FldStats *st, hipStream_t s) {
+                     FldStats *st, hipStream_t s, int expect) {
     L0 a = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
     const size_t n = (size_t)w0 * h0;
     if (precond == SR_PRECOND_LEARNED && !net)
         return fail(SR_ERR_ARGUMENT, "the learned preconditioner has no network (sr_fld_set_network)");
+    if (solve_mode != SOLVE_HOST && precond != SR_PRECOND_LEARNED)
+        return solve_slots(b, x, tol, max_iter, precond, ref, st, s, expect);
     bool use_net = precond == SR_PRECOND_LEARNED;
     const bool mg = precond != 0 && lv.size() > 1;
     *st = FldStats();
@@ -1208,7 +1625,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         }
         if (mg) return vcycle(r32, zc, zo, dots, nbd, s);
         L0 aa = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
-        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, aa, r32, zc, zo, dots);
+        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, aa, r32, zc, zo, dots, (const int *)nullptr);
         *nbd = nbe;
         return SR_OK;
     };
@@ -1221,7 +1638,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + 2047) / 2048));
     const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
     const int nmv = (int)(mv_grid.x * mv_grid.y);
-    for (int outer = 0; outer < 50; outer++) {
+    for (int outer = 0; outer < MAX_REFINEMENTS; outer++) {
         hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, x, r32, ref ? ref->heat : nullptr,
                            ref ? ref->slope : nullptr, part_true);
         work.true_residuals++;
@@ -1256,10 +1673,15 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         double rr_last = rr;
         if (use_net && net_it == 0) rr_net0 = rr;
         while (it < max_iter) {
+            int slot = tick(CLK_MATVEC, s);
             hipLaunchKernelGGL(k_matvec_p, mv_grid, dim3(NT), 0, s, a, zc, pold, pnew, q, dots[k & 1],
-                               dots[(k + 1) & 1], nbd, nbp, k == 0 ? 1 : 0, mv_rows, part_pq);
+                               dots[(k + 1) & 1], nbd, nbp, k == 0 ? 1 : 0, mv_rows, part_pq,
+                               (const SolveCtl *)nullptr);
+            SR_TRY(tock(slot, s));
+            slot = tick(CLK_UPDATE, s);
             hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, x, r32, dots[k & 1], nbd,
-                               part_pq, nmv, part_rr);
+                               part_pq, nmv, part_rr, (const int *)nullptr);
+            SR_TRY(tock(slot, s));
             it++;
             work.iterations++;
             if (use_net) net_it++;
@@ -1306,6 +1728,144 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     if (precond == SR_PRECOND_MULTIGRID && st->converged && it > 0 && rel0 > 0.0 && st->rel > 0.0)
         plain_rate = std::log(st->rel / rel0) / it;
     SR_HIP(hipGetLastError());
+    return SR_OK;
+}
+
+// One slot of a device-decided solve, parity `par` (its state read from ctl[par], written to ctl[1 - par]; the z, p and
+// dot-partial buffers alternate with the parity as the host-decided loop alternates them with the inner index): the
+// head, then z = M r, p = z + beta p with q = A p, and x, r updated, each skipping itself unless the head left the
+// inner iteration running.
+int FldSolver::launch_slot(int par, const double *b, double *x, const RefStop *ref, int precond, hipStream_t s) {
+    L0 a = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
+    const size_t n = (size_t)w0 * h0;
+    SolveCtl *cin = ctl + par, *cout = ctl + (1 - par);
+    const int *gate = &cout->phase;
+    hipLaunchKernelGGL(k_slot_head, dim3(nbe), dim3(NT), 0, s, a, b, x, r32, ref ? ref->heat : nullptr,
+                       ref ? ref->slope : nullptr, part_true, (const double *)part_rr, nbe, (const SolveCtl *)cin,
+                       cout, ticket);
+    float *zc = par ? zb : za, *zo = par ? za : zb;
+    float *pold = par ? p + n : p, *pnew = par ? p : p + n;
+    double *dcur = part_dots + (size_t)par * 2 * nbd_max, *dprev = part_dots + (size_t)(1 - par) * 2 * nbd_max;
+    int nbd = nbe;
+    if (precond != 0 && lv.size() > 1) {
+        SR_TRY(vcycle(r32, zc, zo, dcur, &nbd, s, gate));
+    } else {
+        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, a, (const float *)r32, zc, (const float *)zo, dcur,
+                           gate);
+    }
+    const int gxm = (w0 + NT - 1) / NT;
+    const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + 2047) / 2048));
+    const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
+    const int nmv = (int)(mv_grid.x * mv_grid.y);
+    int slot = tick(CLK_MATVEC, s);
+    hipLaunchKernelGGL(k_matvec_p, mv_grid, dim3(NT), 0, s, a, (const float *)zc, (const float *)pold, pnew, q,
+                       (const double *)dcur, (const double *)dprev, nbd, nbd, 0, mv_rows, part_pq,
+                       (const SolveCtl *)cout);
+    SR_TRY(tock(slot, s));
+    slot = tick(CLK_UPDATE, s);
+    hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, (const float *)pnew, (const float *)q, x, r32,
+                       (const double *)dcur, nbd, (const double *)part_pq, nmv, part_rr, gate);
+    SR_TRY(tock(slot, s));
+    work.precond++;  // an upper count: a slot that skips itself is counted too
+    work.iterations++;
+    SR_HIP(hipGetLastError());
+    return SR_OK;
+}
+
+// The device-decided solve: the state set, then pairs of slots (replayed as one captured graph, or launched one by
+// one with the clocks on or in SOLVE_DEVICE mode) until the state says done. The host reads the state between bursts
+// of pairs, each burst sized from the residual's rate at the last two reads (about 0.7 of the iterations it predicts
+// are left, as the host-decided loop spaces its checks); the pairs past the end cost their launches only.
+int FldSolver::solve_slots(const double *b, double *x, double tol, int max_iter, int precond, const RefStop *ref,
+                           FldStats *st, hipStream_t s, int expect) {
+    *st = FldStats();
+    const FldWork w_before = work;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, tol, max_iter, ref ? 1 : 0,
+                       ref ? ref->u_bound : 0.0, ref ? ref->t_bound : 0.0);
+    SR_HIP(hipGetLastError());
+    // the graph is captured on the caller's stream; the legacy default stream cannot be captured, so there (and with
+    // the clocks on) the same slots are launched one by one
+    bool graph = solve_mode == SOLVE_GRAPH && !timing && s != nullptr;
+    if (graph) {
+        GraphKey k;
+        k.b = b;
+        k.x = x;
+        k.heat = ref ? ref->heat : nullptr;
+        k.slope = ref ? ref->slope : nullptr;
+        k.precond = precond;
+        k.tiles = tiles_version;
+        k.s = s;
+        if (!slot_exec || memcmp(&k, &slot_key, sizeof k) != 0) {
+            if (slot_exec) SR_HIP(hipGraphExecDestroy(slot_exec));
+            slot_exec = nullptr;
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                (void)hipGetLastError();
+                graph = false;
+            }
+        }
+        if (graph && !slot_exec) {
+            hipGraph_t g = nullptr;
+            const int c0 = launch_slot(0, b, x, ref, precond, s);
+            const int c1 = c0 == SR_OK ? launch_slot(1, b, x, ref, precond, s) : c0;
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            SR_TRY(c1);
+            if (e != hipSuccess) return fail(SR_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&slot_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                slot_exec = nullptr;
+                return fail(SR_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            }
+            slot_key = k;
+        }
+    }
+    // the first burst: most of what a similar solve took (the caller's hint, else this solver's last solve); then
+    // bursts sized from the residual's rate, one pair once the goal is near
+    const int guess = expect > 0 ? expect : last_iterations;
+    int pairs = std::max(1, (int)(0.45 * guess));
+    int prev_it = -1;
+    double prev_rr = 0.0;
+    SolveCtl c{};
+    for (;;) {
+        for (int i = 0; i < pairs; i++) {
+            if (graph) {
+                SR_HIP(hipGraphLaunch(slot_exec, s));
+                work.precond += 2;
+                work.iterations += 2;
+            } else {
+                SR_TRY(launch_slot(0, b, x, ref, precond, s));
+                SR_TRY(launch_slot(1, b, x, ref, precond, s));
+            }
+        }
+        SR_HIP(hipMemcpyAsync(host_ctl, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+        SR_HIP(hipStreamSynchronize(s));
+        c = *host_ctl;
+        if (c.phase == PH_DONE) break;
+        // iterations left at the rate since the last read, toward the outer goal (the inner one under the
+        // reference's rule)
+        const double target = c.ref ? c.stop2 : c.tol * c.tol * c.bb;
+        double left = 2.0;
+        if (prev_it >= 0 && c.it > prev_it && c.rr > 0.0 && prev_rr > 0.0 && c.rr < prev_rr && target > 0.0) {
+            const double rate = std::log(c.rr / prev_rr) / (double)(c.it - prev_it);
+            left = std::log(target / c.rr) / rate;
+        }
+        pairs = std::max(1, std::min(16, (int)(0.4 * left)));
+        prev_it = c.it;
+        prev_rr = c.rr;
+    }
+    // the counters: what ran, not what was launched
+    work = w_before;
+    work.iterations += c.it;
+    work.precond += c.it;  // one application opens each inner iteration; none is left over at the end
+    work.true_residuals += c.refinements + 1;
+    st->iterations = c.it;
+    st->refinements = c.refinements;
+    st->rel = c.rel;
+    st->converged = c.converged != 0;
+    st->finite = c.finite != 0;
+    last_iterations = c.it;
+    if (precond == SR_PRECOND_MULTIGRID && st->converged && c.it > 0 && c.rel0 > 0.0 && c.rel > 0.0)
+        plain_rate = std::log(c.rel / c.rel0) / c.it;
     return SR_OK;
 }
 
@@ -1411,6 +1971,14 @@ extern "C" int sr_fld_solve_stats(sr_fld *f, const double *b_dev, double *x_dev,
     if (learned_iterations) *learned_iterations = st.learned_iterations;
     if (fell_back) *fell_back = st.fell_back ? 1 : 0;
     if (!st.finite) return fail(SR_ERR_NON_FINITE, "sr_fld_solve: a sum left the float's range");
+    return SR_OK;
+}
+
+extern "C" int sr_fld_set_solve_mode(sr_fld *f, int mode) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_solve_mode: null solver");
+    if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH)
+        return fail(SR_ERR_ARGUMENT, "sr_fld_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE or _GRAPH");
+    f->s.set_solve_mode(mode);
     return SR_OK;
 }
 

@@ -10,6 +10,11 @@ namespace sr {
 
 class Net;
 struct CoarseSet;
+struct SolveCtl;
+
+// How a solve decides when to stop: on the host from read-back partials (the learned preconditioner always), or on
+// the device by each slot's head, the slots launched one by one or replayed as a captured graph of two
+enum { SOLVE_HOST = SR_SOLVE_HOST, SOLVE_DEVICE = SR_SOLVE_DEVICE, SOLVE_GRAPH = SR_SOLVE_GRAPH };
 
 // tile shapes of the level smoothers
 enum { TILE_16 = SR_TILE_16, TILE_32 = SR_TILE_32, TILE_64 = SR_TILE_64 };
@@ -60,9 +65,13 @@ class FldSolver {
     uint8_t *gas0() { return gas; }
     int rebuild(bool faces, hipStream_t s);
     // A x = b to ||b - Ax|| <= tol ||b|| (or the reference rule when ref != nullptr); x holds the start
+    // `expect`: the iterations a similar solve took (the same Newton pass of the last step), the device-decided
+    // solve's first burst of slots; 0 when unknown
     int solve(const double *b, double *x, double tol, int max_iter, int precond, const RefStop *ref, FldStats *st,
-              hipStream_t s);
+              hipStream_t s, int expect = 0);
     int apply_precond(const float *r, float *z, int precond, hipStream_t s);
+    void set_solve_mode(int m) { solve_mode = m; }
+    int get_solve_mode() const { return solve_mode; }
     // the network of the learned preconditioner (z = V r + N(r - A V r)); after `fallback` iterations without
     // convergence (0: never) the solve continues with plain multigrid
     int set_network(Net *n, int fallback);
@@ -79,15 +88,20 @@ class FldSolver {
     // state)
     double work_bytes(const FldWork &w) const;
     FldWork work;
-    // kernel timing of the level-0 pre-smoother: event pairs recorded on the stream without waiting, summed when
-    // read (smooth_time), so timing does not stall the solve
+    // kernel clocks (timing on): event pairs recorded on the stream around every launch of the level-0 pre- and
+    // post-smoother, the single-block V-cycle, the matvec and the update, without waiting; summed per kernel when read
+    // (clock), so timing does not stall the solve
+    enum { CLK_PRE0, CLK_POST0, CLK_COARSE, CLK_MATVEC, CLK_UPDATE, CLK_N };
     void set_timing(bool on) { timing = on; }
-    int smooth_time(double *ms, unsigned long long *launches);
-    void reset_smooth_time();
+    int clock(int kernel, double *ms, unsigned long long *launches);
+    void reset_clocks();
+    // algorithmic HBM bytes of one launch of a clocked kernel
+    double clock_bytes(int kernel) const;
     bool timing = false;
 
   private:
     int w0 = 0, h0 = 0, nu1 = 2, nu2 = 2, coarse_from = 0;
+    int head = 0, ltx = 0;  // the single-block V-cycle's register-resident levels (0 or 2), log2 of its layout's width
     float omega = 1.0f;
     int nbe = 1, ntiles0 = 1, nbd_max = 1; // blocks of the elementwise kernels, level-0 tiles
     // the tile shape (TILE_16, TILE_32, TILE_64) of each tile-smoothed level's pre- and post-smoother: the autotuner's
@@ -95,9 +109,9 @@ class FldSolver {
     std::vector<int> pre_tile, post_tile;
     bool tuned = false, autotune = true;
     int tune(hipStream_t s);
-    int launch_pre_level(int l, const float *b, hipStream_t s);
+    int launch_pre_level(int l, const float *b, hipStream_t s, const int *gate = nullptr);
     int launch_post_level(int l, const float *b, float *zout, const float *zold, double *part, int *nblocks,
-                          hipStream_t s);
+                          hipStream_t s, const int *gate = nullptr);
     std::vector<FldLevel> lv;
     uint8_t *gas = nullptr;
     // level-0 vectors
@@ -107,14 +121,37 @@ class FldSolver {
     // per-block partial sums (summed by the kernel that consumes them, or on the host)
     double *part_true = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_dots = nullptr;
     double *host_part = nullptr; // pinned
+    // the device-decided solve: two copies of its state (by slot parity), the head's ticket counter, a pinned copy for
+    // the host's checks; the captured pair of slots and what it was captured for
+    int solve_mode = SOLVE_HOST;
+    SolveCtl *ctl = nullptr, *host_ctl = nullptr;
+    unsigned *ticket = nullptr;
+    hipGraphExec_t slot_exec = nullptr;
+    struct GraphKey {
+        const double *b = nullptr;
+        const double *x = nullptr;
+        const float *heat = nullptr, *slope = nullptr;
+        int precond = -1, tiles = -1;
+        hipStream_t s = nullptr;
+    } slot_key;
+    int tiles_version = 0;
+    int last_iterations = 0;  // of the last device-decided solve: where the next one first looks
+    int launch_slot(int parity, const double *b, double *x, const RefStop *ref, int precond, hipStream_t s);
+    int solve_slots(const double *b, double *x, double tol, int max_iter, int precond, const RefStop *ref,
+                    FldStats *st, hipStream_t s, int expect);
     std::vector<hipEvent_t> evs;   // pairs: start, end
+    std::vector<int> ev_kernel;    // the kernel each recorded pair times
     size_t ev_used = 0;
-    double smooth_ms = 0.0;
-    unsigned long long smooth_launches = 0;
+    double clock_ms[CLK_N] = {};
+    unsigned long long clock_n[CLK_N] = {};
+    int tick(int kernel, hipStream_t s);  // records a pair's start when timing; returns its slot (or -1)
+    int tock(int slot, hipStream_t s);
+    int fold_clocks();
     Net *net = nullptr;
     int fallback_it = 0;
     double plain_rate = 0.0;  // ln(residual reduction) per iteration of the last converged plain multigrid solve
-    int vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s);
+    int vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s,
+               const int *gate = nullptr);
     CoarseSet coarse_set() const;
     int learned(const float *r, float *z, const float *zold, double *dots, hipStream_t s);
     int read_partials(const double *part, int k, int nb, double *out, hipStream_t s);

@@ -31,7 +31,9 @@ __global__ void __launch_bounds__(256) k_rows_r2c(const TIn *__restrict__ sigma,
     extern __shared__ float2 lds[];
     const int L = 1 << logL;
     float2 *a = lds, *b = lds + rb * L;
-    const int y0 = blockIdx.x * rb;
+    // row groups remapped so that each XCD takes one contiguous run of them: the groups that share a bin's cache
+    // line of the column-major spectrum then write it through the same L2
+    const int y0 = xcd_tile() * rb;
     for (int t = threadIdx.x; t < rb * L; t += blockDim.x) {
         const int r = t >> logL, m = t & (L - 1), y = y0 + r;
         float2 v = make_float2(0.f, 0.f);
@@ -51,9 +53,11 @@ __global__ void __launch_bounds__(256) k_rows_r2c(const TIn *__restrict__ sigma,
     __syncthreads();
     float2 *Z = lds_fft(a, b, L, logL, rb, tw, tw_stride_half, false);
     // X[k] = (Z_k + conj Z_{L-k}) / 2 - i/2 W^k (Z_k - conj Z_{L-k}),  W = e^{-2 pi i / px}
+    // Consecutive lanes take consecutive rows of one bin, so each bin's rb rows go out as one contiguous segment of
+    // the column-major spectrum (rather than one 8-byte value per lane at stride ld).
     const int nk = L + 1;
     for (int t = threadIdx.x; t < rb * nk; t += blockDim.x) {
-        const int r = t / nk, k = t - r * nk, y = y0 + r;
+        const int k = t / rb, r = t - k * rb, y = y0 + r;
         if (y >= h) continue;
         const float2 zk = Z[r * L + (k & (L - 1))];
         const float2 zc = cconj(Z[r * L + ((L - k) & (L - 1))]);
@@ -103,10 +107,12 @@ __global__ void __launch_bounds__(256) k_rows_c2r(const float2 *__restrict__ in,
     extern __shared__ float2 lds[];
     const int L = 1 << logL;
     float2 *a = lds, *b = lds + rb * L;
-    const int j0 = blockIdx.x * rb;
-    // Z_k = (X_k + conj X_{L-k}) + i W^{-k} (X_k - conj X_{L-k}), k in [0, L)
+    const int j0 = xcd_tile() * rb;  // as in k_rows_r2c: a bin's cache line read through one XCD's L2
+    // Z_k = (X_k + conj X_{L-k}) + i W^{-k} (X_k - conj X_{L-k}), k in [0, L). Consecutive lanes take consecutive
+    // rows of one bin, so the gather reads each bin's rb rows (and its mirror's) as contiguous segments of the
+    // column-major buffer.
     for (int t = threadIdx.x; t < rb * L; t += blockDim.x) {
-        const int r = t >> logL, k = t & (L - 1), j = j0 + r;
+        const int k = t / rb, r = t - k * rb, j = j0 + r;
         float2 v = make_float2(0.f, 0.f);
         if (j < nrows) {
             const float2 xk = in[(size_t)k * ldi + j];
@@ -116,7 +122,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r(const float2 *__restrict__ in,
             const float2 wd = cmul(wk, d);
             v = make_float2(s.x - wd.y, s.y + wd.x); // s + i * wd
         }
-        a[t] = v;
+        a[r * L + k] = v;
     }
     __syncthreads();
     float2 *z = lds_fft(a, b, L, logL, rb, tw, tw_stride_half, true);
@@ -318,7 +324,7 @@ static int rows_per_block(int L) { return std::max(1, std::min(8, 4096 / L)); }
 
 template <typename TIn> int Gravity::potential(const TIn *sigma, float *phi_ringed, hipStream_t s) {
     const int L = px / 2, logL = ilog2(L), logpy = ilog2(py);
-    int rb = rows_per_block(L);
+    const int rb = this->rb > 0 ? this->rb : rows_per_block(L);
     const int stride_half = nmax / L, stride_full = nmax / px;
     hipLaunchKernelGGL(k_rows_r2c<TIn>, dim3((h + rb - 1) / rb), dim3(256), sizeof(float2) * 2 * rb * L, s, sigma, w,
                        h, (TIn)sigma_vac, px, logL, rb, tw, stride_half, stride_full, spec, ld_spec);
@@ -328,6 +334,46 @@ template <typename TIn> int Gravity::potential(const TIn *sigma, float *phi_ring
                        ld_cols, h + 2, w, px, logL, rb, tw, stride_half, stride_full, phi_ringed);
     SR_HIP(hipGetLastError());
     return SR_OK;
+}
+
+// Rows per block 1, 2, 4 or 8 (as the LDS allows two rb x px/2 complex arrays), each timed as the best of three means
+// of four convolutions between two events after one to warm; the heuristic's count is kept unless another is 3 %
+// faster.
+int Gravity::tune(const float *sigma, float *phi_ringed, hipStream_t s) {
+    const int L = px / 2, heur = rows_per_block(L);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SR_HIP(hipEventCreate(&e0));
+    SR_HIP(hipEventCreate(&e1));
+    int best = heur;
+    float best_ms = 0.0f;
+    int rc = SR_OK;
+    std::vector<int> cand = {heur};
+    for (int r : {1, 2, 4, 8})
+        if (r != heur && (size_t)2 * r * L * sizeof(float2) <= (size_t)max_lds_bytes()) cand.push_back(r);
+    for (int r : cand) {
+        rb = r;
+        float ms = 0.0f;
+        if ((rc = potential<float>(sigma, phi_ringed, s)) != SR_OK) break;
+        for (int trial = 0; trial < 3 && rc == SR_OK; trial++) {
+            if (hipEventRecord(e0, s) != hipSuccess) rc = fail(SR_ERR_DEVICE, "hipEventRecord");
+            for (int k = 0; k < 4 && rc == SR_OK; k++) rc = potential<float>(sigma, phi_ringed, s);
+            if (rc != SR_OK) break;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+                rc = fail(SR_ERR_DEVICE, "hipEventSynchronize");
+            float t = 0.0f;
+            if (rc == SR_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = fail(SR_ERR_DEVICE, "timing");
+            if (trial == 0 || t < ms) ms = t;
+        }
+        if (rc != SR_OK) break;
+        if (r == heur || ms < 0.97f * best_ms) {
+            best = r;
+            best_ms = ms;
+        }
+    }
+    rb = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 template int Gravity::potential<float>(const float *, float *, hipStream_t);

@@ -15,6 +15,7 @@ struct Gravity {
     float2 *cols = nullptr;  // column outputs, column-major (px/2+1) x (h+2)
     float *phi_tmp = nullptr;
     int ld_spec = 0, ld_cols = 0;
+    int rb = 0;  // rows per block of the row transforms: the autotuner's choice (0 until it runs: a heuristic)
 
     ~Gravity();
     int init(int w, int h, double G, double dx, double sigma_vac, int max_padded);
@@ -22,6 +23,10 @@ struct Gravity {
     template <typename TIn> int potential(const TIn *sigma, float *phi_ringed, hipStream_t s);
     int gradient(const float *phi_ringed, float *gx, float *gy, hipStream_t s);
     double algorithmic_bytes() const;
+    // the autotuner: the row transforms' rows per block, timed on the convolution of `sigma` into `phi_ringed`
+    // (scratch device arrays of the box's and the ringed box's size); each row's transform is the same whichever
+    // block takes it, so the choice changes no result bit
+    int tune(const float *sigma, float *phi_ringed, hipStream_t s);
 };
 
 } // namespace sr

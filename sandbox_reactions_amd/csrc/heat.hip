@@ -389,6 +389,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     const double eps = hs.stop == SR_STOP_REFERENCE ? hs.solve_tol : 0.0;
     int passes = 0, rounds = 0;
     bool converged = false, faces = true;
+    std::vector<int> step_rounds;
     double last_rel = 0.0;
     // inexact Newton (relative stopping only): loose early passes, the last one to solve_tol
     const bool inexact = hs.forcing > 0.0 && hs.stop == SR_STOP_RELATIVE;
@@ -414,8 +415,11 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
         SR_TRY(solver.rebuild(faces, s));
         faces = false;
         FldStats fs;
+        const int hint = passes <= (int)pass_rounds.size() ? pass_rounds[passes - 1] : 0;
         SR_TRY(solver.solve(rhs, xs, eta, hs.max_rounds - rounds, precond,
-                            hs.stop == SR_STOP_REFERENCE ? &ref : nullptr, &fs, s));
+                            hs.stop == SR_STOP_REFERENCE ? &ref : nullptr, &fs, s, hint));
+        if ((int)step_rounds.size() < passes) step_rounds.resize(passes);
+        step_rounds[passes - 1] = fs.iterations;
         rounds += fs.iterations;
         last_rel = fs.rel;
         if (!fs.finite) break;
@@ -434,6 +438,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     }
     report.passes = passes;
     report.rounds = rounds;
+    pass_rounds = step_rounds;
     report.final_residual = last_rel;
     if (converged) {
         hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, part);

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/sr_hip.h"
 #include "cold.hpp"
 #include "fld.hpp"
@@ -67,6 +69,7 @@ class HeatPass {
   private:
     int read_sums(int k, double *out, hipStream_t s);
     HeatPlanes H;
+    std::vector<int> pass_rounds;  // the last step's solve iterations per Newton pass: the next step's first bursts
     double *rhs = nullptr, *xs = nullptr, *part = nullptr;
     double *host = nullptr;
     int nb = 1;

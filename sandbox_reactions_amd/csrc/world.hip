@@ -23,7 +23,8 @@ namespace sr {
 #define NT 256
 #define NS 10
 
-enum { SCAL_DT = 0, SCAL_DT_NEXT, SCAL_SIM_TIME, SCAL_CAP, SCAL_RATE, SCAL_GMAX, SCAL_NF, SCAL_NF_STEP, SCAL_N = 16 };
+enum { SCAL_DT = 0, SCAL_DT_NEXT, SCAL_SIM_TIME, SCAL_CAP, SCAL_RATE, SCAL_GMAX, SCAL_NF, SCAL_NF_STEP, SCAL_STEPS,
+       SCAL_N = 16 };
 
 __constant__ float w_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f / 20.0f,
                                 13.0f / 24.0f, 15.0f / 28.0f, 17.0f / 32.0f, 27.0f / 56.0f, 1.0f};
@@ -165,7 +166,9 @@ __global__ void k_scal_reset(double *scal) {
     if (threadIdx.x < SCAL_N) scal[threadIdx.x] = threadIdx.x == SCAL_NF ? -HUGE_VAL : 0.0;
 }
 
-__global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, int nb, double *scal, double step) {
+// (the step index a guard finding names is the device's count of the steps whose P9 has run; `count`: this P9 closes a
+// step, so the count moves on)
+__global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, int nb, double *scal, int count) {
     __shared__ double sh[3][4];
     double v[3] = {0.0, 0.0, -HUGE_VAL};
     for (int i = threadIdx.x; i < nb; i += blockDim.x)
@@ -183,8 +186,9 @@ __global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, in
         // the guard keeps its first finding and the index of the step whose P9 made it (step/dt.rs:40-47, NonFinite)
         if (o[2] > -HUGE_VAL && !(scal[SCAL_NF] > -HUGE_VAL)) {
             scal[SCAL_NF] = o[2];
-            scal[SCAL_NF_STEP] = step;
+            scal[SCAL_NF_STEP] = scal[SCAL_STEPS];
         }
+        if (count) scal[SCAL_STEPS] += 1.0;
         double dt = P.cfl * P.dx / o[0];
         if (o[1] > 0.0) dt = fmin(dt, P.eta_g * sqrt(P.dx / o[1]));
         scal[SCAL_DT_NEXT] = dt;
@@ -270,6 +274,10 @@ struct sr_world {
     hipEvent_t ev[12]{};
     sr_step_cost cost{};
     KernelClock k_update, k_mid, k_grav, k_pre0;
+    // the step's fixed parts captured as graphs (with the clocks off): P1..P4 per current buffer and radiation flag,
+    // P8/P9 per current buffer
+    bool graphs = true;
+    hipGraphExec_t g_front[2][2] = {}, g_tail[2] = {};
     // the step's launches outside the heat pass, counted as they are made (sr_read_work)
     struct {
         unsigned long long steps = 0, hydro = 0, mid = 0, conv_first = 0, conv_mid = 0, radiation = 0;
@@ -302,6 +310,11 @@ extern "C" void sr_destroy(sr_world *wd) {
     if (wd->host_scal) (void)hipHostFree(wd->host_scal);
     for (auto &e : wd->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &row : wd->g_front)
+        for (auto &g : row)
+            if (g) (void)hipGraphExecDestroy(g);
+    for (auto &g : wd->g_tail)
+        if (g) (void)hipGraphExecDestroy(g);
     if (wd->s) (void)hipStreamDestroy(wd->s);
     delete wd;
 }
@@ -367,6 +380,9 @@ extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_swit
     if (sw->gravity) {
         if ((c = wd->grav.init(w, h, ph->G, ph->dx, ph->sigma_vac, ph->max_padded > 0 ? ph->max_padded : 4096)))
             return bail(c);
+        // the row transforms' rows per block, timed on the (zeroed) mid-step arrays
+        if ((c = wd->grav.tune(wd->sigma_mid, wd->phi_mid, wd->s))) return bail(c);
+        if (hipStreamSynchronize(wd->s) != hipSuccess) return bail(fail(SR_ERR_DEVICE, "hipStreamSynchronize"));
     } else {
         int px;
         if ((c = padded_size(w, ph->max_padded > 0 ? ph->max_padded : 4096, &px))) return bail(c);
@@ -467,8 +483,7 @@ static int p89(sr_world *wd, int apply) {
     const int nb = grid_n(wd->n);
     hipLaunchKernelGGL(k_floors_dt, dim3(nb), dim3(NT), 0, wd->s, wd->fp, st, wd->sw.gravity ? wd->gx : nullptr,
                        wd->sw.gravity ? wd->gy : nullptr, apply, wd->rpart);
-    hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(256), 0, wd->s, wd->fp, wd->rpart, nb, wd->scal,
-                       (double)wd->steps);
+    hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(256), 0, wd->s, wd->fp, wd->rpart, nb, wd->scal, apply);
     SR_HIP(hipGetLastError());
     return SR_OK;
 }
@@ -604,14 +619,13 @@ static double between(sr_world *wd, int a, int b) {
     return ms;
 }
 
-static int one_step(sr_world *wd) {
+// P1 .. P4 and the take-backs: launches only (no host decision), so the sequence can be captured
+static int step_front(sr_world *wd) {
     hipStream_t s = wd->s;
     mark(wd, 0);
     hipLaunchKernelGGL(k_p1, dim3(1), dim3(64), 0, s, wd->scal);
     mark(wd, 1);
     SR_TRY(p2(wd));
-    wd->wk.steps++;
-    if (wd->sw.gravity) wd->wk.conv_first++;
     mark(wd, 2);
     if (wd->sw.hydro) {
         HydroState &in = wd->st[wd->cur], &out = wd->st[1 - wd->cur];
@@ -622,11 +636,7 @@ static int one_step(sr_world *wd) {
                                 nullptr, wd->sigma_mid, nullptr, wd->escaped, wd->part, s));
             mark(wd, 9);
             SR_TRY(wd->grav.potential<float>(wd->sigma_mid, wd->phi_mid, s));
-            wd->wk.mid++;
-            wd->wk.conv_mid++;
         }
-        wd->wk.hydro++;
-        if (rad) wd->wk.radiation++;
         mark(wd, 10);
         SR_TRY(hydro_launch(HYDRO_UPDATE, wd->hp, in, out, wd->scal, grav ? wd->gx : nullptr,
                             grav ? wd->gy : nullptr, grav ? wd->sigma_mid : nullptr, grav ? wd->phi_mid : nullptr,
@@ -634,6 +644,47 @@ static int one_step(sr_world *wd) {
                             wd->part, s));
         mark(wd, 11);
         if (grav || rad) SR_TRY(hydro_takeback(wd->hp, out, wd->pushmag, wd->part, wd->takeback, rad, s));
+    }
+    return SR_OK;
+}
+
+// `launch` captured on the world's stream into *g (once), then the graph launched
+template <typename F> static int replay(sr_world *wd, hipGraphExec_t *g, F launch) {
+    if (!*g) {
+        hipGraph_t graph = nullptr;
+        SR_HIP(hipStreamBeginCapture(wd->s, hipStreamCaptureModeThreadLocal));
+        const int c = launch();
+        const hipError_t e = hipStreamEndCapture(wd->s, &graph);
+        SR_TRY(c);
+        if (e != hipSuccess) return fail(SR_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(g, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            *g = nullptr;
+            return fail(SR_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    SR_HIP(hipGraphLaunch(*g, wd->s));
+    return SR_OK;
+}
+
+static int one_step(sr_world *wd) {
+    hipStream_t s = wd->s;
+    const bool graphs = wd->graphs && !wd->timing;
+    const bool grav = wd->sw.gravity, rad = wd->sw.heat && wd->have_flux;
+    if (graphs)
+        SR_TRY(replay(wd, &wd->g_front[wd->cur][rad ? 1 : 0], [&] { return step_front(wd); }));
+    else
+        SR_TRY(step_front(wd));
+    wd->wk.steps++;
+    if (grav) wd->wk.conv_first++;
+    if (wd->sw.hydro) {
+        if (grav) {
+            wd->wk.mid++;
+            wd->wk.conv_mid++;
+        }
+        wd->wk.hydro++;
+        if (rad) wd->wk.radiation++;
         wd->cur = 1 - wd->cur;
     }
     mark(wd, 3);
@@ -649,7 +700,10 @@ static int one_step(sr_world *wd) {
         wd->first_built = false;
     }
     mark(wd, 4);
-    SR_TRY(p89(wd, 1));
+    if (graphs)
+        SR_TRY(replay(wd, &wd->g_tail[wd->cur], [&] { return p89(wd, 1); }));
+    else
+        SR_TRY(p89(wd, 1));
     mark(wd, 5);
     wd->steps++;
     if (wd->timing) {
@@ -838,6 +892,14 @@ extern "C" int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations) 
     return SR_OK;
 }
 
+extern "C" int sr_set_solve_mode(sr_world *wd, int mode) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH)
+        return fail(SR_ERR_ARGUMENT, "sr_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE or _GRAPH");
+    if (wd->sw.heat) wd->heat.solver.set_solve_mode(mode);
+    return SR_OK;
+}
+
 extern "C" int sr_set_dt_cap(sr_world *wd, double cap) {
     if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
     if (!(cap > 0.0) || !std::isfinite(cap)) return fail(SR_ERR_ARGUMENT, "a dt cap must be positive and finite");
@@ -853,7 +915,7 @@ extern "C" int sr_set_timing(sr_world *wd, int on) {
     wd->k_update = wd->k_mid = wd->k_grav = wd->k_pre0 = KernelClock{};
     if (wd->sw.heat) {
         wd->heat.solver.set_timing(wd->timing);
-        wd->heat.solver.reset_smooth_time();
+        wd->heat.solver.reset_clocks();
     }
     return SR_OK;
 }
@@ -928,12 +990,15 @@ extern "C" int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launch
     } else if (k == "hydro_mid") {
         c = wd->k_mid;
         bytes = n * (60.0 + 8.0 + 4.0);
-    } else if (k == "mg_pre0") {
+    } else if (k == "mg_pre0" || k == "mg_post0" || k == "mg_coarse" || k == "fld_matvec" || k == "fld_update") {
         if (!wd->sw.heat) return fail(SR_ERR_ARGUMENT, "no heat pass");
-        SR_TRY(wd->heat.solver.smooth_time(&c.ms, &c.launches));
-        // b and the operator (own, left and up faces) in; z and the coarse b out. The stored inverse diagonal is a
-        // derived copy of the operator and is not counted.
-        bytes = n * 21.0;
+        const int id = k == "mg_pre0"     ? FldSolver::CLK_PRE0
+                       : k == "mg_post0"  ? FldSolver::CLK_POST0
+                       : k == "mg_coarse" ? FldSolver::CLK_COARSE
+                       : k == "fld_matvec" ? FldSolver::CLK_MATVEC
+                                           : FldSolver::CLK_UPDATE;
+        SR_TRY(wd->heat.solver.clock(id, &c.ms, &c.launches));
+        bytes = wd->heat.solver.clock_bytes(id);
     } else {
         return fail(SR_ERR_ARGUMENT, "unknown kernel " + k);
     }

@@ -52,6 +52,11 @@ class Fld:
                                         self._s(stream)))
         return it.value, rel.value, li.value, bool(fb.value)
 
+    def set_solve_mode(self, mode: int):
+        """SOLVE_HOST (the default), SOLVE_DEVICE or SOLVE_GRAPH: how a solve decides when to stop."""
+        from . import _check, lib
+        _check(lib().sr_fld_set_solve_mode(self._p, mode))
+
     def set_network(self, net, fallback_iterations: int = 0):
         """The network of the learned preconditioner (None detaches it)."""
         from . import _check, lib

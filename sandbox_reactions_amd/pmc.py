@@ -1,10 +1,12 @@
-"""HBM traffic per kernel launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).
+"""HBM traffic per kernel launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), and the
+per-kernel durations of a rocprofv3 kernel trace.
 
 Each counter is scaled by what it reads for a device copy of known size: the 512 MiB copy `bench.py` makes with
 one float per lane (k_calib_copy1, the access pattern of the stencil kernels), which reads and writes 512 MiB. The
 float4 and runtime copies are reported beside it (wide requests can count differently). Usage (after the two runs, on any machine):
 
     python -m sandbox_reactions_amd.pmc FETCH_DIR WRITE_DIR OUT_JSON [--workload TEXT] [--command TEXT]
+    python -m sandbox_reactions_amd.pmc --trace TRACE_DIR OUT_MD [--workload TEXT] [--command TEXT]
 
 OUT_JSON gets {"copy_scale": {...}, "kernels": {name: {"launches", "fetch_bytes", "write_bytes", "traffic_bytes",
 "grid"}}} for the dispatch grid with the most total traffic per kernel, and "by_grid": {"name|grid": ...} for every
@@ -81,14 +83,46 @@ def summarize(fetch_dir, write_dir):
             "by_grid": by_grid}
 
 
+def trace_table(trace_dir):
+    """Per (kernel, grid): launches, mean and total device time (us, ms) from a kernel trace's CSV
+    (rocprofv3 --kernel-trace -f csv), sorted by total time."""
+    paths = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True)
+    if not paths:
+        raise FileNotFoundError(f"no kernel_trace.csv under {trace_dir}")
+    agg = collections.defaultdict(list)
+    for p in paths:
+        with open(p) as f:
+            for r in csv.DictReader(f):
+                grid = "x".join(r.get(f"Grid_Size_{a}", "1") for a in "XY")
+                dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+                agg[(short(r["Kernel_Name"]), grid, r.get("Workgroup_Size_X", ""))].append(dur)
+    rows = [(k, len(v), float(np.mean(v)), float(np.sum(v)) * 1e-3) for k, v in agg.items()]
+    return sorted(rows, key=lambda r: -r[3])
+
+
+def write_trace(trace_dir, out_md, workload, command):
+    rows = trace_table(trace_dir)
+    with open(out_md, "w") as f:
+        f.write(f"# Kernel trace: {workload}\n\nCommand: `{command}`\n\n")
+        f.write("Per kernel and grid, from the trace's CSV, sorted by total time.\n\n")
+        f.write("| kernel | grid (threads) | wg | launches | avg us | total ms |\n|---|---|---|---|---|---|\n")
+        for (name, grid, wg), n, mean, total in rows:
+            f.write(f"| {name} | {grid} | {wg} | {n} | {mean:.2f} | {total:.2f} |\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("fetch_dir")
-    ap.add_argument("write_dir")
-    ap.add_argument("out_json")
+    ap.add_argument("--trace", action="store_true", help="summarise a kernel trace: TRACE_DIR OUT_MD")
+    ap.add_argument("paths", nargs="+")
     ap.add_argument("--workload", default="")
     ap.add_argument("--command", default="")
     a = ap.parse_args()
+    if a.trace:
+        write_trace(a.paths[0], a.paths[1], a.workload, a.command)
+        return
+    if len(a.paths) != 3:
+        ap.error("FETCH_DIR WRITE_DIR OUT_JSON")
+    a.fetch_dir, a.write_dir, a.out_json = a.paths
     s = summarize(a.fetch_dir, a.write_dir)
     s["workload"] = a.workload
     s["command"] = a.command

@@ -124,6 +124,10 @@ class World:
         self._net = net
         self._c(self.lib.sr_set_network(self._p, net._p if net is not None else None, fallback_iterations))
 
+    def set_solve_mode(self, mode: int):
+        """SOLVE_HOST, SOLVE_DEVICE or SOLVE_GRAPH: how the heat pass's solves decide when to stop."""
+        self._c(self.lib.sr_set_solve_mode(self._p, mode))
+
     def set_dt_cap(self, cap: float):
         self._c(self.lib.sr_set_dt_cap(self._p, cap))
 
