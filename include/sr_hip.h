@@ -40,6 +40,9 @@ enum sr_opacity { SR_OPACITY_REFERENCE = 0, SR_OPACITY_KRAMERS = 1 };
 enum sr_precond { SR_PRECOND_JACOBI = 0, SR_PRECOND_MULTIGRID = 1, SR_PRECOND_LEARNED = 2 };
 enum sr_stop { SR_STOP_REFERENCE = 0, SR_STOP_RELATIVE = 1 };
 enum sr_tile { SR_TILE_16 = 0, SR_TILE_32 = 1, SR_TILE_64 = 2 }; /* smoother tiles: 16x16, 32x32, 64x32 */
+/* the fused smoothers of the last two tile-smoothed levels: separate launches, or one launch per direction with 8x8
+ * coarse / 16x16 fine tiles (small) or 16x16 / 32x32 (large, nu1 and nu2 <= 2) */
+enum sr_fuse { SR_FUSE_OFF = 0, SR_FUSE_SMALL = 1, SR_FUSE_LARGE = 2 };
 enum sr_field { SR_FIELD_PHI = 0, SR_FIELD_GX = 1, SR_FIELD_GY = 2, SR_FIELD_FLUX_X = 3, SR_FIELD_FLUX_Y = 4,
                 SR_FIELD_SIGMA_MID = 5, SR_FIELD_PHI_MID = 6 };
 
@@ -188,6 +191,14 @@ int sr_fld_set_solve_mode(sr_fld *f, int mode);
  * (heat.wgsl:1401-1445: 16x16 with 4 RB pairs). */
 int sr_fld_tiles(sr_fld *f, int *pre, int *post, int max_levels, int *levels);
 int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int levels);
+/* The last two tile-smoothed levels' pre-smoothers in one launch and their post-smoothers in another, when at least
+ * three levels are tile-smoothed (level 0 always apart): each block redoes the finer level's smoothing over the cells
+ * its coarser tile's halo reaches, the two levels' launches being latency-bound. The same bits as separate launches.
+ * The autotuner picks per direction (SR_FUSE_OFF, _SMALL or _LARGE) on the first system built; sr_fld_set_fusion
+ * fixes the choice (for both it stays off when fewer levels are tile-smoothed). No counterpart in the reference (its
+ * V-cycle dispatches each level's passes, heat.wgsl:1401-1445). */
+int sr_fld_fusion(sr_fld *f, int *pre, int *post);
+int sr_fld_set_fusion(sr_fld *f, int pre, int post);
 void sr_fld_destroy(sr_fld *f);
 /* The learned preconditioner (SR_PRECOND_LEARNED): z = V r + N(r - A V r), V the multigrid V-cycle and N the
  * network. With fallback_iterations > 0 the solve continues with plain multigrid from where it stands once the
@@ -276,6 +287,8 @@ int sr_apply_heat_precond(sr_world *wd, const float *r, float *z, size_t n);
 int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
 /* the heat pass's solves: SR_SOLVE_HOST (the default), SR_SOLVE_DEVICE or SR_SOLVE_GRAPH */
 int sr_set_solve_mode(sr_world *wd, int mode);
+/* the heat pass's fused smoother pair, as sr_fld_set_fusion */
+int sr_set_fusion(sr_world *wd, int pre, int post);
 int sr_set_dt_cap(sr_world *wd, double cap);                  /* Twin::cap_dt_next (reference/mod.rs:225) */
 int sr_set_timing(sr_world *wd, int on);                      /* Step::set_timing (step/mod.rs:722) */
 int sr_read_cost(sr_world *wd, sr_step_cost *out);            /* Step::read_cost (step/mod.rs:753) */

@@ -21,6 +21,7 @@ PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 STOP_REFERENCE, STOP_RELATIVE = 0, 1
 TILE_16, TILE_32, TILE_64 = 0, 1, 2
 SOLVE_HOST, SOLVE_DEVICE, SOLVE_GRAPH = 0, 1, 2
+FUSE_OFF, FUSE_SMALL, FUSE_LARGE = 0, 1, 2
 FIELD_PHI, FIELD_GX, FIELD_GY, FIELD_FLUX_X, FIELD_FLUX_Y, FIELD_SIGMA_MID, FIELD_PHI_MID = range(7)
 COMPONENTS = 14
 
@@ -118,6 +119,9 @@ SIGNATURES = [
     ("sr_read_work", _i, [_vp, C.POINTER(StepWork), _i]),
     ("sr_set_solve_mode", _i, [_vp, _i]),
     ("sr_fld_set_solve_mode", _i, [_vp, _i]),
+    ("sr_fld_fusion", _i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    ("sr_fld_set_fusion", _i, [_vp, _i, _i]),
+    ("sr_set_fusion", _i, [_vp, _i, _i]),
     ("sr_destroy", None, [_vp]),
     ("sr_kernel_time", _i, [_vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_d), C.POINTER(_d)]),
     ("sr_fld_set_network", _i, [_vp, _vp, _i]),

@@ -479,6 +479,30 @@ __device__ inline void sweeps(Region<TX, TY, HW> &R, const Lv &L, int x0, int y0
     }
 }
 
+// The residual b - A z summed over the 2x2 cells whose top-left is the tile's cell (tx, ty), at (gx, gy) on the
+// level; cells past the level's right or bottom edge add nothing. `own` holds the diagonal's own term of the tile's
+// cells in rows of OS. The fused and the separate pre-smoothers both restrict through this one expression.
+template <int TX, int TY, int HW, int OS>
+__device__ inline float restrict_residual(const Region<TX, TY, HW> &R, const Lv &L, const float *own, int tx, int ty,
+                                          int gx, int gy) {
+    using RG = Region<TX, TY, HW>;
+    float sum = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            if (gx + dx >= L.w || gy + dy >= L.h) continue;
+            const int c = (HW + ty + dy) * RG::RX + HW + tx + dx;
+            const float zc = R.z[c];
+            const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RG::RX];
+            const float o = own[(ty + dy) * OS + tx + dx];
+            const float az = o * zc + al * (zc - R.z[c - 1]) + ar * (zc - R.z[c + 1]) + au * (zc - R.z[c - RG::RX]) +
+                             ad * (zc - R.z[c + RG::RX]);
+            sum += R.b[c] - az;
+        }
+    return sum;
+}
+
 // Pre-smoothing of level L from z = 0 (nu sweeps, red first), then the residual on the tile and its 2x2
 // restriction into the next level's b. Halo 2 nu + 1.
 template <int TX, int TY, int NU>
@@ -501,22 +525,7 @@ __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, f
         const int cx = t % CX, cy = t / CX;
         const int X = x0 / 2 + cx, Y = y0 / 2 + cy;
         if (2 * X >= L.w || 2 * Y >= L.h) continue;
-        float sum = 0.0f;
-#pragma unroll
-        for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-            for (int dx = 0; dx < 2; dx++) {
-                const int lx = HW + 2 * cx + dx, ly = HW + 2 * cy + dy;
-                if (x0 + lx - HW >= L.w || y0 + ly - HW >= L.h) continue;
-                const int c = ly * RG::RX + lx;
-                const float zc = R.z[c];
-                const float al = R.al[c], ar = R.al[c + 1], au = R.au[c], ad = R.au[c + RG::RX];
-                const float own = own_in[(2 * cy + dy) * TX + 2 * cx + dx];
-                const float az = own * zc + al * (zc - R.z[c - 1]) + ar * (zc - R.z[c + 1]) +
-                                 au * (zc - R.z[c - RG::RX]) + ad * (zc - R.z[c + RG::RX]);
-                sum += R.b[c] - az;
-            }
-        bc[(size_t)Y * wc + X] = sum;
+        bc[(size_t)Y * wc + X] = restrict_residual<TX, TY, HW, TX>(R, L, own_in, 2 * cx, 2 * cy, 2 * X, 2 * Y);
     }
     for (int t = threadIdx.x; t < TX * TY; t += NT) {
         const int lx = HW + t % TX, ly = HW + t / TX;
@@ -574,6 +583,121 @@ __global__ void __launch_bounds__(NT) k_post(Lv L, const float *__restrict__ b, 
         }
     }
     if (part) block_partials<2>(v, part, gridDim.x * gridDim.y, tile);  // partials in tile order
+}
+
+// Two adjacent tile-smoothed levels a and b = a + 1 in one launch per direction (the last two above the
+// single-block set, where every launch is latency-bound). Each block redoes on its own the finer level's work over
+// the cells its coarser tile's halo reaches; neighbouring blocks recompute the overlap alike. Every value a block
+// uses is one the separate kernels compute with the same expressions in the same order (sweeps, residual,
+// restriction, prolongation), so the fused pair gives the same bits.
+
+// Pre: a block owns an FB x FB tile of level b. Level b's pre-smoother reads b on its region's cells at least one
+// cell in (halo HW = 2 nu + 1), so level a's residual is restricted over those: level a's cells under them form the
+// tile of TA = 2 FB + 4 HW - 4 cells starting at 2 (xb0 - HW + 1), smoothed from z = 0 over that tile plus its own
+// halo HW. Writes level a's z and level b's b and z on the cells under this block's tile, and level c's b.
+template <int FB, int NU>
+__global__ void __launch_bounds__(NT) k_pre2(Lv La, const float *__restrict__ ba, float *__restrict__ za, Lv Lb,
+                                             float *__restrict__ bb, float *__restrict__ zb, float *__restrict__ bc,
+                                             int wc, const int *gate) {
+    if (gated_off(gate)) return;
+    constexpr int HW = 2 * NU + 1;
+    constexpr int TA = 2 * FB + 4 * HW - 4;
+    using RGA = Region<TA, TA, HW>;
+    using RGB = Region<FB, FB, HW>;
+    __shared__ RGA A;
+    __shared__ RGB B;
+    __shared__ float own_a[TA * TA];
+    __shared__ float own_b[FB * FB];
+    const int tile = xcd_tile();
+    const int xb0 = (tile % gridDim.x) * FB, yb0 = (tile / gridDim.x) * FB;
+    const int xa0 = 2 * (xb0 - HW + 1), ya0 = 2 * (yb0 - HW + 1);
+    load_region<TA, TA, HW>(A, La, xa0, ya0, ba, own_a);
+    load_region<FB, FB, HW>(B, Lb, xb0, yb0, nullptr, own_b);
+#pragma unroll 4
+    for (int t = threadIdx.x; t < RGA::N; t += NT) A.z[t] = 0.0f;
+    __syncthreads();
+    sweeps<TA, TA, HW, 1, 2 * NU, true>(A, La, xa0, ya0);
+    // level b's b on its region, from level a's residual; the region's outer ring is never read
+    for (int t = threadIdx.x; t < RGB::N; t += NT) {
+        const int cx = t % RGB::RX, cy = t / RGB::RX;
+        const int X = xb0 - HW + cx, Y = yb0 - HW + cy;
+        float v = 0.0f;
+        if (cx >= 1 && cy >= 1 && cx < RGB::RX - 1 && cy < RGB::RY - 1 && X >= 0 && Y >= 0 && 2 * X < La.w &&
+            2 * Y < La.h)
+            v = restrict_residual<TA, TA, HW, TA>(A, La, own_a, 2 * (cx - 1), 2 * (cy - 1), 2 * X, 2 * Y);
+        B.b[t] = v;
+        B.z[t] = 0.0f;
+    }
+    // level a's z on the cells under this block's b tile
+    constexpr int OA = 2 * HW - 2;  // their offset in level a's tile
+    for (int t = threadIdx.x; t < 4 * FB * FB; t += NT) {
+        const int lx = HW + OA + t % (2 * FB), ly = HW + OA + t / (2 * FB);
+        const int gx = xa0 + lx - HW, gy = ya0 + ly - HW;
+        if (gx < La.w && gy < La.h) za[(size_t)gy * La.w + gx] = A.z[ly * RGA::RX + lx];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < FB * FB; t += NT) {
+        const int gx = xb0 + t % FB, gy = yb0 + t / FB;
+        if (gx < Lb.w && gy < Lb.h) bb[(size_t)gy * Lb.w + gx] = B.b[(HW + t / FB) * RGB::RX + HW + t % FB];
+    }
+    sweeps<FB, FB, HW, 1, 2 * NU, true>(B, Lb, xb0, yb0);
+    constexpr int CX = FB / 2;
+    for (int t = threadIdx.x; t < CX * CX; t += NT) {
+        const int cx = t % CX, cy = t / CX;
+        const int X = xb0 / 2 + cx, Y = yb0 / 2 + cy;
+        if (2 * X >= Lb.w || 2 * Y >= Lb.h) continue;
+        bc[(size_t)Y * wc + X] = restrict_residual<FB, FB, HW, FB>(B, Lb, own_b, 2 * cx, 2 * cy, 2 * X, 2 * Y);
+    }
+    for (int t = threadIdx.x; t < FB * FB; t += NT) {
+        const int lx = HW + t % FB, ly = HW + t / FB;
+        const int gx = xb0 + lx - HW, gy = yb0 + ly - HW;
+        if (gx < Lb.w && gy < Lb.h) zb[(size_t)gy * Lb.w + gx] = B.z[ly * RGB::RX + lx];
+    }
+}
+
+// Post: a block owns a TA x TA tile of level a. Its region (halo HW = 2 nu) starts from level b's final z, which
+// level b's post-smoother makes exact on its region's cells at least HW in: level b's tile here is the TA / 2 + HW
+// cells under level a's region, starting at xa0 / 2 - HW / 2. Writes level a's result on its tile.
+template <int TA, int NU>
+__global__ void __launch_bounds__(NT) k_post2(Lv La, const float *__restrict__ ba, const float *__restrict__ za, Lv Lb,
+                                              const float *__restrict__ bb, const float *__restrict__ zb,
+                                              const float *__restrict__ zc, int wc, float omega,
+                                              float *__restrict__ zout, const int *gate) {
+    if (gated_off(gate)) return;
+    constexpr int HW = 2 * NU;
+    constexpr int TB = TA / 2 + HW;
+    using RGA = Region<TA, TA, HW>;
+    using RGB = Region<TB, TB, HW>;
+    __shared__ RGA A;
+    __shared__ RGB B;
+    const int tile = xcd_tile();
+    const int xa0 = (tile % gridDim.x) * TA, ya0 = (tile / gridDim.x) * TA;
+    const int xb0 = xa0 / 2 - HW / 2, yb0 = ya0 / 2 - HW / 2;
+    load_region<TB, TB, HW>(B, Lb, xb0, yb0, bb);
+    load_region<TA, TA, HW>(A, La, xa0, ya0, ba);
+    for (int t = threadIdx.x; t < RGB::N; t += NT) {
+        const int gx = xb0 + t % RGB::RX - HW, gy = yb0 + t / RGB::RX - HW;
+        float v = 0.0f;
+        if (gx >= 0 && gx < Lb.w && gy >= 0 && gy < Lb.h)
+            v = zb[(size_t)gy * Lb.w + gx] + omega * zc[(size_t)(gy >> 1) * wc + (gx >> 1)];
+        B.z[t] = v;
+    }
+    __syncthreads();
+    sweeps<TB, TB, HW, 1, 2 * NU, false>(B, Lb, xb0, yb0);
+    for (int t = threadIdx.x; t < RGA::N; t += NT) {
+        const int gx = xa0 + t % RGA::RX - HW, gy = ya0 + t / RGA::RX - HW;
+        float v = 0.0f;
+        if (gx >= 0 && gx < La.w && gy >= 0 && gy < La.h)
+            v = za[(size_t)gy * La.w + gx] + omega * B.z[((gy >> 1) - yb0 + HW) * RGB::RX + (gx >> 1) - xb0 + HW];
+        A.z[t] = v;
+    }
+    __syncthreads();
+    sweeps<TA, TA, HW, 1, 2 * NU, false>(A, La, xa0, ya0);
+    for (int t = threadIdx.x; t < TA * TA; t += NT) {
+        const int lx = HW + t % TA, ly = HW + t / TA;
+        const int gx = xa0 + lx - HW, gy = ya0 + ly - HW;
+        if (gx < La.w && gy < La.h) zout[(size_t)gy * La.w + gx] = A.z[ly * RGA::RX + lx];
+    }
 }
 
 // The V-cycle over every level of at most COARSE_MAX cells in one block (the "set"). The levels' own, ax and ay
@@ -1350,6 +1474,45 @@ int FldSolver::tune(hipStream_t s) {
             tile[l] = best;
         }
     }
+    // the last two tile-smoothed levels: their two launches with the shapes just chosen against the fused pair in
+    // each of its tilings (the large one for nu <= 2), per direction, the fused form kept when it is 3 % faster
+    const int pa = pair_level();
+    if (pa >= 1 && !fuse_fixed && rc == SR_OK) {
+        std::vector<int> forms = {FUSE_OFF, FUSE_SMALL};
+        if (nu1 <= 2 && nu2 <= 2) forms.push_back(FUSE_LARGE);
+        const FldLevel &fa = lv[pa], &fb = lv[pa + 1];
+        for (int kind = 0; kind < 2 && rc == SR_OK; kind++) {
+            int &form = kind == 0 ? fuse_pre : fuse_post;
+            int best = FUSE_OFF;
+            float best_ms = 0.0f;
+            for (int f : forms) {
+                form = f;
+                float ms = 0.0f;
+                if (kind == 0)
+                    rc = time_of(
+                        [&] {
+                            if (f != FUSE_OFF) return launch_pre_pair(pa, fa.b, s);
+                            SR_TRY(launch_pre_level(pa, fa.b, s));
+                            return launch_pre_level(pa + 1, fb.b, s);
+                        },
+                        &ms);
+                else
+                    rc = time_of(
+                        [&] {
+                            if (f != FUSE_OFF) return launch_post_pair(pa, s);
+                            SR_TRY(launch_post_level(pa + 1, fb.b, fb.zp, nullptr, nullptr, nullptr, s));
+                            return launch_post_level(pa, fa.b, fa.zp, nullptr, nullptr, nullptr, s);
+                        },
+                        &ms);
+                if (rc != SR_OK) break;
+                if (f == forms[0] || ms < 0.97f * best_ms) {
+                    best = f;
+                    best_ms = ms;
+                }
+            }
+            form = best;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
@@ -1383,6 +1546,25 @@ int FldSolver::set_tiles(const int *pre, const int *post, int levels) {
     return SR_OK;
 }
 
+int FldSolver::fusion(int *pre, int *post) const {
+    if (pre) *pre = pair_level() >= 1 ? fuse_pre : FUSE_OFF;
+    if (post) *post = pair_level() >= 1 ? fuse_post : FUSE_OFF;
+    return SR_OK;
+}
+
+int FldSolver::set_fusion(int pre, int post) {
+    for (int v : {pre, post})
+        if (v != FUSE_OFF && v != FUSE_SMALL && v != FUSE_LARGE)
+            return fail(SR_ERR_ARGUMENT, "set_fusion: SR_FUSE_OFF, SR_FUSE_SMALL or SR_FUSE_LARGE");
+    if ((pre == FUSE_LARGE && nu1 > 2) || (post == FUSE_LARGE && nu2 > 2))
+        return fail(SR_ERR_ARGUMENT, "set_fusion: the large fused tiles serve nu1, nu2 <= 2 only");
+    fuse_pre = pre;
+    fuse_post = post;
+    fuse_fixed = true;
+    tiles_version++;
+    return SR_OK;
+}
+
 // the 64x32 tile serves nu <= 2 only (its nu = 3 region would not leave room for two blocks per CU)
 template <int TX, int TY>
 static void launch_pre(int nu, hipStream_t s, Lv L, const float *b, float *z, float *bc, int wc, const int *gate) {
@@ -1409,6 +1591,53 @@ static int launch_post(int nu, hipStream_t s, Lv L, const float *b, const float 
         hipLaunchKernelGGL((k_post<TX, TY, 3>), grid, dim3(NT), 0, s, L, b, zin, zc, wc, omega, zout, zold, part,
                            gate);
     return (int)(grid.x * grid.y);
+}
+
+// the fused pair's launches: level b's FB x FB tiles before, level a's TA x TA tiles after
+template <int FB>
+static void launch_pre2(int nu, hipStream_t s, Lv La, const float *ba, float *za, Lv Lb, float *bb, float *zb,
+                        float *bc, int wc, const int *gate) {
+    const dim3 grid((Lb.w + FB - 1) / FB, (Lb.h + FB - 1) / FB);
+    if (nu == 1)
+        hipLaunchKernelGGL((k_pre2<FB, 1>), grid, dim3(NT), 0, s, La, ba, za, Lb, bb, zb, bc, wc, gate);
+    else if (nu == 2)
+        hipLaunchKernelGGL((k_pre2<FB, 2>), grid, dim3(NT), 0, s, La, ba, za, Lb, bb, zb, bc, wc, gate);
+    else if constexpr (FB <= 8)
+        hipLaunchKernelGGL((k_pre2<FB, 3>), grid, dim3(NT), 0, s, La, ba, za, Lb, bb, zb, bc, wc, gate);
+}
+
+template <int TA>
+static void launch_post2(int nu, hipStream_t s, Lv La, const float *ba, const float *za, Lv Lb, const float *bb,
+                         const float *zb, const float *zc, int wc, float omega, float *zout, const int *gate) {
+    const dim3 grid((La.w + TA - 1) / TA, (La.h + TA - 1) / TA);
+    if (nu == 1)
+        hipLaunchKernelGGL((k_post2<TA, 1>), grid, dim3(NT), 0, s, La, ba, za, Lb, bb, zb, zc, wc, omega, zout, gate);
+    else if (nu == 2)
+        hipLaunchKernelGGL((k_post2<TA, 2>), grid, dim3(NT), 0, s, La, ba, za, Lb, bb, zb, zc, wc, omega, zout, gate);
+    else
+        hipLaunchKernelGGL((k_post2<TA, 3>), grid, dim3(NT), 0, s, La, ba, za, Lb, bb, zb, zc, wc, omega, zout, gate);
+}
+
+// levels a and a + 1 pre-smoothed in one launch: level a's b in, level a + 2's b out
+int FldSolver::launch_pre_pair(int a, const float *ba, hipStream_t s, const int *gate) {
+    const FldLevel &fa = lv[a], &fb = lv[a + 1], &fc = lv[a + 2];
+    const Lv La = {fa.w, fa.h, fa.own, fa.ax, fa.ay, fa.cf}, Lb = {fb.w, fb.h, fb.own, fb.ax, fb.ay, fb.cf};
+    if (fuse_pre == FUSE_LARGE)
+        launch_pre2<16>(nu1, s, La, ba, fa.z, Lb, fb.b, fb.z, fc.b, fc.w, gate);
+    else
+        launch_pre2<8>(nu1, s, La, ba, fa.z, Lb, fb.b, fb.z, fc.b, fc.w, gate);
+    return SR_OK;
+}
+
+// levels a + 1 and a post-smoothed in one launch: level a + 2's correction in, level a's zp out
+int FldSolver::launch_post_pair(int a, hipStream_t s, const int *gate) {
+    const FldLevel &fa = lv[a], &fb = lv[a + 1], &fc = lv[a + 2];
+    const Lv La = {fa.w, fa.h, fa.own, fa.ax, fa.ay, fa.cf}, Lb = {fb.w, fb.h, fb.own, fb.ax, fb.ay, fb.cf};
+    if (fuse_post == FUSE_LARGE)
+        launch_post2<32>(nu2, s, La, fa.b, fa.z, Lb, fb.b, fb.z, fc.zp, fc.w, omega, fa.zp, gate);
+    else
+        launch_post2<16>(nu2, s, La, fa.b, fa.z, Lb, fb.b, fb.z, fc.zp, fc.w, omega, fa.zp, gate);
+    return SR_OK;
 }
 
 // level l's pre-smoother, residual and restriction into level l + 1's b, in the level's tile shape
@@ -1442,14 +1671,21 @@ int FldSolver::launch_post_level(int l, const float *b, float *zout, const float
 // z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
 int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s,
                       const int *gate) {
-    const int nl = (int)lv.size();
+    const int pa = pair_level();
     const float *bl = r;
-    for (int l = 0; l < coarse_from; l++) {
+    for (int l = 0; l < coarse_from;) {
+        if (l == pa && fuse_pre != FUSE_OFF) {
+            SR_TRY(launch_pre_pair(l, bl, s, gate));
+            bl = lv[l + 2].b;
+            l += 2;
+            continue;
+        }
         const FldLevel &c = lv[l + 1];
         const int slot = l == 0 ? tick(CLK_PRE0, s) : -1;
         SR_TRY(launch_pre_level(l, bl, s, gate));
         SR_TRY(tock(slot, s));
         bl = c.b;
+        l++;
     }
     {
         const CoarseSet C = coarse_set();
@@ -1470,6 +1706,11 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
         }
     }
     for (int l = coarse_from - 1; l >= 0; l--) {
+        if (l == pa + 1 && pa >= 1 && fuse_post != FUSE_OFF) {
+            SR_TRY(launch_post_pair(pa, s, gate));
+            l--;
+            continue;
+        }
         const FldLevel &f = lv[l];
         const float *b = l == 0 ? r : f.b;
         float *zo = l == 0 ? z : f.zp;
@@ -1926,6 +2167,16 @@ extern "C" int sr_fld_tiles(sr_fld *f, int *pre, int *post, int max_levels, int 
 extern "C" int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int levels) {
     if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_tiles: null solver");
     return f->s.set_tiles(pre, post, levels);
+}
+
+extern "C" int sr_fld_fusion(sr_fld *f, int *pre, int *post) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_fusion: null solver");
+    return f->s.fusion(pre, post);
+}
+
+extern "C" int sr_fld_set_fusion(sr_fld *f, int pre, int post) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_fusion: null solver");
+    return f->s.set_fusion(pre, post);
 }
 
 extern "C" int sr_fld_set_system(sr_fld *f, const float *own_dev, const float *kx_dev, const float *ky_dev,

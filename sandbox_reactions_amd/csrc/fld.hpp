@@ -18,6 +18,8 @@ enum { SOLVE_HOST = SR_SOLVE_HOST, SOLVE_DEVICE = SR_SOLVE_DEVICE, SOLVE_GRAPH =
 
 // tile shapes of the level smoothers
 enum { TILE_16 = SR_TILE_16, TILE_32 = SR_TILE_32, TILE_64 = SR_TILE_64 };
+// the last two tile-smoothed levels' smoothers: separate launches, or fused with small or large tiles
+enum { FUSE_OFF = SR_FUSE_OFF, FUSE_SMALL = SR_FUSE_SMALL, FUSE_LARGE = SR_FUSE_LARGE };
 
 // One level of the hierarchy: (A v)_i = own_i v_i + sum_f a_f (v_i - v_j); ax is (w+1) x h (face x between cells
 // x-1 and x, 0 at the edges), ay is w x (h+1).
@@ -80,6 +82,10 @@ class FldSolver {
     // (which turns the autotuner off). Level 0's post-smoother keeps 32x32: its dot partials are counted per tile.
     int tiles(int *pre, int *post, int max_levels, int *levels) const;
     int set_tiles(const int *pre, const int *post, int levels);
+    // the fused pair (levels coarse_from - 2 >= 1 and coarse_from - 1, one launch per direction; no result bit depends
+    // on it): the autotuner's choice per direction, or the caller's
+    int fusion(int *pre, int *post) const;
+    int set_fusion(int pre, int post);
     int nlevels() const { return (int)lv.size(); }
     int width() const { return w0; }
     int height() const { return h0; }
@@ -108,6 +114,12 @@ class FldSolver {
     // choice at the first rebuild, or the caller's (set_tiles)
     std::vector<int> pre_tile, post_tile;
     bool tuned = false, autotune = true;
+    int fuse_pre = FUSE_SMALL, fuse_post = FUSE_SMALL;
+    bool fuse_fixed = false;
+    // the finer level of the fused pair, or -1 when fewer than three levels are tile-smoothed (level 0 stays apart)
+    int pair_level() const { return coarse_from >= 3 ? coarse_from - 2 : -1; }
+    int launch_pre_pair(int a, const float *ba, hipStream_t s, const int *gate = nullptr);
+    int launch_post_pair(int a, hipStream_t s, const int *gate = nullptr);
     int tune(hipStream_t s);
     int launch_pre_level(int l, const float *b, hipStream_t s, const int *gate = nullptr);
     int launch_post_level(int l, const float *b, float *zout, const float *zold, double *part, int *nblocks,

@@ -900,6 +900,11 @@ extern "C" int sr_set_solve_mode(sr_world *wd, int mode) {
     return SR_OK;
 }
 
+extern "C" int sr_set_fusion(sr_world *wd, int pre, int post) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    return wd->sw.heat ? wd->heat.solver.set_fusion(pre, post) : SR_OK;
+}
+
 extern "C" int sr_set_dt_cap(sr_world *wd, double cap) {
     if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
     if (!(cap > 0.0) || !std::isfinite(cap)) return fail(SR_ERR_ARGUMENT, "a dt cap must be positive and finite");

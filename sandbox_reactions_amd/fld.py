@@ -94,6 +94,19 @@ class Fld:
         k = len(pre)
         _check(lib().sr_fld_set_tiles(self._p, (C.c_int * max(1, k))(*pre), (C.c_int * max(1, k))(*post), k))
 
+    def fusion(self):
+        """(pre, post): FUSE_OFF, FUSE_SMALL or FUSE_LARGE for the last two tile-smoothed levels' smoothers, the
+        autotuner's choice once a system is set, or set_fusion's; FUSE_OFF for both with fewer than three levels
+        tile-smoothed."""
+        from . import _check, lib
+        pre, post = C.c_int(), C.c_int()
+        _check(lib().sr_fld_fusion(self._p, C.byref(pre), C.byref(post)))
+        return pre.value, post.value
+
+    def set_fusion(self, pre: int, post: int):
+        from . import _check, lib
+        _check(lib().sr_fld_set_fusion(self._p, pre, post))
+
     def __del__(self):
         from . import _LIB
         if getattr(self, "_p", None) and _LIB is not None:

@@ -128,6 +128,10 @@ class World:
         """SOLVE_HOST, SOLVE_DEVICE or SOLVE_GRAPH: how the heat pass's solves decide when to stop."""
         self._c(self.lib.sr_set_solve_mode(self._p, mode))
 
+    def set_fusion(self, pre: int, post: int):
+        """FUSE_OFF, FUSE_SMALL or FUSE_LARGE for the heat solver's last two tile-smoothed levels (as Fld.set_fusion)."""
+        self._c(self.lib.sr_set_fusion(self._p, pre, post))
+
     def set_dt_cap(self, cap: float):
         self._c(self.lib.sr_set_dt_cap(self._p, cap))
 

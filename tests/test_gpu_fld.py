@@ -167,6 +167,51 @@ def test_autotuned_tiles_change_no_result_bit(gpu_lib):
         sr.Fld(n, n, nu1=3).set_tiles([sr.TILE_32] + [sr.TILE_64] * (k - 1), [sr.TILE_32] * k)
 
 
+@pytest.mark.parametrize("w,h,nu", [(512, 512, 2), (600, 380, 2), (600, 380, 1), (600, 380, 3)])
+def test_the_fused_smoother_pair_changes_no_result_bit(gpu_lib, w, h, nu):
+    """The last two tile-smoothed levels' smoothers fused into one launch per direction, in either tiling and mixed,
+    against separate launches and the autotuner's choice: the same V-cycle output and the same solve to the last bit
+    (every value a fused block uses is one the separate kernels compute, with the same expressions in the same
+    order). 512^2 and 600x380 tile-smooth three levels (the pair 256^2/128^2, and the odd 300x190/150x95)."""
+    import torch
+    sr = gpu_lib
+    print(f"fused pair {w}x{h} nu {nu}", flush=True)
+    own, kx, ky, b = ragged_system(w, h, n=1024) if (w, h) != (512, 512) else system(512)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    bt = torch.tensor(b, device="cuda")
+    r = t(np.random.default_rng(5).standard_normal((h, w)))
+    forms = [None, (sr.FUSE_OFF, sr.FUSE_OFF), (sr.FUSE_SMALL, sr.FUSE_SMALL), (sr.FUSE_SMALL, sr.FUSE_OFF)]
+    if nu <= 2:
+        forms += [(sr.FUSE_LARGE, sr.FUSE_LARGE), (sr.FUSE_OFF, sr.FUSE_LARGE)]
+    out = []
+    for form in forms:
+        f = sr.Fld(w, h, nu1=nu, nu2=nu, coarse_scale=1.2)
+        if form is not None:
+            f.set_fusion(*form)
+        f.set_system(t(own), t(kx), t(ky))
+        got = f.fusion()
+        print("  form", form, "in use", got, flush=True)
+        assert got == (form if form is not None else got) and all(v in (0, 1, 2) for v in got)
+        z = torch.empty((h, w), dtype=torch.float32, device="cuda")
+        f.apply_precond(r, z)
+        x = torch.zeros((h, w), dtype=torch.float64, device="cuda")
+        it, rel = f.solve(bt, x, tol=1e-8)
+        assert rel <= 1e-8
+        out.append((it, z.cpu().numpy(), x.cpu().numpy()))
+    for it, z, x in out[1:]:
+        assert it == out[0][0]
+        assert np.array_equal(z.view(np.uint32), out[0][1].view(np.uint32))
+        assert np.array_equal(x.view(np.uint64), out[0][2].view(np.uint64))
+    with pytest.raises(sr.SrError):
+        sr.Fld(w, h).set_fusion(3, 0)
+    if nu == 3:
+        with pytest.raises(sr.SrError):
+            sr.Fld(w, h, nu1=3, nu2=3).set_fusion(sr.FUSE_LARGE, sr.FUSE_OFF)
+    # fewer than three tile-smoothed levels: no pair
+    small = upload(sr, 256, *system(256)[:3])
+    assert small.fusion() == (sr.FUSE_OFF, sr.FUSE_OFF)
+
+
 def ragged_system(w, h, n=256):
     """The stiff lognormal system cut to w x h (not multiples of any tile): the cut's edge faces carry no coupling."""
     own, kx, ky, b = system(n)

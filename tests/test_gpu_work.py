@@ -85,6 +85,27 @@ def test_kernel_clocks_count_every_launch(gpu_lib):
         w.kernel_time("no_such_kernel")
 
 
+def test_the_fused_smoother_pair_in_the_world(gpu_lib):
+    """The heat pass with the last two tile-smoothed levels' smoothers fused (either tiling) or launched apart: the
+    same state to the last bit after two full steps, host- and device-decided. 512^2 tile-smooths three levels."""
+    sr = gpu_lib
+    n = 512
+    print(f"fused pair in the world {n}^2", flush=True)
+    for mode in (sr.SOLVE_HOST, sr.SOLVE_GRAPH):
+        states = []
+        for form in ((sr.FUSE_OFF, sr.FUSE_OFF), (sr.FUSE_SMALL, sr.FUSE_SMALL), (sr.FUSE_LARGE, sr.FUSE_LARGE)):
+            w = make(sr, n)
+            w.set_solve_mode(mode)
+            w.set_fusion(*form)
+            w.run(2)
+            states.append((w.readback(), w.read_dt().sim_time))
+            w.close()
+        for s, t in states[1:]:
+            assert np.array_equal(s.view(np.uint32), states[0][0].view(np.uint32)) and t == states[0][1], mode
+    with pytest.raises(sr.SrError):
+        make(sr, 16).set_fusion(0, 5)
+
+
 def test_host_and_device_decided_solves(gpu_lib):
     """The heat pass's solves decided on the host (partials read back), on the device (slots launched one by one) and
     by the captured graph of slots: the two device modes give the same bits after three steps; all three converge
