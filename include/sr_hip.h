@@ -199,6 +199,11 @@ int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int levels);
  * V-cycle dispatches each level's passes, heat.wgsl:1401-1445). */
 int sr_fld_fusion(sr_fld *f, int *pre, int *post);
 int sr_fld_set_fusion(sr_fld *f, int pre, int post);
+/* The single-block V-cycle's smallest levels (from the first of at most 16 x 16 cells, when one or two levels follow
+ * it and the last is solved by its dense inverse) on one wave, neighbours by cross-lane shuffles and no block
+ * barrier (on, the default), or on the whole block from LDS (off): the same values either way. No counterpart in the
+ * reference (its one coarse workgroup, heat.wgsl:1485-1526). */
+int sr_fld_set_coarse_wave(sr_fld *f, int on);
 void sr_fld_destroy(sr_fld *f);
 /* The learned preconditioner (SR_PRECOND_LEARNED): z = V r + N(r - A V r), V the multigrid V-cycle and N the
  * network. With fallback_iterations > 0 the solve continues with plain multigrid from where it stands once the

@@ -121,6 +121,7 @@ SIGNATURES = [
     ("sr_fld_set_solve_mode", _i, [_vp, _i]),
     ("sr_fld_fusion", _i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     ("sr_fld_set_fusion", _i, [_vp, _i, _i]),
+    ("sr_fld_set_coarse_wave", _i, [_vp, _i]),
     ("sr_set_fusion", _i, [_vp, _i, _i]),
     ("sr_destroy", None, [_vp]),
     ("sr_kernel_time", _i, [_vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_d), C.POINTER(_d)]),

@@ -724,6 +724,7 @@ struct CoarseSet {
     int lw[16];        // log2 of the level's width when a power of two, else -1
     int head, ltx;     // register-resident levels (0 or 2); log2 TX of the block's TX x (COARSE_NT / TX) layout
     int offza, offzb, offrb;  // LDS: the head levels' published z, the second level's residuals
+    int wave;          // the first level the wave tail runs, or -1
 };
 
 // (y, x) of entry t in rows of width w: a shift when w is a power of two (lw >= 0), a division otherwise
@@ -993,9 +994,150 @@ __global__ void __launch_bounds__(COARSE_NT) k_coarse(CoarseSet C, const float *
             __syncthreads();
         }
     }
-    for (int l = T; l < lc; l++) down(l);
-    coarsest();
-    for (int l = lc - 1; l >= T; l--) up(l);
+    // The wave tail: the levels from C.wave on (one of at most 16 x 16 cells, then one or two more, the last solved
+    // by the dense inverse) on the block's first wave alone, with no block barrier. Lane (tx, ty) holds the first
+    // wave level's 2x2 block at (2 tx, 2 ty) in registers and the next level's cell (tx, ty), the block's aggregate;
+    // a neighbour in another lane comes by a cross-lane shuffle. Every cell's expression and every sum's order are
+    // the LDS levels', so the tail computes the same values (a zero coupling's product may differ in the sign of 0).
+    auto wave_tail = [&](int s0) {
+        const int lane = (int)threadIdx.x, qx = lane & 7, qy = lane >> 3;
+        const int w = C.w[s0], h = C.h[s0], s1 = s0 + 1, w1 = C.w[s1], h1 = C.h[s1];
+        auto nb = [&](float v, int src) { return __shfl(v, src, 64); };
+        float b[4], z[4], o[4], rd[4], cl[4], cr[4], cu[4], cd[4];
+        bool in[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int x = 2 * qx + (k & 1), y = 2 * qy + (k >> 1), i = y * w + x;
+            in[k] = x < w && y < h;
+            b[k] = in[k] ? sm[offb[s0] + i] : 0.0f;
+            o[k] = in[k] ? sm[offo[s0] + i] : 0.0f;
+            rd[k] = in[k] ? sm[offd[s0] + i] : 0.0f;
+            cl[k] = in[k] ? sm[offx[s0] + y * (w + 1) + x] : 0.0f;
+            cr[k] = in[k] ? sm[offx[s0] + y * (w + 1) + x + 1] : 0.0f;
+            cu[k] = in[k] ? sm[offy[s0] + i] : 0.0f;
+            cd[k] = in[k] ? sm[offy[s0] + i + w] : 0.0f;
+            z[k] = 0.0f;
+        }
+        // colour 0 holds cells 0 and 3 of the block, colour 1 cells 1 and 2
+        auto sweep0 = [&](int color) {
+            if (color == 0) {
+                const float L1 = nb(z[1], lane - 1), U2 = nb(z[2], lane - 8), R2 = nb(z[2], lane + 1),
+                            D1 = nb(z[1], lane + 8);
+                if (in[0]) z[0] = (b[0] + cl[0] * L1 + cr[0] * z[1] + cu[0] * U2 + cd[0] * z[2]) * rd[0];
+                if (in[3]) z[3] = (b[3] + cl[3] * z[2] + cr[3] * R2 + cu[3] * z[1] + cd[3] * D1) * rd[3];
+            } else {
+                const float R0 = nb(z[0], lane + 1), U3 = nb(z[3], lane - 8), L3 = nb(z[3], lane - 1),
+                            D0 = nb(z[0], lane + 8);
+                if (in[1]) z[1] = (b[1] + cl[1] * z[0] + cr[1] * R0 + cu[1] * U3 + cd[1] * z[3]) * rd[1];
+                if (in[2]) z[2] = (b[2] + cl[2] * L3 + cr[2] * z[3] + cu[2] * z[0] + cd[2] * D0) * rd[2];
+            }
+        };
+        for (int k = 0; k < nu1; k++) {
+            sweep0(0);
+            sweep0(1);
+        }
+        // the residual of the block restricted into the lane's cell of the next level (cells in row order)
+        float b1 = 0.0f;
+        {
+            const float L1 = nb(z[1], lane - 1), U2 = nb(z[2], lane - 8), R0 = nb(z[0], lane + 1),
+                        U3 = nb(z[3], lane - 8), L3 = nb(z[3], lane - 1), D0 = nb(z[0], lane + 8),
+                        R2 = nb(z[2], lane + 1), D1 = nb(z[1], lane + 8);
+            const float zn[4][4] = {{L1, z[1], U2, z[2]}, {z[0], R0, U3, z[3]}, {L3, z[3], z[0], D0},
+                                    {z[2], R2, z[1], D1}};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!in[k]) continue;
+                const float zi = z[k];
+                const float azi = o[k] * zi + cl[k] * (zi - zn[k][0]) + cr[k] * (zi - zn[k][1]) +
+                                  cu[k] * (zi - zn[k][2]) + cd[k] * (zi - zn[k][3]);
+                b1 += b[k] - azi;
+            }
+        }
+        const bool v1 = qx < w1 && qy < h1;
+        const int i1 = qy * w1 + qx;
+        // the coarsest's z = A^-1 b in lane t for its cell t; b of cell j from lane src(j), every exchange issued
+        // before the sum (in j order) starts
+        auto dense = [&](float bv, auto src) {
+            float bj[DENSE_MAX];
+#pragma unroll
+            for (int j = 0; j < DENSE_MAX; j++) bj[j] = nb(bv, src(j));
+            float zt = 0.0f;
+            if (lane < m) {
+#pragma unroll
+                for (int j = 0; j < DENSE_MAX; j++)
+                    if (j < m) zt += sm[offinv + lane * m + j] * bj[j];
+            }
+            return zt;
+        };
+        if (s1 == lc) {
+            // the next level is the coarsest
+            const float zt = dense(b1, [&](int j) { return (j % w1) + 8 * (j / w1); });
+            const float z1 = nb(zt, i1);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (in[k]) z[k] += omega * z1;
+        } else {
+            const float o1 = v1 ? sm[offo[s1] + i1] : 0.0f, rd1 = v1 ? sm[offd[s1] + i1] : 0.0f;
+            const float l1 = v1 ? sm[offx[s1] + qy * (w1 + 1) + qx] : 0.0f;
+            const float r1 = v1 ? sm[offx[s1] + qy * (w1 + 1) + qx + 1] : 0.0f;
+            const float u1 = v1 ? sm[offy[s1] + i1] : 0.0f, d1 = v1 ? sm[offy[s1] + i1 + w1] : 0.0f;
+            float z1 = 0.0f;
+            auto sweep1 = [&](int color) {
+                const float L = nb(z1, lane - 1), R = nb(z1, lane + 1), U = nb(z1, lane - 8), D = nb(z1, lane + 8);
+                if (v1 && ((qx + qy) & 1) == color) z1 = (b1 + l1 * L + r1 * R + u1 * U + d1 * D) * rd1;
+            };
+            for (int k = 0; k < nu1; k++) {
+                sweep1(0);
+                sweep1(1);
+            }
+            // the residual, restricted into the coarsest (lane t forming its cell t), the dense solve
+            float r = 0.0f;
+            {
+                const float L = nb(z1, lane - 1), R = nb(z1, lane + 1), U = nb(z1, lane - 8), D = nb(z1, lane + 8);
+                const float azi = o1 * z1 + l1 * (z1 - L) + r1 * (z1 - R) + u1 * (z1 - U) + d1 * (z1 - D);
+                r = b1 - azi;
+            }
+            const int w2 = C.w[lc], X = lane % w2, Y = lane / w2;
+            float b2 = 0.0f;
+#pragma unroll
+            for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                for (int dx = 0; dx < 2; dx++) {
+                    const int x = 2 * X + dx, y = 2 * Y + dy;
+                    const float rv = nb(r, x + 8 * y);
+                    if (lane < m && x < w1 && y < h1) b2 += rv;
+                }
+            const float zt = dense(b2, [](int j) { return j; });
+            // the coarsest's correction prolonged, the post-smoothing
+            const float zc = nb(zt, (qy >> 1) * w2 + (qx >> 1));
+            if (v1) z1 += omega * zc;
+            for (int k = 0; k < nu2; k++) {
+                sweep1(1);
+                sweep1(0);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (in[k]) z[k] += omega * z1;
+        }
+        for (int k = 0; k < nu2; k++) {
+            sweep0(1);
+            sweep0(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int x = 2 * qx + (k & 1), y = 2 * qy + (k >> 1);
+            if (in[k]) sm[offz[s0] + y * w + x] = z[k];
+        }
+    };
+    const int S = C.wave >= 0 ? C.wave : lc;
+    for (int l = T; l < S; l++) down(l);
+    if (C.wave >= 0) {
+        if (threadIdx.x < 64) wave_tail(S);
+        __syncthreads();
+    } else {
+        coarsest();
+    }
+    for (int l = S - 1; l >= T; l--) up(l);
     if constexpr (HEAD) {
         // the second level: the tail's correction prolonged, published, post-smoothed
         if (vb) {
@@ -1388,6 +1530,15 @@ CoarseSet FldSolver::coarse_set() const {
         o += C.w[1] * C.h[1];
     }
     C.total = o;
+    // the wave tail: from the first level past the head of at most 16 x 16 cells, when one or two levels follow it
+    // and the last is solved by the dense inverse
+    C.wave = -1;
+    if (coarse_wave) {
+        int s = head;
+        while (s < C.n && (C.w[s] > 16 || C.h[s] > 16)) s++;
+        const int lc = C.n - 1;
+        if (s < C.n && (lc - s == 1 || lc - s == 2) && C.w[lc] * C.h[lc] <= DENSE_MAX) C.wave = s;
+    }
     return C;
 }
 
@@ -2167,6 +2318,12 @@ extern "C" int sr_fld_tiles(sr_fld *f, int *pre, int *post, int max_levels, int 
 extern "C" int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int levels) {
     if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_tiles: null solver");
     return f->s.set_tiles(pre, post, levels);
+}
+
+extern "C" int sr_fld_set_coarse_wave(sr_fld *f, int on) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_coarse_wave: null solver");
+    f->s.set_coarse_wave(on != 0);
+    return SR_OK;
 }
 
 extern "C" int sr_fld_fusion(sr_fld *f, int *pre, int *post) {

@@ -86,6 +86,12 @@ class FldSolver {
     // on it): the autotuner's choice per direction, or the caller's
     int fusion(int *pre, int *post) const;
     int set_fusion(int pre, int post);
+    // the single-block V-cycle's smallest levels on one wave with no block barrier (on) or on the whole block from
+    // LDS (off); the same values either way
+    void set_coarse_wave(bool on) {
+        coarse_wave = on;
+        tiles_version++;
+    }
     int nlevels() const { return (int)lv.size(); }
     int width() const { return w0; }
     int height() const { return h0; }
@@ -115,6 +121,7 @@ class FldSolver {
     std::vector<int> pre_tile, post_tile;
     bool tuned = false, autotune = true;
     int fuse_pre = FUSE_SMALL, fuse_post = FUSE_SMALL;
+    bool coarse_wave = true;  // the single-block V-cycle's levels of at most 16 x 16 cells on one wave
     bool fuse_fixed = false;
     // the finer level of the fused pair, or -1 when fewer than three levels are tile-smoothed (level 0 stays apart)
     int pair_level() const { return coarse_from >= 3 ? coarse_from - 2 : -1; }

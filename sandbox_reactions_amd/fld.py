@@ -107,6 +107,11 @@ class Fld:
         from . import _check, lib
         _check(lib().sr_fld_set_fusion(self._p, pre, post))
 
+    def set_coarse_wave(self, on: bool):
+        """The single-block V-cycle's smallest levels on one wave (True, the default) or on the whole block."""
+        from . import _check, lib
+        _check(lib().sr_fld_set_coarse_wave(self._p, int(on)))
+
     def __del__(self):
         from . import _LIB
         if getattr(self, "_p", None) and _LIB is not None:

@@ -212,6 +212,38 @@ def test_the_fused_smoother_pair_changes_no_result_bit(gpu_lib, w, h, nu):
     assert small.fusion() == (sr.FUSE_OFF, sr.FUSE_OFF)
 
 
+@pytest.mark.parametrize("w,h", [(1024, 1024), (512, 512), (600, 380), (200, 120), (97, 61), (33, 250)])
+def test_the_coarse_wave_tail_is_the_block_tail(gpu_lib, w, h):
+    """The single-block V-cycle's smallest levels on one wave (registers and cross-lane shuffles) against the same
+    levels on the whole block from LDS: the same V-cycle output and the same solve (values equal; a zero coupling's
+    product may differ in the sign of 0). The shapes put the wave's first level at 16x16 (after the register-resident
+    head), 10x6 (no head) and other ragged sides."""
+    import torch
+    sr = gpu_lib
+    print(f"coarse wave tail {w}x{h}", flush=True)
+    if (w, h) in ((1024, 1024), (512, 512)):
+        own, kx, ky, b = system(w)
+    else:
+        own, kx, ky, b = ragged_system(w, h, n=1024 if max(w, h) > 256 else 256)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    bt = torch.tensor(b, device="cuda")
+    r = t(np.random.default_rng(9).standard_normal((h, w)))
+    out = []
+    for on in (True, False):
+        f = sr.Fld(w, h, coarse_scale=1.2)
+        f.set_coarse_wave(on)
+        f.set_system(t(own), t(kx), t(ky))
+        z = torch.empty((h, w), dtype=torch.float32, device="cuda")
+        f.apply_precond(r, z)
+        x = torch.zeros((h, w), dtype=torch.float64, device="cuda")
+        it, rel = f.solve(bt, x, tol=1e-8)
+        assert rel <= 1e-8
+        out.append((it, z.cpu().numpy(), x.cpu().numpy()))
+    (ia, za, xa), (ib, zb, xb) = out
+    print(f"  iterations {ia} / {ib}", flush=True)
+    assert ia == ib and np.array_equal(za, zb) and np.array_equal(xa, xb)
+
+
 def ragged_system(w, h, n=256):
     """The stiff lognormal system cut to w x h (not multiples of any tile): the cut's edge faces carry no coupling."""
     own, kx, ky, b = system(n)
