@@ -7,6 +7,7 @@ float4 and runtime copies are reported beside it (wide requests can count differ
 
     python -m sandbox_reactions_amd.pmc FETCH_DIR WRITE_DIR OUT_JSON [--workload TEXT] [--command TEXT]
     python -m sandbox_reactions_amd.pmc --trace TRACE_DIR OUT_MD [--workload TEXT] [--command TEXT]
+    python -m sandbox_reactions_amd.pmc --counters OUT_MD PASS_DIR... [--kernel REGEX] [--workload TEXT] [--command TEXT]
 
 OUT_JSON gets {"copy_scale": {...}, "kernels": {name: {"launches", "fetch_bytes", "write_bytes", "traffic_bytes",
 "grid"}}} for the dispatch grid with the most total traffic per kernel, and "by_grid": {"name|grid": ...} for every
@@ -27,13 +28,23 @@ COPY_BYTES = 512 * 2**20
 
 
 def read_counters(d):
+    """The counter rows of a rocprofv3 --pmc run: its CSV output (-f csv) or its rocpd database (the default)."""
     paths = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
-    if not paths:
-        raise FileNotFoundError(f"no counter_collection.csv under {d}")
     rows = []
     for p in paths:
         with open(p) as f:
             rows += list(csv.DictReader(f))
+    if not paths:
+        import sqlite3
+        dbs = glob.glob(os.path.join(d, "**", "*_results.db"), recursive=True)
+        if not dbs:
+            raise FileNotFoundError(f"no counter_collection.csv or rocpd database under {d}")
+        for p in dbs:
+            con = sqlite3.connect(p)
+            q = "select kernel_name, grid_size, counter_name, value from counters_collection"
+            rows += [{"Kernel_Name": k, "Grid_Size": g, "Counter_Name": c, "Counter_Value": v}
+                     for k, g, c, v in con.execute(q)]
+            con.close()
     return rows
 
 
@@ -110,15 +121,48 @@ def write_trace(trace_dir, out_md, workload, command):
             f.write(f"| {name} | {grid} | {wg} | {n} | {mean:.2f} | {total:.2f} |\n")
 
 
+def counters_table(dirs, kernel_re):
+    """Per (kernel, grid) matching kernel_re: each counter's median per dispatch over the runs in dirs (one counter
+    pass each), and the dispatches it was read on."""
+    import re
+    pat = re.compile(kernel_re)
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in dirs:
+        for r in read_counters(d):
+            name = short(r["Kernel_Name"])
+            if not pat.search(name):
+                continue
+            agg[(name, int(r["Grid_Size"]))][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: (float(np.median(v)), len(v)) for c, v in cs.items()} for k, cs in agg.items()}
+
+
+def write_counters(dirs, out_md, kernel_re, workload, command):
+    t = counters_table(dirs, kernel_re)
+    with open(out_md, "w") as f:
+        f.write(f"# Counters: {workload}\n\nCommands: `{command}`\n\n")
+        f.write("Median per dispatch of each counter (one rocprofv3 --pmc pass per group of counters), per kernel and "
+                "grid (threads).\n\n")
+        for (name, grid), cs in sorted(t.items()):
+            f.write(f"## {name}, grid {grid}\n\n| counter | median per dispatch | dispatches |\n|---|---|---|\n")
+            for c, (v, n) in sorted(cs.items()):
+                f.write(f"| {c} | {v:.6g} | {n} |\n")
+            f.write("\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", action="store_true", help="summarise a kernel trace: TRACE_DIR OUT_MD")
+    ap.add_argument("--counters", action="store_true", help="tabulate counter passes: OUT_MD PASS_DIR...")
+    ap.add_argument("--kernel", default=".", help="with --counters: the kernels to keep (a regular expression)")
     ap.add_argument("paths", nargs="+")
     ap.add_argument("--workload", default="")
     ap.add_argument("--command", default="")
     a = ap.parse_args()
     if a.trace:
         write_trace(a.paths[0], a.paths[1], a.workload, a.command)
+        return
+    if a.counters:
+        write_counters(a.paths[1:], a.paths[0], a.kernel, a.workload, a.command)
         return
     if len(a.paths) != 3:
         ap.error("FETCH_DIR WRITE_DIR OUT_JSON")
