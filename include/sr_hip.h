@@ -294,6 +294,29 @@ int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
 int sr_set_solve_mode(sr_world *wd, int mode);
 /* the heat pass's fused smoother pair, as sr_fld_set_fusion */
 int sr_set_fusion(sr_world *wd, int pre, int post);
+/* The active box (step/boxfit.rs:1-14, BoxMode and BoxReport :27-55): the bounding box of the cells with Sigma >=
+ * Sigma_vac, grown by 8 cells, rounded out to multiples of 8, clamped to the world (empty, all 0, with no such cell),
+ * its FFT size per axis the smallest power of two >= 2 x the box's, at least 32. It is fit at the upload (before
+ * step 0's dt) and after P9 at step indices = 0 (mod 16); P8's floors and P9's dt, wave-speed maximum and non-finite
+ * guard run over its cells only, so vacuum outside it keeps its uploaded values. SR_BOX_WHOLE_WORLD (the default
+ * here; the twin's world) holds it at the world on the same schedule; SR_BOX_FIT is the reference's Step::new
+ * (step/mod.rs:368). Set before sr_upload. Hydro, gravity and the heat pass leave cells outside the box as they are
+ * either way (every cell there and its neighbours are below Sigma_vac) and run over the world: gravity's FFT is the
+ * world's, where the reference sizes it to the box (the same field up to round-off). */
+enum { SR_BOX_WHOLE_WORLD = 0, SR_BOX_FIT = 1 };
+typedef struct {
+    uint32_t x0, y0, w, h; /* origin and size in cells, multiples of 8; all 0 when empty */
+    uint32_t fft[2];       /* the FFT size per axis */
+    uint32_t from_step;    /* the first step that runs inside this box (0: the upload's fit) */
+    uint32_t fits;         /* fits written since the upload, its own included */
+    uint32_t tiles[3];     /* the reference's indirect sizes over the box: 8 x 8 tiles, then 256-cell groups */
+    uint32_t cells[3];
+} sr_box_report;
+int sr_set_box_mode(sr_world *wd, int mode);
+int sr_read_box(sr_world *wd, sr_box_report *out);
+/* load_dump's restore of the box (Step::restore, step/latch.rs:496-505): the record (x0, y0, w, h, fft_w, fft_h,
+ * from_step, fits) a dump holds and the step index the refit schedule and the guard count from; after sr_resume */
+int sr_resume_box(sr_world *wd, const uint32_t *record, uint64_t step);
 int sr_set_dt_cap(sr_world *wd, double cap);                  /* Twin::cap_dt_next (reference/mod.rs:225) */
 int sr_set_timing(sr_world *wd, int on);                      /* Step::set_timing (step/mod.rs:722) */
 int sr_read_cost(sr_world *wd, sr_step_cost *out);            /* Step::read_cost (step/mod.rs:753) */

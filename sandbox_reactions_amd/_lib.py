@@ -22,6 +22,7 @@ STOP_REFERENCE, STOP_RELATIVE = 0, 1
 TILE_16, TILE_32, TILE_64 = 0, 1, 2
 SOLVE_HOST, SOLVE_DEVICE, SOLVE_GRAPH = 0, 1, 2
 FUSE_OFF, FUSE_SMALL, FUSE_LARGE = 0, 1, 2
+BOX_WHOLE_WORLD, BOX_FIT = 0, 1
 FIELD_PHI, FIELD_GX, FIELD_GY, FIELD_FLUX_X, FIELD_FLUX_Y, FIELD_SIGMA_MID, FIELD_PHI_MID = range(7)
 COMPONENTS = 14
 
@@ -77,6 +78,12 @@ class ResumeRecords(C.Structure):
     _fields_ = [("dt", C.c_double), ("dt_next", C.c_double), ("sim_time", C.c_double)]
 
 
+class BoxReport(C.Structure):
+    _fields_ = [("x0", C.c_uint32), ("y0", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
+                ("fft", C.c_uint32 * 2), ("from_step", C.c_uint32), ("fits", C.c_uint32),
+                ("tiles", C.c_uint32 * 3), ("cells", C.c_uint32 * 3)]
+
+
 # (name, restype, argtypes): every entry point include/sr_hip.h declares
 _vp, _i, _d, _sz, _u32 = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_uint32
 _pp = C.POINTER(C.c_void_p)
@@ -123,6 +130,9 @@ SIGNATURES = [
     ("sr_fld_set_fusion", _i, [_vp, _i, _i]),
     ("sr_fld_set_coarse_wave", _i, [_vp, _i]),
     ("sr_set_fusion", _i, [_vp, _i, _i]),
+    ("sr_set_box_mode", _i, [_vp, _i]),
+    ("sr_read_box", _i, [_vp, C.POINTER(BoxReport)]),
+    ("sr_resume_box", _i, [_vp, C.POINTER(C.c_uint32), C.c_uint64]),
     ("sr_destroy", None, [_vp]),
     ("sr_kernel_time", _i, [_vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_d), C.POINTER(_d)]),
     ("sr_fld_set_network", _i, [_vp, _vp, _i]),

@@ -71,10 +71,17 @@ __global__ void k_p1(double *scal) {
 // P8 floors (floors.rs:124-175, no hold) then P9's dt (hydro.rs:618-634) and the non-finite guard: the largest
 // wave-speed rate and |g| by a max reduction, the first non-finite (cell, channel) as the max of -key.
 __global__ void __launch_bounds__(NT) k_floors_dt(FloorParams P, HydroState st, const float *__restrict__ gx,
-                                                  const float *__restrict__ gy, int apply, double *part) {
+                                                  const float *__restrict__ gy, int apply, double *part,
+                                                  const int *__restrict__ box) {
     const int n = P.w * P.h;
+    // the active box: P8 and P9 read and write its cells only (the whole world when the box is held at it)
+    const int bx0 = box[0], by0 = box[1], bw = box[2], bh = box[3];
     double v[3] = {0.0, 0.0, -HUGE_VAL};
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+        {
+            const int x = i % P.w - bx0, y = i / P.w - by0;
+            if (x < 0 || y < 0 || x >= bw || y >= bh) continue;
+        }
         float xs[NS];
         bool finite = true;
         for (int s = 0; s < NS; s++) {
@@ -195,6 +202,109 @@ __global__ void k_finalize_dt(FloorParams P, const double *__restrict__ part, in
     }
 }
 
+// The active box (step/boxfit.rs:1-14, the test oracle of tests/gpu/box.rs:21-90): the bounding box of the cells with
+// Sigma >= Sigma_vac, grown by BOX_MARGIN cells, rounded out to multiples of BOX_ROUND, clamped to the world, in that
+// order; empty (all 0) with no such cell; the FFT size per axis the smallest power of two >= 2 x the box's, at least
+// BOX_FFT_MIN. Fit before step 0 (at the upload) and after P9 at step indices = 0 (mod BOX_REFIT_EVERY); the record
+// holds x0, y0, w, h, fft_w, fft_h, the first step that runs inside it, and the fits written since the upload.
+constexpr int BOX_MARGIN = 8, BOX_ROUND = 8, BOX_FFT_MIN = 32, BOX_REFIT_EVERY = 16;
+enum { BOX_AT_UPLOAD = 0, BOX_AFTER_P9 = 1 };
+
+// after a P9: whether the step whose P9 just ran (index = the count of P9s closing a step, less one) refits the box
+__device__ inline bool box_due(const double *scal) {
+    const long long k = (long long)scal[SCAL_STEPS] - 1;
+    return k >= 0 && k % BOX_REFIT_EVERY == 0;
+}
+
+// per block: the min x, min y, max x + 1, max y + 1 of its cells holding matter (min > max when none)
+__global__ void __launch_bounds__(NT) k_box_cells(const double *__restrict__ sigma, int w, int h, double svac,
+                                                  const double *__restrict__ scal, int when, int *__restrict__ part) {
+    if (when == BOX_AFTER_P9 && !box_due(scal)) return;
+    __shared__ int sh[4][NT / 64];
+    int v[4] = {w, h, 0, 0};
+    const int n = w * h;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT)
+        if (sigma[i] >= svac) {
+            const int x = i % w, y = i / w;
+            v[0] = min(v[0], x);
+            v[1] = min(v[1], y);
+            v[2] = max(v[2], x + 1);
+            v[3] = max(v[3], y + 1);
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        v[0] = min(v[0], __shfl_xor(v[0], o, 64));
+        v[1] = min(v[1], __shfl_xor(v[1], o, 64));
+        v[2] = max(v[2], __shfl_xor(v[2], o, 64));
+        v[3] = max(v[3], __shfl_xor(v[3], o, 64));
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; k++) sh[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        int r = sh[k][0];
+        for (int j = 1; j < NT / 64; j++) r = k < 2 ? min(r, sh[k][j]) : max(r, sh[k][j]);
+        part[4 * blockIdx.x + k] = r;
+    }
+}
+
+__device__ inline int box_fft(int side) {
+    int f = 1;
+    while (f < 2 * side) f <<= 1;
+    return max(f, BOX_FFT_MIN);
+}
+
+// one block: the partials joined (integer min and max: no order to fix) and the record written
+__global__ void __launch_bounds__(NT) k_box_fit(const int *__restrict__ part, int nb, int w, int h, int whole,
+                                                const double *__restrict__ scal, int when, int *__restrict__ box) {
+    if (when == BOX_AFTER_P9 && !box_due(scal)) return;
+    __shared__ int sh[4][NT / 64];
+    int v[4] = {w, h, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += NT) {
+        v[0] = min(v[0], part[4 * b]);
+        v[1] = min(v[1], part[4 * b + 1]);
+        v[2] = max(v[2], part[4 * b + 2]);
+        v[3] = max(v[3], part[4 * b + 3]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        v[0] = min(v[0], __shfl_xor(v[0], o, 64));
+        v[1] = min(v[1], __shfl_xor(v[1], o, 64));
+        v[2] = max(v[2], __shfl_xor(v[2], o, 64));
+        v[3] = max(v[3], __shfl_xor(v[3], o, 64));
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; k++) sh[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 1; j < NT / 64; j++) {
+            sh[0][0] = min(sh[0][0], sh[0][j]);
+            sh[1][0] = min(sh[1][0], sh[1][j]);
+            sh[2][0] = max(sh[2][0], sh[2][j]);
+            sh[3][0] = max(sh[3][0], sh[3][j]);
+        }
+        int x0 = 0, y0 = 0, bw = 0, bh = 0;
+        if (whole) {
+            bw = w;
+            bh = h;
+        } else if (sh[0][0] < sh[2][0]) {
+            // v - MARGIN rounded down and v + MARGIN rounded up to the 8-grid, then clamped (a negative quotient
+            // truncates toward 0 where the floor would go lower, and both clamp to 0)
+            x0 = max(0, (sh[0][0] - BOX_MARGIN) / BOX_ROUND * BOX_ROUND);
+            y0 = max(0, (sh[1][0] - BOX_MARGIN) / BOX_ROUND * BOX_ROUND);
+            bw = min(w, (sh[2][0] + BOX_MARGIN + BOX_ROUND - 1) / BOX_ROUND * BOX_ROUND) - x0;
+            bh = min(h, (sh[3][0] + BOX_MARGIN + BOX_ROUND - 1) / BOX_ROUND * BOX_ROUND) - y0;
+        }
+        box[0] = x0;
+        box[1] = y0;
+        box[2] = bw;
+        box[3] = bh;
+        box[4] = box_fft(bw);
+        box[5] = box_fft(bh);
+        box[6] = when == BOX_AT_UPLOAD ? 0 : (int)scal[SCAL_STEPS];
+        box[7] = when == BOX_AT_UPLOAD ? 1 : box[7] + 1;
+    }
+}
+
 // FP64 totals: Sigma, Sigma u, Sigma v, E, Sigma X_s (hydro.rs:210-224), and the escaped sums per component
 __global__ void __launch_bounds__(NT) k_totals(HydroState st, int n, double *part) {
     double v[14];
@@ -282,6 +392,10 @@ struct sr_world {
     struct {
         unsigned long long steps = 0, hydro = 0, mid = 0, conv_first = 0, conv_mid = 0, radiation = 0;
     } wk;
+    // the active box: its record (x0, y0, w, h, fft_w, fft_h, from_step, fits) and the cell pass's partials
+    bool box_fit = false;
+    int *box = nullptr, *box_part = nullptr;
+    int box_nb = 1;
     std::vector<void *> allocs;
 };
 
@@ -361,11 +475,14 @@ extern "C" int sr_world_create(int w, int h, const sr_physics *ph, const sr_swit
     if (hipHostMalloc(&wd->host_scal, sizeof(double) * SCAL_N) != hipSuccess)
         return bail(fail(SR_ERR_DEVICE, "hipHostMalloc"));
     const int nb = hydro_blocks(w, h);
+    wd->box_nb = (int)std::min<size_t>(1024, (n + NT - 1) / NT);
     if ((c = world_alloc(wd, (void **)&wd->takeback, sizeof(double) * 8)) ||
         (c = world_alloc(wd, (void **)&wd->part.sums, sizeof(double) * 5 * nb)) ||
         (c = world_alloc(wd, (void **)&wd->part.keys, sizeof(unsigned long long) * nb)) ||
         (c = world_alloc(wd, (void **)&wd->rpart, sizeof(double) * 16 * 4096)) ||
         (c = world_alloc(wd, (void **)&wd->counters, sizeof(unsigned) * 8)) ||
+        (c = world_alloc(wd, (void **)&wd->box, sizeof(int) * 8)) ||
+        (c = world_alloc(wd, (void **)&wd->box_part, sizeof(int) * 4 * 1024)) ||
         (c = world_alloc(wd, (void **)&wd->phi, sizeof(float) * (w + 2) * (h + 2))) ||
         (c = world_alloc(wd, (void **)&wd->phi_mid, sizeof(float) * (w + 2) * (h + 2))) ||
         (c = world_alloc(wd, (void **)&wd->gx, sizeof(float) * n)) ||
@@ -478,13 +595,25 @@ static int p2(sr_world *wd) {
     return wd->grav.gradient(wd->phi, wd->gx, wd->gy, wd->s);
 }
 
+// the box fit on the current state: at the upload, or after a P9 (each kernel skipping itself unless the step index
+// is one the box is refit at, so the sequence is the same every step and can be captured)
+static int box_refit(sr_world *wd, int when) {
+    hipLaunchKernelGGL(k_box_cells, dim3(wd->box_nb), dim3(NT), 0, wd->s, wd->st[wd->cur].sigma, wd->w, wd->h,
+                       (double)wd->fp.svac32, wd->scal, when, wd->box_part);
+    hipLaunchKernelGGL(k_box_fit, dim3(1), dim3(NT), 0, wd->s, wd->box_part, wd->box_nb, wd->w, wd->h,
+                       wd->box_fit ? 0 : 1, wd->scal, when, wd->box);
+    SR_HIP(hipGetLastError());
+    return SR_OK;
+}
+
 static int p89(sr_world *wd, int apply) {
     HydroState &st = wd->st[wd->cur];
     const int nb = grid_n(wd->n);
     hipLaunchKernelGGL(k_floors_dt, dim3(nb), dim3(NT), 0, wd->s, wd->fp, st, wd->sw.gravity ? wd->gx : nullptr,
-                       wd->sw.gravity ? wd->gy : nullptr, apply, wd->rpart);
+                       wd->sw.gravity ? wd->gy : nullptr, apply, wd->rpart, (const int *)wd->box);
     hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(256), 0, wd->s, wd->fp, wd->rpart, nb, wd->scal, apply);
     SR_HIP(hipGetLastError());
+    if (apply) SR_TRY(box_refit(wd, BOX_AFTER_P9));
     return SR_OK;
 }
 
@@ -509,7 +638,9 @@ extern "C" int sr_upload(sr_world *wd, const float *planes, size_t n) {
         wd->heat.work = HeatWork{};
         wd->heat.solver.work = FldWork{};
     }
-    // the twin's start: P2 on the state and dt_0 from it (reference/mod.rs:173-175), no floors
+    // the box fit before step 0, ahead of the initial dt (step/boxfit.rs:10); then the twin's start: P2 on the
+    // state and dt_0 from it (reference/mod.rs:173-175), no floors
+    SR_TRY(box_refit(wd, BOX_AT_UPLOAD));
     SR_TRY(p2(wd));
     SR_TRY(p89(wd, 0));
     SR_HIP(hipStreamSynchronize(wd->s));
@@ -897,6 +1028,53 @@ extern "C" int sr_set_solve_mode(sr_world *wd, int mode) {
     if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH)
         return fail(SR_ERR_ARGUMENT, "sr_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE or _GRAPH");
     if (wd->sw.heat) wd->heat.solver.set_solve_mode(mode);
+    return SR_OK;
+}
+
+extern "C" int sr_set_box_mode(sr_world *wd, int mode) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (mode != SR_BOX_WHOLE_WORLD && mode != SR_BOX_FIT)
+        return fail(SR_ERR_ARGUMENT, "sr_set_box_mode: SR_BOX_WHOLE_WORLD or SR_BOX_FIT");
+    wd->box_fit = mode == SR_BOX_FIT;
+    return SR_OK;
+}
+
+extern "C" int sr_read_box(sr_world *wd, sr_box_report *out) {
+    if (!wd || !out) return fail(SR_ERR_ARGUMENT, "sr_read_box: null pointer");
+    int r[8];
+    SR_HIP(hipMemcpyAsync(r, wd->box, sizeof r, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    sr_box_report b{};
+    b.x0 = (uint32_t)r[0];
+    b.y0 = (uint32_t)r[1];
+    b.w = (uint32_t)r[2];
+    b.h = (uint32_t)r[3];
+    b.fft[0] = (uint32_t)r[4];
+    b.fft[1] = (uint32_t)r[5];
+    b.from_step = (uint32_t)r[6];
+    b.fits = (uint32_t)r[7];
+    b.tiles[0] = b.w / 8;
+    b.tiles[1] = b.h / 8;
+    b.tiles[2] = 1;
+    b.cells[0] = (b.w * b.h + 255) / 256;
+    b.cells[1] = b.cells[2] = 1;
+    *out = b;
+    return SR_OK;
+}
+
+extern "C" int sr_resume_box(sr_world *wd, const uint32_t *record, uint64_t step) {
+    if (!wd || !record) return fail(SR_ERR_ARGUMENT, "sr_resume_box: null pointer");
+    const uint32_t x0 = record[0], y0 = record[1], bw = record[2], bh = record[3];
+    if (x0 % 8 || y0 % 8 || bw % 8 || bh % 8 || (uint64_t)x0 + bw > (uint64_t)wd->w ||
+        (uint64_t)y0 + bh > (uint64_t)wd->h || (bw == 0) != (bh == 0) || step > (1ull << 52))
+        return fail(SR_ERR_ARGUMENT, "sr_resume_box: a box of multiples of 8 inside the world is required");
+    int r[8];
+    for (int k = 0; k < 8; k++) r[k] = (int)record[k];
+    SR_HIP(hipMemcpyAsync(wd->box, r, sizeof r, hipMemcpyHostToDevice, wd->s));
+    const double st = (double)step;
+    SR_HIP(hipMemcpyAsync(wd->scal + SCAL_STEPS, &st, sizeof st, hipMemcpyHostToDevice, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    wd->steps = step;
     return SR_OK;
 }
 

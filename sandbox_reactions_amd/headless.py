@@ -157,7 +157,9 @@ def run(argv=None) -> int:
     refused = (sr._lib.SR_ERR_WORLD_SIZE, sr._lib.SR_ERR_BOX_AXIS, sr._lib.SR_ERR_ARGUMENT, sr._lib.SR_ERR_LENGTH)
     try:
         try:
-            world = scene_json.world(scene, gamma=a.gamma, planes=d["planes"] if d is not None else None)
+            # the passes over the active box, as the reference's headless runs them (headless.rs:1081)
+            world = scene_json.world(scene, gamma=a.gamma, planes=d["planes"] if d is not None else None,
+                                     box_mode=sr.BOX_FIT)
         except sr.SrError as e:
             if e.code in refused:
                 print(f"SR-ERROR bad arguments: {e}", flush=True)
@@ -170,6 +172,8 @@ def run(argv=None) -> int:
             step0 = int(d["step"])
             flux = d["flux"] if scene.switches["heat"] else None
             world.resume(float(d["dt"]), float(d["dt_next"]), float(d["sim_time"]), flux=flux, sigma=d["sigma64"])
+            # and its active box with the step index its refits count from (Step::restore, step/latch.rs:496-505)
+            world.resume_box(d["box"], step0)
         if a.timing:
             world.set_timing(True)
         dumps = os.path.join(a.out, "dumps") if a.out else None
@@ -180,8 +184,11 @@ def run(argv=None) -> int:
             # F follows the channels whenever the run runs P5 (dump.rs:16-19); every dump here follows a step
             flux = np.stack([world.read_field(sr.FIELD_FLUX_X), world.read_field(sr.FIELD_FLUX_Y)]) \
                 if scene.switches["heat"] else None
+            b = world.read_box()
             dump.write(os.path.join(dumps, f"s_{step0 + done}.bin"), world.readback(), step=step0 + done,
-                       sim_time=r.sim_time, dt=r.dt, dt_next=r.dt_next, flux=flux, sigma64=world.readback_sigma())
+                       sim_time=r.sim_time, dt=r.dt, dt_next=r.dt_next, flux=flux, sigma64=world.readback_sigma(),
+                       box=[b.x0, b.y0, b.w, b.h, b.fft[0], b.fft[1], b.from_step, b.fits],
+                       box_args=list(b.tiles) + list(b.cells))
 
         t0 = time.perf_counter()
         target = min(a.steps if a.steps is not None else a.max_steps, a.max_steps)

@@ -293,15 +293,18 @@ def parse(text: str) -> Scene:
     return Scene(json.loads(text))
 
 
-def world(scene: Scene, heat_settings=None, planes=None, **physics_over):
+def world(scene: Scene, heat_settings=None, planes=None, box_mode=None, **physics_over):
     """A World for the scene (its world size, switches and constants; gamma 2 and unit cells unless overridden),
-    with its planes uploaded (`planes`, (14, h, w), in place of the scene's own: a dump's cells). Refuses a scene whose
-    passes this path does not carry."""
+    with its planes uploaded (`planes`, (14, h, w), in place of the scene's own: a dump's cells) and, given
+    `box_mode`, its active box in that mode (BOX_FIT, BOX_WHOLE_WORLD). Refuses a scene whose passes this path does
+    not carry."""
     import sandbox_reactions_amd as sr
     scene.check_path()
     kw = scene.physics_kwargs()
     kw.update(physics_over)
     w = sr.World(scene.width, scene.height, physics=sr.physics(**kw), gravity=scene.switches["gravity"],
                  hydro=scene.switches["hydro"], heat=scene.switches["heat"], heat_settings=heat_settings)
+    if box_mode is not None:
+        w.set_box_mode(box_mode)
     w.upload(scene.planes(kw["gamma"]) if planes is None else planes)
     return w

@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import DtReport, HeatReport, StepCost, StepWork, Switches
+from ._lib import BoxReport, DtReport, HeatReport, StepCost, StepWork, Switches
 
 
 class World:
@@ -64,6 +64,25 @@ class World:
         self._c(self.lib.sr_resume(self._p, C.byref(rec), None if f is None else f.ctypes.data,
                                    0 if f is None else f.size, None if s is None else s.ctypes.data,
                                    0 if s is None else s.size))
+
+    def set_box_mode(self, mode: int):
+        """BOX_WHOLE_WORLD (the default) or BOX_FIT (the reference's Step::new): P8 and P9 over the active box. Before
+        upload."""
+        self._c(self.lib.sr_set_box_mode(self._p, mode))
+
+    def read_box(self) -> BoxReport:
+        """The active box's record and the reference's indirect sizes over it (BoxReport, step/boxfit.rs:38-55)."""
+        r = BoxReport()
+        self._c(self.lib.sr_read_box(self._p, C.byref(r)))
+        return r
+
+    def resume_box(self, record, step: int):
+        """The box record a dump holds (8 words) and the step index the refit schedule counts from; after resume."""
+        rec = (C.c_uint32 * 8)(*[int(v) for v in record])
+        if len(record) != 8:
+            from . import SrError
+            raise SrError(_lib.SR_ERR_ARGUMENT, "the box record has 8 words")
+        self._c(self.lib.sr_resume_box(self._p, rec, int(step)))
 
     def readback_sigma(self) -> np.ndarray:
         """Sigma's FP64 plane, (h, w)."""

@@ -137,3 +137,19 @@ def test_flux_and_sigma64_round_trip(tmp_path):
         f.write(b"SRSIG64\0" + b"\0" * 8)
     with pytest.raises(dump.DumpError, match="sigma64"):
         dump.read(path)
+
+
+def test_the_box_record_round_trips_and_defaults_to_the_whole_world(tmp_path):
+    """A dump's resume.box (x0, y0, w, h, fft_w, fft_h, from_step, fits) and box_args (8 x 8 tiles, 256-cell groups)
+    as given, and without them the whole world's record as the reference's twin writes it (headless.rs:635-651)."""
+    planes = np.zeros((14, 16, 24), np.float32)
+    b = dump.to_bytes(planes, step=5, box=[8, 0, 16, 16, 32, 32, 1, 2], box_args=[2, 2, 1, 1, 1, 1])
+    p = tmp_path / "a.bin"
+    p.write_bytes(b)
+    d = dump.read(str(p))
+    assert d["box"] == [8, 0, 16, 16, 32, 32, 1, 2] and d["box_args"] == [2, 2, 1, 1, 1, 1]
+    p.write_bytes(dump.to_bytes(planes))
+    d = dump.read(str(p))
+    assert d["box"] == [0, 0, 24, 16, 64, 32, 0, 0] and d["box_args"] == [3, 2, 1, 2, 1, 1]
+    with pytest.raises(dump.DumpError):
+        dump.to_bytes(planes, box=[0, 0, 24, 16], box_args=[3, 2, 1, 2, 1, 1])

@@ -92,6 +92,32 @@ def test_a_run_writes_the_summary_and_dumps(gpu_lib, tmp_path):
     assert np.array_equal(a["flux"].view(np.uint32), b["flux"].view(np.uint32))
     assert np.array_equal(a["sigma64"].view(np.uint64), b["sigma64"].view(np.uint64))
     assert (a["sim_time"], a["dt"], a["dt_next"], a["step"]) == (b["sim_time"], b["dt"], b["dt_next"], b["step"])
+    assert a["box"] == b["box"] and a["box_args"] == b["box_args"]
+
+
+@pytest.mark.gpu
+def test_a_resume_across_a_box_refit_goes_on_bit_for_bit(gpu_lib, tmp_path):
+    """box_blob.json (the blob moving through its active box): 20 steps with a dump every 10, and 10 steps resumed
+    from the dump at 10 (the box refit after step 16's P9 falls in the resumed part): the two dumps at 20 hold the same
+    cells, box record and clock; the box is the blob's, not the world's."""
+    from sandbox_reactions_amd import dump
+    print("headless box_blob resume across a refit", flush=True)
+    scene = os.path.join(SCENES, "box_blob.json")
+    code, out = headless("--scene", scene, "--steps", "20", "--out", str(tmp_path / "a"), "--dump-every", "10")
+    assert code == 0, out
+    code, out = headless("--scene", scene, "--steps", "10", "--out", str(tmp_path / "b"), "--load-dump",
+                         str(tmp_path / "a" / "dumps" / "s_10.bin"))
+    assert code == 0, out
+    a = dump.read(str(tmp_path / "a" / "dumps" / "s_20.bin"))
+    b = dump.read(str(tmp_path / "b" / "dumps" / "s_20.bin"))
+    assert np.array_equal(a["planes"].view(np.uint32), b["planes"].view(np.uint32))
+    assert np.array_equal(a["sigma64"].view(np.uint64), b["sigma64"].view(np.uint64))
+    assert a["box"] == b["box"] and a["box_args"] == b["box_args"]
+    assert (a["sim_time"], a["dt"], a["dt_next"]) == (b["sim_time"], b["dt"], b["dt_next"])
+    x0, y0, w, h, fw, fh, frm, fits = a["box"]
+    fft = lambda n: max(32, 1 << (2 * n - 1).bit_length())
+    assert (frm, fits) == (17, 3) and 0 < w < 600 and 0 < h < 400 and (fw, fh) == (fft(w), fft(h))
+    assert a["box_args"] == [w // 8, h // 8, 1, -(-(w * h) // 256), 1, 1]
 
 
 @pytest.mark.gpu
