@@ -127,9 +127,13 @@ typedef struct {
  * does), or on the device, each PCG iteration opened by a head kernel that reads the last one's residual and decides
  * (the FP64 true residual and restart, or the next inner iteration, or done), the later kernels of the iteration
  * skipping themselves once it is done; the iterations launched one by one (SR_SOLVE_DEVICE) or replayed as a
- * captured hipGraph of two (SR_SOLVE_GRAPH). The two device modes give the same bits. A world replays its step's
- * fixed parts (P1..P4 with the take-backs, P8/P9) as captured graphs in every mode (with the clocks off). */
-enum { SR_SOLVE_HOST = 0, SR_SOLVE_DEVICE = 1, SR_SOLVE_GRAPH = 2 };
+ * captured hipGraph of two (SR_SOLVE_GRAPH). The two device modes give the same bits. SR_SOLVE_HOST looks ahead: at
+ * a check the last rate says will not end the inner loop, the next iteration's preconditioner application is queued
+ * before the host waits for the partials, so the device does not idle meanwhile (when the check ends the loop after
+ * all, that application goes unused); SR_SOLVE_HOST_WAIT is the same loop waiting at every check on an idle device.
+ * The two host modes give the same bits. A world replays its step's fixed parts (P1..P4 with the take-backs, P8/P9)
+ * as captured graphs in every mode (with the clocks off). */
+enum { SR_SOLVE_HOST = 0, SR_SOLVE_DEVICE = 1, SR_SOLVE_GRAPH = 2, SR_SOLVE_HOST_WAIT = 3 };
 
 typedef struct sr_world sr_world;
 typedef struct sr_gravity sr_gravity;

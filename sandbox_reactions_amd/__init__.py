@@ -17,8 +17,8 @@ from . import _lib
 from ._lib import (BOX_FIT, BOX_WHOLE_WORLD, COMPONENTS, EDGE_BOUNCE, EDGE_LEAVE, FIELD_FLUX_X, FIELD_FLUX_Y, FIELD_GX,
                    FIELD_GY, FIELD_PHI, FIELD_PHI_MID, FIELD_SIGMA_MID, FUSE_LARGE, FUSE_OFF, FUSE_SMALL,
                    OPACITY_KRAMERS, OPACITY_REFERENCE, PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID, SOLVE_DEVICE,
-                   SOLVE_GRAPH, SOLVE_HOST, STOP_REFERENCE, STOP_RELATIVE, TILE_16, TILE_32, TILE_64, DtReport,
-                   HeatReport, HeatSettings, Physics, StepCost, Switches)
+                   SOLVE_GRAPH, SOLVE_HOST, SOLVE_HOST_WAIT, STOP_REFERENCE, STOP_RELATIVE, TILE_16, TILE_32, TILE_64,
+                   DtReport, HeatReport, HeatSettings, Physics, StepCost, Switches)
 
 _LIB = None
 

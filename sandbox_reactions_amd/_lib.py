@@ -20,7 +20,7 @@ OPACITY_REFERENCE, OPACITY_KRAMERS = 0, 1
 PRECOND_JACOBI, PRECOND_MULTIGRID, PRECOND_LEARNED = 0, 1, 2
 STOP_REFERENCE, STOP_RELATIVE = 0, 1
 TILE_16, TILE_32, TILE_64 = 0, 1, 2
-SOLVE_HOST, SOLVE_DEVICE, SOLVE_GRAPH = 0, 1, 2
+SOLVE_HOST, SOLVE_DEVICE, SOLVE_GRAPH, SOLVE_HOST_WAIT = 0, 1, 2, 3
 FUSE_OFF, FUSE_SMALL, FUSE_LARGE = 0, 1, 2
 BOX_WHOLE_WORLD, BOX_FIT = 0, 1
 FIELD_PHI, FIELD_GX, FIELD_GY, FIELD_FLUX_X, FIELD_FLUX_Y, FIELD_SIGMA_MID, FIELD_PHI_MID = range(7)

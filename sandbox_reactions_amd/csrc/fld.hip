@@ -1333,6 +1333,9 @@ FldSolver::~FldSolver() {
         if (x) (void)hipFree(x);
     if (host_part) (void)hipHostFree(host_part);
     if (host_ctl) (void)hipHostFree(host_ctl);
+    if (ev_part) (void)hipEventDestroy(ev_part);
+    for (hipGraphExec_t g : rebuild_exec)
+        if (g) (void)hipGraphExecDestroy(g);
     if (ctl) (void)hipFree(ctl);
     if (ticket) (void)hipFree(ticket);
     if (slot_exec) (void)hipGraphExecDestroy(slot_exec);
@@ -1447,6 +1450,7 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
     SR_HIP(hipMalloc(&part_dots, sizeof(double) * 2 * 2 * nbd_max));
     SR_HIP(hipMemset(part_dots, 0, sizeof(double) * 2 * 2 * nbd_max));
     SR_HIP(hipHostMalloc(&host_part, sizeof(double) * 4 * nbe));
+    SR_HIP(hipEventCreateWithFlags(&ev_part, hipEventDisableTiming));
     SR_HIP(hipMalloc(&ctl, sizeof(SolveCtl) * 2));
     SR_HIP(hipMemset(ctl, 0, sizeof(SolveCtl) * 2));
     SR_HIP(hipMalloc(&ticket, sizeof(unsigned)));
@@ -1542,29 +1546,67 @@ CoarseSet FldSolver::coarse_set() const {
     return C;
 }
 
+// The hierarchy from level 0's arrays: a chain of small launches (the coarsenings level by level, the tile levels'
+// records, the single block's preparation) whose arguments are fixed for the solver's life, so on a stream it is
+// captured once per variant (with or without the faces) and layout of the single-block set, and replayed: the
+// device then runs the chain back to back instead of waiting on each launch from the host.
 int FldSolver::rebuild(bool faces, hipStream_t s) {
     if (faces)
         work.builds_faces++;
     else
         work.builds_own++;
-    for (size_t l = 0; l + 1 < lv.size(); l++) {
-        const FldLevel &f = lv[l], &c = lv[l + 1];
-        Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
-        const int nc = c.w * c.h;
-        hipLaunchKernelGGL(k_coarsen_own, dim3((nc + 255) / 256), dim3(256), 0, s, fl, l == 0 ? gas : nullptr,
-                           c.own, c.w, c.h);
-        if (faces) {
-            const int nf = std::max((c.w + 1) * c.h, c.w * (c.h + 1));
-            hipLaunchKernelGGL(k_coarsen_faces, dim3((nf + 255) / 256), dim3(256), 0, s, fl, c.ax, c.ay, c.w, c.h);
+    auto launch = [&]() -> int {
+        for (size_t l = 0; l + 1 < lv.size(); l++) {
+            const FldLevel &f = lv[l], &c = lv[l + 1];
+            Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
+            const int nc = c.w * c.h;
+            hipLaunchKernelGGL(k_coarsen_own, dim3((nc + 255) / 256), dim3(256), 0, s, fl, l == 0 ? gas : nullptr,
+                               c.own, c.w, c.h);
+            if (faces) {
+                const int nf = std::max((c.w + 1) * c.h, c.w * (c.h + 1));
+                hipLaunchKernelGGL(k_coarsen_faces, dim3((nf + 255) / 256), dim3(256), 0, s, fl, c.ax, c.ay, c.w,
+                                   c.h);
+            }
         }
+        for (int l = 0; l < coarse_from; l++) {
+            const FldLevel &f = lv[l];
+            Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
+            hipLaunchKernelGGL(k_pack_cf, dim3((f.w * f.h + 255) / 256), dim3(256), 0, s, fl, f.cf);
+        }
+        hipLaunchKernelGGL(k_coarse_prep, dim3(1), dim3(COARSE_NT), 0, s, coarse_set(), pack);
+        SR_HIP(hipGetLastError());
+        return SR_OK;
+    };
+    hipGraphExec_t &g = rebuild_exec[faces ? 1 : 0];
+    if (g && rebuild_key[faces ? 1 : 0] != tiles_version) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
     }
-    for (int l = 0; l < coarse_from; l++) {
-        const FldLevel &f = lv[l];
-        Lv fl = {f.w, f.h, f.own, f.ax, f.ay, nullptr};
-        hipLaunchKernelGGL(k_pack_cf, dim3((f.w * f.h + 255) / 256), dim3(256), 0, s, fl, f.cf);
+    bool replayed = false;
+    bool capturing = false;
+    if (s != nullptr && !g) {
+        capturing = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (!capturing) (void)hipGetLastError();  // a stream that cannot be captured: direct launches
     }
-    hipLaunchKernelGGL(k_coarse_prep, dim3(1), dim3(COARSE_NT), 0, s, coarse_set(), pack);
-    SR_HIP(hipGetLastError());
+    if (capturing) {
+        hipGraph_t graph = nullptr;
+        const int rc = launch();
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        SR_TRY(rc);
+        if (e != hipSuccess) return fail(SR_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g = nullptr;
+            return fail(SR_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        rebuild_key[faces ? 1 : 0] = tiles_version;
+    }
+    if (s != nullptr && g) {
+        SR_HIP(hipGraphLaunch(g, s));
+        replayed = true;
+    }
+    if (!replayed) SR_TRY(launch());
     if (!tuned) {
         tuned = true;
         if (autotune) SR_TRY(tune(s));
@@ -1987,8 +2029,20 @@ int FldSolver::matvec64(const double *v, double *out, hipStream_t s) {
 }
 
 int FldSolver::read_partials(const double *part, int k, int nb, double *out, hipStream_t s) {
+    SR_TRY(copy_partials(part, k, nb, s));
+    return wait_partials(k, nb, out);
+}
+
+// the partials' copy to the host queued, with the event that marks it done
+int FldSolver::copy_partials(const double *part, int k, int nb, hipStream_t s) {
     SR_HIP(hipMemcpyAsync(host_part, part, sizeof(double) * k * nb, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipStreamSynchronize(s));
+    SR_HIP(hipEventRecord(ev_part, s));
+    return SR_OK;
+}
+
+// the host's wait for that copy alone (work queued after it keeps the device busy), then the sums in block order
+int FldSolver::wait_partials(int k, int nb, double *out) {
+    SR_HIP(hipEventSynchronize(ev_part));
     for (int j = 0; j < k; j++) {
         double t = 0.0;
         for (int i = 0; i < nb; i++) t += host_part[j * nb + i];
@@ -2003,7 +2057,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     const size_t n = (size_t)w0 * h0;
     if (precond == SR_PRECOND_LEARNED && !net)
         return fail(SR_ERR_ARGUMENT, "the learned preconditioner has no network (sr_fld_set_network)");
-    if (solve_mode != SOLVE_HOST && precond != SR_PRECOND_LEARNED)
+    if ((solve_mode == SOLVE_DEVICE || solve_mode == SOLVE_GRAPH) && precond != SR_PRECOND_LEARNED)
         return solve_slots(b, x, tol, max_iter, precond, ref, st, s, expect);
     bool use_net = precond == SR_PRECOND_LEARNED;
     const bool mg = precond != 0 && lv.size() > 1;
@@ -2062,8 +2116,9 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
         work.precond++;
         int last_check = it, interval = 1;
-        double rr_last = rr;
+        double rr_last = rr, left = HUGE_VAL;  // iterations left, as the last check's rate predicted them
         if (use_net && net_it == 0) rr_net0 = rr;
+        bool queued = false;  // the next iteration's z = M r already queued (the lookahead below)
         while (it < max_iter) {
             int slot = tick(CLK_MATVEC, s);
             hipLaunchKernelGGL(k_matvec_p, mv_grid, dim3(NT), 0, s, a, zc, pold, pnew, q, dots[k & 1],
@@ -2078,8 +2133,17 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
             work.iterations++;
             if (use_net) net_it++;
             if (it - last_check >= interval || it >= max_iter) {
+                // Lookahead: where the last rate predicts more than one iteration beyond this check, the next
+                // iteration's z = M r is queued behind the partials' copy before the host waits for them, so the
+                // device works while the host decides. It writes the next z and its dot partials only (x and r are
+                // as this check reads them): when the check ends the loop after all, that z goes unused.
+                const bool ahead = solve_mode == SOLVE_HOST && !use_net && !timing && it < max_iter &&
+                                   left - (double)(it - last_check) > 1.5;
+                SR_TRY(copy_partials(part_rr, 1, nbe, s));
+                int nbd_next = nbe;
+                if (ahead) SR_TRY(precondition(zo, zc, dots[(k + 1) & 1], &nbd_next));
                 double rri;
-                SR_TRY(read_partials(part_rr, 1, nbe, &rri, s));
+                SR_TRY(wait_partials(1, nbe, &rri));
                 if (!std::isfinite(rri)) {
                     st->finite = false;
                     break;
@@ -2099,18 +2163,27 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                 // iterations left at the rate seen since the last check: check again a little before then
                 const double rate =
                     std::log(std::max(rri, 1e-300) / std::max(rr_last, 1e-300)) / (double)(it - last_check);
-                const double left = rate < 0.0 ? std::log(stop2 / rri) / rate : 1.0;
+                left = rate < 0.0 ? std::log(stop2 / rri) / rate : 1.0;
                 interval = std::max(1, std::min(8, (int)(0.7 * left)));
                 if (use_net && fallback_it > 0 && plain_rate < 0.0 && net_it < NET_RATE_AT)
                     interval = std::min(interval, NET_RATE_AT - net_it);  // the first rate check comes on time
                 last_check = it;
                 rr_last = rri;
+                if (ahead) {
+                    queued = true;
+                    nbp = nbd;
+                    nbd = nbd_next;
+                }
             }
             std::swap(zc, zo);
             std::swap(pold, pnew);
             k++;
-            nbp = nbd;
-            SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
+            if (queued) {
+                queued = false;
+            } else {
+                nbp = nbd;
+                SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
+            }
             work.precond++;
         }
         if (!st->finite) break;
@@ -2384,8 +2457,8 @@ extern "C" int sr_fld_solve_stats(sr_fld *f, const double *b_dev, double *x_dev,
 
 extern "C" int sr_fld_set_solve_mode(sr_fld *f, int mode) {
     if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_solve_mode: null solver");
-    if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH)
-        return fail(SR_ERR_ARGUMENT, "sr_fld_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE or _GRAPH");
+    if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH && mode != SR_SOLVE_HOST_WAIT)
+        return fail(SR_ERR_ARGUMENT, "sr_fld_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE, _GRAPH or _HOST_WAIT");
     f->s.set_solve_mode(mode);
     return SR_OK;
 }

@@ -14,7 +14,8 @@ struct SolveCtl;
 
 // How a solve decides when to stop: on the host from read-back partials (the learned preconditioner always), or on
 // the device by each slot's head, the slots launched one by one or replayed as a captured graph of two
-enum { SOLVE_HOST = SR_SOLVE_HOST, SOLVE_DEVICE = SR_SOLVE_DEVICE, SOLVE_GRAPH = SR_SOLVE_GRAPH };
+enum { SOLVE_HOST = SR_SOLVE_HOST, SOLVE_DEVICE = SR_SOLVE_DEVICE, SOLVE_GRAPH = SR_SOLVE_GRAPH,
+       SOLVE_HOST_WAIT = SR_SOLVE_HOST_WAIT };
 
 // tile shapes of the level smoothers
 enum { TILE_16 = SR_TILE_16, TILE_32 = SR_TILE_32, TILE_64 = SR_TILE_64 };
@@ -174,6 +175,12 @@ class FldSolver {
     CoarseSet coarse_set() const;
     int learned(const float *r, float *z, const float *zold, double *dots, hipStream_t s);
     int read_partials(const double *part, int k, int nb, double *out, hipStream_t s);
+    int copy_partials(const double *part, int k, int nb, hipStream_t s);
+    int wait_partials(int k, int nb, double *out);
+    hipEvent_t ev_part = nullptr;
+    // the hierarchy's rebuild captured per variant (without, with the faces) and the layout it was captured for
+    hipGraphExec_t rebuild_exec[2] = {nullptr, nullptr};
+    int rebuild_key[2] = {-1, -1};
 };
 
 } // namespace sr

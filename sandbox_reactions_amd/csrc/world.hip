@@ -1025,8 +1025,8 @@ extern "C" int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations) 
 
 extern "C" int sr_set_solve_mode(sr_world *wd, int mode) {
     if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
-    if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH)
-        return fail(SR_ERR_ARGUMENT, "sr_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE or _GRAPH");
+    if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH && mode != SR_SOLVE_HOST_WAIT)
+        return fail(SR_ERR_ARGUMENT, "sr_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE, _GRAPH or _HOST_WAIT");
     if (wd->sw.heat) wd->heat.solver.set_solve_mode(mode);
     return SR_OK;
 }

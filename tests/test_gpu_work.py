@@ -107,15 +107,15 @@ def test_the_fused_smoother_pair_in_the_world(gpu_lib):
 
 
 def test_host_and_device_decided_solves(gpu_lib):
-    """The heat pass's solves decided on the host (partials read back), on the device (slots launched one by one) and
-    by the captured graph of slots: the two device modes give the same bits after three steps; all three converge
-    every pass, and the host-decided run (which stops only at its checks, so a few iterations later) agrees with them
-    within the solves' tolerance in E."""
+    """The heat pass's solves decided on the host (partials read back, with and without the lookahead), on the device
+    (slots launched one by one) and by the captured graph of slots: the two device modes give the same bits after
+    three steps, and so do the two host modes; all converge every pass, and the host-decided run (which stops only at
+    its checks, so a few iterations later) agrees with them within the solves' tolerance in E."""
     sr = gpu_lib
     n = 128
     print(f"solve modes {n}^2", flush=True)
     out = {}
-    for mode in (sr.SOLVE_HOST, sr.SOLVE_DEVICE, sr.SOLVE_GRAPH):
+    for mode in (sr.SOLVE_HOST, sr.SOLVE_HOST_WAIT, sr.SOLVE_DEVICE, sr.SOLVE_GRAPH):
         w = make(sr, n)
         w.set_solve_mode(mode)
         reps = []
@@ -125,10 +125,35 @@ def test_host_and_device_decided_solves(gpu_lib):
         assert all(r.converged and r.final_residual <= 1e-8 for r in reps), mode
         out[mode] = (w.readback(), [r.rounds for r in reps], w.read_dt().sim_time)
         w.close()
-    d, g, h = out[sr.SOLVE_DEVICE], out[sr.SOLVE_GRAPH], out[sr.SOLVE_HOST]
+    d, g, h, hw = out[sr.SOLVE_DEVICE], out[sr.SOLVE_GRAPH], out[sr.SOLVE_HOST], out[sr.SOLVE_HOST_WAIT]
     assert np.array_equal(d[0].view(np.uint32), g[0].view(np.uint32)) and d[1] == g[1] and d[2] == g[2]
+    # the host's lookahead queues preconditioner applications that a check may leave unused: the same bits
+    assert np.array_equal(h[0].view(np.uint32), hw[0].view(np.uint32)) and h[1] == hw[1] and h[2] == hw[2]
     print(f"rounds per step: host {h[1]}, device {d[1]}", flush=True)
     e_h, e_d = h[0][3].astype(np.float64), d[0][3].astype(np.float64)
     assert np.abs(e_h - e_d).sum() / np.abs(e_h).sum() <= 1e-6
     with pytest.raises(sr.SrError):
         make(sr, 16).set_solve_mode(7)
+
+
+def test_the_lookahead_changes_no_solve_bit(gpu_lib):
+    """The standalone stiff solve at 512^2 with the host's lookahead and waiting at every check: the same iterations
+    and the same FP64 answer to the last bit."""
+    import torch
+    from sandbox_reactions_amd import scenes
+    sr = gpu_lib
+    n = 512
+    print(f"lookahead {n}^2", flush=True)
+    c, kx, ky, b, _ = scenes.fld_system(n)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    bt = torch.tensor(b, device="cuda")
+    res = []
+    for mode in (sr.SOLVE_HOST, sr.SOLVE_HOST_WAIT):
+        f = sr.Fld(n, n, coarse_scale=1.2)
+        f.set_solve_mode(mode)
+        f.set_system(t(c), t(kx), t(ky))
+        x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        it, rel = f.solve(bt, x, tol=1e-8)
+        assert rel <= 1e-8
+        res.append((it, x.cpu().numpy()))
+    assert res[0][0] == res[1][0] and np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
