@@ -298,6 +298,57 @@ int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
 int sr_set_solve_mode(sr_world *wd, int mode);
 /* the heat pass's fused smoother pair, as sr_fld_set_fusion */
 int sr_set_fusion(sr_world *wd, int pre, int post);
+/* P6 burning (crates/sr-physics/src/reference/react.rs; the rate law crates/sr-physics/src/rates.rs; the records
+ * assets/reactions.json resolved against physics.json, registry.rs:485-535, :680-753). One enabled record: its inputs
+ * (species, share, order), outputs (species, share), group (H He C Ne O Si N, registry.rs:460), q = q_ratio Q_H, the
+ * law omega = A Sigma^a prod X_i^n_i f(T) g(Y_e Sigma) with f's Gamow-shaped terms (T_k, nu, w) above T_thr (no terms:
+ * f = 1) and the gate ((Y_e Sigma / Sigma_g - 1)^p above Sigma_g), and whether its rate sets the collapse latch. The
+ * species are assets/elements.json's order: H He C O Ne Mg Si S Fe n. */
+#define SR_REACT_RECORDS 16
+#define SR_REACT_GROUPS 7
+typedef struct {
+    int n_in, n_out;
+    int in_species[4];
+    double in_share[4], in_order[4];
+    int out_species[4];
+    double out_share[4];
+    int group;
+    double q, a_coef, a_sigma;
+    int n_terms;
+    double t_k[4], nu[4], w[4];
+    double t_thr;
+    int gated;
+    double sigma_g, gate_p;
+    int latch;
+} sr_react_record;
+/* The pass: the enabled records in the registry's order, burn_dx, burn_dt, K_burn, the cooling's A_nu, T_nu, m, the
+ * scene's neutrinos switch (the cooling) and pin_temperature, and the record stepped in closed form (N_Fe: one input
+ * of order 1 into neutrons, no temperature law, p = 1; -1 none; react.rs:197-219). */
+typedef struct {
+    int n_records;
+    sr_react_record rec[SR_REACT_RECORDS];
+    double burn_dx, burn_dt, burn_k, a_nu, t_nu, m_nu;
+    int neutrinos, pin_temperature, nfe;
+} sr_react_params;
+/* What the last pass did (react.rs:120-157's Pass, summed over the world): energy per area booked per group, the
+ * neutrino loss, the accepted and retried sub-steps, the most in one cell, the cells that ran out of attempts, the
+ * largest change of a fraction and of T in one accepted sub-step, and the latch (an N_Fe rate > 0 at a pass's start) */
+typedef struct {
+    double nuclear[SR_REACT_GROUPS];
+    double neutrino_lost;
+    uint64_t substeps, retried, max_substeps, unfinished;
+    double max_dx, max_dt_rel;
+    int latch;
+} sr_react_report;
+/* P6 in the world's step, between P5 and P8 (reference/mod.rs:263-297), with these records (NULL: off, the default);
+ * the scene's `reactions` switch (Step::with_reactions, step/mod.rs:427-440) */
+int sr_set_reactions(sr_world *wd, const sr_react_params *p);
+/* P6 alone over dt on the world's state (the reference's rig, tests/gpu/burn.rs:120-235) */
+int sr_react_apply(sr_world *wd, double dt);
+int sr_read_react(sr_world *wd, sr_react_report *out);
+/* the last pass's per-cell energy booked per group (7 w h, group-major), neutrino loss and S_nu (w h each) */
+int sr_read_react_planes(sr_world *wd, double *nuclear, double *lost, double *s_nu, size_t n);
+
 /* The active box (step/boxfit.rs:1-14, BoxMode and BoxReport :27-55): the bounding box of the cells with Sigma >=
  * Sigma_vac, grown by 8 cells, rounded out to multiples of 8, clamped to the world (empty, all 0, with no such cell),
  * its FFT size per axis the smallest power of two >= 2 x the box's, at least 32. It is fit at the upload (before

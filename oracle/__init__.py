@@ -330,6 +330,43 @@ def floors_apply(ph: Physics, planes: np.ndarray):
     return p, vr.value, fa.value
 
 
+class ReactPass(C.Structure):
+    """sro_react_pass (the same layout as the library's sr_react_report)."""
+    _fields_ = [("nuclear", C.c_double * 7), ("neutrino_lost", C.c_double), ("substeps", C.c_uint64),
+                ("retried", C.c_uint64), ("max_substeps", C.c_uint64), ("unfinished", C.c_uint64),
+                ("max_dx", C.c_double), ("max_dt_rel", C.c_double), ("latch", C.c_int)]
+
+
+lib.sro_react_step.argtypes = [_PP, C.c_void_p, _dp, _sz, _sz, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.POINTER(ReactPass)]
+lib.sro_react_step.restype = C.c_int
+lib.sro_react_omega.argtypes = [C.c_void_p, C.c_void_p, C.c_double, _dp, C.c_double, C.c_double, C.c_int]
+lib.sro_react_omega.restype = C.c_double
+lib.sro_react_table.argtypes = [C.c_void_p, _dp]
+
+
+def react_step(ph: Physics, params, planes: np.ndarray, dt: float):
+    """P6 over dt on (14, h, w) f64 planes in place (reference/react.rs:563-603). `params` is an sr_react_params
+    structure (sandbox_reactions_amd.react.params: the same layout as sro_react_params). Returns (the pass's report,
+    the energy booked per group (7, h, w), the neutrino loss (h, w), S_nu (h, w))."""
+    assert planes.dtype == np.float64 and planes.flags.c_contiguous and planes.shape[0] == COMPONENTS
+    _, h, w = planes.shape
+    nuclear = np.zeros((7, h, w))
+    lost, s_nu = np.zeros((h, w)), np.zeros((h, w))
+    rep = ReactPass()
+    _check(lib.sro_react_step(C.byref(ph), C.byref(params), planes, w, h, dt, nuclear.ctypes.data, lost.ctypes.data,
+                              s_nu.ctypes.data, C.byref(rep)))
+    return rep, nuclear, lost, s_nu
+
+
+def react_omega(record, sigma: float, x, t: float, y_e: float, table: bool = True) -> float:
+    """One record's rate omega (reference rates.rs:121-131), f through its table or by the direct form."""
+    ln_f = np.zeros(256)
+    lib.sro_react_table(C.byref(record), ln_f)
+    return lib.sro_react_omega(C.byref(record), ln_f.ctypes.data, sigma, np.ascontiguousarray(x, np.float64), t,
+                               y_e, int(table))
+
+
 def lcg(seed: int, count: int) -> np.ndarray:
     """The reference's 64-bit LCG (fft.rs:292-300), uniform on [0, 1)."""
     out = np.empty(count)

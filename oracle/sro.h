@@ -146,6 +146,54 @@ void sro_twin_cap_dt_next(sro_twin *t, double cap);
 void sro_twin_last_heat(const sro_twin *t, sro_heat_report *rep);
 void sro_twin_escaped(const sro_twin *t, double *escaped);
 
+/* ---------------- P6 burning (reference crates/sr-physics/src/reference/react.rs, rates.rs) ---------------- */
+#define SRO_REACT_RECORDS 16
+#define SRO_REACT_GROUPS 7 /* registry.rs:460 GROUPS: H He C Ne O Si N */
+#define SRO_REACT_TABLE 256
+#define SRO_REACT_SPAN 2.5
+/* One enabled record with its physics.json keys resolved (registry.rs:485-535, rates.rs:31-60): inputs (species,
+ * share, order), outputs (species, share), its group, q = q_ratio Q_H, the law's A, a, Gamow terms (T_k, nu, w),
+ * T_thr (0: f = 1), the gate (Sigma_g, p) and whether it is N_Fe (its rate sets the latch). */
+typedef struct {
+    int n_in, n_out;
+    int in_species[4];
+    double in_share[4], in_order[4];
+    int out_species[4];
+    double out_share[4];
+    int group;
+    double q, a_coef, a_sigma;
+    int n_terms;
+    double t_k[4], nu[4], w[4];
+    double t_thr;
+    int gated;
+    double sigma_g, gate_p;
+    int latch;
+} sro_react_record;
+/* The pass's constants: the records, burn_dx, burn_dt, K_burn, the cooling's A_nu, T_nu, m (physics.json), the
+ * scene's neutrinos switch and pin_temperature, and the record N_Fe's closed form steps (-1: none). */
+typedef struct {
+    int n_records;
+    sro_react_record rec[SRO_REACT_RECORDS];
+    double burn_dx, burn_dt, burn_k, a_nu, t_nu, m_nu;
+    int neutrinos, pin_temperature, nfe;
+} sro_react_params;
+typedef struct {
+    double nuclear[SRO_REACT_GROUPS]; /* energy per area released, summed over the world */
+    double neutrino_lost;
+    uint64_t substeps, retried, max_substeps, unfinished;
+    double max_dx, max_dt_rel;
+    int latch;
+} sro_react_pass;
+/* ln f at the table's nodes of one record (SRO_REACT_TABLE values; none without terms) */
+void sro_react_table(const sro_react_record *r, double *ln_f);
+/* one record's rate omega at (Sigma, X, T, Y_e), f through the table (table != 0) or by the direct form */
+double sro_react_omega(const sro_react_record *r, const double *ln_f, double sigma, const double *x, double t,
+                       double y_e, int table);
+/* P6 over dt on channel-major planes (react.rs:563-603): per-cell energy booked by group (7 n, may be NULL),
+ * neutrino loss (n), S_nu (n) */
+int sro_react_step(const sro_physics *ph, const sro_react_params *rp, double *planes, size_t w, size_t h, double dt,
+                   double *nuclear, double *lost, double *s_nu, sro_react_pass *pass);
+
 /* ---------------- the reference's RNG (reference fft.rs:292-300) ---------------- */
 void sro_lcg_fill(uint64_t seed, size_t count, double *out); /* uniform [0, 1) */
 

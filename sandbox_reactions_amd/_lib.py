@@ -78,6 +78,33 @@ class ResumeRecords(C.Structure):
     _fields_ = [("dt", C.c_double), ("dt_next", C.c_double), ("sim_time", C.c_double)]
 
 
+REACT_RECORDS, REACT_GROUPS = 16, 7
+
+
+class ReactRecord(C.Structure):
+    """sr_react_record: one enabled record with its physics keys resolved."""
+    _fields_ = [("n_in", C.c_int), ("n_out", C.c_int), ("in_species", C.c_int * 4), ("in_share", C.c_double * 4),
+                ("in_order", C.c_double * 4), ("out_species", C.c_int * 4), ("out_share", C.c_double * 4),
+                ("group", C.c_int), ("q", C.c_double), ("a_coef", C.c_double), ("a_sigma", C.c_double),
+                ("n_terms", C.c_int), ("t_k", C.c_double * 4), ("nu", C.c_double * 4), ("w", C.c_double * 4),
+                ("t_thr", C.c_double), ("gated", C.c_int), ("sigma_g", C.c_double), ("gate_p", C.c_double),
+                ("latch", C.c_int)]
+
+
+class ReactParams(C.Structure):
+    """sr_react_params: the records and the pass's constants."""
+    _fields_ = [("n_records", C.c_int), ("rec", ReactRecord * REACT_RECORDS), ("burn_dx", C.c_double),
+                ("burn_dt", C.c_double), ("burn_k", C.c_double), ("a_nu", C.c_double), ("t_nu", C.c_double),
+                ("m_nu", C.c_double), ("neutrinos", C.c_int), ("pin_temperature", C.c_int), ("nfe", C.c_int)]
+
+
+class ReactReport(C.Structure):
+    """sr_react_report: what the last P6 pass did, summed over the world."""
+    _fields_ = [("nuclear", C.c_double * REACT_GROUPS), ("neutrino_lost", C.c_double), ("substeps", C.c_uint64),
+                ("retried", C.c_uint64), ("max_substeps", C.c_uint64), ("unfinished", C.c_uint64),
+                ("max_dx", C.c_double), ("max_dt_rel", C.c_double), ("latch", C.c_int)]
+
+
 class BoxReport(C.Structure):
     _fields_ = [("x0", C.c_uint32), ("y0", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
                 ("fft", C.c_uint32 * 2), ("from_step", C.c_uint32), ("fits", C.c_uint32),
@@ -131,6 +158,10 @@ SIGNATURES = [
     ("sr_fld_set_coarse_wave", _i, [_vp, _i]),
     ("sr_set_fusion", _i, [_vp, _i, _i]),
     ("sr_set_box_mode", _i, [_vp, _i]),
+    ("sr_set_reactions", _i, [_vp, C.POINTER(ReactParams)]),
+    ("sr_react_apply", _i, [_vp, _d]),
+    ("sr_read_react", _i, [_vp, C.POINTER(ReactReport)]),
+    ("sr_read_react_planes", _i, [_vp, _vp, _vp, _vp, _sz]),
     ("sr_read_box", _i, [_vp, C.POINTER(BoxReport)]),
     ("sr_resume_box", _i, [_vp, C.POINTER(C.c_uint32), C.c_uint64]),
     ("sr_destroy", None, [_vp]),

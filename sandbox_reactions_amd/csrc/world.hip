@@ -17,6 +17,7 @@
 #include "grav.hpp"
 #include "heat.hpp"
 #include "hydro.hpp"
+#include "react.hpp"
 
 namespace sr {
 
@@ -392,6 +393,11 @@ struct sr_world {
     struct {
         unsigned long long steps = 0, hydro = 0, mid = 0, conv_first = 0, conv_mid = 0, radiation = 0;
     } wk;
+    // P6: the records and constants on the device (null: off), the last pass's per-cell bookings and partials
+    ReactDev *react = nullptr, *react_dev_mem = nullptr;
+    double *react_nuc = nullptr, *react_lost = nullptr, *react_snu = nullptr, *react_psum = nullptr,
+           *react_pmax = nullptr;
+    int react_nb = 0;
     // the active box: its record (x0, y0, w, h, fft_w, fft_h, from_step, fits) and the cell pass's partials
     bool box_fit = false;
     int *box = nullptr, *box_part = nullptr;
@@ -830,6 +836,11 @@ static int one_step(sr_world *wd) {
         wd->heat_ran = true;
         wd->first_built = false;
     }
+    if (wd->react) {  // P6 at the step's dt
+        HydroState &st = wd->st[wd->cur];
+        SR_TRY(react_launch(wd->react, st, (int)wd->n, -1.0, wd->scal, wd->react_nuc, wd->react_lost, wd->react_snu,
+                            wd->react_psum, wd->react_pmax, &wd->react_nb, s));
+    }
     mark(wd, 4);
     if (graphs)
         SR_TRY(replay(wd, &wd->g_tail[wd->cur], [&] { return p89(wd, 1); }));
@@ -1028,6 +1039,86 @@ extern "C" int sr_set_solve_mode(sr_world *wd, int mode) {
     if (mode != SR_SOLVE_HOST && mode != SR_SOLVE_DEVICE && mode != SR_SOLVE_GRAPH && mode != SR_SOLVE_HOST_WAIT)
         return fail(SR_ERR_ARGUMENT, "sr_set_solve_mode: mode must be SR_SOLVE_HOST, _DEVICE, _GRAPH or _HOST_WAIT");
     if (wd->sw.heat) wd->heat.solver.set_solve_mode(mode);
+    return SR_OK;
+}
+
+extern "C" int sr_set_reactions(sr_world *wd, const sr_react_params *p) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (!p) {
+        wd->react = nullptr;  // its device memory stays with the world until sr_destroy
+        return SR_OK;
+    }
+    if (react_check(p) != SR_OK)
+        return fail(SR_ERR_ARGUMENT, "sr_set_reactions: a record out of range (species, group, order, terms, gate) "
+                                     "or N_Fe's closed form on a record not of its shape");
+    const size_t n = wd->n;
+    if (!wd->react_nuc) {
+        void *d = nullptr;
+        SR_TRY(world_alloc(wd, &d, sizeof(ReactDev)));
+        wd->react = nullptr;
+        wd->react_dev_mem = (ReactDev *)d;
+        SR_TRY(world_alloc(wd, (void **)&wd->react_nuc, sizeof(double) * SR_REACT_GROUPS * n));
+        SR_TRY(world_alloc(wd, (void **)&wd->react_lost, sizeof(double) * n));
+        SR_TRY(world_alloc(wd, (void **)&wd->react_snu, sizeof(double) * n));
+        SR_TRY(world_alloc(wd, (void **)&wd->react_psum, sizeof(double) * REACT_SUMS * REACT_BLOCKS));
+        SR_TRY(world_alloc(wd, (void **)&wd->react_pmax, sizeof(double) * REACT_MAXES * REACT_BLOCKS));
+    }
+    ReactDev *h = new ReactDev;
+    react_build(p, wd->ph.sigma_vac, wd->ph.t_floor, wd->fp.cold, h);
+    const hipError_t e = hipMemcpyAsync(wd->react_dev_mem, h, sizeof(ReactDev), hipMemcpyHostToDevice, wd->s);
+    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(wd->s) : e;
+    delete h;
+    if (e2 != hipSuccess) return fail(SR_ERR_DEVICE, std::string("sr_set_reactions: ") + hipGetErrorString(e2));
+    wd->react = wd->react_dev_mem;
+    return SR_OK;
+}
+
+extern "C" int sr_react_apply(sr_world *wd, double dt) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (!wd->react) return fail(SR_ERR_ARGUMENT, "sr_react_apply: no reactions (sr_set_reactions)");
+    if (!(dt >= 0.0) || !std::isfinite(dt)) return fail(SR_ERR_ARGUMENT, "sr_react_apply: dt >= 0 is required");
+    SR_TRY(react_launch(wd->react, wd->st[wd->cur], (int)wd->n, dt, wd->scal, wd->react_nuc, wd->react_lost,
+                        wd->react_snu, wd->react_psum, wd->react_pmax, &wd->react_nb, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+extern "C" int sr_read_react(sr_world *wd, sr_react_report *out) {
+    if (!wd || !out) return fail(SR_ERR_ARGUMENT, "sr_read_react: null pointer");
+    *out = sr_react_report{};
+    const int nb = wd->react_nb;
+    if (!wd->react_nuc || nb == 0) return SR_OK;
+    std::vector<double> ps((size_t)REACT_SUMS * nb), pm((size_t)REACT_MAXES * nb);
+    SR_HIP(hipMemcpyAsync(ps.data(), wd->react_psum, sizeof(double) * ps.size(), hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipMemcpyAsync(pm.data(), wd->react_pmax, sizeof(double) * pm.size(), hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    double sum[REACT_SUMS] = {}, mx[REACT_MAXES] = {};
+    for (int k = 0; k < REACT_SUMS; k++)
+        for (int b = 0; b < nb; b++) sum[k] += ps[(size_t)k * nb + b];
+    for (int k = 0; k < REACT_MAXES; k++)
+        for (int b = 0; b < nb; b++) mx[k] = std::max(mx[k], pm[(size_t)k * nb + b]);
+    out->substeps = (uint64_t)sum[0];
+    out->retried = (uint64_t)sum[1];
+    out->unfinished = (uint64_t)sum[2];
+    for (int g = 0; g < SR_REACT_GROUPS; g++) out->nuclear[g] = sum[3 + g];
+    out->neutrino_lost = sum[3 + SR_REACT_GROUPS];
+    out->max_substeps = (uint64_t)mx[0];
+    out->max_dx = mx[1];
+    out->max_dt_rel = mx[2];
+    out->latch = mx[3] > 0.0;
+    return SR_OK;
+}
+
+extern "C" int sr_read_react_planes(sr_world *wd, double *nuclear, double *lost, double *s_nu, size_t n) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (n != wd->n) return fail(SR_ERR_LENGTH, "sr_read_react_planes: w h values per plane");
+    if (!wd->react_nuc) return fail(SR_ERR_ARGUMENT, "sr_read_react_planes: no reactions (sr_set_reactions)");
+    if (nuclear)
+        SR_HIP(hipMemcpyAsync(nuclear, wd->react_nuc, sizeof(double) * SR_REACT_GROUPS * n, hipMemcpyDeviceToHost,
+                              wd->s));
+    if (lost) SR_HIP(hipMemcpyAsync(lost, wd->react_lost, sizeof(double) * n, hipMemcpyDeviceToHost, wd->s));
+    if (s_nu) SR_HIP(hipMemcpyAsync(s_nu, wd->react_snu, sizeof(double) * n, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
     return SR_OK;
 }
 

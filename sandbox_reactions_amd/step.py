@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import BoxReport, DtReport, HeatReport, StepCost, StepWork, Switches
+from ._lib import BoxReport, DtReport, HeatReport, ReactReport, StepCost, StepWork, Switches
 
 
 class World:
@@ -64,6 +64,28 @@ class World:
         self._c(self.lib.sr_resume(self._p, C.byref(rec), None if f is None else f.ctypes.data,
                                    0 if f is None else f.size, None if s is None else s.ctypes.data,
                                    0 if s is None else s.size))
+
+    def set_reactions(self, params):
+        """P6 in the step with these records (sandbox_reactions_amd.react.params(...)); None switches it off."""
+        self._react = params
+        self._c(self.lib.sr_set_reactions(self._p, None if params is None else C.byref(params)))
+
+    def react_apply(self, dt: float):
+        """P6 alone over dt on the current state."""
+        self._c(self.lib.sr_react_apply(self._p, dt))
+
+    def read_react(self) -> ReactReport:
+        r = ReactReport()
+        self._c(self.lib.sr_read_react(self._p, C.byref(r)))
+        return r
+
+    def read_react_planes(self):
+        """The last pass's (energy booked per group (7, h, w), neutrino loss (h, w), S_nu (h, w))."""
+        nuc = np.empty((7, self.h, self.w))
+        lost, snu = np.empty((self.h, self.w)), np.empty((self.h, self.w))
+        self._c(self.lib.sr_read_react_planes(self._p, nuc.ctypes.data, lost.ctypes.data, snu.ctypes.data,
+                                              self.w * self.h))
+        return nuc, lost, snu
 
     def set_box_mode(self, mode: int):
         """BOX_WHOLE_WORLD (the default) or BOX_FIT (the reference's Step::new): P8 and P9 over the active box. Before

@@ -168,7 +168,7 @@ def test_p8_and_p9_run_over_the_box(gpu_lib):
     x0, y0, bw, bh = oracle_box(start[0].astype(np.float64), svac32(sr))
     assert [x0, y0, bw, bh] == [88, 40, 40, 32]
     start[4, y0:y0 + bh, x0:x0 + bw] = 0.0
-    start[12, y0:y0 + bh, x0:x0 + bw] = 1.0  # x_Fe (channels: x_H, x_He, x_C, x_N, x_O, x_Ne, x_Mg, x_Si, x_Fe, x_n)
+    start[12, y0:y0 + bh, x0:x0 + bw] = 1.0  # x_Fe (channels: x_H, x_He, x_C, x_O, x_Ne, x_Mg, x_Si, x_S, x_Fe, x_n)
     crop = sr.World(bw, bh, physics=sr.physics(), gravity=False, hydro=False, heat=False)
     crop.upload(np.ascontiguousarray(start[:, y0:y0 + bh, x0:x0 + bw]))
     crop.run(1)
