@@ -7,8 +7,8 @@ is Mcell-steps/s of that step with the state resident in HBM, timed with the ker
 steps one at a time (median), an instrumented window (per-pass times and per-kernel clocks, from which the dominant
 kernel per step is named), the step's algorithmic HBM bytes from the library's work counters (the full step's
 roofline), the stiff FLD solve alone at 2048^2 (plain MG-PCG on the tuned and on the stated 7-level shape, and the
-learned preconditioner; medians of 5), the 64-scene ensemble, and the oracle (the f64 CPU twin restated in C) on the
-same 1024^2 workload on one host core and on all of them.
+learned preconditioner; medians of 5), the 64-scene ensemble, P6 burning alone over a 1024^2 burning field, and the
+oracle (the f64 CPU twin restated in C) on the same 1024^2 workload on one host core and on all of them.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-fld", action="store_true", help="skip the standalone FLD solve (counter runs)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-scene ensemble")
     ap.add_argument("--ensemble-threads", type=int, default=16)
+    ap.add_argument("--no-burn", action="store_true", help="skip the P6 burning field")
     return ap.parse_args()
 
 
@@ -274,6 +275,59 @@ def ensemble(sr, scenes, threads, steps=2):
     return res
 
 
+# P6's algorithmic bytes per cell: Sigma (f64), E and the 10 fractions (f32) in and out, the bookings per group, the
+# neutrino loss and S_nu (f64) out
+BURN_BYTES_PER_CELL = 8 + 2 * (4 + 40) + 8 * 7 + 8 + 8
+
+
+def burn(sr, scenes, n=N_STEP, passes=5, cpu_n=128):
+    """P6 alone over the burning field (the reference's ref_burn.json made uneven, scenes.burn_scene) at n^2: each pass
+    from the same upload, at the field's step dt (5e-3: one sub-step per cell) and at ten times it (sub-cycled, with
+    retries); medians of `passes` after one untimed. Beside them the oracle's P6 on one host thread over a cpu_n^2
+    sample of the same field."""
+    import torch
+    import oracle
+    from sandbox_reactions_amd import react
+    planes, kw = scenes.burn_scene(n)
+    params = react.params(overrides={"burn_k": 10.0})
+    w = sr.World(n, n, physics=sr.physics(**kw), gravity=False, hydro=False, heat=False)
+    w.set_reactions(params)
+    out = {"workload": f"P6 burning alone, {n}^2: Sigma 30 x lognormal (0.5), T 100 x lognormal (0.5), fresh (H 0.75, "
+                       f"He 0.25) and burnt fractions mixed by a smooth field, the cold pressure on, gamma 2, every "
+                       f"shipped record, burn_k 10",
+           "algorithmic_bytes_per_cell": BURN_BYTES_PER_CELL}
+    sample = np.ascontiguousarray(planes[:, :cpu_n, :cpu_n]).astype(np.float64)
+    ph = oracle.reference_physics(**kw)
+    for dt in (5e-3, 5e-2):
+        times = []
+        for k in range(passes + 1):
+            w.upload(planes)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            w.react_apply(dt)
+            torch.cuda.synchronize()
+            if k:
+                times.append(time.perf_counter() - t0)
+        rep = w.read_react()
+        ms = float(np.median(times)) * 1e3
+        q = sample.copy()
+        t0 = time.perf_counter()
+        orep, *_ = oracle.react_step(ph, params, q, dt)
+        cpu_s = time.perf_counter() - t0
+        gbs = n * n * BURN_BYTES_PER_CELL / (ms * 1e-3) / 1e9
+        out[f"dt_{dt:g}"] = {"ms_per_pass": round(ms, 4), "mcells_per_s": round(n * n / ms / 1e3, 1),
+                             "substeps_per_cell": round(rep.substeps / (n * n), 3), "retried": rep.retried,
+                             "max_substeps": rep.max_substeps, "unfinished": rep.unfinished,
+                             "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                             "cpu_oracle_mcells_per_s": round(cpu_n * cpu_n / cpu_s / 1e6, 4),
+                             "cpu_sample": f"oracle P6 (f64 C) on a {cpu_n}^2 corner of the field, one thread, "
+                                           f"{orep.substeps} sub-steps in {cpu_s:.3f} s"}
+        log(f"P6 {n}^2 dt {dt:g}: {ms:.3f} ms per pass ({n * n / ms / 1e3:.1f} Mcells/s), "
+            f"{rep.substeps / (n * n):.2f} sub-steps per cell; CPU oracle {cpu_n * cpu_n / cpu_s / 1e6:.3f} Mcells/s")
+    w.close()
+    return out
+
+
 def cpu_baseline(scenes, n=N_STEP, steps=1, omp_steps=2, ref_budget_s=4.0):
     """The oracle twin (f64 C) on the headline workload at its full size: its heat pass solved by multigrid PCG to
     1e-11 (the device's method; the reference twin's Jacobi rule would take thousands of rounds per pass here), one
@@ -373,6 +427,7 @@ def main():
     dominant = max(per_step, key=lambda k: per_step[k]["ms_per_step"])
     fld = None if args.no_fld else fld_solve(sr, scenes, args.fld_n)
     ens = None if args.no_ensemble else ensemble(sr, scenes, args.ensemble_threads)
+    burning = None if args.no_burn else burn(sr, scenes)
     copy_gbs = copy_bandwidth()
     bracket = event_bracket_ms()
     log(f"device copies (GB/s): {copy_gbs}; timing-event bracket {bracket} ms")
@@ -427,6 +482,7 @@ def main():
                          f"tuned solver (all levels, V(2,2), coarse scale {COARSE_SCALE}), `stated_shape` on 7 "
                          f"levels, V(2,2), plain prolongation; medians of 5", **fld),
             "ensemble_64": ens,
+            "burning_1024": burning,
         },
         # the kernel with the most device time per step (from the clocks of the instrumented window), with every other
         # clocked kernel beside it; `full_step` is the whole step's algorithmic bytes over the headline window's time.

@@ -123,6 +123,8 @@ def _declare_twin(l):
     l.sro_twin_cap_dt_next.argtypes = [C.c_void_p, C.c_double]
     l.sro_twin_last_heat.argtypes = [C.c_void_p, C.POINTER(HeatReport)]
     l.sro_twin_escaped.argtypes = [C.c_void_p, _dp]
+    l.sro_twin_set_reactions.argtypes = [C.c_void_p, C.c_void_p]
+    l.sro_twin_last_react.argtypes = [C.c_void_p, C.c_void_p]
     l.sro_heat_set_mg.argtypes = [C.c_double]
     l.sro_heat_set_mg.restype = None
 
@@ -375,13 +377,17 @@ def lcg(seed: int, count: int) -> np.ndarray:
 
 
 class Twin:
-    """The twin's step driver (mod.rs:129-297) over f64 planes shaped (14, h, w)."""
+    """The twin's step driver (mod.rs:129-297) over f64 planes shaped (14, h, w); `reactions` (an sr_react_params
+    structure, sandbox_reactions_amd.react.params) runs P6 between P5 and P8."""
 
-    def __init__(self, ph: Physics, planes: np.ndarray, gravity=True, hydro=True, heat=False, library=None):
+    def __init__(self, ph: Physics, planes: np.ndarray, gravity=True, hydro=True, heat=False, library=None,
+                 reactions=None):
         _, self.h, self.w = planes.shape
         self._l = library if library is not None else lib
         p = np.ascontiguousarray(planes, dtype=np.float64)
         self._t = self._l.sro_twin_new(C.byref(ph), self.w, self.h, p, int(gravity), int(hydro), int(heat))
+        if reactions is not None:
+            self._l.sro_twin_set_reactions(self._t, C.byref(reactions))
 
     def step(self, n=1):
         for _ in range(n):
@@ -404,6 +410,11 @@ class Twin:
     def last_heat(self) -> HeatReport:
         r = HeatReport()
         self._l.sro_twin_last_heat(self._t, C.byref(r))
+        return r
+
+    def last_react(self) -> "ReactPass":
+        r = ReactPass()
+        self._l.sro_twin_last_react(self._t, C.byref(r))
         return r
 
     def escaped(self) -> np.ndarray:

@@ -193,6 +193,9 @@ double sro_react_omega(const sro_react_record *r, const double *ln_f, double sig
  * neutrino loss (n), S_nu (n) */
 int sro_react_step(const sro_physics *ph, const sro_react_params *rp, double *planes, size_t w, size_t h, double dt,
                    double *nuclear, double *lost, double *s_nu, sro_react_pass *pass);
+/* P6 in the twin's step, between P5 and P8 (NULL: off) */
+void sro_twin_set_reactions(sro_twin *t, const sro_react_params *rp);
+void sro_twin_last_react(const sro_twin *t, sro_react_pass *pass);
 
 /* ---------------- the reference's RNG (reference fft.rs:292-300) ---------------- */
 void sro_lcg_fill(uint64_t seed, size_t count, double *out); /* uniform [0, 1) */

@@ -3,8 +3,8 @@
  * Sigma_floor read as the f32 the device holds, floors.rs:110; the cold energy of eos.rs in the vacuum reset and the
  * thermal floor) without the hold (floors.rs:200-260: no cell is held below Sigma_hold = 10, nor with the cold pressure
  * off), and crates/sr-physics/src/reference/mod.rs:129-297 (Twin::new, Twin::step: P1 dt, P2 gravity,
- * P4 hydro with the last P5 flux as the radiation force, P5 heat, P8 floors, P9 dt_{n+1} with P2's g of the step's
- * start state).
+ * P4 hydro with the last P5 flux as the radiation force, P5 heat, P6 reactions at the step's dt, P8 floors, P9
+ * dt_{n+1} with P2's g of the step's start state).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -78,6 +78,9 @@ struct sro_twin {
     double dt, dt_next, sim_time;
     uint64_t steps;
     sro_heat_report last_heat;
+    int react_on;
+    sro_react_params rp;
+    sro_react_pass last_react;
 };
 
 static void solve_gravity(sro_twin *t) {
@@ -140,6 +143,10 @@ int sro_twin_step(sro_twin *t) {
         free(fx);
         free(fy);
     }
+    if (t->react_on) { /* mod.rs:284-287 */
+        int e = sro_react_step(&t->ph, &t->rp, t->planes, w, h, dt, NULL, NULL, NULL, &t->last_react);
+        if (e != SRO_OK) return e;
+    }
     sro_floors_apply(&t->ph, t->planes, w, h, NULL, NULL);
     t->dt_next = sro_hydro_dt(&t->ph, t->planes, w, h, t->gravity ? t->gx : NULL, t->gravity ? t->gy : NULL);
     t->steps++;
@@ -161,6 +168,14 @@ void sro_twin_cap_dt_next(sro_twin *t, double cap) {
 }
 
 void sro_twin_last_heat(const sro_twin *t, sro_heat_report *rep) { *rep = t->last_heat; }
+
+/* P6 in the twin's step with these records (Twin::new's Burning, mod.rs:94-123); NULL switches it off */
+void sro_twin_set_reactions(sro_twin *t, const sro_react_params *rp) {
+    t->react_on = rp != NULL;
+    if (rp) t->rp = *rp;
+}
+
+void sro_twin_last_react(const sro_twin *t, sro_react_pass *pass) { *pass = t->last_react; }
 
 void sro_twin_escaped(const sro_twin *t, double *escaped) {
     memcpy(escaped, t->escaped, sizeof(double) * NC * 2 * (t->w + t->h));

@@ -53,8 +53,8 @@ __device__ __forceinline__ float eps_cold(const ColdEos &C, float s, float ye, f
     return u / s;
 }
 
-// the host's table of one pair (f64), ln u at the nodes
-inline void cold_table(double k1, double k2, float *out) {
+// the host's table of one pair (built in f64, stored as T), ln u at the nodes
+template <typename T> inline void cold_table(double k1, double k2, T *out) {
     auto p = [&](double x) { return x <= 0.0 ? 0.0 : 1.0 / (1.0 / (k1 * x * x) + 1.0 / (k2 * std::pow(x, 1.5))); };
     auto seg = [&](double t0, double t1, int n) {
         const double h = (t1 - t0) / n;
@@ -71,7 +71,7 @@ inline void cold_table(double k1, double k2, float *out) {
         const double x = std::exp(std::log(1e-8) + (double)i / (COLD_POINTS - 1) * (std::log(1e8) - std::log(1e-8)));
         const double t = std::sqrt(x);
         j += seg(tp, t, i == 0 ? 512 : 64);
-        out[i] = (float)std::log(x * j);
+        out[i] = (T)std::log(x * j);
         tp = t;
     }
 }

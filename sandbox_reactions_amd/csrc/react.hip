@@ -5,8 +5,8 @@
 // and the record's group; the burn clock slowing a sub-step whose heating passes eps_th c_s / K_burn; the neutrino loss
 // eps_nu h with the cooling on; with T pinned the energy follows the final composition at T0. A cell below Sigma_vac is
 // not touched and a cell no record converts is left bit for bit. The state's fractions and energy are the world's f32
-// planes, Sigma its FP64 plane; every expression and sum order is the oracle's, so the cells differ from it only by the
-// device's f64 elementary functions and the f32 planes.
+// planes, Sigma its FP64 plane; the cold energy is read from FP64 tables; every expression and sum order is the
+// oracle's, so the cells differ from it only by the device's f64 elementary functions and the f32 planes.
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -64,7 +64,7 @@ int react_check(const sr_react_params *p) {
     return SR_OK;
 }
 
-void react_build(const sr_react_params *p, double sigma_vac, double t_floor, const ColdEos &cold, ReactDev *out) {
+void react_build(const sr_react_params *p, const sr_physics &ph, ReactDev *out) {
     std::memset(out, 0, sizeof *out);
     out->P = *p;
     for (int r = 0; r < p->n_records; r++) {
@@ -76,9 +76,15 @@ void react_build(const sr_react_params *p, double sigma_vac, double t_floor, con
                 out->ln_f[r][i] =
                     std::log(f_sum(c, c.t_thr * std::exp(std::log(2.5) * (double)i / (double)(REACT_TABLE - 1))));
     }
-    out->sigma_vac = sigma_vac;
-    out->t_floor = t_floor;
-    out->cold = cold;
+    out->sigma_vac = ph.sigma_vac;
+    out->t_floor = ph.t_floor;
+    const double k[2][2] = {{ph.k1e, ph.k2e}, {ph.k1n, ph.k2n}};
+    for (int q = 0; q < 2; q++) {
+        out->cold_pair[q] = k[q][0] > 0.0 && k[q][1] > 0.0;
+        if (out->cold_pair[q]) cold_table(k[q][0], k[q][1], out->cold_lnu[q]);
+    }
+    out->cold_ln_min = std::log(1e-8);
+    out->cold_step = (std::log(1e8) - std::log(1e-8)) / (COLD_POINTS - 1);
 }
 
 // -------------------------------------------------------------------------------------------------- the device side
@@ -93,9 +99,22 @@ __device__ inline void comp(const double *x, double *mu, double *y_e) {
     *y_e = ye;
 }
 
+// u(x) of one pair from its FP64 table: linear in (ln x, ln u), the K1 x^2 power law below the first node
+__device__ inline double cold_u_d(const ReactDev &D, const double *lnu, double x) {
+    if (x <= 0.0) return 0.0;
+    const double pos = (log(x) - D.cold_ln_min) / D.cold_step;
+    if (pos < 0.0) return exp(lnu[0]) * (x / 1e-8) * (x / 1e-8);
+    int i = (int)floor(pos);
+    if (i > COLD_POINTS - 2) i = COLD_POINTS - 2;
+    const double frac = pos - (double)i;
+    return exp(lnu[i] + frac * (lnu[i + 1] - lnu[i]));
+}
+
 __device__ inline double eps_cold_d(const ReactDev &D, double sigma, double y_e, double x_n) {
-    if (!D.cold.on) return 0.0;
-    return (double)eps_cold(D.cold, (float)sigma, (float)y_e, (float)x_n);
+    double u = 0.0;
+    if (D.cold_pair[0]) u += cold_u_d(D, D.cold_lnu[0], y_e * sigma);
+    if (D.cold_pair[1]) u += cold_u_d(D, D.cold_lnu[1], x_n * sigma);
+    return u / sigma;
 }
 
 // omega through the table (rates.rs:98-131)

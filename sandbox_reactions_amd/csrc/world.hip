@@ -382,7 +382,7 @@ struct sr_world {
     uint64_t steps = 0;
     sr_heat_report last_heat{};
     bool timing = false;
-    hipEvent_t ev[12]{};
+    hipEvent_t ev[13]{};
     sr_step_cost cost{};
     KernelClock k_update, k_mid, k_grav, k_pre0;
     // the step's fixed parts captured as graphs (with the clocks off): P1..P4 per current buffer and radiation flag,
@@ -836,6 +836,7 @@ static int one_step(sr_world *wd) {
         wd->heat_ran = true;
         wd->first_built = false;
     }
+    mark(wd, 12);
     if (wd->react) {  // P6 at the step's dt
         HydroState &st = wd->st[wd->cur];
         SR_TRY(react_launch(wd->react, st, (int)wd->n, -1.0, wd->scal, wd->react_nuc, wd->react_lost, wd->react_snu,
@@ -853,7 +854,8 @@ static int one_step(sr_world *wd) {
         wd->cost.pass_ms[1] += between(wd, 0, 1);
         wd->cost.pass_ms[2] += between(wd, 1, 2);
         wd->cost.pass_ms[4] += between(wd, 2, 3);
-        wd->cost.pass_ms[5] += between(wd, 3, 4);
+        wd->cost.pass_ms[5] += between(wd, 3, 12);
+        wd->cost.pass_ms[6] += between(wd, 12, 4);
         wd->cost.pass_ms[8] += between(wd, 4, 5);
         wd->cost.step_ms += between(wd, 0, 5);
         wd->cost.steps++;
@@ -1064,7 +1066,7 @@ extern "C" int sr_set_reactions(sr_world *wd, const sr_react_params *p) {
         SR_TRY(world_alloc(wd, (void **)&wd->react_pmax, sizeof(double) * REACT_MAXES * REACT_BLOCKS));
     }
     ReactDev *h = new ReactDev;
-    react_build(p, wd->ph.sigma_vac, wd->ph.t_floor, wd->fp.cold, h);
+    react_build(p, wd->ph, h);
     const hipError_t e = hipMemcpyAsync(wd->react_dev_mem, h, sizeof(ReactDev), hipMemcpyHostToDevice, wd->s);
     const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(wd->s) : e;
     delete h;

@@ -42,7 +42,9 @@ import time
 
 import numpy as np
 
-PASS_LABELS = {1: "P1 dt", 2: "P2 gravity", 4: "P4 hydro", 5: "P5 heat", 8: "P8 floors"}
+PASS_LABELS = {1: "P1 dt", 2: "P2 gravity", 4: "P4 hydro", 5: "P5 heat", 6: "P6 reactions", 8: "P8 floors"}
+# the switch that records each optional pass (the reference names only the passes a step records, headless.rs:521-528)
+PASS_SWITCH = {2: "gravity", 4: "hydro", 5: "heat", 6: "reactions"}
 MAX_STEPS = 2_000_000  # --max-steps's default (headless.rs:77)
 # the state's channel names, as the guard's line names them (crates/sr-engine/src/state.rs:29-31)
 STATE_CHANNELS = ["sigma", "mom_x", "mom_y", "energy", "x_H", "x_He", "x_C", "x_O", "x_Ne", "x_Mg", "x_Si", "x_S",
@@ -232,7 +234,8 @@ def run(argv=None) -> int:
             summary["until"] = {"condition": f"sim_time:{a.until.partition(':')[2]}", "met": met}
         if a.timing:
             c = world.read_cost()
-            summary["timing"] = {"per_pass_ms": {PASS_LABELS[k]: c.pass_ms[k] for k in sorted(PASS_LABELS)},
+            ran = [k for k in sorted(PASS_LABELS) if k not in PASS_SWITCH or scene.switches[PASS_SWITCH[k]]]
+            summary["timing"] = {"per_pass_ms": {PASS_LABELS[k]: c.pass_ms[k] for k in ran},
                                  "step_ms_mean": c.step_ms, "source": "gpu-timestamp"}
         with open(os.path.join(a.out, "summary.json"), "w") as f:
             f.write(json.dumps(summary, indent=2) + "\n")

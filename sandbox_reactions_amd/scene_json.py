@@ -5,9 +5,9 @@ Covered: `rect` and `disk` objects (Maclaurin profile sampled at cell centres, s
 `pressure`, a `composition` of mass fractions summing to 1 (within 1e-6, scene.rs:46, :215-245) and a velocity `v`;
 overlapping objects add their mass, momentum, energy and species mass (scene.rs:485-497); cells no object covers
 hold the floor state, Sigma_floor of hydrogen at rest at T_floor (scene.rs:466-470). Refused with a message: anything
-the path does not carry -- `preset` objects (they need the calibration file), the reactions / neutrinos / sinks
-switches, and, with the cold pressure on, a start with a cell at or above sigma_hold (the hold of floors.rs:200-260,
-which the path does not carry, can only take such a cell).
+the path does not carry -- `preset` objects (they need the calibration file), the neutrinos and sinks switches, and, with the cold pressure on, a start with a cell the hold would take (Sigma >= sigma_hold
+and cold: the hold of floors.rs:200-260, which the path does not carry). The reactions switch gives the world P6 with
+every shipped record (`Scene.react_params`).
 
 The energy follows the EOS with this library's adiabatic index gamma (the reference's gamma = 2 by default) and the
 cold pressure of the shipped constants (k1e, k2e, k1n, k2n; a scene's overrides may turn a pair off):
@@ -179,10 +179,22 @@ class Scene:
                 raise SceneError(f"{at}.kind: must be \"preset\", \"disk\" or \"rect\"")
 
     def check_path(self):
-        """Refuses a scene that switches on a pass this path does not carry (reactions, neutrinos, sinks)."""
-        for k in ("reactions", "neutrinos", "sinks"):
+        """Refuses a scene that switches on a pass this path does not carry: neutrinos (P7's heating, and P6's cooling
+        that feeds it) and sinks (P3)."""
+        for k in ("neutrinos", "sinks"):
             if self.switches[k]:
-                raise SceneError(f"switches.{k}: this path carries gravity, hydro and heat only (set {k} false)")
+                raise SceneError(f"switches.{k}: this path carries gravity, hydro, heat and reactions only "
+                                 f"(set {k} false)")
+
+    def react_params(self):
+        """P6's parameters for this scene (Step::with_reactions's ReactSetup, cpu_gpu.rs:72-85): every shipped record,
+        resolved against the shipped constants with the scene's overrides, its pin_temperature test flag and its
+        neutrinos switch; None with the reactions switch off."""
+        if not self.switches["reactions"]:
+            return None
+        from . import react
+        return react.params(overrides=self.overrides, pin_temperature=self.test["pin_temperature"],
+                            neutrinos=self.switches["neutrinos"])
 
     def constants(self) -> dict:
         """The physics constants this path reads, shipped values with the scene's overrides."""
@@ -261,9 +273,8 @@ class Scene:
         floor_sigma = c["sigma_floor"]
         floor_e = floor_sigma * (c["t_floor"] / ((gamma - 1.0) / inv_mu([1.0] + [0.0] * 9)) +
                                  float(cold_eos.eps_cold(c, floor_sigma, 1.0, 0.0)))
-        if self.cold_on() and np.max(sig, initial=0.0) >= self.sigma_hold:
-            raise SceneError(f"a cell starts at Sigma {np.max(sig):g} >= sigma_hold {self.sigma_hold:g} with the "
-                             "cold pressure on: the hold (floors.rs:200-260) is not carried by this path")
+        if self.cold_on():
+            self._refuse_held(c, sig, mx, my, en, spm)
         out = np.zeros((14, n), np.float64)
         empty = sig <= 0.0
         out[0] = np.where(empty, floor_sigma, sig)
@@ -274,6 +285,22 @@ class Scene:
         for k in range(len(ELEMENTS)):
             out[4 + k] = np.where(empty, 1.0 if k == 0 else 0.0, spm[k] / safe)
         return out.reshape(14, h, w).astype(np.float32)
+
+    def _refuse_held(self, c, sig, mx, my, en, spm):
+        """Refuses a start with a cell the hold would take (floors.rs:200-219, Floors::holds for a cell not yet held):
+        Sigma >= sigma_hold, eps_th <= eta_hold eps_cold, and X_Fe < 1e-3 or X_n >= 0.5; the disk-mass gate is read
+        as passing, so a cell it would free is refused too. The hold is not carried by this path."""
+        eta = float(self.overrides.get("eta_hold", 0.1))
+        for i in np.flatnonzero(sig >= self.sigma_hold):
+            x = np.maximum(spm[:, i] / sig[i], 0.0)
+            x = x / x.sum()
+            cold = float(cold_eos.eps_cold(c, sig[i], float(cold_eos.y_e(x)), x[-1]))
+            eps_th = en[i] / sig[i] - 0.5 * (mx[i] ** 2 + my[i] ** 2) / sig[i] ** 2 - cold
+            if eps_th <= eta * cold and (x[8] < 1e-3 or x[9] >= 0.5):
+                h, w = divmod(int(i), self.width)
+                raise SceneError(f"the cell ({w}, {h}) starts held (Sigma {sig[i]:g} >= sigma_hold "
+                                 f"{self.sigma_hold:g}, eps_th {eps_th:g} <= eta_hold eps_cold {eta * cold:g}): the "
+                                 "hold (floors.rs:200-260) is not carried by this path")
 
     def physics_kwargs(self, gamma: float = 2.0, dx: float = 1.0) -> dict:
         """Keyword arguments of sandbox_reactions_amd.physics() for this scene (the reference opacity)."""
@@ -294,7 +321,8 @@ def parse(text: str) -> Scene:
 
 
 def world(scene: Scene, heat_settings=None, planes=None, box_mode=None, **physics_over):
-    """A World for the scene (its world size, switches and constants; gamma 2 and unit cells unless overridden),
+    """A World for the scene (its world size, switches, P6's records when reactions are on, and constants; gamma 2
+    and unit cells unless overridden),
     with its planes uploaded (`planes`, (14, h, w), in place of the scene's own: a dump's cells) and, given
     `box_mode`, its active box in that mode (BOX_FIT, BOX_WHOLE_WORLD). Refuses a scene whose passes this path does
     not carry."""
@@ -306,5 +334,8 @@ def world(scene: Scene, heat_settings=None, planes=None, box_mode=None, **physic
                  hydro=scene.switches["hydro"], heat=scene.switches["heat"], heat_settings=heat_settings)
     if box_mode is not None:
         w.set_box_mode(box_mode)
+    rp = scene.react_params()
+    if rp is not None:
+        w.set_reactions(rp)
     w.upload(scene.planes(kw["gamma"]) if planes is None else planes)
     return w

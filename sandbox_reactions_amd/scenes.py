@@ -110,6 +110,32 @@ def turbulent_scene(n: int = 1024, gamma: float = 5.0 / 3.0, seed: int = SEED_BA
     return planes_from(rho, T, u, v, gamma=gamma)
 
 
+# mass fractions in the planes' order H He C O Ne Mg Si S Fe n
+BURN_FRESH = np.array([0.75, 0.25, 0, 0, 0, 0, 0, 0, 0, 0], float)
+BURN_ASHES = np.array([0.05, 0.35, 0.2, 0.2, 0.08, 0.04, 0.04, 0.02, 0.02, 0.0], float)
+
+
+def burn_scene(n: int = 1024, seed: int = SEED_BASE + 0x2000):
+    """The burning workload: the reference's burning field (scenes/ref_burn.json: Sigma 30, T 100, H 0.75 | He 0.25,
+    the cold pressure on, gamma 2, at rest) made uneven so that every regime of P6 occurs: Sigma = 30 x lognormal
+    (sigma_ln 0.5), T = 100 x lognormal (0.5), the fractions a mix w X_ashes + (1 - w) X_fresh with w from a third
+    smooth field (ashes: He, C, O, Ne, Mg, Si, S, Fe). Returns (planes (14, n, n) f32, the physics keywords)."""
+    from . import cold
+    sigma = 30.0 * lognormal(n, 0.5, seed)
+    t = 100.0 * lognormal(n, 0.5, seed + 1)
+    wgt = 0.5 * (1.0 + np.tanh(gaussian_random_field(n, seed + 2)))
+    x = (1.0 - wgt)[None] * BURN_FRESH[:, None, None] + wgt[None] * BURN_ASHES[:, None, None]
+    inv_mu = np.einsum("s,shw->hw", np.array([2.0, 0.75, 7 / 12, 9 / 16, 11 / 20, 13 / 24, 15 / 28, 17 / 32,
+                                              27 / 56, 1.0]), x)
+    ye = np.einsum("s,shw->hw", cold.Z_OVER_A, x)
+    gamma = 2.0
+    p = np.zeros((14, n, n))
+    p[0] = sigma
+    p[3] = sigma * (t * inv_mu / (gamma - 1.0) + cold.eps_cold(cold.SHIPPED, sigma, ye, x[9]))
+    p[4:] = x
+    return p.astype(np.float32), dict(gamma=gamma, **cold.SHIPPED)
+
+
 def sod_scene(w: int = 400, h: int = 8) -> np.ndarray:
     """The reference's scenes/sod.json on a w x h strip: left half Sigma 1, Pi 1; right Sigma 0.125, Pi 0.1; H;
     gamma = 2 (Pi = Sigma eps, so E = Pi)."""

@@ -287,3 +287,33 @@ def test_p6_runs_in_the_step_at_the_steps_dt(gpu_lib):
         bad = react.params()
         bad.rec[0].group = 9
         a.world.set_reactions(bad)
+
+
+@pytest.mark.parametrize("dt", [5e-3, 5e-2])
+def test_the_burning_field_is_the_oracles(gpu_lib, dt):
+    """The benchmark's burning field (the reference's ref_burn.json made uneven: Sigma 30 and T 100 lognormal, fresh
+    and burnt fractions mixed, the cold pressure on, burn_k 10) at 128^2, one pass at the step's dt (one sub-step per
+    cell) and at ten times it (sub-cycled, with retries): the fractions within 1e-5 of the oracle's from the same f32
+    planes, E within 1e-5 relative, the bookings within 1e-5 of the pass's total, the same sub-steps within 0.1 %."""
+    from sandbox_reactions_amd import scenes
+    sr = gpu_lib
+    n = 128
+    print(f"P6 burning field {n}^2 dt {dt}", flush=True)
+    planes, kw = scenes.burn_scene(n)
+    params = react.params(overrides={"burn_k": 10.0})
+    w = sr.World(n, n, physics=sr.physics(**kw), gravity=False, hydro=False, heat=False)
+    w.set_reactions(params)
+    w.upload(planes)
+    w.react_apply(dt)
+    rep = w.read_react()
+    p = w.readback().astype(np.float64)
+    nuc, lost, _ = w.read_react_planes()
+    o = planes.astype(np.float64)
+    orep, onuc, olost, _ = oracle.react_step(oracle.reference_physics(**kw), params, o, dt)
+    print(f"P6 field: {rep.substeps} sub-steps (oracle {orep.substeps}), retried {rep.retried} ({orep.retried}), "
+          f"most {rep.max_substeps}", flush=True)
+    assert rep.unfinished == orep.unfinished == 0 and orep.substeps >= n * n * 0.9
+    assert np.abs(p[4:] - o[4:]).max() <= 1e-5
+    assert (np.abs(p[3] - o[3]) / np.abs(o[3])).max() <= 1e-5
+    assert np.abs(nuc - onuc).sum() <= 1e-5 * np.abs(onuc).sum()
+    assert abs(rep.substeps - orep.substeps) <= 1e-3 * orep.substeps

@@ -13,9 +13,11 @@ properties that do not depend on size:
 - the 64-scene ensemble (sides 128..1024) for one step: every world converged, finite and holding its mass, and three
   of them bit for bit as stepped alone;
 
-and the reference's G-REF comparison (crates/sr-engine/tests/gpu/cpu_gpu.rs:37-40, :94-170): ref_sod.json and
-ref_disk.json run 200 steps on the GPU, the twin follows to the GPU's sim time; the relative L1 of Sigma and E within
-1e-3, |dX| within 1e-4 where both hold matter, and the clocks after 200 steps within 5e-4."""
+and the reference's G-REF comparison (crates/sr-engine/tests/gpu/cpu_gpu.rs:37-40, :94-196): ref_sod.json,
+ref_disk.json and ref_burn.json (heat and P6 burning, the cold pressure on) run 200 steps on the GPU, the twin follows
+to the GPU's sim time; the relative L1 of Sigma and E within 1e-3, |dX| within 1e-4 where both hold matter, the clocks
+after 200 steps within 5e-4, and with reactions on the twin's hydrogen mass down by at least 1 % (a field too cool to
+burn compares nothing)."""
 import os
 
 import numpy as np
@@ -215,7 +217,7 @@ def test_the_64_scene_ensemble_one_step(gpu_lib):
         w.close()
 
 
-@pytest.mark.parametrize("name", ["ref_sod.json", "ref_disk.json"])
+@pytest.mark.parametrize("name", ["ref_sod.json", "ref_disk.json", "ref_burn.json"])
 def test_gref_200_steps_against_the_twin(gpu_lib, name):
     sr = gpu_lib
     steps = 200
@@ -234,7 +236,7 @@ def test_gref_200_steps_against_the_twin(gpu_lib, name):
 
     def make_twin():
         return oracle.Twin(op, start.astype(np.float64), gravity=s.switches["gravity"], hydro=s.switches["hydro"],
-                           heat=s.switches["heat"])
+                           heat=s.switches["heat"], reactions=s.react_params())
 
     twin, clock = make_twin(), None
     while True:  # the twin on its own dt to the GPU's sim time, the last step capped to land there
@@ -253,13 +255,18 @@ def test_gref_200_steps_against_the_twin(gpu_lib, name):
         full.step(steps)
         clock = full.times()[2]
     tw = twin.state()
+    h_mass = lambda p: float(np.sum(p[0].astype(np.float64) * p[4]))
+    fell = 1.0 - h_mass(tw) / h_mass(start.astype(np.float64))
     sig, e = rel_l1(g[0], tw[0]), rel_l1(g[3], tw[3])
     matter = kw.get("sigma_floor", 1e-8) / 1e-4
     live = (g[0] >= matter) & (tw[0] >= matter)
     dx = max(float(np.max(np.abs(g[c][live] - tw[c][live]))) for c in range(4, 14))
     off = abs(t_gpu / clock - 1.0)
     print(f"{name}: sim time GPU {t_gpu:.5f} / twin {clock:.5f} ({100 * (t_gpu / clock - 1):+.4f} %); twin in "
-          f"{landed} steps; rel-L1 Sigma {sig:.3e}, E {e:.3e}; max |dX| {dx:.3e}", flush=True)
+          f"{landed} steps; rel-L1 Sigma {sig:.3e}, E {e:.3e}; max |dX| {dx:.3e}; H mass in the twin "
+          f"{-100 * fell:+.3f} %", flush=True)
     assert off <= 5e-4
     assert sig <= 1e-3 and e <= 1e-3
     assert dx <= 1e-4
+    if s.switches["reactions"]:
+        assert fell >= 0.01

@@ -25,7 +25,7 @@ def test_a_missing_scene_is_exit_4():
 
 def test_a_scene_the_path_does_not_carry_is_exit_4(tmp_path):
     code, out = headless("--scene", os.path.join(SCENES, "asym_blob.json"), "--steps", "1")
-    assert code == 4 and "reactions" in out  # its default switches run P6
+    assert code == 4 and "neutrinos" in out  # its default switches run P7 and P3
     bad = {"version": 1, "world": {"width": 32, "height": 32},
            "switches": {"reactions": False, "neutrinos": False, "sinks": False},
            "objects": [{"kind": "rect", "x0": 0, "y0": 0, "w": 8, "h": 8, "sigma": 1.0, "pressure": 1e-3,
@@ -93,6 +93,29 @@ def test_a_run_writes_the_summary_and_dumps(gpu_lib, tmp_path):
     assert np.array_equal(a["sigma64"].view(np.uint64), b["sigma64"].view(np.uint64))
     assert (a["sim_time"], a["dt"], a["dt_next"], a["step"]) == (b["sim_time"], b["dt"], b["dt_next"], b["step"])
     assert a["box"] == b["box"] and a["box_args"] == b["box_args"]
+
+
+@pytest.mark.gpu
+def test_a_burning_scene_runs_p6(gpu_lib, tmp_path):
+    """ref_burn.json (heat and reactions on, gravity and hydro off) for 4 steps with timing: summary.json names the
+    passes the steps ran, P6 among them with its own time, and none that they did not; resuming from the dump at 2
+    lands on the dump at 4 bit for bit (P6 keeps no state between steps)."""
+    from sandbox_reactions_amd import dump
+    print("headless ref_burn 4 steps", flush=True)
+    scene = os.path.join(SCENES, "ref_burn.json")
+    code, out = headless("--scene", scene, "--steps", "4", "--out", str(tmp_path), "--timing", "--dump-every", "2")
+    assert code == 0, out
+    s = json.load(open(tmp_path / "summary.json"))
+    passes = s["timing"]["per_pass_ms"]
+    assert set(passes) == {"P1 dt", "P5 heat", "P6 reactions", "P8 floors"} and passes["P6 reactions"] > 0
+    code, out = headless("--scene", scene, "--steps", "2", "--out", str(tmp_path / "resumed"), "--load-dump",
+                         str(tmp_path / "dumps" / "s_2.bin"))
+    assert code == 0, out
+    a = dump.read(str(tmp_path / "resumed" / "dumps" / "s_4.bin"))
+    b = dump.read(str(tmp_path / "dumps" / "s_4.bin"))
+    assert np.array_equal(a["planes"].view(np.uint32), b["planes"].view(np.uint32))
+    start = dump.read(str(tmp_path / "dumps" / "s_2.bin"))["planes"]
+    assert not np.array_equal(b["planes"][4:], start[4:])
 
 
 @pytest.mark.gpu

@@ -345,3 +345,34 @@ def test_the_rate_table_holds_its_budget(rid):
     for t in (3.0 * t_thr, 10.0 * t_thr):
         assert oracle.react_omega(rec, 1.0, x, t, 0.5) == oracle.react_omega(rec, 1.0, x, t, 0.5, table=False)
     assert oracle.react_omega(rec, 1.0, x, 0.999 * t_thr, 0.5) == 0.0
+
+
+def test_the_twin_runs_p6_between_p5_and_p8():
+    """The twin's step with P6 (mod.rs:263-297) on a burning field with gravity, hydro and heat off: P1's dt, P6 over
+    it, P8's floors, the next dt -- the same f64 values as the sequence done by hand; with the records off the step is
+    P8 alone."""
+    rng = np.random.default_rng(5)
+    w = h = 8
+    planes = np.zeros((14, h, w))
+    planes[0] = rng.uniform(5.0, 60.0, (h, w))
+    x = rng.dirichlet(np.ones(NS), size=(h, w)).transpose(2, 0, 1)
+    x[9] = 0.0
+    x /= x.sum(axis=0)
+    planes[4:] = x
+    inv = np.einsum("s,shw->hw", (Z + 1.0) / A, x)
+    planes[3] = planes[0] * rng.uniform(100.0, 400.0, (h, w)) * inv
+    params = react.params()
+    tw = oracle.Twin(PH, planes, gravity=False, hydro=False, heat=False, reactions=params)
+    dt = tw.times()[1]
+    tw.step()
+    by_hand = planes.copy()
+    assert dt == oracle.hydro_dt(PH, by_hand)
+    rep, *_ = oracle.react_step(PH, params, by_hand, dt)
+    by_hand, _, _ = oracle.floors_apply(PH, by_hand)
+    assert np.array_equal(tw.state(), by_hand)
+    assert tw.times()[1] == oracle.hydro_dt(PH, by_hand)
+    got = tw.last_react()
+    assert got.substeps == rep.substeps > 0 and list(got.nuclear) == list(rep.nuclear)
+    off = oracle.Twin(PH, planes, gravity=False, hydro=False, heat=False)
+    off.step()
+    assert np.array_equal(off.state(), oracle.floors_apply(PH, planes)[0])

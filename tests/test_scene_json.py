@@ -104,7 +104,8 @@ def base():  # an 8 x 8 rect of hydrogen in a 64 x 64 world, cold pressure off
     (lambda d: (d.pop("overrides"), d.update(switches=OFF), d["objects"][0].pop("t"),
                 d["objects"][0].update(pressure=1e-3)), "below the cold pressure"),
     (lambda d: d["objects"].append({"kind": "preset", "preset": "sun", "x": 1, "y": 1}), "calibration"),
-    (lambda d: d.update(switches={"reactions": True, "neutrinos": False, "sinks": False}), "reactions"),
+    (lambda d: d.update(switches={"reactions": True, "neutrinos": True, "sinks": False}), "neutrinos"),
+    (lambda d: d.update(switches={"reactions": True, "neutrinos": False, "sinks": True}), "sinks"),
     (lambda d: d["world"].update(width=60), "world size"),
     (lambda d: d["overrides"].update(warp=1.0), "unknown key"),
     (lambda d: d.update(version=2), "version"),
@@ -190,3 +191,23 @@ def test_a_resized_world_keeps_the_objects_about_its_centre():
     assert np.array_equal(r.planes(), load("sod.json").planes())
     with pytest.raises(scene_json.SceneError, match="world size"):
         r.resize(100, 64)
+
+
+def test_the_burning_scene_carries_p6():
+    """ref_burn.json (heat and reactions on, burn_k 10): every shipped record with burn_k from the scene's overrides,
+    the cooling off with the neutrinos switch; a hot start at Sigma 30 >= sigma_hold is not held (eps_th far above
+    eta_hold eps_cold), while a cold one is refused; reactions off gives no parameters."""
+    from sandbox_reactions_amd import react
+    s = load("ref_burn.json")
+    s.check_path()
+    p = s.react_params()
+    assert p.n_records == len(react.RECORDS) and p.burn_k == 10.0 and p.neutrinos == 0 and p.pin_temperature == 0
+    assert p.burn_dx == react.CONSTANTS["burn_dx"]
+    s.planes()
+    d = json.load(open(os.path.join(HERE, "ref_burn.json")))
+    d["objects"][0]["t"] = 1.0
+    with pytest.raises(scene_json.SceneError, match="starts held"):
+        scene_json.parse(json.dumps(d)).planes()
+    d["switches"]["reactions"] = False
+    d["objects"][0]["t"] = 100.0
+    assert scene_json.parse(json.dumps(d)).react_params() is None
