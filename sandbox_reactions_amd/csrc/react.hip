@@ -24,6 +24,10 @@ namespace sr {
 
 __constant__ double r_z[NS] = {1, 2, 6, 8, 10, 12, 14, 16, 26, 0};
 __constant__ double r_a[NS] = {1, 4, 12, 16, 20, 24, 28, 32, 56, 1};
+// (Z + 1) / A and Z / A per species
+__constant__ double r_zp1_a[NS] = {2.0, 3.0 / 4.0, 7.0 / 12.0, 9.0 / 16.0, 11.0 / 20.0, 13.0 / 24.0, 15.0 / 28.0,
+                                   17.0 / 32.0, 27.0 / 56.0, 1.0};
+__constant__ double r_z_a[NS] = {1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 26.0 / 56.0, 0.0};
 
 // ln of one Gamow-shaped term and f's direct form (rates.rs:44-53), on either side
 __host__ __device__ inline double term_ln(double t_k, double nu, double w, double t) {
@@ -89,11 +93,13 @@ void react_build(const sr_react_params *p, const sr_physics &ph, ReactDev *out) 
 
 // -------------------------------------------------------------------------------------------------- the device side
 
+// 1/mu and Y_e (registry.rs:146-154): sums of X (Z + 1) / A and X Z / A in species order
 __device__ inline void comp(const double *x, double *mu, double *y_e) {
     double im = 0.0, ye = 0.0;
+#pragma unroll
     for (int s = 0; s < NS; s++) {
-        im += x[s] * (r_z[s] + 1.0) / r_a[s];
-        ye += x[s] * r_z[s] / r_a[s];
+        im += x[s] * r_zp1_a[s];
+        ye += x[s] * r_z_a[s];
     }
     *mu = 1.0 / im;
     *y_e = ye;
@@ -117,40 +123,50 @@ __device__ inline double eps_cold_d(const ReactDev &D, double sigma, double y_e,
     return u / sigma;
 }
 
-// omega through the table (rates.rs:98-131)
+// x^n for the whole orders 0..3 by products, pow() otherwise
+__device__ __forceinline__ double ipow(double x, double n) {
+    if (n == 1.0) return x;
+    if (n == 2.0) return x * x;
+    if (n == 3.0) return x * x * x;
+    if (n == 0.0) return 1.0;
+    return pow(x, n);
+}
+
+// f(T) through the table (rates.rs:98-118): 0 below T_thr, the direct sum past the table's span
+__device__ inline double temp_factor(const ReactDev &D, int r, double t) {
+    const sr_react_record &c = D.P.rec[r];
+    if (c.n_terms == 0) return 1.0;
+    if (t < c.t_thr) return 0.0;
+    const double step = log(2.5) / (double)(REACT_TABLE - 1);
+    const double pos = log(t / c.t_thr) / step;
+    if (pos >= (double)(REACT_TABLE - 1)) return f_sum(c, t);
+    const int i = (int)floor(pos);
+    const double frac = pos - (double)i;
+    return exp(D.ln_f[r][i] + frac * (D.ln_f[r][i + 1] - D.ln_f[r][i]));
+}
+
+// the gate g(Y_e Sigma) (rates.rs:119-125)
+__device__ inline double gate(const sr_react_record &c, double sigma, double y_e) {
+    if (!c.gated) return 1.0;
+    const double yes = y_e * sigma;
+    return yes > c.sigma_g ? pow(yes / c.sigma_g - 1.0, c.gate_p) : 0.0;
+}
+
+// omega = A Sigma^a prod X^n f(T) g(Y_e Sigma) (rates.rs:121-131)
 __device__ inline double omega(const ReactDev &D, int r, double sigma, const double *x, double t, double y_e) {
     const sr_react_record &c = D.P.rec[r];
-    double f = 1.0;
-    if (c.n_terms > 0) {
-        if (t < c.t_thr) {
-            f = 0.0;
-        } else {
-            const double step = log(2.5) / (double)(REACT_TABLE - 1);
-            const double pos = log(t / c.t_thr) / step;
-            if (pos >= (double)(REACT_TABLE - 1)) {
-                f = f_sum(c, t);
-            } else {
-                const int i = (int)floor(pos);
-                const double frac = pos - (double)i;
-                f = exp(D.ln_f[r][i] + frac * (D.ln_f[r][i + 1] - D.ln_f[r][i]));
-            }
-        }
-    }
+    const double f = temp_factor(D, r, t);
     double orders = 1.0;
-    for (int k = 0; k < c.n_in; k++) orders *= pow(x[c.in_species[k]], c.in_order[k]);
-    double g = 1.0;
-    if (c.gated) {
-        const double yes = y_e * sigma;
-        g = yes > c.sigma_g ? pow(yes / c.sigma_g - 1.0, c.gate_p) : 0.0;
-    }
-    return c.a_coef * pow(sigma, c.a_sigma) * orders * f * g;
+    for (int k = 0; k < c.n_in; k++) orders *= ipow(x[c.in_species[k]], c.in_order[k]);
+    const double g = gate(c, sigma, y_e);
+    return c.a_coef * ipow(sigma, c.a_sigma) * orders * f * g;
 }
 
 // N_Fe over dt in closed form (react.rs:470-513)
 __device__ double nfe_exact(const ReactDev &D, double sigma, double *x, double dt) {
     const sr_react_record &c = D.P.rec[D.P.nfe];
     const int fe = c.in_species[0], n = c.out_species[0];
-    const double y_fe = r_z[fe] / r_a[fe];
+    const double y_fe = r_z_a[fe];
     const double b = c.sigma_g / sigma;
     double others[NS], mu, a_y;
     for (int s = 0; s < NS; s++) others[s] = x[s];
@@ -160,7 +176,7 @@ __device__ double nfe_exact(const ReactDev &D, double sigma, double *x, double d
     const double v = y_fe * x[fe];
     const double u = v - cc;
     if (!(u > 0.0 && v > 0.0) || dt <= 0.0) return 0.0;
-    const double k = c.a_coef * pow(sigma, c.a_sigma) / b;
+    const double k = c.a_coef * ipow(sigma, c.a_sigma) / b;
     const double w = cc >= 0.0 ? u : v, s = cc >= 0.0 ? cc : -cc;
     const double ks = k * s * dt;
     const double phi = ks > 0.0 ? -expm1(-ks) / s : k * dt;
@@ -277,17 +293,32 @@ __device__ void burn(const ReactDev &D, double sigma, const double *x0, double e
             for (int j = 0; j < NS; j++) grad[r][j] = 0.0;
             if (r == P.nfe) continue;
             const sr_react_record &c = P.rec[r];
-            const double w = omega(D, r, sigma, x, t, y_e);
+            // omega at X and, for each input j, at X with X_j = 1 (whose factor 1^n is exactly 1): f, g and A Sigma^a
+            // once, each input's power once, the products in omega()'s order
+            const double f = temp_factor(D, r, t), g = gate(c, sigma, y_e);
+            const double ap = c.a_coef * ipow(sigma, c.a_sigma);
+            const int n_in = c.n_in;
+            double pw[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) pw[k] = k < n_in ? ipow(x[c.in_species[k]], c.in_order[k]) : 1.0;
+            double orders = 1.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < n_in) orders *= pw[k];
+            const double w = ap * orders * f * g;
             omegas[r] = w;
             cell.nfe |= c.latch && w > 0.0;
-            for (int k = 0; k < c.n_in; k++) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k >= n_in) break;
                 const int j = c.in_species[k];
                 const double n = c.in_order[k];
                 if (n == 0.0) continue;
-                double xj[NS];
-                for (int q = 0; q < NS; q++) xj[q] = x[q];
-                xj[j] = 1.0;
-                grad[r][j] += n * pow(x[j], n - 1.0) * omega(D, r, sigma, xj, t, y_e);
+                double oj = 1.0;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (q < n_in) oj *= c.in_species[q] == j ? 1.0 : pw[q];
+                grad[r][j] += n * ipow(x[j], n - 1.0) * (ap * oj * f * g);
             }
             for (int i = 0; i < NS; i++) s[i] += D.nu[r][i] * w;
             qdot += c.q * w;
@@ -310,7 +341,7 @@ __device__ void burn(const ReactDev &D, double sigma, const double *x0, double e
         if (s_max > 0.0) h = fmin(h, P.burn_dx / s_max);
         if (!pin) {
             double dinv_mu = 0.0;
-            for (int i = 0; i < NS; i++) dinv_mu += s[i] * ((r_z[i] + 1.0) / r_a[i]);
+            for (int i = 0; i < NS; i++) dinv_mu += s[i] * r_zp1_a[i];
             const double dt_dt = mu * (qdot - enu) - t * mu * dinv_mu;
             if (dt_dt != 0.0) h = fmin(h, P.burn_dt * t / fabs(dt_dt));
         }
