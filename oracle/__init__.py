@@ -369,6 +369,123 @@ def react_omega(record, sigma: float, x, t: float, y_e: float, table: bool = Tru
                                y_e, int(table))
 
 
+MAX_SINKS = 16
+
+
+class Sink(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("vx", C.c_double), ("vy", C.c_double), ("m", C.c_double)]
+
+
+class SinksState(C.Structure):
+    """sro_sinks: the sinks, what they swallowed, and the constants P3 reads."""
+    _fields_ = [("count", C.c_int), ("s", Sink * MAX_SINKS), ("swallowed", C.c_double), ("g", C.c_double),
+                ("sigma_bh", C.c_double), ("sigma_vac", C.c_double), ("sigma_floor", C.c_double),
+                ("c_sb", C.c_double), ("m_tov", C.c_double)]
+
+
+class SinksReport(C.Structure):
+    _fields_ = [("formed_x", C.c_int), ("formed_y", C.c_int), ("accreted", C.c_double), ("merged", C.c_int)]
+
+
+lib.sro_r_acc.argtypes = [C.c_double, C.c_double, C.c_double]
+lib.sro_r_acc.restype = C.c_double
+_SP = C.POINTER(SinksState)
+lib.sro_sinks_form.argtypes = [_SP, _PP, _dp, _sz, _sz, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+lib.sro_sinks_accrete.argtypes = [_SP, _PP, _dp, _sz, _sz]
+lib.sro_sinks_accrete.restype = C.c_double
+lib.sro_sinks_pull.argtypes = [_SP, _dp, _sz, _sz, _dp]
+lib.sro_sinks_advance.argtypes = [_SP, _dp, _sz, _sz, C.c_double]
+lib.sro_sinks_merge.argtypes = [_SP]
+lib.sro_sinks_add_field.argtypes = [_SP, C.c_void_p, C.c_void_p, C.c_void_p, _sz, _sz]
+lib.sro_sinks_step.argtypes = [_SP, _PP, _dp, _sz, _sz, _dp, C.c_double, _dp, C.POINTER(SinksReport)]
+
+
+def r_acc(g: float, m: float, c_sb: float) -> float:
+    return lib.sro_r_acc(g, m, c_sb)
+
+
+class Sinks:
+    """P3's twin (reference/sinks.rs) over (14, h, w) f64 planes changed in place; the constants from `ph`
+    (G, Sigma_vac, Sigma_floor, c_sb) with Sigma_BH and m_tov_sb given."""
+
+    def __init__(self, ph: Physics, sigma_bh: float = 6000.0, m_tov: float = float("inf")):
+        self.ph = ph
+        self.k = SinksState(count=0, swallowed=0.0, g=ph.G, sigma_bh=sigma_bh, sigma_vac=ph.sigma_vac,
+                            sigma_floor=ph.sigma_floor, c_sb=ph.c_sb, m_tov=m_tov)
+
+    @property
+    def sinks(self):
+        return [(s.x, s.y, s.vx, s.vy, s.m) for s in self.k.s[:self.k.count]]
+
+    def push(self, x, y, vx, vy, m):
+        assert self.k.count < MAX_SINKS
+        self.k.s[self.k.count] = Sink(x, y, vx, vy, m)
+        self.k.count += 1
+
+    def pop(self):
+        self.k.count -= 1
+
+    @property
+    def swallowed(self):
+        return self.k.swallowed
+
+    def mass(self):
+        return sum(s[4] for s in self.sinks)
+
+    def momentum(self):
+        return [sum(s[4] * s[2] for s in self.sinks), sum(s[4] * s[3] for s in self.sinks)]
+
+    def r_acc(self, i):
+        return r_acc(self.k.g, self.k.s[i].m, self.k.c_sb)
+
+    @staticmethod
+    def _check(planes):
+        assert planes.dtype == np.float64 and planes.flags.c_contiguous and planes.shape[0] == COMPONENTS
+        return planes.shape[2], planes.shape[1]
+
+    def form(self, planes, phi):
+        w, h = self._check(planes)
+        fx, fy = C.c_int(), C.c_int()
+        ok = lib.sro_sinks_form(C.byref(self.k), C.byref(self.ph), planes, w, h,
+                                np.ascontiguousarray(phi, np.float64).ravel(), C.byref(fx), C.byref(fy))
+        return (fx.value, fy.value) if ok else None
+
+    def accrete(self, planes):
+        w, h = self._check(planes)
+        return lib.sro_sinks_accrete(C.byref(self.k), C.byref(self.ph), planes, w, h)
+
+    def pull(self, planes):
+        w, h = self._check(planes)
+        a = np.zeros(2 * MAX_SINKS)
+        lib.sro_sinks_pull(C.byref(self.k), planes, w, h, a)
+        return a[:2 * self.k.count].reshape(-1, 2)
+
+    def advance(self, planes, dt):
+        w, h = self._check(planes)
+        lib.sro_sinks_advance(C.byref(self.k), planes, w, h, dt)
+
+    def merge(self):
+        return lib.sro_sinks_merge(C.byref(self.k))
+
+    def add_field(self, phi=None, gx=None, gy=None):
+        ref = next(a for a in (phi, gx, gy) if a is not None)
+        h, w = ref.shape
+        ptr = lambda a: None if a is None else a.ctypes.data
+        for a in (phi, gx, gy):
+            assert a is None or (a.dtype == np.float64 and a.flags.c_contiguous and a.shape == (h, w))
+        lib.sro_sinks_add_field(C.byref(self.k), ptr(phi), ptr(gx), ptr(gy), w, h)
+
+    def step(self, planes, phi_gas, dt):
+        """One P3 step; returns (formed (x, y) or None, accreted, merged, phi with the sinks' potential)."""
+        w, h = self._check(planes)
+        work = np.zeros((h, w))
+        rep = SinksReport()
+        _check(lib.sro_sinks_step(C.byref(self.k), C.byref(self.ph), planes, w, h,
+                                  np.ascontiguousarray(phi_gas, np.float64).ravel(), dt, work, C.byref(rep)))
+        formed = None if rep.formed_x < 0 else (rep.formed_x, rep.formed_y)
+        return formed, rep.accreted, rep.merged, work
+
+
 def lcg(seed: int, count: int) -> np.ndarray:
     """The reference's 64-bit LCG (fft.rs:292-300), uniform on [0, 1)."""
     out = np.empty(count)

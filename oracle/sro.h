@@ -193,6 +193,37 @@ double sro_react_omega(const sro_react_record *r, const double *ln_f, double sig
  * neutrino loss (n), S_nu (n) */
 int sro_react_step(const sro_physics *ph, const sro_react_params *rp, double *planes, size_t w, size_t h, double dt,
                    double *nuclear, double *lost, double *s_nu, sro_react_pass *pass);
+/* ---------------- P3 sinks (reference crates/sr-physics/src/reference/sinks.rs, grav.rs:79-96) ---------------- */
+#define SRO_MAX_SINKS 16
+typedef struct {
+    double x, y, vx, vy, m; /* position (cells, cell (i, j) centred at (i + 1/2, j + 1/2)), velocity, mass */
+} sro_sink;
+/* the sinks, what they swallowed, and the constants P3 reads: G, Sigma_BH, Sigma_vac, Sigma_floor, c_sb, m_tov_sb */
+typedef struct {
+    int count;
+    sro_sink s[SRO_MAX_SINKS];
+    double swallowed;
+    double g, sigma_bh, sigma_vac, sigma_floor, c_sb, m_tov;
+} sro_sinks;
+typedef struct {
+    int formed_x, formed_y; /* -1 when none formed */
+    double accreted;
+    int merged;
+} sro_sinks_report;
+double sro_r_acc(double g, double m, double c_sb);
+int sro_sinks_form(sro_sinks *k, const sro_physics *ph, double *planes, size_t w, size_t h, const double *phi,
+                   int *fx, int *fy);
+double sro_sinks_accrete(sro_sinks *k, const sro_physics *ph, double *planes, size_t w, size_t h);
+void sro_sinks_pull(const sro_sinks *k, const double *planes, size_t w, size_t h, double *a);
+void sro_sinks_advance(sro_sinks *k, const double *planes, size_t w, size_t h, double dt);
+int sro_sinks_merge(sro_sinks *k);
+/* the sinks' softened potential and exact pull added to a field (any of the three may be NULL) */
+void sro_sinks_add_field(const sro_sinks *k, double *phi, double *gx, double *gy, size_t w, size_t h);
+/* one P3 step (sinks.rs:401-417): formation against phi_gas with the sinks' potential added (phi_work, w h values,
+ * receives it), accretion, the kick-drift-kick in the gas accretion left, merging */
+int sro_sinks_step(sro_sinks *k, const sro_physics *ph, double *planes, size_t w, size_t h, const double *phi_gas,
+                   double dt, double *phi_work, sro_sinks_report *rep);
+
 /* P6 in the twin's step, between P5 and P8 (NULL: off) */
 void sro_twin_set_reactions(sro_twin *t, const sro_react_params *rp);
 void sro_twin_last_react(const sro_twin *t, sro_react_pass *pass);
