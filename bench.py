@@ -7,8 +7,9 @@ is Mcell-steps/s of that step with the state resident in HBM, timed with the ker
 steps one at a time (median), an instrumented window (per-pass times and per-kernel clocks, from which the dominant
 kernel per step is named), the step's algorithmic HBM bytes from the library's work counters (the full step's
 roofline), the stiff FLD solve alone at 2048^2 (plain MG-PCG on the tuned and on the stated 7-level shape, and the
-learned preconditioner; medians of 5), the 64-scene ensemble, P6 burning alone over a 1024^2 burning field, and the
-oracle (the f64 CPU twin restated in C) on the same 1024^2 workload on one host core and on all of them.
+learned preconditioner; medians of 5), the 64-scene ensemble, P6 burning alone over a 1024^2 burning field, P3 alone
+with 16 sinks on the turbulent 1024^2 field, and the oracle (the f64 CPU twin restated in C) on the same 1024^2
+workload on one host core and on all of them.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -38,7 +39,7 @@ def parse():
     ap.add_argument("--no-fld", action="store_true", help="skip the standalone FLD solve (counter runs)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-scene ensemble")
     ap.add_argument("--ensemble-threads", type=int, default=16)
-    ap.add_argument("--no-burn", action="store_true", help="skip the P6 burning field")
+    ap.add_argument("--no-burn", action="store_true", help="skip the P6 burning field and the P3 pass")
     return ap.parse_args()
 
 
@@ -328,6 +329,35 @@ def burn(sr, scenes, n=N_STEP, passes=5, cpu_n=128):
     return out
 
 
+def sinks_leg(sr, scenes, n=N_STEP, passes=5):
+    """P3 alone on the turbulent n^2 field with 16 sinks spread over it (each pass from the same upload: candidates,
+    accretion, the gas's pull on every sink twice over all cells, the kicks and merging), median of `passes` after one
+    untimed."""
+    import torch
+    kw = dict(scenes.config_physics_kwargs(n), dx=1.0)
+    planes = scenes.turbulent_scene(n).astype(np.float32)
+    w = sr.World(n, n, physics=sr.physics(**kw), gravity=False, hydro=False, heat=False)
+    w.set_sinks(6000.0)
+    g = np.linspace(0.1, 0.9, 4) * n
+    sinks = [(float(x) + 0.5, float(y) + 0.5, 0.0, 0.0, 1.0) for y in g for x in g]
+    times = []
+    for k in range(passes + 1):
+        w.upload(planes)
+        w.put_sinks(sinks)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        w.sinks_apply(1e-3)
+        torch.cuda.synchronize()
+        if k:
+            times.append(time.perf_counter() - t0)
+    ms = float(np.median(times)) * 1e3
+    w.close()
+    log(f"P3 {n}^2 with 16 sinks: {ms:.3f} ms per pass")
+    return {"workload": f"P3 alone on the turbulent {n}^2 field with 16 sinks on a 4 x 4 lattice (dx 1, Sigma_BH "
+                        f"6000)", "ms_per_pass": round(ms, 4), "sinks": 16,
+            "gas_pull_terms_per_pass": 2 * 16 * n * n}
+
+
 def cpu_baseline(scenes, n=N_STEP, steps=1, omp_steps=2, ref_budget_s=4.0):
     """The oracle twin (f64 C) on the headline workload at its full size: its heat pass solved by multigrid PCG to
     1e-11 (the device's method; the reference twin's Jacobi rule would take thousands of rounds per pass here), one
@@ -428,6 +458,7 @@ def main():
     fld = None if args.no_fld else fld_solve(sr, scenes, args.fld_n)
     ens = None if args.no_ensemble else ensemble(sr, scenes, args.ensemble_threads)
     burning = None if args.no_burn else burn(sr, scenes)
+    sinks = None if args.no_burn else sinks_leg(sr, scenes)
     copy_gbs = copy_bandwidth()
     bracket = event_bracket_ms()
     log(f"device copies (GB/s): {copy_gbs}; timing-event bracket {bracket} ms")
@@ -483,6 +514,7 @@ def main():
                          f"levels, V(2,2), plain prolongation; medians of 5", **fld),
             "ensemble_64": ens,
             "burning_1024": burning,
+            "sinks_1024": sinks,
         },
         # the kernel with the most device time per step (from the clocks of the instrumented window), with every other
         # clocked kernel beside it; `full_step` is the whole step's algorithmic bytes over the headline window's time.

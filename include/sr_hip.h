@@ -349,6 +349,42 @@ int sr_read_react(sr_world *wd, sr_react_report *out);
 /* the last pass's per-cell energy booked per group (7 w h, group-major), neutrino loss and S_nu (w h each) */
 int sr_read_react_planes(sr_world *wd, double *nuclear, double *lost, double *s_nu, size_t n);
 
+/* P3 sinks (reference/sinks.rs; Step::with_sinks, step/sinks.rs): point masses in cells (cell (i, j) centred at
+ * (i + 1/2, j + 1/2)), at most SR_MAX_SINKS. Each step with P3 runs, after P2 and before P4: formation at the densest
+ * cell meeting all four criteria (Sigma >= Sigma_BH, or a local Sigma maximum whose disk r <= 4 holds m_tov_sb with a
+ * disk r <= 3 at Y_e <= 0.40; converging; the potential's minimum among its 8 neighbours; bound; the world's outer
+ * ring never forms; one a step), accretion of the bound gas within each sink's r_acc = max(2, 2GM/c_sb^2) in index
+ * order (the share 1 - Sigma_floor / Sigma of a cell's Sigma, momentum and E; its thermal and cold energy booked
+ * `swallowed`), the kick-drift-kick by the gas's softened pull and the pairs' (softened by the larger r_acc), then
+ * merging of pairs closer than the larger r_acc, the lowest pair first. P2 adds the sinks' softened potential to phi
+ * and their exact pull to g (reference/grav.rs:79-96), and the half step's field the same part (hydro.rs:836-846). */
+#define SR_MAX_SINKS 16
+typedef struct {
+    double x, y, vx, vy, m;
+} sr_sink;
+/* Sigma_BH, and m_tov_sb: the sheet's neutron-matter ceiling for the run's physics, which this library does not
+ * compute (sinks.rs:104-115): the caller gives it (INFINITY: no sink from mass, the reference's own with the
+ * neutrons' cold pressure off) */
+typedef struct {
+    double sigma_bh, m_tov;
+} sr_sinks_params;
+/* the last pass: the cell a sink formed at (-1, -1: none), the mass accreted, the sinks lost to merging; and the
+ * thermal and cold energy swallowed since the sinks were last set */
+typedef struct {
+    int count, formed_x, formed_y, merged;
+    double accreted, swallowed;
+} sr_sinks_report;
+/* P3 in the world's step (NULL: off, the default); the world's dx must be 1 (the sinks' law is in cells). Setting it
+ * clears the sinks. */
+int sr_set_sinks(sr_world *wd, const sr_sinks_params *p);
+/* the sinks placed as given (a dump's header, a test's), count <= SR_MAX_SINKS; swallowed restarts at 0 */
+int sr_put_sinks(sr_world *wd, const sr_sink *sinks, int count);
+/* the sinks (up to max of them into out) and the last pass's report (Step::read_sinks, step/sinks.rs) */
+int sr_read_sinks(sr_world *wd, sr_sink *out, int max, sr_sinks_report *rep);
+/* P3 alone over dt on the world's state, against the potential phi (w h values, the cells' phi; NULL: the world's own
+ * from its last P2, the sinks' term included): the reference's rig with phi uploaded by hand (tests/gpu/sink.rs) */
+int sr_sinks_apply(sr_world *wd, double dt, const float *phi, size_t n);
+
 /* The active box (step/boxfit.rs:1-14, BoxMode and BoxReport :27-55): the bounding box of the cells with Sigma >=
  * Sigma_vac, grown by 8 cells, rounded out to multiples of 8, clamped to the world (empty, all 0, with no such cell),
  * its FFT size per axis the smallest power of two >= 2 x the box's, at least 32. It is fit at the upload (before

@@ -105,6 +105,25 @@ class ReactReport(C.Structure):
                 ("max_dx", C.c_double), ("max_dt_rel", C.c_double), ("latch", C.c_int)]
 
 
+MAX_SINKS = 16
+
+
+class Sink(C.Structure):
+    """sr_sink: a sink's position (cells), velocity and mass."""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("vx", C.c_double), ("vy", C.c_double), ("m", C.c_double)]
+
+
+class SinksParams(C.Structure):
+    """sr_sinks_params: Sigma_BH and m_tov_sb (infinite: no sink from mass)."""
+    _fields_ = [("sigma_bh", C.c_double), ("m_tov", C.c_double)]
+
+
+class SinksReport(C.Structure):
+    """sr_sinks_report: the sinks' count, the last pass's formation, merges and accretion, what they swallowed."""
+    _fields_ = [("count", C.c_int), ("formed_x", C.c_int), ("formed_y", C.c_int), ("merged", C.c_int),
+                ("accreted", C.c_double), ("swallowed", C.c_double)]
+
+
 class BoxReport(C.Structure):
     _fields_ = [("x0", C.c_uint32), ("y0", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
                 ("fft", C.c_uint32 * 2), ("from_step", C.c_uint32), ("fits", C.c_uint32),
@@ -158,6 +177,10 @@ SIGNATURES = [
     ("sr_fld_set_coarse_wave", _i, [_vp, _i]),
     ("sr_set_fusion", _i, [_vp, _i, _i]),
     ("sr_set_box_mode", _i, [_vp, _i]),
+    ("sr_set_sinks", _i, [_vp, C.POINTER(SinksParams)]),
+    ("sr_put_sinks", _i, [_vp, C.POINTER(Sink), C.c_int]),
+    ("sr_read_sinks", _i, [_vp, C.POINTER(Sink), C.c_int, C.POINTER(SinksReport)]),
+    ("sr_sinks_apply", _i, [_vp, _d, _vp, _sz]),
     ("sr_set_reactions", _i, [_vp, C.POINTER(ReactParams)]),
     ("sr_react_apply", _i, [_vp, _d]),
     ("sr_read_react", _i, [_vp, C.POINTER(ReactReport)]),

@@ -180,14 +180,14 @@ template <bool COLD> struct Lds {
     unsigned long long redk[8];
 };
 
-template <int MODE, bool COLD>
+template <int MODE, bool COLD, bool SINKS = false>
 __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, HydroState out,
                                                  const double *__restrict__ scal, const float *__restrict__ gx,
                                                  const float *__restrict__ gy, const float *__restrict__ sigma_mid,
                                                  const float *__restrict__ phi_mid, const float *__restrict__ rfx,
                                                  const float *__restrict__ rfy, float *__restrict__ sigma_mid_out,
                                                  float *__restrict__ pushmag, double *__restrict__ escaped,
-                                                 HydroPartials part) {
+                                                 HydroPartials part, SinkPlanes SP) {
     __shared__ Lds<COLD> L;
     const int w = P.w, h = P.h;
     const int tile = xcd_tile();
@@ -454,12 +454,25 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
         if (grav) {
             const int rw = w + 2;
             const float *pr = phi_mid + (size_t)(gyc + 1) * rw + gxc + 1;
-            phi_c = pr[0];
             const float src = sigma_mid[ci];
             const float gmx = -(pr[1] - pr[-1]) * 0.5f * P.inv_dx, gmy = -(pr[rw] - pr[-rw]) * 0.5f * P.inv_dx;
             float kick[2] = {dtf * src * gmx, dtf * src * gmy};
+            // the net kick taken back is the gas's own field's; the sinks' pull is the sinks' to book
             kick_net[0] = (double)kick[0];
             kick_net[1] = (double)kick[1];
+            // phi_mid as the faces' work reads it: the gas's with the sinks' potential
+            float pw[5] = {pr[0], pr[1], pr[-1], pr[rw], pr[-rw]};
+            if constexpr (SINKS) {
+                const float *ps = SP.ps + (size_t)(gyc + 1) * rw + gxc + 1;
+                pw[0] += ps[0];
+                pw[1] += ps[1];
+                pw[2] += ps[-1];
+                pw[3] += ps[rw];
+                pw[4] += ps[-rw];
+                kick[0] += dtf * src * SP.gsx[ci];
+                kick[1] += dtf * src * SP.gsy[ci];
+            }
+            phi_c = pw[0];
             if (src > 0.0f) dkey = ((unsigned long long)__float_as_uint(src) << 32) | (0xffffffffu - (unsigned)ci);
             const float m0 = L.mid[0][tid], m1 = L.mid[1][tid], m2 = L.mid[2][tid];
             float r[3], l[3], dn[3], up[3];
@@ -467,28 +480,28 @@ __global__ void __launch_bounds__(NSLOT) k_hydro(HydroParams P, HydroState in, H
             {
                 const bool has = gxc + 1 < w;
                 const int ns = slot_of(sx_ + 1, sy_);
-                const float pn = has ? pr[1] : phi_c;
+                const float pn = has ? pw[1] : phi_c;
                 face_work(F[1][0], m0, has ? L.mid[0][ns] : m0, m1, has ? L.mid[1][ns] : m1, (phi_c - pn) * P.inv_dx,
                           dtf, r);
             }
             {
                 const bool has = gxc > 0;
                 const int ns = slot_of(sx_ - 1, sy_);
-                const float pn = has ? pr[-1] : phi_c;
+                const float pn = has ? pw[2] : phi_c;
                 face_work(F[0][0], has ? L.mid[0][ns] : m0, m0, has ? L.mid[1][ns] : m1, m1, (pn - phi_c) * P.inv_dx,
                           dtf, l);
             }
             {
                 const bool has = gyc + 1 < h;
                 const int ns = slot_of(sx_, sy_ + 1);
-                const float pn = has ? pr[rw] : phi_c;
+                const float pn = has ? pw[3] : phi_c;
                 face_work(F[3][0], m0, has ? L.mid[0][ns] : m0, m2, has ? L.mid[2][ns] : m2, (phi_c - pn) * P.inv_dx,
                           dtf, dn);
             }
             {
                 const bool has = gyc > 0;
                 const int ns = slot_of(sx_, sy_ - 1);
-                const float pn = has ? pr[-rw] : phi_c;
+                const float pn = has ? pw[4] : phi_c;
                 face_work(F[2][0], has ? L.mid[0][ns] : m0, m0, has ? L.mid[2][ns] : m2, m2, (pn - phi_c) * P.inv_dx,
                           dtf, up);
             }
@@ -726,15 +739,20 @@ __global__ void k_hydro_takeback(HydroState st, const float *__restrict__ pushma
 int hydro_launch(int mode, const HydroParams &P, const HydroState &in, const HydroState &out, const double *scal,
                  const float *gx, const float *gy, const float *sigma_mid, const float *phi_mid, const float *rfx,
                  const float *rfy, float *sigma_mid_out, float *pushmag, double *escaped, const HydroPartials &part,
-                 hipStream_t s) {
+                 hipStream_t s, const SinkPlanes *sinks) {
     dim3 grid((P.w + T - 1) / T, (P.h + T - 1) / T);
-    // the cold pressure's variant only where a pair is on: the plain gas keeps the kernel without it
+    const SinkPlanes sp = sinks ? *sinks : SinkPlanes{nullptr, nullptr, nullptr};
+    const bool with_sinks = mode == HYDRO_UPDATE && sp.ps && phi_mid;
+    // the cold pressure's variant only where a pair is on, the sinks' only with sinks: the plain gas keeps the kernel
+    // without them
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(NSLOT), 0, s, P, in, out, scal, gx, gy, sigma_mid, phi_mid, rfx, rfy,
-                           sigma_mid_out, pushmag, escaped, part);
+                           sigma_mid_out, pushmag, escaped, part, sp);
     };
     if (mode == HYDRO_MID)
         P.cold.on ? go(k_hydro<HYDRO_MID, true>) : go(k_hydro<HYDRO_MID, false>);
+    else if (with_sinks)
+        P.cold.on ? go(k_hydro<HYDRO_UPDATE, true, true>) : go(k_hydro<HYDRO_UPDATE, false, true>);
     else
         P.cold.on ? go(k_hydro<HYDRO_UPDATE, true>) : go(k_hydro<HYDRO_UPDATE, false>);
     SR_HIP(hipGetLastError());

@@ -33,10 +33,18 @@ struct HydroPartials {
     unsigned long long *keys;  // per block: the densest free cell's key (Sigma_mid bits, ~index)
 };
 
+// The sinks' part of the half step's field (reference/hydro.rs:836-846): their softened potential on the ringed grid of
+// phi_mid ((w + 2) x (h + 2)) and their exact pull per cell; all null without sinks
+struct SinkPlanes {
+    const float *ps, *gsx, *gsy;
+};
+
+// `sinks` (UPDATE only; null or all-null planes: none): the kick takes the sinks' exact pull beside the gas's g_mid,
+// the net kick taken back is the gas's own, and the faces' work reads phi_mid with the sinks' potential
 int hydro_launch(int mode, const HydroParams &P, const HydroState &in, const HydroState &out, const double *scal,
                  const float *gx, const float *gy, const float *sigma_mid, const float *phi_mid, const float *rfx,
                  const float *rfy, float *sigma_mid_out, float *pushmag, double *escaped, const HydroPartials &part,
-                 hipStream_t s);
+                 hipStream_t s, const SinkPlanes *sinks = nullptr);
 int hydro_takeback(const HydroParams &P, const HydroState &st, const float *pushmag, const HydroPartials &part,
                    double *takeback, bool radiation, hipStream_t s);
 int hydro_blocks(int w, int h);

@@ -18,6 +18,7 @@
 #include "heat.hpp"
 #include "hydro.hpp"
 #include "react.hpp"
+#include "sinks.hpp"
 
 namespace sr {
 
@@ -382,7 +383,7 @@ struct sr_world {
     uint64_t steps = 0;
     sr_heat_report last_heat{};
     bool timing = false;
-    hipEvent_t ev[13]{};
+    hipEvent_t ev[14]{};
     sr_step_cost cost{};
     KernelClock k_update, k_mid, k_grav, k_pre0;
     // the step's fixed parts captured as graphs (with the clocks off): P1..P4 per current buffer and radiation flag,
@@ -398,6 +399,9 @@ struct sr_world {
     double *react_nuc = nullptr, *react_lost = nullptr, *react_snu = nullptr, *react_psum = nullptr,
            *react_pmax = nullptr;
     int react_nb = 0;
+    bool sinks_on = false;  // P3 in the step
+    SinkParams sk_par{};
+    SinkBufs sk{};
     // the active box: its record (x0, y0, w, h, fft_w, fft_h, from_step, fits) and the cell pass's partials
     bool box_fit = false;
     int *box = nullptr, *box_part = nullptr;
@@ -763,6 +767,11 @@ static int step_front(sr_world *wd) {
     hipLaunchKernelGGL(k_p1, dim3(1), dim3(64), 0, s, wd->scal);
     mark(wd, 1);
     SR_TRY(p2(wd));
+    mark(wd, 13);
+    if (wd->sinks_on) {  // P3 after P2's field, with the sinks' part added to it
+        if (wd->sw.gravity) SR_TRY(sinks_field(wd->sk_par, wd->sk, wd->phi, wd->gx, wd->gy, s));
+        SR_TRY(sinks_pass(wd->sk_par, wd->sk, wd->st[wd->cur], wd->phi, -1.0, wd->scal, s));
+    }
     mark(wd, 2);
     if (wd->sw.hydro) {
         HydroState &in = wd->st[wd->cur], &out = wd->st[1 - wd->cur];
@@ -775,10 +784,11 @@ static int step_front(sr_world *wd) {
             SR_TRY(wd->grav.potential<float>(wd->sigma_mid, wd->phi_mid, s));
         }
         mark(wd, 10);
+        const SinkPlanes sp{wd->sk.ps, wd->sk.gsx, wd->sk.gsy};
         SR_TRY(hydro_launch(HYDRO_UPDATE, wd->hp, in, out, wd->scal, grav ? wd->gx : nullptr,
                             grav ? wd->gy : nullptr, grav ? wd->sigma_mid : nullptr, grav ? wd->phi_mid : nullptr,
                             rad ? wd->rfx : nullptr, rad ? wd->rfy : nullptr, nullptr, wd->pushmag, wd->escaped,
-                            wd->part, s));
+                            wd->part, s, wd->sinks_on && grav ? &sp : nullptr));
         mark(wd, 11);
         if (grav || rad) SR_TRY(hydro_takeback(wd->hp, out, wd->pushmag, wd->part, wd->takeback, rad, s));
     }
@@ -852,7 +862,8 @@ static int one_step(sr_world *wd) {
     if (wd->timing) {
         SR_HIP(hipEventSynchronize(wd->ev[5]));
         wd->cost.pass_ms[1] += between(wd, 0, 1);
-        wd->cost.pass_ms[2] += between(wd, 1, 2);
+        wd->cost.pass_ms[2] += between(wd, 1, 13);
+        wd->cost.pass_ms[3] += between(wd, 13, 2);
         wd->cost.pass_ms[4] += between(wd, 2, 3);
         wd->cost.pass_ms[5] += between(wd, 3, 12);
         wd->cost.pass_ms[6] += between(wd, 12, 4);
@@ -1120,6 +1131,109 @@ extern "C" int sr_read_react_planes(sr_world *wd, double *nuclear, double *lost,
                               wd->s));
     if (lost) SR_HIP(hipMemcpyAsync(lost, wd->react_lost, sizeof(double) * n, hipMemcpyDeviceToHost, wd->s));
     if (s_nu) SR_HIP(hipMemcpyAsync(s_nu, wd->react_snu, sizeof(double) * n, hipMemcpyDeviceToHost, wd->s));
+    SR_HIP(hipStreamSynchronize(wd->s));
+    return SR_OK;
+}
+
+// the captured step front holds P3's launches and the hydro variant: a change of P3 captures it again
+static void drop_front_graphs(sr_world *wd) {
+    for (auto &row : wd->g_front)
+        for (auto &g : row) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+extern "C" int sr_set_sinks(sr_world *wd, const sr_sinks_params *p) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "sr_set_sinks: null world");
+    SR_HIP(hipStreamSynchronize(wd->s));
+    if (!p) {
+        if (wd->sinks_on) drop_front_graphs(wd);
+        wd->sinks_on = false;
+        return SR_OK;
+    }
+    if (!(p->sigma_bh > 0.0) || !(p->m_tov > 0.0))
+        return fail(SR_ERR_ARGUMENT, "sr_set_sinks: Sigma_BH and m_tov_sb must be > 0 (m_tov_sb may be infinite)");
+    if (wd->ph.dx != 1.0) return fail(SR_ERR_ARGUMENT, "sr_set_sinks: the sinks' law is in cells: dx must be 1");
+    if (!wd->sk.dev) {
+        SinkBufs &B = wd->sk;
+        sinks_alloc_sizes(wd->w, wd->h, &B.tiles, &B.pull_blocks);
+        SR_TRY(world_alloc(wd, (void **)&B.dev, sizeof(SinkDev)));
+        SR_TRY(world_alloc(wd, (void **)&B.cand_s, sizeof(double) * B.tiles));
+        SR_TRY(world_alloc(wd, (void **)&B.cand_i, sizeof(int) * B.tiles));
+        SR_TRY(world_alloc(wd, (void **)&B.pull, sizeof(double) * 2 * SR_MAX_SINKS * B.pull_blocks));
+        SR_TRY(world_alloc(wd, (void **)&B.ps, sizeof(float) * (wd->w + 2) * (wd->h + 2)));
+        SR_TRY(world_alloc(wd, (void **)&B.gsx, sizeof(float) * wd->n));
+        SR_TRY(world_alloc(wd, (void **)&B.gsy, sizeof(float) * wd->n));
+    }
+    SinkParams &K = wd->sk_par;
+    K.w = wd->w;
+    K.h = wd->h;
+    K.g = wd->ph.G;
+    K.sigma_bh = p->sigma_bh;
+    K.sigma_vac = wd->ph.sigma_vac;
+    K.sigma_floor = wd->ph.sigma_floor;
+    K.c_sb = wd->ph.c_sb;
+    K.m_tov = p->m_tov;
+    K.t_floor = wd->ph.t_floor;
+    K.gm1 = wd->ph.gamma - 1.0;
+    K.fixed_imu = wd->ph.mu > 0.0 ? 1.0 / wd->ph.mu : 0.0;
+    K.cold = wd->fp.cold;
+    SinkDev h{};
+    h.formed_x = h.formed_y = -1;
+    SR_HIP(hipMemcpy(wd->sk.dev, &h, sizeof h, hipMemcpyHostToDevice));
+    if (!wd->sinks_on) drop_front_graphs(wd);
+    wd->sinks_on = true;
+    return SR_OK;
+}
+
+extern "C" int sr_put_sinks(sr_world *wd, const sr_sink *sinks, int count) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "sr_put_sinks: null world");
+    if (!wd->sinks_on) return fail(SR_ERR_ARGUMENT, "sr_put_sinks: no P3 (sr_set_sinks)");
+    if (count < 0 || count > SR_MAX_SINKS || (count > 0 && !sinks))
+        return fail(SR_ERR_ARGUMENT, "sr_put_sinks: 0 to 16 sinks");
+    SinkDev h{};
+    h.count = count;
+    h.formed_x = h.formed_y = -1;
+    for (int q = 0; q < count; q++) {
+        if (!(sinks[q].m > 0.0) || !std::isfinite(sinks[q].x) || !std::isfinite(sinks[q].y) ||
+            !std::isfinite(sinks[q].vx) || !std::isfinite(sinks[q].vy) || !std::isfinite(sinks[q].m))
+            return fail(SR_ERR_ARGUMENT, "sr_put_sinks: a sink's mass must be > 0 and every value finite");
+        h.x[q] = sinks[q].x;
+        h.y[q] = sinks[q].y;
+        h.vx[q] = sinks[q].vx;
+        h.vy[q] = sinks[q].vy;
+        h.m[q] = sinks[q].m;
+    }
+    SR_HIP(hipStreamSynchronize(wd->s));
+    SR_HIP(hipMemcpy(wd->sk.dev, &h, sizeof h, hipMemcpyHostToDevice));
+    return SR_OK;
+}
+
+extern "C" int sr_read_sinks(sr_world *wd, sr_sink *out, int max, sr_sinks_report *rep) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "sr_read_sinks: null world");
+    SinkDev h{};
+    h.formed_x = h.formed_y = -1;
+    if (wd->sk.dev) {
+        SR_HIP(hipMemcpyAsync(&h, wd->sk.dev, sizeof h, hipMemcpyDeviceToHost, wd->s));
+        SR_HIP(hipStreamSynchronize(wd->s));
+    }
+    if (!wd->sinks_on) h.count = 0;
+    for (int q = 0; q < h.count && q < max && out; q++) out[q] = sr_sink{h.x[q], h.y[q], h.vx[q], h.vy[q], h.m[q]};
+    if (rep) *rep = sr_sinks_report{h.count, h.formed_x, h.formed_y, h.merged, h.accreted, h.swallowed};
+    return SR_OK;
+}
+
+extern "C" int sr_sinks_apply(sr_world *wd, double dt, const float *phi, size_t n) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "sr_sinks_apply: null world");
+    if (!wd->sinks_on) return fail(SR_ERR_ARGUMENT, "sr_sinks_apply: no P3 (sr_set_sinks)");
+    if (!(dt >= 0.0) || !std::isfinite(dt)) return fail(SR_ERR_ARGUMENT, "sr_sinks_apply: dt >= 0 is required");
+    if (phi) {
+        if (n != wd->n) return fail(SR_ERR_LENGTH, "sr_sinks_apply: phi holds w h values");
+        SR_HIP(hipMemcpy2DAsync(wd->phi + (wd->w + 2) + 1, sizeof(float) * (wd->w + 2), phi, sizeof(float) * wd->w,
+                                sizeof(float) * wd->w, wd->h, hipMemcpyHostToDevice, wd->s));
+    }
+    SR_TRY(sinks_pass(wd->sk_par, wd->sk, wd->st[wd->cur], wd->phi, dt, wd->scal, wd->s));
     SR_HIP(hipStreamSynchronize(wd->s));
     return SR_OK;
 }

@@ -130,7 +130,8 @@ def from_bytes(b: bytes) -> dict:
         raise DumpError(f"the planes hold {len(body)} bytes, the header says {4 * cells * planes}")
     data = np.frombuffer(body, dtype="<f4").reshape(planes, hh, w)
     out = {"header": h, "width": w, "height": hh, "step": step, "sim_time": sim_time, "dt": dt, "dt_next": dt_next,
-           "planes": data[:14].astype(np.float32), "box": list(r["box"]), "box_args": list(r["box_args"])}
+           "planes": data[:14].astype(np.float32), "box": list(r["box"]), "box_args": list(r["box_args"]),
+           "sinks": [tuple(float(s[k]) for k in ("x", "y", "vx", "vy", "m")) for s in sinks]}
     k = 14
     out["flux"] = data[k:k + 2].astype(np.float32) if flux else None
     k += 2 if flux else 0
@@ -154,24 +155,27 @@ def whole_world_box(w: int, h: int):
 
 
 def to_bytes(planes: np.ndarray, step: int = 0, sim_time: float = 0.0, dt: float = 0.0, dt_next: float = 0.0,
-             flux: np.ndarray | None = None, box=None, box_args=None) -> bytes:
+             flux: np.ndarray | None = None, box=None, box_args=None, sinks=()) -> bytes:
     """The dump's bytes. dt and dt_next are written at full f64 (this library's clock is FP64; the reference reads them
     as f32), sim_time at full f64 as the reference writes it. `box` (8 words: x0, y0, w, h, fft_w, fft_h, from_step,
     fits) and `box_args` (6: the 8 x 8 tiles, then the 256-cell groups over it) are the active box's (World.read_box);
-    without them the whole world's record."""
+    without them the whole world's record. `sinks`: P3's sinks, (x, y, vx, vy, m) each (dump.rs:87-99)."""
     p = np.ascontiguousarray(planes, dtype="<f4")
     if p.ndim != 3 or p.shape[0] != 14:
         raise DumpError("planes: (14, h, w) expected")
     _, hh, w = p.shape
     if flux is not None and np.shape(flux) != (2, hh, w):
         raise DumpError(f"flux: (2, {hh}, {w}) expected")
+    if len(sinks) > MAX_SINKS or any(len(sk) != 5 for sk in sinks):
+        raise DumpError(f"sinks: at most {MAX_SINKS}, (x, y, vx, vy, m) each")
     if box is None:
         box, box_args = whole_world_box(w, hh)
     box, box_args = [int(v) for v in box], [int(v) for v in box_args]
     if len(box) != 8 or len(box_args) != 6 or not all(0 <= v < 2 ** 32 for v in box + box_args):
         raise DumpError("box: 8 words and box_args: 6 words below 2^32 expected")
     header = {"version": 1, "width": w, "height": hh, "step": int(step), "sim_time": float(sim_time),
-              "channels": CHANNELS, "sinks": [],
+              "channels": CHANNELS,
+              "sinks": [dict(zip(("x", "y", "vx", "vy", "m"), (float(v) for v in sk))) for sk in sinks],
               "resume": {"dt": float(dt), "dt_next": float(dt_next), "latch_since": 0.0,
                          "box": box, "box_args": box_args, "flux": flux is not None, "rate": False}}
     hb = json.dumps(header).encode("utf-8")

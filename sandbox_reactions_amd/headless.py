@@ -42,9 +42,10 @@ import time
 
 import numpy as np
 
-PASS_LABELS = {1: "P1 dt", 2: "P2 gravity", 4: "P4 hydro", 5: "P5 heat", 6: "P6 reactions", 8: "P8 floors"}
+PASS_LABELS = {1: "P1 dt", 2: "P2 gravity", 3: "P3 sinks", 4: "P4 hydro", 5: "P5 heat", 6: "P6 reactions",
+               8: "P8 floors"}
 # the switch that records each optional pass (the reference names only the passes a step records, headless.rs:521-528)
-PASS_SWITCH = {2: "gravity", 4: "hydro", 5: "heat", 6: "reactions"}
+PASS_SWITCH = {2: "gravity", 3: "sinks", 4: "hydro", 5: "heat", 6: "reactions"}
 MAX_STEPS = 2_000_000  # --max-steps's default (headless.rs:77)
 # the state's channel names, as the guard's line names them (crates/sr-engine/src/state.rs:29-31)
 STATE_CHANNELS = ["sigma", "mom_x", "mom_y", "energy", "x_H", "x_He", "x_C", "x_O", "x_Ne", "x_Mg", "x_Si", "x_S",
@@ -176,6 +177,9 @@ def run(argv=None) -> int:
             world.resume(float(d["dt"]), float(d["dt_next"]), float(d["sim_time"]), flux=flux, sigma=d["sigma64"])
             # and its active box with the step index its refits count from (Step::restore, step/latch.rs:496-505)
             world.resume_box(d["box"], step0)
+            # and P3's sinks from its header (dump.rs:87-99)
+            if scene.switches["sinks"]:
+                world.put_sinks(d["sinks"])
         if a.timing:
             world.set_timing(True)
         dumps = os.path.join(a.out, "dumps") if a.out else None
@@ -190,7 +194,8 @@ def run(argv=None) -> int:
             dump.write(os.path.join(dumps, f"s_{step0 + done}.bin"), world.readback(), step=step0 + done,
                        sim_time=r.sim_time, dt=r.dt, dt_next=r.dt_next, flux=flux, sigma64=world.readback_sigma(),
                        box=[b.x0, b.y0, b.w, b.h, b.fft[0], b.fft[1], b.from_step, b.fits],
-                       box_args=list(b.tiles) + list(b.cells))
+                       box_args=list(b.tiles) + list(b.cells),
+                       sinks=world.read_sinks()[0] if scene.switches["sinks"] else ())
 
         t0 = time.perf_counter()
         target = min(a.steps if a.steps is not None else a.max_steps, a.max_steps)

@@ -5,9 +5,10 @@ Covered: `rect` and `disk` objects (Maclaurin profile sampled at cell centres, s
 `pressure`, a `composition` of mass fractions summing to 1 (within 1e-6, scene.rs:46, :215-245) and a velocity `v`;
 overlapping objects add their mass, momentum, energy and species mass (scene.rs:485-497); cells no object covers
 hold the floor state, Sigma_floor of hydrogen at rest at T_floor (scene.rs:466-470). Refused with a message: anything
-the path does not carry -- `preset` objects (they need the calibration file), the neutrinos and sinks switches, and, with the cold pressure on, a start with a cell the hold would take (Sigma >= sigma_hold
-and cold: the hold of floors.rs:200-260, which the path does not carry). The reactions switch gives the world P6 with
-every shipped record (`Scene.react_params`).
+the path does not carry -- `preset` objects (they need the calibration file), the neutrinos switch, sinks with a
+neutron pair other than the shipped one, and, with the cold pressure on, a start with a cell the hold would take
+(Sigma >= sigma_hold and cold: the hold of floors.rs:200-260, which the path does not carry). The reactions switch
+gives the world P6 with every shipped record (`Scene.react_params`), the sinks switch P3 (`Scene.sinks_params`).
 
 The energy follows the EOS with this library's adiabatic index gamma (the reference's gamma = 2 by default) and the
 cold pressure of the shipped constants (k1e, k2e, k1n, k2n; a scene's overrides may turn a pair off):
@@ -32,6 +33,8 @@ COMPOSITION_TOLERANCE = 1e-6
 SHIPPED = {"sigma_floor": 1e-8, "sigma_vac": 1e-6, "t_floor": 0.05, "a_r": 6.25e-7, "c_sb": 12800.0, "kappa0": 1.0,
            "k_cond": 0.0, "cfl": 0.4, "eta_g": 0.3, "kappa_dust": 0.0, "t_dust": 0.0}
 COLD = ("k1e", "k2e", "k1n", "k2n")
+# m_tov_sb for the shipped constants: the sheet's neutron-matter ceiling as the reference computes it (sinks.rs:104-115)
+M_TOV_SHIPPED = 4853.17
 PHYSICS_KEYS = {"t_h", "t_he", "t_c", "t_ne", "t_o", "t_si", "q_h", "a_h_burn", "a_he_burn", "a_c_alpha", "a_c_burn",
                 "a_ne_burn", "a_o_burn", "a_mg_burn", "a_si_burn", "a_s_burn", "a_n_fe", "nu_c_burn", "nu_ne_burn",
                 "nu_o_burn", "nu_mg_burn", "nu_si_burn", "nu_s_burn", "sigma_n", "burn_dx", "burn_dt", "burn_k",
@@ -180,11 +183,28 @@ class Scene:
 
     def check_path(self):
         """Refuses a scene that switches on a pass this path does not carry: neutrinos (P7's heating, and P6's cooling
-        that feeds it) and sinks (P3)."""
-        for k in ("neutrinos", "sinks"):
-            if self.switches[k]:
-                raise SceneError(f"switches.{k}: this path carries gravity, hydro, heat and reactions only "
-                                 f"(set {k} false)")
+        that feeds it); and sinks with a neutron pair other than the shipped one or none (sinks_params)."""
+        if self.switches["neutrinos"]:
+            raise SceneError("switches.neutrinos: this path carries gravity, hydro, heat, reactions and sinks only "
+                             "(set neutrinos false)")
+        self.sinks_params()
+
+    def sinks_params(self):
+        """P3's (Sigma_BH, m_tov_sb) for this scene, None with the sinks switch off. m_tov_sb is the sheet's
+        neutron-matter ceiling (sinks.rs:104-115), which this path does not compute: the reference's value for the
+        shipped pair (4,853.17), infinite with the neutrons' cold pressure off (no sink from mass); another pair is
+        refused."""
+        if not self.switches["sinks"]:
+            return None
+        c = self.constants()
+        if c["k1n"] == 0.0 and c["k2n"] == 0.0:
+            m_tov = math.inf
+        elif (c["k1n"], c["k2n"]) == (cold_eos.SHIPPED["k1n"], cold_eos.SHIPPED["k2n"]):
+            m_tov = M_TOV_SHIPPED
+        else:
+            raise SceneError("switches.sinks: m_tov_sb for a neutron pair other than the shipped one (the sheet's "
+                             "ceiling) is not carried by this path")
+        return float(self.overrides.get("sigma_bh", 6000.0)), m_tov
 
     def react_params(self):
         """P6's parameters for this scene (Step::with_reactions's ReactSetup, cpu_gpu.rs:72-85): every shipped record,
@@ -337,5 +357,8 @@ def world(scene: Scene, heat_settings=None, planes=None, box_mode=None, **physic
     rp = scene.react_params()
     if rp is not None:
         w.set_reactions(rp)
+    sp = scene.sinks_params()
+    if sp is not None:
+        w.set_sinks(*sp)
     w.upload(scene.planes(kw["gamma"]) if planes is None else planes)
     return w

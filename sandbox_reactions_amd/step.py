@@ -7,7 +7,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import BoxReport, DtReport, HeatReport, ReactReport, StepCost, StepWork, Switches
+from ._lib import (MAX_SINKS, BoxReport, DtReport, HeatReport, ReactReport, Sink, SinksParams, SinksReport, StepCost,
+                   StepWork, Switches)
 
 
 class World:
@@ -86,6 +87,32 @@ class World:
         self._c(self.lib.sr_read_react_planes(self._p, nuc.ctypes.data, lost.ctypes.data, snu.ctypes.data,
                                               self.w * self.h))
         return nuc, lost, snu
+
+    def set_sinks(self, sigma_bh: float = 6000.0, m_tov: float = float("inf"), on: bool = True):
+        """P3 in the step with Sigma_BH and m_tov_sb (infinite: no sink from mass); on=False switches it off. Clears
+        the sinks."""
+        p = SinksParams(sigma_bh, m_tov)
+        self._c(self.lib.sr_set_sinks(self._p, C.byref(p) if on else None))
+
+    def put_sinks(self, sinks):
+        """The sinks placed as given: (x, y, vx, vy, m) each, at most 16."""
+        arr = (Sink * MAX_SINKS)(*[Sink(*map(float, s)) for s in sinks][:MAX_SINKS])
+        self._c(self.lib.sr_put_sinks(self._p, arr, len(sinks)))
+
+    def read_sinks(self):
+        """((x, y, vx, vy, m) per sink, the last pass's SinksReport)."""
+        arr = (Sink * MAX_SINKS)()
+        rep = SinksReport()
+        self._c(self.lib.sr_read_sinks(self._p, arr, MAX_SINKS, C.byref(rep)))
+        return [(s.x, s.y, s.vx, s.vy, s.m) for s in arr[:rep.count]], rep
+
+    def sinks_apply(self, dt: float, phi=None):
+        """P3 alone over dt against phi ((h, w) f32 values the cells' potential; None: the world's own)."""
+        if phi is None:
+            self._c(self.lib.sr_sinks_apply(self._p, dt, None, 0))
+            return
+        a = np.ascontiguousarray(phi, dtype=np.float32)
+        self._c(self.lib.sr_sinks_apply(self._p, dt, a.ctypes.data, a.size))
 
     def set_box_mode(self, mode: int):
         """BOX_WHOLE_WORLD (the default) or BOX_FIT (the reference's Step::new): P8 and P9 over the active box. Before

@@ -153,3 +153,17 @@ def test_the_box_record_round_trips_and_defaults_to_the_whole_world(tmp_path):
     assert d["box"] == [0, 0, 24, 16, 64, 32, 0, 0] and d["box_args"] == [3, 2, 1, 2, 1, 1]
     with pytest.raises(dump.DumpError):
         dump.to_bytes(planes, box=[0, 0, 24, 16], box_args=[3, 2, 1, 2, 1, 1])
+
+
+def test_the_sinks_round_trip(tmp_path):
+    """P3's sinks in the header (dump.rs:87-99), each number at full f64; none by default; more than 16 refused."""
+    planes = np.zeros((14, 16, 24), np.float32)
+    sinks = [(12.5 + 1e-13, 3.25, 0.1, -0.3, 2000.0 / 3.0), (1.0, 2.0, 0.0, 0.0, 6000.0)]
+    p = tmp_path / "s.bin"
+    p.write_bytes(dump.to_bytes(planes, sinks=sinks))
+    d = dump.read(str(p))
+    assert d["sinks"] == sinks
+    p.write_bytes(dump.to_bytes(planes))
+    assert dump.read(str(p))["sinks"] == []
+    with pytest.raises(dump.DumpError):
+        dump.to_bytes(planes, sinks=[(0.0, 0.0, 0.0, 0.0, 1.0)] * 17)

@@ -190,3 +190,34 @@ def test_bad_dumps_are_exit_4(tmp_path):
     assert code == 4 and out.startswith("SR-ERROR bad dump") and "planes hold" in out, out
     code, out = headless("--scene", scene, "--steps", "1", "--load-dump", src, "--world", "128x64")
     assert code == 4 and out.startswith("SR-ERROR bad arguments") and "against the dump's 64x64" in out, out
+
+
+@pytest.mark.gpu
+def test_a_sink_scene_runs_p3_and_resumes_its_sinks(gpu_lib, tmp_path):
+    """sink_cloud.json with a sink placed at its centre by a dump (the header's sinks, dump.rs:87-99): steps 2 to 4
+    with timing name P3 among the passes; the sink accretes; resuming from the dump at 2 lands on the dump at 4 bit for
+    bit, the sinks included."""
+    from sandbox_reactions_amd import dump
+    print("headless sink_cloud 4 steps", flush=True)
+    scene = os.path.join(SCENES, "sink_cloud.json")
+    code, out = headless("--scene", scene, "--steps", "1", "--out", str(tmp_path / "a0"))
+    assert code == 0, out
+    first = str(tmp_path / "a0" / "dumps" / "s_1.bin")
+    d1 = dump.read(first)
+    start = tmp_path / "start.bin"  # the first step's dump with a sink added to its header
+    dump.write(str(start), d1["planes"], sigma64=dump.read_sigma64(first + ".sigma64", 64, 64), step=1,
+               sim_time=d1["sim_time"], dt=d1["dt"], dt_next=d1["dt_next"], box=d1["box"], box_args=d1["box_args"],
+               sinks=[(32.5, 32.5, 0.0, 0.0, 2000.0)])
+    code, out = headless("--scene", scene, "--steps", "3", "--out", str(tmp_path / "a"), "--timing",
+                         "--dump-every", "2", "--load-dump", str(start))
+    assert code == 0, out
+    passes = json.load(open(tmp_path / "a" / "summary.json"))["timing"]["per_pass_ms"]
+    assert {"P1 dt", "P2 gravity", "P3 sinks", "P4 hydro", "P8 floors"} == set(passes) and passes["P3 sinks"] > 0
+    d2 = dump.read(str(tmp_path / "a" / "dumps" / "s_2.bin"))
+    d4 = dump.read(str(tmp_path / "a" / "dumps" / "s_4.bin"))
+    assert len(d4["sinks"]) == 1 and d4["sinks"][0][4] > d2["sinks"][0][4] > 2000.0
+    code, out = headless("--scene", scene, "--steps", "2", "--out", str(tmp_path / "b"), "--load-dump",
+                         str(tmp_path / "a" / "dumps" / "s_2.bin"))
+    assert code == 0, out
+    b4 = dump.read(str(tmp_path / "b" / "dumps" / "s_4.bin"))
+    assert np.array_equal(b4["planes"].view(np.uint32), d4["planes"].view(np.uint32)) and b4["sinks"] == d4["sinks"]

@@ -105,7 +105,8 @@ def base():  # an 8 x 8 rect of hydrogen in a 64 x 64 world, cold pressure off
                 d["objects"][0].update(pressure=1e-3)), "below the cold pressure"),
     (lambda d: d["objects"].append({"kind": "preset", "preset": "sun", "x": 1, "y": 1}), "calibration"),
     (lambda d: d.update(switches={"reactions": True, "neutrinos": True, "sinks": False}), "neutrinos"),
-    (lambda d: d.update(switches={"reactions": True, "neutrinos": False, "sinks": True}), "sinks"),
+    (lambda d: (d.update(switches={"reactions": False, "neutrinos": False, "sinks": True}),
+                d["overrides"].update(k1n=3.0, k2n=50.0)), "neutron pair other than the shipped"),
     (lambda d: d["world"].update(width=60), "world size"),
     (lambda d: d["overrides"].update(warp=1.0), "unknown key"),
     (lambda d: d.update(version=2), "version"),
@@ -211,3 +212,16 @@ def test_the_burning_scene_carries_p6():
     d["switches"]["reactions"] = False
     d["objects"][0]["t"] = 100.0
     assert scene_json.parse(json.dumps(d)).react_params() is None
+
+
+def test_the_sink_scene_carries_p3():
+    """sink_cloud.json (gravity, hydro and sinks on, the cold pressure off by its overrides): P3 with Sigma_BH 6000 and
+    no sink from mass; with the shipped neutron pair m_tov_sb is the reference's 4,853.17; sinks off gives none."""
+    s = load("sink_cloud.json")
+    s.check_path()
+    assert s.sinks_params() == (6000.0, math.inf)
+    d = json.load(open(os.path.join(HERE, "sink_cloud.json")))
+    d["overrides"] = {"sigma_bh": 5000.0}
+    assert scene_json.parse(json.dumps(d)).sinks_params() == (5000.0, scene_json.M_TOV_SHIPPED)
+    d["switches"]["sinks"] = False
+    assert scene_json.parse(json.dumps(d)).sinks_params() is None
