@@ -208,6 +208,12 @@ int sr_fld_set_fusion(sr_fld *f, int pre, int post);
  * barrier (on, the default), or on the whole block from LDS (off): the same values either way. No counterpart in the
  * reference (its one coarse workgroup, heat.wgsl:1485-1526). */
 int sr_fld_set_coarse_wave(sr_fld *f, int on);
+/* Host-decided solves: each PCG iteration's update (x += alpha p, r -= alpha q, ||r||^2 partials) folded into the next
+ * preconditioner application's level-0 pre-smoother (on, the default, where level 0 has 32 x 32 tiles apart from the
+ * fused pair), or launched on its own (off). x, r and z take the same values either way; the ||r||^2 partials are
+ * summed per tile when folded. No counterpart in the reference (its update is a round of its chained dispatches,
+ * heat.wgsl:605-877). */
+int sr_fld_set_update_fusion(sr_fld *f, int on);
 void sr_fld_destroy(sr_fld *f);
 /* The learned preconditioner (SR_PRECOND_LEARNED): z = V r + N(r - A V r), V the multigrid V-cycle and N the
  * network. With fallback_iterations > 0 the solve continues with plain multigrid from where it stands once the
@@ -298,6 +304,8 @@ int sr_set_network(sr_world *wd, sr_net *n, int fallback_iterations);
 int sr_set_solve_mode(sr_world *wd, int mode);
 /* the heat pass's fused smoother pair, as sr_fld_set_fusion */
 int sr_set_fusion(sr_world *wd, int pre, int post);
+/* the heat pass's update folded into the next pre-smoother, as sr_fld_set_update_fusion */
+int sr_set_update_fusion(sr_world *wd, int on);
 /* P6 burning (crates/sr-physics/src/reference/react.rs; the rate law crates/sr-physics/src/rates.rs; the records
  * assets/reactions.json resolved against physics.json, registry.rs:485-535, :680-753). One enabled record: its inputs
  * (species, share, order), outputs (species, share), group (H He C Ne O Si N, registry.rs:460), q = q_ratio Q_H, the
@@ -418,8 +426,9 @@ void sr_destroy(sr_world *wd);
 /* Timing of a kernel over the steps run with timing on (since sr_set_timing switched it on): launches, mean device ms
  * per launch (HIP events on the world's stream around each launch), and the algorithmic HBM bytes of one launch.
  * `kernel`: "hydro_update", "hydro_mid" (P4), "mg_pre0", "mg_post0" (the level-0 smoothers), "mg_coarse" (the
- * single-block V-cycle), "fld_matvec", "fld_update" (the PCG iteration's two kernels). No counterpart in the reference
- * (its read_cost times passes). */
+ * single-block V-cycle), "fld_matvec", "fld_update" (the PCG iteration's two kernels), "fld_update_pre0" (the update
+ * folded into the next level-0 pre-smoother, sr_set_update_fusion). No counterpart in the reference (its read_cost
+ * times passes). */
 int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launches, double *mean_ms, double *bytes_per_launch);
 
 #ifdef __cplusplus

@@ -175,6 +175,8 @@ SIGNATURES = [
     ("sr_fld_fusion", _i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     ("sr_fld_set_fusion", _i, [_vp, _i, _i]),
     ("sr_fld_set_coarse_wave", _i, [_vp, _i]),
+    ("sr_fld_set_update_fusion", _i, [_vp, _i]),
+    ("sr_set_update_fusion", _i, [_vp, _i]),
     ("sr_set_fusion", _i, [_vp, _i, _i]),
     ("sr_set_box_mode", _i, [_vp, _i]),
     ("sr_set_sinks", _i, [_vp, C.POINTER(SinksParams)]),

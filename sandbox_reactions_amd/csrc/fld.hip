@@ -534,6 +534,89 @@ __global__ void __launch_bounds__(NT) k_pre(Lv L, const float *__restrict__ b, f
     }
 }
 
+// The PCG update of the iteration before, folded into the next V-cycle's level-0 pre-smoother: alpha = r.z / p.q from
+// the partials, x += alpha p (FP64) and r -= alpha q in the update's own expressions, ||r||^2 partials per tile. Each
+// tile computes the new residual over its whole region (its cells and the halo) from the old r and q, stores x and r
+// for its own cells only, r into the other residual buffer (the tiles beside it read the old one as halo), and smooths
+// from it as k_pre does. x, r and z are the separate kernels' values; the ||r||^2 partials are summed per tile.
+struct PendingUpdate {
+    const float *p, *q, *rold;
+    float *rnew;
+    double *x;
+    const double *dots_cur;
+    int nbd;
+    const double *pq;
+    int nbq;
+    double *part;  // ||r||^2 per level-0 tile
+};
+
+template <int TX, int TY, int NU>
+__global__ void __launch_bounds__(NT) k_pre_upd(Lv L, PendingUpdate U, float *__restrict__ z, float *__restrict__ bc,
+                                                int wc) {
+    constexpr int HW = 2 * NU + 1;
+    using RG = Region<TX, TY, HW>;
+    __shared__ RG R;
+    __shared__ float own_in[TX * TY];
+    const double rz = sum_partials(U.dots_cur, U.nbd), pq = sum_partials(U.pq, U.nbq);
+    const double alpha = pq != 0.0 ? rz / pq : 0.0;
+    const float af = (float)alpha;
+    const int tile = xcd_tile();
+    const int x0 = (tile % gridDim.x) * TX, y0 = (tile / gridDim.x) * TY;
+    constexpr int IT = (RG::N + NT - 1) / NT;
+    float4 vc[IT];
+    float vr[IT], vq[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int t = threadIdx.x + k * NT;
+        const int lx = t % RG::RX, ly = t / RG::RX;
+        const int gx = x0 + lx - HW, gy = y0 + ly - HW;
+        const bool in = t < RG::N && gx >= 0 && gx < L.w && gy >= 0 && gy < L.h;
+        const size_t i = (size_t)gy * L.w + gx;
+        vc[k] = in ? L.cf[i] : make_float4(1.0f, 0.0f, 0.0f, 1.0f);
+        vr[k] = in ? U.rold[i] : 0.0f;
+        vq[k] = in ? U.q[i] : 0.0f;
+    }
+    double v[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int t = threadIdx.x + k * NT;
+        if (t < RG::N) {
+            const int lx = t % RG::RX - HW, ly = t / RG::RX - HW;
+            const float ri = vr[k] - af * vq[k];
+            const bool mine = lx >= 0 && lx < TX && ly >= 0 && ly < TY;
+            if (mine) own_in[ly * TX + lx] = vc[k].x;
+            R.al[t] = vc[k].y;
+            R.au[t] = vc[k].z;
+            R.rd[t] = vc[k].w;
+            R.b[t] = ri;
+            const int gx = x0 + lx, gy = y0 + ly;
+            if (mine && gx < L.w && gy < L.h) {
+                const size_t i = (size_t)gy * L.w + gx;
+                U.rnew[i] = ri;
+                U.x[i] = U.x[i] + alpha * (double)U.p[i];
+                v[0] += (double)ri * ri;
+            }
+        }
+    }
+#pragma unroll 4
+    for (int t = threadIdx.x; t < RG::N; t += NT) R.z[t] = 0.0f;
+    __syncthreads();
+    sweeps<TX, TY, HW, 1, 2 * NU, true>(R, L, x0, y0);
+    constexpr int CX = TX / 2, CY = TY / 2;
+    for (int t = threadIdx.x; t < CX * CY; t += NT) {
+        const int cx = t % CX, cy = t / CX;
+        const int X = x0 / 2 + cx, Y = y0 / 2 + cy;
+        if (2 * X >= L.w || 2 * Y >= L.h) continue;
+        bc[(size_t)Y * wc + X] = restrict_residual<TX, TY, HW, TX>(R, L, own_in, 2 * cx, 2 * cy, 2 * X, 2 * Y);
+    }
+    for (int t = threadIdx.x; t < TX * TY; t += NT) {
+        const int lx = HW + t % TX, ly = HW + t / TX;
+        const int gx = x0 + lx - HW, gy = y0 + ly - HW;
+        if (gx < L.w && gy < L.h) z[(size_t)gy * L.w + gx] = R.z[ly * RG::RX + lx];
+    }
+    block_partials<1>(v, U.part, gridDim.x * gridDim.y, tile);
+}
+
 // Prolongation of the coarse correction (scaled by omega) and post-smoothing of level L (nu sweeps, black first).
 // Halo 2 nu. With `part` (level 0), partials of r.z_new and r.z_old.
 template <int TX, int TY, int NU>
@@ -1328,7 +1411,7 @@ FldSolver::~FldSolver() {
         (void)hipFree(lv[l].zp);
         if (lv[l].cf) (void)hipFree(lv[l].cf);
     }
-    void *ps[] = {gas, r32, za, zb, p, q, pack, part_true, part_pq, part_rr, part_dots};
+    void *ps[] = {gas, r32, r32b, za, zb, p, q, pack, part_true, part_pq, part_rr, part_rr_t, part_dots};
     for (void *x : ps)
         if (x) (void)hipFree(x);
     if (host_part) (void)hipHostFree(host_part);
@@ -1437,6 +1520,7 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
     SR_HIP(hipMalloc(&gas, n));
     SR_HIP(hipMemset(gas, 1, n));
     SR_HIP(hipMalloc(&r32, sizeof(float) * n));
+    SR_HIP(hipMalloc(&r32b, sizeof(float) * n));
     SR_HIP(hipMalloc(&za, sizeof(float) * n));
     SR_HIP(hipMalloc(&zb, sizeof(float) * n));
     SR_HIP(hipMalloc(&p, sizeof(float) * 2 * n));
@@ -1447,9 +1531,10 @@ int FldSolver::init(int w, int h, int levels, int nu1_, int nu2_, double omega_)
     SR_HIP(hipMalloc(&part_true, sizeof(double) * 4 * nbe));
     SR_HIP(hipMalloc(&part_pq, sizeof(double) * ((w + NT - 1) / NT) * h));
     SR_HIP(hipMalloc(&part_rr, sizeof(double) * nbe));
+    SR_HIP(hipMalloc(&part_rr_t, sizeof(double) * ntiles0));
     SR_HIP(hipMalloc(&part_dots, sizeof(double) * 2 * 2 * nbd_max));
     SR_HIP(hipMemset(part_dots, 0, sizeof(double) * 2 * 2 * nbd_max));
-    SR_HIP(hipHostMalloc(&host_part, sizeof(double) * 4 * nbe));
+    SR_HIP(hipHostMalloc(&host_part, sizeof(double) * std::max(4 * nbe, ntiles0)));
     SR_HIP(hipEventCreateWithFlags(&ev_part, hipEventDisableTiming));
     SR_HIP(hipMalloc(&ctl, sizeof(SolveCtl) * 2));
     SR_HIP(hipMemset(ctl, 0, sizeof(SolveCtl) * 2));
@@ -1771,6 +1856,17 @@ static void launch_pre(int nu, hipStream_t s, Lv L, const float *b, float *z, fl
 }
 
 template <int TX, int TY>
+static void launch_pre_upd(int nu, hipStream_t s, Lv L, const PendingUpdate &U, float *z, float *bc, int wc) {
+    const dim3 grid((L.w + TX - 1) / TX, (L.h + TY - 1) / TY);
+    if (nu == 1)
+        hipLaunchKernelGGL((k_pre_upd<TX, TY, 1>), grid, dim3(NT), 0, s, L, U, z, bc, wc);
+    else if (nu == 2)
+        hipLaunchKernelGGL((k_pre_upd<TX, TY, 2>), grid, dim3(NT), 0, s, L, U, z, bc, wc);
+    else
+        hipLaunchKernelGGL((k_pre_upd<TX, TY, 3>), grid, dim3(NT), 0, s, L, U, z, bc, wc);
+}
+
+template <int TX, int TY>
 static int launch_post(int nu, hipStream_t s, Lv L, const float *b, const float *zin, const float *zc, int wc,
                        float omega, float *zout, const float *zold, double *part, const int *gate) {
     const dim3 grid((L.w + TX - 1) / TX, (L.h + TY - 1) / TY);
@@ -1862,8 +1958,13 @@ int FldSolver::launch_post_level(int l, const float *b, float *zout, const float
 }
 
 // z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
+bool FldSolver::update_fusable() const {
+    return fuse_update && coarse_from > 0 && pair_level() != 0 && !pre_tile.empty() && pre_tile[0] == TILE_32 &&
+           nu1 >= 1 && nu1 <= 3;
+}
+
 int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s,
-                      const int *gate) {
+                      const int *gate, const PendingUpdate *upd, int stage) {
     const int pa = pair_level();
     const float *bl = r;
     for (int l = 0; l < coarse_from;) {
@@ -1874,9 +1975,22 @@ int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots,
             continue;
         }
         const FldLevel &c = lv[l + 1];
-        const int slot = l == 0 ? tick(CLK_PRE0, s) : -1;
-        SR_TRY(launch_pre_level(l, bl, s, gate));
-        SR_TRY(tock(slot, s));
+        if (l > 0 || stage != 2) {
+            if (l == 0 && upd) {
+                const int slot = tick(CLK_UPDATE_PRE0, s);
+                const FldLevel &f = lv[0];
+                launch_pre_upd<32, 32>(nu1, s, Lv{f.w, f.h, f.own, f.ax, f.ay, f.cf}, *upd, f.z, c.b, c.w);
+                SR_TRY(tock(slot, s));
+            } else {
+                const int slot = l == 0 ? tick(CLK_PRE0, s) : -1;
+                SR_TRY(launch_pre_level(l, bl, s, gate));
+                SR_TRY(tock(slot, s));
+            }
+            if (l == 0 && stage == 1) {
+                SR_HIP(hipGetLastError());
+                return SR_OK;
+            }
+        }
         bl = c.b;
         l++;
     }
@@ -1973,7 +2087,9 @@ void FldSolver::reset_clocks() {
 // Per launch at level 0 (f32 unless noted): the pre-smoother reads b, own, ax, ay and writes z and the coarse b (21
 // B per fine cell); the post-smoother reads b, own, ax, ay, z, the coarse z and z_old and writes z (33); the matvec
 // reads z and p and own, ax, ay and writes p and q (28); the update reads p, q, r and writes r, and reads and writes x
-// in f64 (32); the single-block V-cycle reads the pack and its first level's b and writes that level's z.
+// in f64 (32); the update folded into the next pre-smoother reads r, q, p and the coefficients, reads and writes x in
+// f64 and writes r, z and the coarse b (49); the single-block V-cycle reads the pack and its first level's b and writes
+// that level's z.
 double FldSolver::clock_bytes(int kernel) const {
     const double n = (double)w0 * h0;
     switch (kernel) {
@@ -1981,6 +2097,7 @@ double FldSolver::clock_bytes(int kernel) const {
     case CLK_POST0: return 33.0 * n;
     case CLK_MATVEC: return 28.0 * n;
     case CLK_UPDATE: return 32.0 * n;
+    case CLK_UPDATE_PRE0: return 49.0 * n;  // the update's 32 and the pre-smoother's 21 less its read of b
     case CLK_COARSE: return 4.0 * (pack_floats + 2.0 * lv[coarse_from].w * lv[coarse_from].h);
     default: return 0.0;
     }
@@ -2062,16 +2179,19 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     bool use_net = precond == SR_PRECOND_LEARNED;
     const bool mg = precond != 0 && lv.size() > 1;
     *st = FldStats();
+    // the current residual's buffer and the other one: an update folded into the next pre-smoother writes the new
+    // residual into the other buffer (the tiles read the old one as halo), and the two swap
+    float *rc = r32, *ro = r32b;
     // z = M r with partials of r.z and r.z_old into `dots` (their count into nbd)
     auto precondition = [&](float *zc, const float *zo, double *dots, int *nbd) -> int {
         if (use_net) {
-            SR_TRY(learned(r32, zc, zo, dots, s));
+            SR_TRY(learned(rc, zc, zo, dots, s));
             *nbd = nbe;
             return SR_OK;
         }
-        if (mg) return vcycle(r32, zc, zo, dots, nbd, s);
+        if (mg) return vcycle(rc, zc, zo, dots, nbd, s);
         L0 aa = {w0, h0, lv[0].own, lv[0].ax, lv[0].ay, gas};
-        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, aa, r32, zc, zo, dots, (const int *)nullptr);
+        hipLaunchKernelGGL(k_jacobi, dim3(nbe), dim3(NT), 0, s, aa, rc, zc, zo, dots, (const int *)nullptr);
         *nbd = nbe;
         return SR_OK;
     };
@@ -2085,7 +2205,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
     const int nmv = (int)(mv_grid.x * mv_grid.y);
     for (int outer = 0; outer < MAX_REFINEMENTS; outer++) {
-        hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, x, r32, ref ? ref->heat : nullptr,
+        hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, x, rc, ref ? ref->heat : nullptr,
                            ref ? ref->slope : nullptr, part_true);
         work.true_residuals++;
         double tr[4];
@@ -2118,32 +2238,50 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         int last_check = it, interval = 1;
         double rr_last = rr, left = HUGE_VAL;  // iterations left, as the last check's rate predicted them
         if (use_net && net_it == 0) rr_net0 = rr;
-        bool queued = false;  // the next iteration's z = M r already queued (the lookahead below)
+        // how much of the next iteration's z = M r is queued: 0 none, 1 its first launch (this iteration's update
+        // folded in), 2 all of it (the lookahead below)
+        int queued = 0;
         while (it < max_iter) {
             int slot = tick(CLK_MATVEC, s);
             hipLaunchKernelGGL(k_matvec_p, mv_grid, dim3(NT), 0, s, a, zc, pold, pnew, q, dots[k & 1],
                                dots[(k + 1) & 1], nbd, nbp, k == 0 ? 1 : 0, mv_rows, part_pq,
                                (const SolveCtl *)nullptr);
             SR_TRY(tock(slot, s));
-            slot = tick(CLK_UPDATE, s);
-            hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, x, r32, dots[k & 1], nbd,
-                               part_pq, nmv, part_rr, (const int *)nullptr);
-            SR_TRY(tock(slot, s));
+            // the update: folded into the next z = M r's level-0 pre-smoother where the V-cycle allows it (the new
+            // residual into the other buffer), else on its own, in place
+            const bool fold = mg && !use_net && update_fusable();
+            int nbd_next = nbe;
+            if (fold) {
+                const PendingUpdate U = {pnew, q, rc, ro, x, dots[k & 1], nbd, part_pq, nmv, part_rr_t};
+                SR_TRY(vcycle(ro, zo, zc, dots[(k + 1) & 1], &nbd_next, s, nullptr, &U, 1));
+                std::swap(rc, ro);
+                queued = 1;
+            } else {
+                slot = tick(CLK_UPDATE, s);
+                hipLaunchKernelGGL(k_update, dim3(nbe), dim3(NT), 0, s, (int)n, pnew, q, x, rc, dots[k & 1], nbd,
+                                   part_pq, nmv, part_rr, (const int *)nullptr);
+                SR_TRY(tock(slot, s));
+            }
             it++;
             work.iterations++;
             if (use_net) net_it++;
             if (it - last_check >= interval || it >= max_iter) {
-                // Lookahead: where the last rate predicts more than one iteration beyond this check, the next
-                // iteration's z = M r is queued behind the partials' copy before the host waits for them, so the
+                // Lookahead: where the last rate predicts more than one iteration beyond this check, the rest of the
+                // next iteration's z = M r is queued behind the partials' copy before the host waits for them, so the
                 // device works while the host decides. It writes the next z and its dot partials only (x and r are
                 // as this check reads them): when the check ends the loop after all, that z goes unused.
                 const bool ahead = solve_mode == SOLVE_HOST && !use_net && !timing && it < max_iter &&
                                    left - (double)(it - last_check) > 1.5;
-                SR_TRY(copy_partials(part_rr, 1, nbe, s));
-                int nbd_next = nbe;
-                if (ahead) SR_TRY(precondition(zo, zc, dots[(k + 1) & 1], &nbd_next));
+                const int nrr = fold ? ntiles0 : nbe;
+                SR_TRY(copy_partials(fold ? part_rr_t : part_rr, 1, nrr, s));
+                if (ahead) {
+                    if (fold)
+                        SR_TRY(vcycle(rc, zo, zc, dots[(k + 1) & 1], &nbd_next, s, nullptr, nullptr, 2));
+                    else
+                        SR_TRY(precondition(zo, zc, dots[(k + 1) & 1], &nbd_next));
+                }
                 double rri;
-                SR_TRY(wait_partials(1, nbe, &rri));
+                SR_TRY(wait_partials(1, nrr, &rri));
                 if (!std::isfinite(rri)) {
                     st->finite = false;
                     break;
@@ -2169,21 +2307,19 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                     interval = std::min(interval, NET_RATE_AT - net_it);  // the first rate check comes on time
                 last_check = it;
                 rr_last = rri;
-                if (ahead) {
-                    queued = true;
-                    nbp = nbd;
-                    nbd = nbd_next;
-                }
+                if (ahead) queued = 2;
             }
             std::swap(zc, zo);
             std::swap(pold, pnew);
             k++;
-            if (queued) {
-                queued = false;
-            } else {
-                nbp = nbd;
+            nbp = nbd;
+            if (queued == 2)
+                nbd = nbd_next;
+            else if (queued == 1)
+                SR_TRY(vcycle(rc, zc, zo, dots[k & 1], &nbd, s, nullptr, nullptr, 2));
+            else
                 SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
-            }
+            queued = 0;
             work.precond++;
         }
         if (!st->finite) break;
@@ -2396,6 +2532,12 @@ extern "C" int sr_fld_set_tiles(sr_fld *f, const int *pre, const int *post, int 
 extern "C" int sr_fld_set_coarse_wave(sr_fld *f, int on) {
     if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_coarse_wave: null solver");
     f->s.set_coarse_wave(on != 0);
+    return SR_OK;
+}
+
+extern "C" int sr_fld_set_update_fusion(sr_fld *f, int on) {
+    if (!f) return fail(SR_ERR_ARGUMENT, "sr_fld_set_update_fusion: null solver");
+    f->s.set_update_fusion(on != 0);
     return SR_OK;
 }
 

@@ -10,6 +10,7 @@ namespace sr {
 
 class Net;
 struct CoarseSet;
+struct PendingUpdate;
 struct SolveCtl;
 
 // How a solve decides when to stop: on the host from read-back partials (the learned preconditioner always), or on
@@ -89,6 +90,8 @@ class FldSolver {
     int set_fusion(int pre, int post);
     // the single-block V-cycle's smallest levels on one wave with no block barrier (on) or on the whole block from
     // LDS (off); the same values either way
+    void set_update_fusion(bool on) { fuse_update = on; }
+    bool update_fusion() const { return fuse_update; }
     void set_coarse_wave(bool on) {
         coarse_wave = on;
         tiles_version++;
@@ -104,7 +107,7 @@ class FldSolver {
     // kernel clocks (timing on): event pairs recorded on the stream around every launch of the level-0 pre- and
     // post-smoother, the single-block V-cycle, the matvec and the update, without waiting; summed per kernel when read
     // (clock), so timing does not stall the solve
-    enum { CLK_PRE0, CLK_POST0, CLK_COARSE, CLK_MATVEC, CLK_UPDATE, CLK_N };
+    enum { CLK_PRE0, CLK_POST0, CLK_COARSE, CLK_MATVEC, CLK_UPDATE, CLK_UPDATE_PRE0, CLK_N };
     void set_timing(bool on) { timing = on; }
     int clock(int kernel, double *ms, unsigned long long *launches);
     void reset_clocks();
@@ -123,6 +126,10 @@ class FldSolver {
     bool tuned = false, autotune = true;
     int fuse_pre = FUSE_SMALL, fuse_post = FUSE_SMALL;
     bool coarse_wave = true;  // the single-block V-cycle's levels of at most 16 x 16 cells on one wave
+    // host-decided solves: each PCG update folded into the next V-cycle's level-0 pre-smoother (needs 32 x 32 tiles
+    // at level 0 apart from the fused pair)
+    bool fuse_update = true;
+    bool update_fusable() const;
     bool fuse_fixed = false;
     // the finer level of the fused pair, or -1 when fewer than three levels are tile-smoothed (level 0 stays apart)
     int pair_level() const { return coarse_from >= 3 ? coarse_from - 2 : -1; }
@@ -135,11 +142,11 @@ class FldSolver {
     std::vector<FldLevel> lv;
     uint8_t *gas = nullptr;
     // level-0 vectors
-    float *r32 = nullptr, *za = nullptr, *zb = nullptr, *p = nullptr, *q = nullptr;
+    float *r32 = nullptr, *r32b = nullptr, *za = nullptr, *zb = nullptr, *p = nullptr, *q = nullptr;
     float *pack = nullptr; // the single-block levels' own/ax/ay back to back, the coarsest's inverse, 1/diag
     int pack_floats = 0;
     // per-block partial sums (summed by the kernel that consumes them, or on the host)
-    double *part_true = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_dots = nullptr;
+    double *part_true = nullptr, *part_pq = nullptr, *part_rr = nullptr, *part_rr_t = nullptr, *part_dots = nullptr;
     double *host_part = nullptr; // pinned
     // the device-decided solve: two copies of its state (by slot parity), the head's ticket counter, a pinned copy for
     // the host's checks; the captured pair of slots and what it was captured for
@@ -170,8 +177,10 @@ class FldSolver {
     Net *net = nullptr;
     int fallback_it = 0;
     double plain_rate = 0.0;  // ln(residual reduction) per iteration of the last converged plain multigrid solve
+    // `upd`: the pending update folded into level 0's pre-smoother (r is then the new residual's buffer); `stage`:
+    // 0 the whole V-cycle, 1 its first launch only, 2 the rest after it
     int vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s,
-               const int *gate = nullptr);
+               const int *gate = nullptr, const PendingUpdate *upd = nullptr, int stage = 0);
     CoarseSet coarse_set() const;
     int learned(const float *r, float *z, const float *zold, double *dots, hipStream_t s);
     int read_partials(const double *part, int k, int nb, double *out, hipStream_t s);

@@ -1290,6 +1290,12 @@ extern "C" int sr_set_fusion(sr_world *wd, int pre, int post) {
     return wd->sw.heat ? wd->heat.solver.set_fusion(pre, post) : SR_OK;
 }
 
+extern "C" int sr_set_update_fusion(sr_world *wd, int on) {
+    if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
+    if (wd->sw.heat) wd->heat.solver.set_update_fusion(on != 0);
+    return SR_OK;
+}
+
 extern "C" int sr_set_dt_cap(sr_world *wd, double cap) {
     if (!wd) return fail(SR_ERR_ARGUMENT, "null world");
     if (!(cap > 0.0) || !std::isfinite(cap)) return fail(SR_ERR_ARGUMENT, "a dt cap must be positive and finite");
@@ -1380,13 +1386,15 @@ extern "C" int sr_kernel_time(sr_world *wd, const char *kernel, uint64_t *launch
     } else if (k == "hydro_mid") {
         c = wd->k_mid;
         bytes = n * (60.0 + 8.0 + 4.0);
-    } else if (k == "mg_pre0" || k == "mg_post0" || k == "mg_coarse" || k == "fld_matvec" || k == "fld_update") {
+    } else if (k == "mg_pre0" || k == "mg_post0" || k == "mg_coarse" || k == "fld_matvec" || k == "fld_update" ||
+               k == "fld_update_pre0") {
         if (!wd->sw.heat) return fail(SR_ERR_ARGUMENT, "no heat pass");
-        const int id = k == "mg_pre0"     ? FldSolver::CLK_PRE0
-                       : k == "mg_post0"  ? FldSolver::CLK_POST0
-                       : k == "mg_coarse" ? FldSolver::CLK_COARSE
+        const int id = k == "mg_pre0"      ? FldSolver::CLK_PRE0
+                       : k == "mg_post0"   ? FldSolver::CLK_POST0
+                       : k == "mg_coarse"  ? FldSolver::CLK_COARSE
                        : k == "fld_matvec" ? FldSolver::CLK_MATVEC
-                                           : FldSolver::CLK_UPDATE;
+                       : k == "fld_update" ? FldSolver::CLK_UPDATE
+                                           : FldSolver::CLK_UPDATE_PRE0;
         SR_TRY(wd->heat.solver.clock(id, &c.ms, &c.launches));
         bytes = wd->heat.solver.clock_bytes(id);
     } else {

@@ -107,6 +107,11 @@ class Fld:
         from . import _check, lib
         _check(lib().sr_fld_set_fusion(self._p, pre, post))
 
+    def set_update_fusion(self, on: bool):
+        """Each PCG update folded into the next V-cycle's level-0 pre-smoother (True, the default) or on its own."""
+        from . import _check, lib
+        _check(lib().sr_fld_set_update_fusion(self._p, int(on)))
+
     def set_coarse_wave(self, on: bool):
         """The single-block V-cycle's smallest levels on one wave (True, the default) or on the whole block."""
         from . import _check, lib

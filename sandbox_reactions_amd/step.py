@@ -200,6 +200,10 @@ class World:
         """FUSE_OFF, FUSE_SMALL or FUSE_LARGE for the heat solver's last two tile-smoothed levels (as Fld.set_fusion)."""
         self._c(self.lib.sr_set_fusion(self._p, pre, post))
 
+    def set_update_fusion(self, on: bool):
+        """The heat solver's update folded into the next pre-smoother (True, the default) or on its own."""
+        self._c(self.lib.sr_set_update_fusion(self._p, int(on)))
+
     def set_dt_cap(self, cap: float):
         self._c(self.lib.sr_set_dt_cap(self._p, cap))
 

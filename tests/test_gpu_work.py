@@ -58,15 +58,19 @@ def test_kernel_clocks_count_every_launch(gpu_lib):
     w.run(2)
     k = w.read_work()
     clocks = {name: w.kernel_time(name) for name in ("hydro_update", "hydro_mid", "mg_pre0", "mg_post0",
-                                                       "mg_coarse", "fld_matvec", "fld_update")}
+                                                       "mg_coarse", "fld_matvec", "fld_update", "fld_update_pre0")}
     w.set_timing(False)
     for name, (launches, mean_ms, nbytes) in clocks.items():
-        assert launches > 0 and 0.0 < mean_ms < 50.0 and nbytes > 0, name
+        if name != "fld_update":  # every update is folded into the next pre-smoother here
+            assert launches > 0 and 0.0 < mean_ms < 50.0 and nbytes > 0, name
     assert clocks["hydro_update"][0] == 2 and clocks["hydro_mid"][0] == 2
-    # host-decided: every launch is counted work
-    for name in ("mg_pre0", "mg_post0", "mg_coarse"):
+    # host-decided: every launch is counted work; each update once, on its own or folded into the next pre-smoother
+    # (the fold at a solve's last check starts an application that the check ends)
+    for name in ("mg_post0", "mg_coarse"):
         assert clocks[name][0] == k.precond_applications, name
-    assert clocks["fld_matvec"][0] == clocks["fld_update"][0] == k.pcg_iterations
+    assert clocks["fld_matvec"][0] == clocks["fld_update"][0] + clocks["fld_update_pre0"][0] == k.pcg_iterations
+    assert clocks["mg_pre0"][0] + clocks["fld_update_pre0"][0] >= k.precond_applications
+    assert clocks["fld_update_pre0"][2] == 49 * n * n
     # device-decided, launched one by one with the clocks on: every slot's launches are clocked, the ones that skip
     # themselves past a solve's end included, and each inner iteration has exactly one application
     w.read_work(reset=True)
@@ -151,6 +155,38 @@ def test_the_lookahead_changes_no_solve_bit(gpu_lib):
     for mode in (sr.SOLVE_HOST, sr.SOLVE_HOST_WAIT):
         f = sr.Fld(n, n, coarse_scale=1.2)
         f.set_solve_mode(mode)
+        f.set_system(t(c), t(kx), t(ky))
+        x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        it, rel = f.solve(bt, x, tol=1e-8)
+        assert rel <= 1e-8
+        res.append((it, x.cpu().numpy()))
+    assert res[0][0] == res[1][0] and np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
+
+
+def test_the_folded_update_changes_no_bit(gpu_lib):
+    """The heat pass with each PCG update folded into the next level-0 pre-smoother and with it launched apart: the same
+    state to the last bit after two full steps at 512^2, and the standalone stiff solve at 512^2 in the same iterations
+    to the same FP64 answer."""
+    import torch
+    sr = gpu_lib
+    n = 512
+    print(f"folded update {n}^2", flush=True)
+    states = []
+    for on in (True, False):
+        w = make(sr, n)
+        w.set_update_fusion(on)
+        w.run(2)
+        states.append((w.readback(), w.read_dt().sim_time, w.read_work().pcg_iterations))
+        w.close()
+    assert np.array_equal(states[0][0].view(np.uint32), states[1][0].view(np.uint32))
+    assert states[0][1:] == states[1][1:]
+    c, kx, ky, b, _ = scenes.fld_system(n)
+    t = lambda a: torch.tensor(a.astype(np.float32), device="cuda").contiguous()
+    bt = torch.tensor(b, device="cuda")
+    res = []
+    for on in (True, False):
+        f = sr.Fld(n, n, coarse_scale=1.2)
+        f.set_update_fusion(on)
         f.set_system(t(c), t(kx), t(ky))
         x = torch.zeros((n, n), dtype=torch.float64, device="cuda")
         it, rel = f.solve(bt, x, tol=1e-8)
