@@ -54,7 +54,8 @@ COARSE_SCALE = 1.2
 
 
 # the kernels the world clocks when timing is on: the P4 passes and the PCG iteration's kernels
-KERNELS = ("hydro_update", "hydro_mid", "mg_pre0", "mg_post0", "mg_coarse", "fld_matvec", "fld_update")
+KERNELS = ("hydro_update", "hydro_mid", "mg_pre0", "mg_post0", "mg_coarse", "fld_matvec", "fld_update",
+           "fld_update_pre0")
 
 
 def full_step(sr, scenes, n, steps, warmup):
@@ -438,11 +439,14 @@ def main():
     trace = {"mg_pre0": ("sr::k_pre<32, 32, 2>", tiles32), "mg_post0": ("sr::k_post<32, 32, 2>", tiles32),
              "fld_matvec": ("sr::k_matvec_p", gxm * 256 * ((n + mv_rows - 1) // mv_rows)),
              "fld_update": ("sr::k_update", min(1024, (n * n + 1023) // 1024) * 256),
+             "fld_update_pre0": ("sr::k_pre_upd<32, 32, 2>", tiles32),
              "mg_coarse": ("sr::k_coarse<true>", 1024),
              "hydro_update": ("sr::k_hydro<1, false>", ((n + 15) // 16) ** 2 * 320),
              "hydro_mid": ("sr::k_hydro<0, false>", ((n + 15) // 16) ** 2 * 320)}
     per_step = {}
     for name, (launches, mean_ms, bytes_per) in fs["kernels"].items():
+        if launches == 0:  # e.g. the update on its own, when every update is folded into the next pre-smoother
+            continue
         per_step[name] = {"launches_per_step": round(launches / fs["inst"], 2), "mean_us": round(mean_ms * 1e3, 3),
                           "ms_per_step": round(launches * mean_ms / fs["inst"], 4),
                           "algorithmic_bytes_per_launch": bytes_per}

@@ -2204,12 +2204,24 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + 2047) / 2048));
     const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
     const int nmv = (int)(mv_grid.x * mv_grid.y);
+    bool again = false;  // the last inner loop stopped at inner_rtol, short of the outer goal
     for (int outer = 0; outer < MAX_REFINEMENTS; outer++) {
         hipLaunchKernelGGL(k_true_residual, dim3(nbe), dim3(NT), 0, s, a, b, x, rc, ref ? ref->heat : nullptr,
                            ref ? ref->slope : nullptr, part_true);
         work.true_residuals++;
         double tr[4];
-        SR_TRY(read_partials(part_true, 4, nbe, tr, s));
+        // Lookahead: on the first refinement, and after an inner loop that stopped at inner_rtol rather than at the
+        // goal, the next inner loop's first z = M r is queued behind the true residual's copy before the host waits;
+        // it writes z, the old z and their dot partials only, which go unused when the check ends the solve.
+        const bool ahead = solve_mode == SOLVE_HOST && !use_net && !timing && it < max_iter && (outer == 0 || again);
+        int nbd_ahead = nbe;
+        SR_TRY(copy_partials(part_true, 4, nbe, s));
+        if (ahead) {
+            SR_HIP(hipMemsetAsync(zb, 0, sizeof(float) * n, s));
+            SR_TRY(precondition(za, zb, dots[0], &nbd_ahead));
+        }
+        again = false;
+        SR_TRY(wait_partials(4, nbe, tr));
         const double rr = tr[0], bb = tr[1] > 0 ? tr[1] : 1.0;
         st->rel = std::sqrt(rr / bb);
         st->refinements = outer;
@@ -2230,10 +2242,14 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         const double stop2 = std::max(inner_rtol * inner_rtol * rr, 0.25 * goal * goal);
         float *zc = za, *zo = zb;
         float *pold = p, *pnew = p + n;
-        SR_HIP(hipMemsetAsync(zo, 0, sizeof(float) * n, s));
         int k = 0, nbd = nbe, nbp = nbe;
         // z = M r, partials of r.z (restart: beta = 0)
-        SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
+        if (ahead) {
+            nbd = nbd_ahead;
+        } else {
+            SR_HIP(hipMemsetAsync(zo, 0, sizeof(float) * n, s));
+            SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
+        }
         work.precond++;
         int last_check = it, interval = 1;
         double rr_last = rr, left = HUGE_VAL;  // iterations left, as the last check's rate predicted them
@@ -2286,7 +2302,10 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
                     st->finite = false;
                     break;
                 }
-                if (rri <= stop2) break;
+                if (rri <= stop2) {
+                    again = inner_rtol * inner_rtol * rr > 0.25 * goal * goal;
+                    break;
+                }
                 // with a fallback budget, the learned preconditioner gives way to plain multigrid once it has used
                 // the budget, or when from NET_RATE_AT iterations on it reduces the residual more slowly per
                 // iteration than plain multigrid did on its last solve of this system (checked at every check)

@@ -311,6 +311,7 @@ HeatPass::~HeatPass() {
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (host) (void)hipHostFree(host);
+    if (ev_sums) (void)hipEventDestroy(ev_sums);
 }
 
 int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
@@ -329,7 +330,24 @@ int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
     SR_HIP(hipMalloc(&xs, sizeof(double) * n));
     SR_HIP(hipMalloc(&part, sizeof(double) * 4 * nb));
     SR_HIP(hipHostMalloc(&host, sizeof(double) * 4 * nb));
+    SR_HIP(hipEventCreateWithFlags(&ev_sums, hipEventDisableTiming));
     return solver.init(P.w, P.h, hs.levels, hs.nu1, hs.nu2, hs.coarse_scale);
+}
+
+int HeatPass::copy_sums(int k, hipStream_t s) {
+    SR_HIP(hipMemcpyAsync(host, part, sizeof(double) * k * nb, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipEventRecord(ev_sums, s));
+    return SR_OK;
+}
+
+int HeatPass::wait_sums(int k, double *out) {
+    SR_HIP(hipEventSynchronize(ev_sums));
+    for (int j = 0; j < k; j++) {
+        double t = 0.0;
+        for (int i = 0; i < nb; i++) t += host[j * nb + i];
+        out[j] = t;
+    }
+    return SR_OK;
 }
 
 // the k sums of the last kernel's partials, in block order
@@ -375,17 +393,29 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     hipLaunchKernelGGL(k_heat_cells, dim3(g), dim3(NT), 0, s, P, st.sigma, st.mx, st.my, st.e, st.x, H, part);
     work.steps++;
     double heat0 = 0.0;
-    SR_TRY(read_sums(1, &heat0, s));
+    // Lookahead (relative stopping): the face law, the vacuum sums, the first pass's system and the hierarchy's
+    // rebuild are queued behind the heat sum's copy before the host waits for it; none of them writes what the commit
+    // of a step that does not solve reads (the system, its right-hand side and start, the solver's coefficients).
+    const bool ahead = hs.stop == SR_STOP_RELATIVE && !P.pinned && dt_host > 0.0;
+    SR_TRY(copy_sums(1, s));
     const int nf = std::max((P.w + 1) * P.h, P.w * (P.h + 1));
     hipLaunchKernelGGL(k_heat_law, dim3((nf + 255) / 256), dim3(256), 0, s, P, H, solver.ax0(), solver.ay0());
     hipLaunchKernelGGL(k_heat_kvac, dim3((n + 255) / 256), dim3(256), 0, s, P, H);
     SR_HIP(hipGetLastError());
+    bool built = false;  // the next pass's system built and the hierarchy rebuilt already
+    if (ahead) {
+        SR_HIP(hipMemcpyAsync(solver.gas0(), H.gas, n, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
+        SR_TRY(solver.rebuild(true, s));
+        built = true;
+    }
+    SR_TRY(wait_sums(1, &heat0));
     if (P.pinned || !(dt_host > 0.0) || !(heat0 > 0.0)) {
         hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, 0.0, st.e, fx, fy, 1);
         SR_HIP(hipGetLastError());
         return SR_OK;
     }
-    SR_HIP(hipMemcpyAsync(solver.gas0(), H.gas, n, hipMemcpyDeviceToDevice, s));
+    if (!built) SR_HIP(hipMemcpyAsync(solver.gas0(), H.gas, n, hipMemcpyDeviceToDevice, s));
     const double eps = hs.stop == SR_STOP_REFERENCE ? hs.solve_tol : 0.0;
     int passes = 0, rounds = 0;
     bool converged = false, faces = true;
@@ -397,7 +427,8 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     while (passes < hs.max_passes && rounds < hs.max_rounds) {
         passes++;
         work.passes++;
-        hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
+        if (!built)
+            hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
         // the reference's two bounds are needed (and read back) only under its stopping rule
         RefStop ref;
         if (hs.stop == SR_STOP_REFERENCE) {
@@ -412,7 +443,8 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
                 break;
             }
         }
-        SR_TRY(solver.rebuild(faces, s));
+        if (!built) SR_TRY(solver.rebuild(faces, s));
+        built = false;
         faces = false;
         FldStats fs;
         const int hint = passes <= (int)pass_rounds.size() ? pass_rounds[passes - 1] : 0;
@@ -425,7 +457,17 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
         if (!fs.finite) break;
         hipLaunchKernelGGL(k_heat_update, dim3(g), dim3(NT), 0, s, P, H, xs, hs.newton_cell, part);
         double mv[2];
-        SR_TRY(read_sums(2, mv, s));
+        // Lookahead (relative stopping): after a loose pass, which the tight one always follows, the next pass's
+        // system and rebuild are queued behind the sums' copy before the host waits; after a tight pass, which mostly
+        // ends the loop, they are not (the commit would wait behind them)
+        const bool next = hs.stop == SR_STOP_RELATIVE && eta > hs.solve_tol && passes < hs.max_passes;
+        SR_TRY(copy_sums(2, s));
+        if (next) {
+            hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
+            SR_TRY(solver.rebuild(false, s));
+            built = true;
+        }
+        SR_TRY(wait_sums(2, mv));
         if (mv[0] <= hs.newton_heat * heat0 && mv[1] == 0.0) {
             if (eta > hs.solve_tol) {  // converged on a loose solve: one more pass, solved to solve_tol
                 eta = hs.solve_tol;

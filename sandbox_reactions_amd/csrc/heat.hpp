@@ -68,6 +68,11 @@ class HeatPass {
 
   private:
     int read_sums(int k, double *out, hipStream_t s);
+    // the partials' copy queued with an event (copy_sums), work queued behind it, then the host's wait for the copy
+    // alone and the sums in block order (wait_sums)
+    int copy_sums(int k, hipStream_t s);
+    int wait_sums(int k, double *out);
+    hipEvent_t ev_sums = nullptr;
     HeatPlanes H;
     std::vector<int> pass_rounds;  // the last step's solve iterations per Newton pass: the next step's first bursts
     double *rhs = nullptr, *xs = nullptr, *part = nullptr;
