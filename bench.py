@@ -441,8 +441,8 @@ def main():
              "fld_update": ("sr::k_update", min(1024, (n * n + 1023) // 1024) * 256),
              "fld_update_pre0": ("sr::k_pre_upd<32, 32, 2>", tiles32),
              "mg_coarse": ("sr::k_coarse<true>", 1024),
-             "hydro_update": ("sr::k_hydro<1, false>", ((n + 15) // 16) ** 2 * 320),
-             "hydro_mid": ("sr::k_hydro<0, false>", ((n + 15) // 16) ** 2 * 320)}
+             "hydro_update": ("sr::k_hydro<1, false, false>", ((n + 15) // 16) ** 2 * 320),
+             "hydro_mid": ("sr::k_hydro<0, false, false>", ((n + 15) // 16) ** 2 * 320)}
     per_step = {}
     for name, (launches, mean_ms, bytes_per) in fs["kernels"].items():
         if launches == 0:  # e.g. the update on its own, when every update is folded into the next pre-smoother

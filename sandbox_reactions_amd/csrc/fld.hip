@@ -1958,9 +1958,13 @@ int FldSolver::launch_post_level(int l, const float *b, float *zout, const float
 }
 
 // z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
+// The fold pays where the update alone is latency-bound and loses where it streams: at 1024^2 the folded launch takes
+// 21.7 us against 25.2 for the update and the pre-smoother apart; at 2048^2 88.8 against 61.7, the update then moving
+// its bytes faster on its own (5.2 TB/s) than inside the tile kernel. So it is taken up to FOLD_MAX_CELLS.
+#define FOLD_MAX_CELLS (1u << 21)
 bool FldSolver::update_fusable() const {
     return fuse_update && coarse_from > 0 && pair_level() != 0 && !pre_tile.empty() && pre_tile[0] == TILE_32 &&
-           nu1 >= 1 && nu1 <= 3;
+           nu1 >= 1 && nu1 <= 3 && (size_t)w0 * h0 <= FOLD_MAX_CELLS;
 }
 
 int FldSolver::vcycle(const float *r, float *z, const float *zold, double *dots, int *nbd, hipStream_t s,
