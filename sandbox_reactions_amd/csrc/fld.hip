@@ -1315,48 +1315,71 @@ __global__ void k_coarsen_faces(Lv f, float *__restrict__ axc, float *__restrict
 
 // Once per rebuild, for the single-block V-cycle: every cell's inverse diagonal 1 / (own + its four couplings) on
 // every level of the set, and, when the coarsest holds at most DENSE_MAX cells, the inverse of its dense matrix by
-// Gauss-Jordan elimination, one thread per row (the matrix is SPD, so no pivoting); both into the pack.
+// Gauss-Jordan elimination (the matrix is SPD, so no pivoting); both into the pack. The elimination runs on the first
+// wave alone, row i of [A | I] in lane i's registers and the pivot row read across lanes, so it needs no block
+// barrier; the other waves form the inverse diagonals meanwhile.
 __global__ void __launch_bounds__(COARSE_NT) k_coarse_prep(CoarseSet C, float *__restrict__ pack) {
+    const int lc = C.n - 1, wl = C.w[lc], m = wl * C.h[lc];
+    const bool dense = m <= DENSE_MAX;
+    if (dense && threadIdx.x < 64) {
+        const int i = threadIdx.x;
+        const float *own = pack + C.offo[lc], *ax = pack + C.offx[lc], *ay = pack + C.offy[lc];
+        float A[2 * DENSE_MAX];
+#pragma unroll
+        for (int j = 0; j < 2 * DENSE_MAX; j++) A[j] = 0.0f;
+        if (i < m) {
+            const int x = i % wl, y = i / wl;
+            const float a[4] = {ax[y * (wl + 1) + x], ax[y * (wl + 1) + x + 1], ay[y * wl + x], ay[(y + 1) * wl + x]};
+            const int nb[4] = {i - 1, i + 1, i - wl, i + wl};
+            float d = own[i];
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                if (a[f] != 0.0f) {
+                    d += a[f];
+#pragma unroll
+                    for (int j = 0; j < DENSE_MAX; j++)
+                        if (j == nb[f]) A[j] -= a[f];
+                }
+#pragma unroll
+            for (int j = 0; j < DENSE_MAX; j++)
+                if (j == i) A[j] += d;
+#pragma unroll
+            for (int j = 0; j < 2 * DENSE_MAX; j++)
+                if (j == m + i) A[j] = 1.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < DENSE_MAX; k++) {
+            if (k >= m) break;
+            const float piv = __shfl(A[k], k, 64);
+            const float f = (i < m && i != k) ? A[k] / piv : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 2 * DENSE_MAX; j++) {
+                const float pk = __shfl(A[j], k, 64);
+                if (i < m && i != k) A[j] -= f * pk;
+            }
+        }
+        if (i < m) {
+            float d = 0.0f;
+#pragma unroll
+            for (int j = 0; j < DENSE_MAX; j++)
+                if (j == i) d = A[j];
+            float *ainv = pack + C.offinv;
+#pragma unroll
+            for (int j = 0; j < 2 * DENSE_MAX; j++)
+                if (j >= m && j < 2 * m) ainv[i * m + j - m] = A[j] / d;
+        }
+    }
+    const int t0 = dense ? (int)threadIdx.x - 64 : (int)threadIdx.x, nt = dense ? COARSE_NT - 64 : COARSE_NT;
+    if (t0 < 0) return;
     for (int l = 0; l < C.n; l++) {
         const int w = C.w[l], h = C.h[l];
         const float *own = pack + C.offo[l], *al = pack + C.offx[l], *au = pack + C.offy[l];
         float *rd = pack + C.offd[l];
-        for (int t = threadIdx.x; t < w * h; t += COARSE_NT) {
+        for (int t = t0; t < w * h; t += nt) {
             const int y = t / w, x = t - y * w;
             rd[t] = 1.0f / (own[t] + al[y * (w + 1) + x] + al[y * (w + 1) + x + 1] + au[t] + au[t + w]);
         }
     }
-    const int lc = C.n - 1, w = C.w[lc], h = C.h[lc], m = w * h, i = threadIdx.x;
-    if (m > DENSE_MAX) return;
-    const float *own = pack + C.offo[lc], *ax = pack + C.offx[lc], *ay = pack + C.offy[lc];
-    float *ainv = pack + C.offinv;
-    __shared__ float A[DENSE_MAX][2 * DENSE_MAX];
-    if (i < m) {
-        for (int j = 0; j < 2 * m; j++) A[i][j] = 0.0f;
-        const int x = i % w, y = i / w;
-        const float a[4] = {ax[y * (w + 1) + x], ax[y * (w + 1) + x + 1], ay[y * w + x], ay[(y + 1) * w + x]};
-        const int nb[4] = {i - 1, i + 1, i - w, i + w};
-        float d = own[i];
-        for (int f = 0; f < 4; f++)
-            if (a[f] != 0.0f) {
-                d += a[f];
-                A[i][nb[f]] -= a[f];
-            }
-        A[i][i] += d;
-        A[i][m + i] = 1.0f;
-    }
-    __syncthreads();
-    for (int k = 0; k < m; k++) {
-        const float piv = A[k][k];
-        float f = 0.0f;
-        if (i < m && i != k) f = A[i][k] / piv;
-        __syncthreads();
-        if (i < m && i != k)
-            for (int j = 0; j < 2 * m; j++) A[i][j] -= f * A[k][j];
-        __syncthreads();
-    }
-    if (i < m)
-        for (int j = 0; j < m; j++) ainv[i * m + j] = A[i][m + j] / A[i][i];
 }
 
 // the grid's edge faces carry no coupling (a neighbour past the edge does not exist)
