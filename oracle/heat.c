@@ -13,9 +13,12 @@
  *   commit_limited               heat.rs:544-592
  * With cell size dx a face's coefficient is k = D / dx^2 and its flux F = -D (u_b - u_a) / dx; R uses the gradient
  * |u_b - u_a| / dx. Heat capacity C = Sigma / (mu (gamma - 1)) (the reference's Sigma / mu at gamma = 2).
- * Parity unpinned: the exact Newton pass counts and delta of Heat::step are restated only; its converged result is
- * checked by the tests (the nonlinear balance cell by cell on the reference's radiating slab, the constant-diffusivity
- * moments, the solve against a direct solve, commit_limited on the reference's ceiling dump).
+ * Pinned by the tests: the Newton passes on a single gas cell in vacuum (a scalar balance, so each pass is a step of
+ * Newton's method and each solve exact) take as many passes as Newton's method takes steps under the reference's
+ * stopping rule, with and without the [T/2, 2T] clamp engaging, and commit the balance's root; delta by conservation
+ * (the grid loses exactly what it books as radiated); the converged result by the nonlinear balance cell by cell on
+ * the reference's radiating slab, the constant-diffusivity moments, the solve against a direct solve, and
+ * commit_limited by the reference's ceiling dump.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -121,3 +121,55 @@ def test_the_gpu_slab_books_what_it_lost_and_matches_the_twin(gpu_lib):
     assert abs(radiated - twin_radiated) <= 1e-4 * twin_radiated
     assert rel_l1(g[3], twin.state()[3]) <= 1e-5
     assert np.array_equal(g[:, ~slab], st.astype(np.float32)[:, ~slab])
+
+
+def scalar_newton(c, dt, kvac, t0, a_r, uvac):
+    """Newton's method on one cell's backward-Euler balance F(u) = (C/dt)(T(u) - T0) + k_vac (u - u_vac), u = a_r T^4,
+    from u(T0): u' = u - F(u) / F'(u), each new T clamped to [T/2, 2T], stopped by the reference's constants (the heat
+    moved at most 1e-12 of C T0 and the cell at most 1e-10 of its T, heat.rs:55-60). Returns (steps, T, clamped)."""
+    t, clamped = t0, False
+    for k in range(1, 129):
+        u = a_r * t ** 4
+        f = c / dt * (t - t0) + kvac * (u - uvac)
+        un = u - f / (c / dt / (4.0 * a_r * t ** 3) + kvac)
+        tn = (max(un, uvac) / a_r) ** 0.25
+        clamped |= not (0.5 * t <= tn <= 2.0 * t)
+        tn = min(max(tn, 0.5 * t), 2.0 * t)
+        d = abs(tn - t)
+        t_last, t = t, tn
+        if c * d <= 1e-12 * c * t0 and d <= 1e-10 * t_last:
+            return k, t, clamped
+    return None, t, clamped
+
+
+@pytest.mark.parametrize("dt, clamps", [(1e-3, False), (2.0, True), (1e5, True)])
+def test_one_cell_newton_passes_are_newtons_method(dt, clamps):
+    """A single gas cell (Sigma 1, T 30, hydrogen) in an 8 x 8 vacuum is a scalar equation, so the heat pass's Newton passes are
+    Newton's method on it and its linear solves are exact: the pass count is the number of Newton steps the
+    reference's stopping rule takes (the clamp to [T/2, 2T] engaged in the two longer steps), the committed T is the balance's
+    root found by bracketing (scipy's brentq), and the radiated heat is dt k_vac (u - u_vac) at that root. k_vac and C
+    come from the first pass's system (own = C/dt dT/du + k_vac, rhs = C/dt dT/du u0 + k_vac u_vac)."""
+    from scipy.optimize import brentq
+    s = rect_world(w=8, h=8, rect=(3, 3, 1, 1))
+    kw = s.physics_kwargs()
+    op, _ = both(**kw)
+    st = s.planes(kw["gamma"]).astype(np.float64)
+    a_r, uvac = op.a_r, op.a_r * op.t_floor ** 4
+    t0 = temps(st)[3, 3]
+    c = st[0, 3, 3] * INV_MU / (kw["gamma"] - 1.0)
+    own, _, _, rhs, gas, u0 = oracle.heat_first_system(op, st, dt)
+    assert gas.sum() == 1 and gas[3, 3]
+    ci = c / dt / (4.0 * a_r * t0 ** 3)
+    kvac = own[3, 3] - ci
+    assert kvac > 0 and rhs[3, 3] == pytest.approx(ci * u0[3, 3] + kvac * uvac, rel=1e-13)
+    steps, t_newton, clamped = scalar_newton(c, dt, kvac, t0, a_r, uvac)
+    assert steps is not None and clamped == clamps
+    root = brentq(lambda t: c / dt * (t - t0) + kvac * (a_r * t ** 4 - uvac), op.t_floor, t0, xtol=1e-15, rtol=1e-15)
+    p, rep, _, _, em = oracle.heat_step(op, st, dt)
+    print(f"dt {dt}: {rep.passes} passes, Newton's method {steps} steps; T {temps(p)[3, 3]:.12f}, root {root:.12f}",
+          flush=True)
+    assert rep.converged and rep.passes == steps
+    assert temps(p)[3, 3] == pytest.approx(root, rel=1e-12) and t_newton == pytest.approx(root, rel=1e-12)
+    assert rep.radiated == pytest.approx(dt * kvac * (a_r * root ** 4 - uvac), rel=1e-10)
+    assert em[3, 3] == rep.radiated and np.array_equal(np.delete(p.reshape(14, -1), 27, axis=1),
+                                                       np.delete(st.reshape(14, -1), 27, axis=1))
