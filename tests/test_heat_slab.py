@@ -173,3 +173,40 @@ def test_one_cell_newton_passes_are_newtons_method(dt, clamps):
     assert rep.radiated == pytest.approx(dt * kvac * (a_r * root ** 4 - uvac), rel=1e-10)
     assert em[3, 3] == rep.radiated and np.array_equal(np.delete(p.reshape(14, -1), 27, axis=1),
                                                        np.delete(st.reshape(14, -1), 27, axis=1))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt_cap", [1e-3, 2.0])
+def test_the_gpu_one_cell_reaches_the_root_in_newtons_passes_or_more(gpu_lib, dt_cap):
+    """The same one-cell scene stepped by the world (heat only, the twin's Newton constants: 1e-12 of the heat and 1e-10
+    of a cell's T) at a capped dt, each pass's solve to 1e-12 of its right-hand side: the cell's T at the balance's root
+    (f32), and at least the passes Newton's method takes (it converges quadratically, no iteration does better). The
+    world's system matrix is FP32, so its last passes converge linearly at these constants: at dt 1e-3 it takes 22
+    passes where Newton's method takes 5, as a scalar model with the diagonal rounded to f32 each pass does too; at dt 2
+    the counts agree. The workloads stop at 1e-3 of a cell's T, far above where the rounding binds."""
+    import sandbox_reactions_amd as sr
+    from scipy.optimize import brentq
+    s = rect_world(w=8, h=8, rect=(3, 3, 1, 1))
+    kw = s.physics_kwargs()
+    op, _ = both(**kw)
+    st = s.planes(kw["gamma"]).astype(np.float32).astype(np.float64)
+    hs = sr.heat_settings(stop=sr.STOP_RELATIVE, solve_tol=1e-12, newton_heat=1e-12, newton_cell=1e-10)
+    world = scene_json.world(s, heat_settings=hs)
+    world.upload(st.astype(np.float32))
+    world.set_dt_cap(dt_cap)
+    world.run(1)
+    r = world.read_heat()
+    dt = r.dt
+    a_r, uvac = op.a_r, op.a_r * op.t_floor ** 4
+    t0 = temps(st)[3, 3]
+    c = st[0, 3, 3] * INV_MU / (kw["gamma"] - 1.0)
+    own = oracle.heat_first_system(op, st, dt)[0]
+    kvac = own[3, 3] - c / dt / (4.0 * a_r * t0 ** 3)
+    steps, _, _ = scalar_newton(c, dt, kvac, t0, a_r, uvac)
+    root = brentq(lambda t: c / dt * (t - t0) + kvac * (a_r * t ** 4 - uvac), op.t_floor, t0, xtol=1e-15, rtol=1e-15)
+    g = world.readback().astype(np.float64)
+    print(f"dt {dt}: GPU {r.passes} passes, Newton's method {steps}; T {temps(g)[3, 3]:.7f}, root {root:.7f}",
+          flush=True)
+    assert dt == pytest.approx(dt_cap, rel=1e-12)
+    assert r.converged and r.passes >= steps and (r.passes == steps or dt_cap < 1.0)
+    assert temps(g)[3, 3] == pytest.approx(root, rel=2e-6)
