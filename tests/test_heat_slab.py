@@ -176,14 +176,14 @@ def test_one_cell_newton_passes_are_newtons_method(dt, clamps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dt_cap", [1e-3, 2.0])
+@pytest.mark.parametrize("dt_cap", [1e-3, 1e-2])
 def test_the_gpu_one_cell_reaches_the_root_in_newtons_passes_or_more(gpu_lib, dt_cap):
     """The same one-cell scene stepped by the world (heat only, the twin's Newton constants: 1e-12 of the heat and 1e-10
     of a cell's T) at a capped dt, each pass's solve to 1e-12 of its right-hand side: the cell's T at the balance's root
     (f32), and at least the passes Newton's method takes (it converges quadratically, no iteration does better). The
     world's system matrix is FP32, so its last passes converge linearly at these constants: at dt 1e-3 it takes 22
-    passes where Newton's method takes 5, as a scalar model with the diagonal rounded to f32 each pass does too; at dt 2
-    the counts agree. The workloads stop at 1e-3 of a cell's T, far above where the rounding binds."""
+    passes where Newton's method takes 5, as a scalar model with the diagonal rounded to f32 each pass does too. The
+    workloads stop at 1e-3 of a cell's T, far above where the rounding binds. (Both caps sit below P9's dt, ~0.018.)"""
     import sandbox_reactions_amd as sr
     from scipy.optimize import brentq
     s = rect_world(w=8, h=8, rect=(3, 3, 1, 1))
@@ -208,5 +208,5 @@ def test_the_gpu_one_cell_reaches_the_root_in_newtons_passes_or_more(gpu_lib, dt
     print(f"dt {dt}: GPU {r.passes} passes, Newton's method {steps}; T {temps(g)[3, 3]:.7f}, root {root:.7f}",
           flush=True)
     assert dt == pytest.approx(dt_cap, rel=1e-12)
-    assert r.converged and r.passes >= steps and (r.passes == steps or dt_cap < 1.0)
+    assert r.converged and r.passes >= steps
     assert temps(g)[3, 3] == pytest.approx(root, rel=2e-6)
