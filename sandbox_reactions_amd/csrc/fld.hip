@@ -253,6 +253,8 @@ __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__
     const int x = (tile % gridDim.x) * NT + threadIdx.x, y0 = (tile / gridDim.x) * rows, lane = threadIdx.x & 63;
     const bool in = x < w;
     const int xc = in ? x : w - 1;
+    // the coefficients never alias the stores, so the next row's loads may be issued ahead of this row's
+    const float *__restrict__ own = a.own, *__restrict__ axf = a.ax, *__restrict__ ayf = a.ay;
     auto P = [&](size_t j) { return z[j] + beta * pold[j]; };
     double v[1] = {0.0};
     const int y1 = min(y0 + rows, h);
@@ -260,15 +262,15 @@ __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__
         size_t i = (size_t)y0 * w + xc;
         float pu = y0 > 0 ? P(i - w) : 0.0f;
         float pc = P(i);
-        float au = a.ay[i];
+        float au = ayf[i];
         for (int y = y0; y < y1; y++, i += w) {
             const float pd = y + 1 < h ? P(i + w) : 0.0f;
-            const float ad = a.ay[i + w];
-            const float al = a.ax[(size_t)y * (w + 1) + xc], ar = a.ax[(size_t)y * (w + 1) + xc + 1];
+            const float ad = ayf[i + w];
+            const float al = axf[(size_t)y * (w + 1) + xc], ar = axf[(size_t)y * (w + 1) + xc + 1];
             float pl = __shfl_up(pc, 1, 64), pr = __shfl_down(pc, 1, 64);
             if (lane == 0 && xc > 0) pl = P(i - 1);
             if (lane == 63 && xc + 1 < w) pr = P(i + 1);
-            float s = a.own[i] * pc;
+            float s = own[i] * pc;
             if (al != 0.0f) s += al * (pc - pl);
             if (ar != 0.0f) s += ar * (pc - pr);
             if (au != 0.0f) s += au * (pc - pu);

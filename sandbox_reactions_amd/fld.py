@@ -16,7 +16,8 @@ class Fld:
         self.w, self.h = w, h
         hs = heat_settings(levels=levels, nu1=nu1, nu2=nu2, coarse_scale=coarse_scale)
         p = C.c_void_p()
-        _check(lib().sr_fld_create(w, h, C.byref(hs), C.byref(p)))
+        self._lib = lib()  # held for the destructor, which may run while the interpreter shuts down
+        _check(self._lib.sr_fld_create(w, h, C.byref(hs), C.byref(p)))
         self._p = p
 
     @staticmethod
@@ -117,8 +118,11 @@ class Fld:
         from . import _check, lib
         _check(lib().sr_fld_set_coarse_wave(self._p, int(on)))
 
-    def __del__(self):
-        from . import _LIB
-        if getattr(self, "_p", None) and _LIB is not None:
-            _LIB.sr_fld_destroy(self._p)
+    def close(self):
+        """Release the solver's device memory now (the destructor does it otherwise)."""
+        if getattr(self, "_p", None):
+            self._lib.sr_fld_destroy(self._p)
             self._p = None
+
+    def __del__(self):
+        self.close()

@@ -53,7 +53,8 @@ class Net:
     def __init__(self, max_w: int, max_h: int, params: dict | None = None):
         from . import _check, lib
         p = C.c_void_p()
-        _check(lib().sr_net_create(max_w, max_h, C.byref(p)))
+        self._lib = lib()  # held for the destructor, which may run while the interpreter shuts down
+        _check(self._lib.sr_net_create(max_w, max_h, C.byref(p)))
         self._p = p
         n = C.c_size_t()
         _check(lib().sr_net_param_count(self._p, C.byref(n)))
@@ -109,9 +110,8 @@ class Net:
         return out
 
     def __del__(self):
-        from . import _LIB
-        if getattr(self, "_p", None) and _LIB is not None:
-            _LIB.sr_net_destroy(self._p)
+        if getattr(self, "_p", None):
+            self._lib.sr_net_destroy(self._p)
             self._p = None
 
 
