@@ -311,7 +311,8 @@ HeatPass::~HeatPass() {
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (host) (void)hipHostFree(host);
-    if (ev_sums) (void)hipEventDestroy(ev_sums);
+    for (hipEvent_t e : ev_sums)
+        if (e) (void)hipEventDestroy(e);
 }
 
 int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
@@ -329,22 +330,23 @@ int HeatPass::init(const HeatParams &p, const sr_heat_settings &hs_) {
     SR_HIP(hipMalloc(&rhs, sizeof(double) * n));
     SR_HIP(hipMalloc(&xs, sizeof(double) * n));
     SR_HIP(hipMalloc(&part, sizeof(double) * 4 * nb));
-    SR_HIP(hipHostMalloc(&host, sizeof(double) * 4 * nb));
-    SR_HIP(hipEventCreateWithFlags(&ev_sums, hipEventDisableTiming));
+    SR_HIP(hipHostMalloc(&host, sizeof(double) * 2 * 4 * nb));
+    for (hipEvent_t &e : ev_sums) SR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return solver.init(P.w, P.h, hs.levels, hs.nu1, hs.nu2, hs.coarse_scale);
 }
 
-int HeatPass::copy_sums(int k, hipStream_t s) {
-    SR_HIP(hipMemcpyAsync(host, part, sizeof(double) * k * nb, hipMemcpyDeviceToHost, s));
-    SR_HIP(hipEventRecord(ev_sums, s));
+int HeatPass::copy_sums(int k, hipStream_t s, int slot) {
+    SR_HIP(hipMemcpyAsync(host + (size_t)slot * 4 * nb, part, sizeof(double) * k * nb, hipMemcpyDeviceToHost, s));
+    SR_HIP(hipEventRecord(ev_sums[slot], s));
     return SR_OK;
 }
 
-int HeatPass::wait_sums(int k, double *out) {
-    SR_HIP(hipEventSynchronize(ev_sums));
+int HeatPass::wait_sums(int k, double *out, int slot) {
+    SR_HIP(hipEventSynchronize(ev_sums[slot]));
+    const double *h = host + (size_t)slot * 4 * nb;
     for (int j = 0; j < k; j++) {
         double t = 0.0;
-        for (int i = 0; i < nb; i++) t += host[j * nb + i];
+        for (int i = 0; i < nb; i++) t += h[j * nb + i];
         out[j] = t;
     }
     return SR_OK;
@@ -418,7 +420,7 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     if (!built) SR_HIP(hipMemcpyAsync(solver.gas0(), H.gas, n, hipMemcpyDeviceToDevice, s));
     const double eps = hs.stop == SR_STOP_REFERENCE ? hs.solve_tol : 0.0;
     int passes = 0, rounds = 0;
-    bool converged = false, faces = true;
+    bool converged = false, faces = true, commit_ahead = false;
     std::vector<int> step_rounds;
     double last_rel = 0.0;
     // inexact Newton (relative stopping only): loose early passes, the last one to solve_tol
@@ -462,10 +464,16 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
         // ends the loop, they are not (the commit would wait behind them)
         const bool next = hs.stop == SR_STOP_RELATIVE && eta > hs.solve_tol && passes < hs.max_passes;
         SR_TRY(copy_sums(2, s));
+        // after an inexact solve's tight pass, which mostly ends the loop, the commit's sums instead (they read the
+        // iterate alone; unused when the loop goes on)
+        commit_ahead = inexact && !next;
         if (next) {
             hipLaunchKernelGGL(k_heat_system, dim3(g), dim3(NT), 0, s, P, H, scal, solver.own0(), rhs, xs, part);
             SR_TRY(solver.rebuild(false, s));
             built = true;
+        } else if (commit_ahead) {
+            hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, part);
+            SR_TRY(copy_sums(3, s, 1));
         }
         SR_TRY(wait_sums(2, mv));
         if (mv[0] <= hs.newton_heat * heat0 && mv[1] == 0.0) {
@@ -483,10 +491,14 @@ int HeatPass::step(const HydroLike &st, const double *scal, double dt_host, floa
     pass_rounds = step_rounds;
     report.final_residual = last_rel;
     if (converged) {
-        hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, part);
         work.commits++;
         double c3[3];
-        SR_TRY(read_sums(3, c3, s));
+        if (commit_ahead) {
+            SR_TRY(wait_sums(3, c3, 1));
+        } else {
+            hipLaunchKernelGGL(k_heat_commit_sums, dim3(g), dim3(NT), 0, s, P, H, scal, part);
+            SR_TRY(read_sums(3, c3, s));
+        }
         const double delta = c3[2] != 0.0 ? -(c3[1] + c3[0]) / c3[2] : 0.0;
         hipLaunchKernelGGL(k_heat_commit, dim3((n + 255) / 256), dim3(256), 0, s, P, H, delta, st.e, fx, fy, 0);
         SR_HIP(hipGetLastError());

@@ -70,9 +70,10 @@ class HeatPass {
     int read_sums(int k, double *out, hipStream_t s);
     // the partials' copy queued with an event (copy_sums), work queued behind it, then the host's wait for the copy
     // alone and the sums in block order (wait_sums)
-    int copy_sums(int k, hipStream_t s);
-    int wait_sums(int k, double *out);
-    hipEvent_t ev_sums = nullptr;
+    // into one of two pinned slots (4 nb doubles each), so a second copy can be in flight
+    int copy_sums(int k, hipStream_t s, int slot = 0);
+    int wait_sums(int k, double *out, int slot = 0);
+    hipEvent_t ev_sums[2] = {nullptr, nullptr};
     HeatPlanes H;
     std::vector<int> pass_rounds;  // the last step's solve iterations per Newton pass: the next step's first bursts
     double *rhs = nullptr, *xs = nullptr, *part = nullptr;
