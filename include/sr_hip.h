@@ -210,8 +210,8 @@ int sr_fld_set_fusion(sr_fld *f, int pre, int post);
 int sr_fld_set_coarse_wave(sr_fld *f, int on);
 /* Host-decided solves: each PCG iteration's update (x += alpha p, r -= alpha q, ||r||^2 partials) folded into the next
  * preconditioner application's level-0 pre-smoother (on, the default, where level 0 has 32 x 32 tiles apart from the
- * fused pair and at most 2^21 cells: on larger grids the update streams faster on its own), or launched on its own
- * (off). x, r and z take the same values either way; the ||r||^2 partials are
+ * fused pair and at most 2^20 + 2^18 cells: on larger grids the update streams faster on its own), or launched on
+ * its own (off). x, r and z take the same values either way; the ||r||^2 partials are
  * summed per tile when folded. No counterpart in the reference (its update is a round of its chained dispatches,
  * heat.wgsl:605-877). */
 int sr_fld_set_update_fusion(sr_fld *f, int on);

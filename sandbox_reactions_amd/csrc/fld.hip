@@ -1984,9 +1984,11 @@ int FldSolver::launch_post_level(int l, const float *b, float *zout, const float
 
 // z = M^-1 r on level 0; with `dots`, partials of r.z and r.zold into `dots` (their count returned in *nbd)
 // The fold pays where the update alone is latency-bound and loses where it streams: at 1024^2 the folded launch takes
-// 21.7 us against 25.2 for the update and the pre-smoother apart; at 2048^2 88.8 against 61.7, the update then moving
-// its bytes faster on its own (5.2 TB/s) than inside the tile kernel. So it is taken up to FOLD_MAX_CELLS.
-#define FOLD_MAX_CELLS (1u << 21)
+// 21.7 us against 25.2 for the update and the pre-smoother apart (the stiff solve 91.2 against 91.8 us per iteration);
+// at 1280^2 the solve takes 129.3 against 126.0 us per iteration, at 1448^2 159.3 against 146.7, at 2048^2 the folded
+// launch 88.8 us against 61.7, the update then moving its bytes faster on its own (5.2 TB/s) than inside the tile
+// kernel. So it is taken up to FOLD_MAX_CELLS (about 1145^2).
+#define FOLD_MAX_CELLS ((1u << 20) + (1u << 18))
 bool FldSolver::update_fusable() const {
     return fuse_update && coarse_from > 0 && pair_level() != 0 && !pre_tile.empty() && pre_tile[0] == TILE_32 &&
            nu1 >= 1 && nu1 <= 3 && (size_t)w0 * h0 <= FOLD_MAX_CELLS;
