@@ -239,6 +239,9 @@ __device__ inline float beta_of(const double *dots_cur, const double *dots_prev,
 // past the grid's right edge shadow its last column and store nothing. Every product is the one the face form
 // computes cell by cell, in the same order.
 #define MV_ROWS 16
+#ifndef MV_BLOCKS
+#define MV_BLOCKS 2048  // the matvec's target block count: rows per block = the grid's column strips over it
+#endif
 __global__ void __launch_bounds__(NT) k_matvec_p(L0 a, const float *__restrict__ z, const float *__restrict__ pold,
                                                  float *__restrict__ pnew, float *__restrict__ q,
                                                  const double *dots_cur, const double *dots_prev, int nbd, int nbp,
@@ -2232,7 +2235,7 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
     double *dots[2] = {part_dots, part_dots + 2 * nbd_max};
     // rows per matvec block: about 2048 blocks, 1 to 16 rows each
     const int gxm = (w0 + NT - 1) / NT;
-    const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + 2047) / 2048));
+    const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + MV_BLOCKS - 1) / MV_BLOCKS));
     const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
     const int nmv = (int)(mv_grid.x * mv_grid.y);
     bool again = false;  // the last inner loop stopped at inner_rtol, short of the outer goal
@@ -2405,7 +2408,7 @@ int FldSolver::launch_slot(int par, const double *b, double *x, const RefStop *r
                            gate);
     }
     const int gxm = (w0 + NT - 1) / NT;
-    const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + 2047) / 2048));
+    const int mv_rows = std::max(1, std::min(MV_ROWS, (h0 * gxm + MV_BLOCKS - 1) / MV_BLOCKS));
     const dim3 mv_grid(gxm, (h0 + mv_rows - 1) / mv_rows);
     const int nmv = (int)(mv_grid.x * mv_grid.y);
     int slot = tick(CLK_MATVEC, s);
