@@ -73,31 +73,59 @@ __global__ void __launch_bounds__(NT) k_true_residual(L0 a, const double *__rest
                                                       const float *__restrict__ slope, double *part) {
     const int n = a.w * a.h;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
-        const int xx = i % a.w, yy = i / a.w;
-        double r = 0.0;
-        if (is_gas(a, i)) {
-            const double xi = x[i];
-            const double al = a.ax[(size_t)yy * (a.w + 1) + xx], ar = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
-            const double au = a.ay[(size_t)yy * a.w + xx], ad = a.ay[(size_t)(yy + 1) * a.w + xx];
-            double ax = (double)a.own[i] * xi;
-            if (al != 0.0) ax += al * (xi - x[i - 1]);
-            if (ar != 0.0) ax += ar * (xi - x[i + 1]);
-            if (au != 0.0) ax += au * (xi - x[i - a.w]);
-            if (ad != 0.0) ax += ad * (xi - x[i + a.w]);
-            r = b[i] - ax;
-            v[1] += b[i] * b[i];
-            if (heat) {
-                const double d = (double)a.own[i] + al + ar + au + ad;
-                v[2] += r * r / d;
-                const double e = r * (double)slope[i] / d;
-                v[3] += (double)heat[i] * e * e;
+    const int stride = gridDim.x * NT;
+    for (int base = blockIdx.x * NT + threadIdx.x; base < n; base += stride * UNROLL) {
+        // the loads of UNROLL cells issued together (each cell's neighbours whatever its couplings, an index past the
+        // grid reading the cell itself; only a nonzero coupling uses the value), then the cells in order
+        double xv[UNROLL][5], bv[UNROLL];
+        float cv[UNROLL][5];
+        bool gv[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int i = base + u * stride;
+            if (i < n) {
+                const int xx = i % a.w, yy = i / a.w;
+                gv[u] = is_gas(a, i);
+                xv[u][0] = x[i];
+                xv[u][1] = x[xx > 0 ? i - 1 : i];
+                xv[u][2] = x[xx + 1 < a.w ? i + 1 : i];
+                xv[u][3] = x[yy > 0 ? i - a.w : i];
+                xv[u][4] = x[yy + 1 < a.h ? i + a.w : i];
+                bv[u] = b[i];
+                cv[u][0] = a.own[i];
+                cv[u][1] = a.ax[(size_t)yy * (a.w + 1) + xx];
+                cv[u][2] = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
+                cv[u][3] = a.ay[(size_t)yy * a.w + xx];
+                cv[u][4] = a.ay[(size_t)(yy + 1) * a.w + xx];
             }
-        } else {
-            x[i] = 0.0;
         }
-        r32[i] = (float)r;
-        v[0] += r * r;
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int i = base + u * stride;
+            if (i >= n) break;
+            double r = 0.0;
+            if (gv[u]) {
+                const double xi = xv[u][0];
+                const double al = cv[u][1], ar = cv[u][2], au = cv[u][3], ad = cv[u][4];
+                double ax = (double)cv[u][0] * xi;
+                if (al != 0.0) ax += al * (xi - xv[u][1]);
+                if (ar != 0.0) ax += ar * (xi - xv[u][2]);
+                if (au != 0.0) ax += au * (xi - xv[u][3]);
+                if (ad != 0.0) ax += ad * (xi - xv[u][4]);
+                r = bv[u] - ax;
+                v[1] += bv[u] * bv[u];
+                if (heat) {
+                    const double d = (double)cv[u][0] + al + ar + au + ad;
+                    v[2] += r * r / d;
+                    const double e = r * (double)slope[i] / d;
+                    v[3] += (double)heat[i] * e * e;
+                }
+            } else {
+                x[i] = 0.0;
+            }
+            r32[i] = (float)r;
+            v[0] += r * r;
+        }
     }
     block_partials<4>(v, part, gridDim.x);
 }
@@ -154,31 +182,59 @@ __global__ void __launch_bounds__(NT) k_slot_head(L0 a, const double *__restrict
     }
     const int n = a.w * a.h;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
-        const int xx = i % a.w, yy = i / a.w;
-        double r = 0.0;
-        if (is_gas(a, i)) {
-            const double xi = x[i];
-            const double al = a.ax[(size_t)yy * (a.w + 1) + xx], ar = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
-            const double au = a.ay[(size_t)yy * a.w + xx], ad = a.ay[(size_t)(yy + 1) * a.w + xx];
-            double ax = (double)a.own[i] * xi;
-            if (al != 0.0) ax += al * (xi - x[i - 1]);
-            if (ar != 0.0) ax += ar * (xi - x[i + 1]);
-            if (au != 0.0) ax += au * (xi - x[i - a.w]);
-            if (ad != 0.0) ax += ad * (xi - x[i + a.w]);
-            r = b[i] - ax;
-            v[1] += b[i] * b[i];
-            if (heat) {
-                const double d = (double)a.own[i] + al + ar + au + ad;
-                v[2] += r * r / d;
-                const double e = r * (double)slope[i] / d;
-                v[3] += (double)heat[i] * e * e;
+    const int stride = gridDim.x * NT;
+    for (int base = blockIdx.x * NT + threadIdx.x; base < n; base += stride * UNROLL) {
+        // the loads of UNROLL cells issued together (each cell's neighbours whatever its couplings, an index past the
+        // grid reading the cell itself; only a nonzero coupling uses the value), then the cells in order
+        double xv[UNROLL][5], bv[UNROLL];
+        float cv[UNROLL][5];
+        bool gv[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int i = base + u * stride;
+            if (i < n) {
+                const int xx = i % a.w, yy = i / a.w;
+                gv[u] = is_gas(a, i);
+                xv[u][0] = x[i];
+                xv[u][1] = x[xx > 0 ? i - 1 : i];
+                xv[u][2] = x[xx + 1 < a.w ? i + 1 : i];
+                xv[u][3] = x[yy > 0 ? i - a.w : i];
+                xv[u][4] = x[yy + 1 < a.h ? i + a.w : i];
+                bv[u] = b[i];
+                cv[u][0] = a.own[i];
+                cv[u][1] = a.ax[(size_t)yy * (a.w + 1) + xx];
+                cv[u][2] = a.ax[(size_t)yy * (a.w + 1) + xx + 1];
+                cv[u][3] = a.ay[(size_t)yy * a.w + xx];
+                cv[u][4] = a.ay[(size_t)(yy + 1) * a.w + xx];
             }
-        } else {
-            x[i] = 0.0;
         }
-        r32[i] = (float)r;
-        v[0] += r * r;
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int i = base + u * stride;
+            if (i >= n) break;
+            double r = 0.0;
+            if (gv[u]) {
+                const double xi = xv[u][0];
+                const double al = cv[u][1], ar = cv[u][2], au = cv[u][3], ad = cv[u][4];
+                double ax = (double)cv[u][0] * xi;
+                if (al != 0.0) ax += al * (xi - xv[u][1]);
+                if (ar != 0.0) ax += ar * (xi - xv[u][2]);
+                if (au != 0.0) ax += au * (xi - xv[u][3]);
+                if (ad != 0.0) ax += ad * (xi - xv[u][4]);
+                r = bv[u] - ax;
+                v[1] += bv[u] * bv[u];
+                if (heat) {
+                    const double d = (double)cv[u][0] + al + ar + au + ad;
+                    v[2] += r * r / d;
+                    const double e = r * (double)slope[i] / d;
+                    v[3] += (double)heat[i] * e * e;
+                }
+            } else {
+                x[i] = 0.0;
+            }
+            r32[i] = (float)r;
+            v[0] += r * r;
+        }
     }
     block_partials<4>(v, part, gridDim.x);
     __shared__ int last;
