@@ -13,6 +13,7 @@
 namespace sr {
 
 #define NT 256
+#define CELLS 4  // cells per thread whose loads a pass issues together (the grids hold n / (NT CELLS) blocks)
 #define NS 10
 
 __constant__ float h_imu[NS] = {2.0f, 0.75f, 7.0f / 12.0f, 9.0f / 16.0f, 11.0f / 20.0f,
@@ -163,24 +164,45 @@ __global__ void k_heat_system(HeatParams P, HeatPlanes H, const double *__restri
     const double dt = scal[0];
     const double uvac = u_of(P, P.t_floor);
     double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
-        if (!H.gas[i]) {
-            own[i] = 1.0f;
-            rhs[i] = 0.0;
-            xs[i] = 0.0;
-            H.heat[i] = 0.0f;
-            H.slope[i] = 0.0f;
-            continue;
+    const int stride = gridDim.x * NT;
+    for (int base = blockIdx.x * NT + threadIdx.x; base < n; base += stride * CELLS) {
+        // the loads of CELLS cells issued together, then the cells in order
+        uint8_t g[CELLS];
+        double tv[CELLS], t0v[CELLS];
+        float cv[CELLS], kvv[CELLS];
+#pragma unroll
+        for (int k = 0; k < CELLS; k++) {
+            const int i = base + k * stride;
+            g[k] = i < n ? H.gas[i] : 0;
+            if (i < n) {
+                tv[k] = H.t[i];
+                cv[k] = H.cap[i];
+                kvv[k] = H.kvac[i];
+                t0v[k] = H.t0[i];
+            }
         }
-        const double t = H.t[i], heat = (double)H.cap[i] / dt, slope = dt_du(P, t), ci = heat * slope;
-        const double u = u_of(P, t), kv = H.kvac[i];
-        own[i] = (float)(ci + kv);
-        rhs[i] = ci * u - heat * (t - (double)H.t0[i]) + kv * uvac;
-        xs[i] = u;
-        H.heat[i] = (float)heat;
-        H.slope[i] = (float)slope;
-        v[0] += ci * u * u;
-        v[1] += heat * t * t;
+#pragma unroll
+        for (int k = 0; k < CELLS; k++) {
+            const int i = base + k * stride;
+            if (i >= n) break;
+            if (!g[k]) {
+                own[i] = 1.0f;
+                rhs[i] = 0.0;
+                xs[i] = 0.0;
+                H.heat[i] = 0.0f;
+                H.slope[i] = 0.0f;
+                continue;
+            }
+            const double t = tv[k], heat = (double)cv[k] / dt, slope = dt_du(P, t), ci = heat * slope;
+            const double u = u_of(P, t), kv = kvv[k];
+            own[i] = (float)(ci + kv);
+            rhs[i] = ci * u - heat * (t - t0v[k]) + kv * uvac;
+            xs[i] = u;
+            H.heat[i] = (float)heat;
+            H.slope[i] = (float)slope;
+            v[0] += ci * u * u;
+            v[1] += heat * t * t;
+        }
     }
     block_partials<2>(v, part, gridDim.x);
 }
@@ -192,15 +214,34 @@ __global__ void k_heat_update(HeatParams P, HeatPlanes H, const double *__restri
     const int n = P.w * P.h;
     const double uvac = u_of(P, P.t_floor);
     double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
-        if (!H.gas[i]) continue;
-        const double t = H.t[i];
-        double nx = t_of(P, fmax(xs[i], uvac));
-        nx = fmin(fmax(nx, 0.5 * t), 2.0 * t);
-        const double d = fabs(nx - t);
-        v[0] += (double)H.cap[i] * d;
-        if (!(d <= newton_cell * t)) v[1] += 1.0;
-        H.t[i] = nx;
+    const int stride = gridDim.x * NT;
+    for (int base = blockIdx.x * NT + threadIdx.x; base < n; base += stride * CELLS) {
+        uint8_t g[CELLS];
+        double tv[CELLS], xv[CELLS];
+        float cv[CELLS];
+#pragma unroll
+        for (int k = 0; k < CELLS; k++) {
+            const int i = base + k * stride;
+            g[k] = i < n ? H.gas[i] : 0;
+            if (i < n) {
+                tv[k] = H.t[i];
+                xv[k] = xs[i];
+                cv[k] = H.cap[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CELLS; k++) {
+            const int i = base + k * stride;
+            if (i >= n) break;
+            if (!g[k]) continue;
+            const double t = tv[k];
+            double nx = t_of(P, fmax(xv[k], uvac));
+            nx = fmin(fmax(nx, 0.5 * t), 2.0 * t);
+            const double d = fabs(nx - t);
+            v[0] += (double)cv[k] * d;
+            if (!(d <= newton_cell * t)) v[1] += 1.0;
+            H.t[i] = nx;
+        }
     }
     block_partials<2>(v, part, gridDim.x);
 }
