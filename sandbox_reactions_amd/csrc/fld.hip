@@ -2302,14 +2302,11 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         double tr[4];
         // Lookahead: on the first refinement, and after an inner loop that stopped at inner_rtol rather than at the
         // goal, the next inner loop's first z = M r is queued behind the true residual's copy before the host waits;
-        // it writes z, the old z and their dot partials only, which go unused when the check ends the solve.
+        // it writes z and its dot partials only, which go unused when the check ends the solve.
         const bool ahead = solve_mode == SOLVE_HOST && !use_net && !timing && it < max_iter && (outer == 0 || again);
         int nbd_ahead = nbe;
         SR_TRY(copy_partials(part_true, 4, nbe, s));
-        if (ahead) {
-            SR_HIP(hipMemsetAsync(zb, 0, sizeof(float) * n, s));
-            SR_TRY(precondition(za, zb, dots[0], &nbd_ahead));
-        }
+        if (ahead) SR_TRY(precondition(za, nullptr, dots[0], &nbd_ahead));
         again = false;
         SR_TRY(wait_partials(4, nbe, tr));
         const double rr = tr[0], bb = tr[1] > 0 ? tr[1] : 1.0;
@@ -2333,13 +2330,11 @@ int FldSolver::solve(const double *b, double *x, double tol, int max_iter, int p
         float *zc = za, *zo = zb;
         float *pold = p, *pnew = p + n;
         int k = 0, nbd = nbe, nbp = nbe;
-        // z = M r, partials of r.z (restart: beta = 0)
-        if (ahead) {
+        // z = M r, partials of r.z (restart: beta = 0, so no z_old and its partial, which stays 0, is not read)
+        if (ahead)
             nbd = nbd_ahead;
-        } else {
-            SR_HIP(hipMemsetAsync(zo, 0, sizeof(float) * n, s));
-            SR_TRY(precondition(zc, zo, dots[k & 1], &nbd));
-        }
+        else
+            SR_TRY(precondition(zc, nullptr, dots[k & 1], &nbd));
         work.precond++;
         int last_check = it, interval = 1;
         double rr_last = rr, left = HUGE_VAL;  // iterations left, as the last check's rate predicted them
